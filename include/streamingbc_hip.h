@@ -1,0 +1,226 @@
+/*
+ * streamingbc_hip.h -- C ABI of the MI355X streaming betweenness-centrality
+ * engine (libstreamingbc_hip.so).
+ *
+ * Drop-in for the insertion path of the reference's public header
+ * src/streamingbc.h: the functions in section 1 keep the reference's names,
+ * argument order and meaning, so a C caller written against streamingbc.h
+ * (e.g. the InsertEdge/updateEdgeNEW loop of src/sbcMain_utils.c:458-574)
+ * compiles against this header and runs on the GPU.  What changes underneath:
+ *
+ *   - `struct stinger` is an opaque device-resident CSR with per-vertex append
+ *     slack (int32 neighbour ids, per-slot insertion sequence numbers) plus a
+ *     host mirror of the adjacency, instead of STINGER's blocked lists;
+ *   - `bcForest` keeps its public `totalBC` / `NV` fields, but `totalBC` is a
+ *     host mirror refreshed by streamingBCGetBC(); the per-root trees live in
+ *     HBM as structure-of-arrays (int32 dist, int64 sigma, FP64 delta) and are
+ *     read with streamingBCGetTree();
+ *   - `extraArraysPerThread` is an opaque device work context (HIP stream,
+ *     timing events, staging buffers).  Only element 0 of the array is used.
+ *
+ * Legacy (void / struct-returning) entry points abort with a message on
+ * error, as the reference's xmalloc does (src/stinger/xmalloc.c:24-55).  The
+ * sbc* entry points of section 2 return an sbcStatus instead;
+ * sbcLastErrorString() describes the last failure of the calling thread.
+ *
+ * Threading: one host thread at a time per forest/graph/context triple;
+ * different triples may be driven from different threads.
+ */
+#ifndef STREAMINGBC_HIP_H
+#define STREAMINGBC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* src/streamingbc.h:9-22 */
+typedef enum { COARSE, FINE } granularity_t;
+typedef enum { DELETE, INSERT } operation_t;
+typedef enum { NO_BALANCE, BALANCE } lbalance_t;
+
+/* src/streamingbc.h:29-34 */
+typedef struct {
+    int64_t NV;
+    int64_t NE;
+    int64_t *edgeArray;
+    int64_t *vertexPointerArray;
+} csrGraph;
+
+/* src/streamingbc_aux.h:42-48 (returned zeroed, as the reference does:
+ * src/streamingbc.c:220-225; per-root case data is in sbcItemRecord) */
+typedef struct {
+    int64_t movement;
+    int64_t adjacent;
+    int64_t sameLevel;
+    int64_t connectedComponents;
+} StreamingExtraInfo;
+
+struct stinger;                          /* device CSR + slack (replaces STINGER) */
+typedef struct sbcForestDev sbcForestDev;
+
+/* src/streamingbc_aux.h:35-39, with the tree pointer table replaced by an
+ * opaque device handle. */
+typedef struct {
+    sbcForestDev *dev;
+    double *totalBC; /* host mirror, valid after streamingBCGetBC() */
+    int64_t NV;
+} bcForest;
+typedef bcForest *bcForestPtr;
+
+typedef struct extraArraysPerThread extraArraysPerThread;
+
+typedef enum {
+    SBC_OK = 0,
+    SBC_ERR_ARG = 1,       /* bad argument (id out of range, size mismatch, ...) */
+    SBC_ERR_HIP = 2,       /* a HIP runtime call failed */
+    SBC_ERR_NOMEM = 3,     /* device or host allocation failed */
+    SBC_ERR_STATE = 4,     /* call order violated (e.g. update before static pass) */
+    SBC_ERR_EXISTS = 5,    /* edge already present or self loop */
+    SBC_ERR_LEVELS = 6,    /* BFS depth beyond the forest's level capacity */
+    SBC_ERR_BUDGET = 7     /* admission refused: footprint above the HBM budget */
+} sbcStatus;
+
+/* ------------------------------------------- 1. reference entry points */
+
+/* src/streamingbc.h:40 -- one tree per vertex (roots 0..NV-1) */
+bcForest *streamingBCCreateForestExact(int64_t NV);
+/* src/streamingbc.h:41 -- trees at roots[0..NK-1] (caller-owned array) */
+bcForest *streamingBCCreateForestApproximate(int64_t NV, int64_t NK, int64_t *rootArrayForApproximation);
+/* src/streamingbc.h:43-44 */
+void streamingBCDeleteForestExact(bcForestPtr *deadForest);
+void streamingBCDeleteForestApproximate(bcForestPtr *deadForest, int64_t rootArraySize, int64_t *rootArray);
+
+/* src/streamingbc.h:47-48 -- returns threadCount handles; element 0 is the
+ * device work context, the others alias it */
+extraArraysPerThread **streamingBCCreateAuxilary(int64_t threadCount, int64_t NV);
+void streamingBCDeleteAuxilary(extraArraysPerThread **parallelExtra, int64_t threadCount, int64_t NV);
+
+/* src/streamingbc.h:51-52 -- initial Brandes pass on the device */
+void streamingBCInitStaticExact(bcForest *forest, struct stinger *stingerGraph, int64_t NT,
+                                extraArraysPerThread **auxilary);
+void streamingBCInitStaticApproximate(bcForest *forest, struct stinger *stingerGraph, int64_t NT,
+                                      extraArraysPerThread **auxilary, int64_t NK,
+                                      int64_t *rootArrayForApproximation);
+
+/* src/streamingbc.h:55-56 -- the edge (newU,newV) must already have been
+ * added with stinger_insert_edge in both directions.  loadBalancing is
+ * accepted and ignored (every order gives identical trees);
+ * granularity COARSE runs one wavefront per root, FINE one workgroup per
+ * root.  NT is ignored. */
+StreamingExtraInfo insertEdgeStreamingBC(bcForest *forest, struct stinger *sStinger, int64_t newU,
+                                         int64_t newV, int64_t *rootArrayForApproximation, int64_t NK,
+                                         int64_t NV, int64_t NT, extraArraysPerThread **eAPT,
+                                         uint32_t loadBalancing, uint32_t granularity);
+
+/* src/streamingbc.h:68 -- builds the device graph from a CSR whose
+ * vertexPointerArray has NV+1 entries and edgeArray NE entries */
+void CreateStingerFromCSR(csrGraph *csr, struct stinger **stingerGraph);
+/* src/streamingbc.h:88 -- DIMACS10 file (1-based ids, phantom vertex 0) */
+void ParseDimacsGraph(char initial_graph_name[1024], struct stinger **stingerGraph, int64_t *NV,
+                      int64_t *NE);
+/* src/stinger/stinger.c:778-869 -- appends `to` to `from`'s list.  Returns 1
+ * if inserted, 0 if the edge already exists, -1 on error (self loop, bad id).
+ * type, weight and timestamp are ignored. */
+int stinger_insert_edge(struct stinger *G, int64_t type, int64_t from, int64_t to, int64_t weight,
+                        int64_t timestamp);
+/* src/stinger/stinger.c:72-95 */
+int64_t stinger_outdegree(struct stinger *G, int64_t v);
+/* src/stinger/stinger.c:558-590 -- frees the graph, returns NULL */
+struct stinger *stinger_free_all(struct stinger *S);
+
+/* ------------------------------------------------ 2. additions (status) */
+
+/* Status-returning forms of the creation, static and single-insertion calls
+ * (the section-1 forms abort on error instead). */
+sbcStatus sbcGraphCreate(int64_t NV, const int64_t *vertexPointerArray, const int64_t *edgeArray,
+                         struct stinger **out);
+/* roots == NULL: exact forest (NK must equal NV, roots 0..NV-1) */
+sbcStatus sbcForestCreate(int64_t NV, int64_t NK, const int64_t *roots, bcForest **out);
+sbcStatus sbcInitStatic(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT);
+/* Adds (u,v) to the graph in both directions, then runs the update:
+ * stinger_insert_edge x2 + insertEdgeStreamingBC as InsertEdge does
+ * (src/sbcMain_utils.c:458-471). */
+sbcStatus sbcInsertEdge(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t u,
+                        int64_t v, uint32_t granularity);
+
+/* Copies BC (NV doubles) to out (may be NULL) and refreshes forest->totalBC. */
+sbcStatus streamingBCGetBC(bcForest *forest, double *out);
+
+/* Per-root readout for parity checks: level and sigma use the reference's
+ * sentinel LONG_MAX for unreachable vertices (src/streamingbc_init.c:93-95).
+ * rootIndex indexes the root array given at creation. Any output may be NULL. */
+sbcStatus streamingBCGetTree(bcForest *forest, int64_t rootIndex, int64_t *level, int64_t *sigma,
+                             double *delta);
+
+/* Recomputes BC as sum over roots (in root-array order) of each tree's delta,
+ * skipping each tree's own root.  Deterministic: the result depends only on
+ * the trees, not on the order in which updates ran. */
+sbcStatus streamingBCRecomputeBC(bcForest *forest, extraArraysPerThread **eAPT);
+
+/* Per (root, insertion) record of the last update call. */
+typedef struct {
+    int32_t root_index;
+    int32_t insertion;  /* index within the call's batch */
+    int32_t kase;       /* 0: gap 0, 1: |gap| == 1, 2: |gap| >= 2 */
+    int32_t gap;        /* dist[u] - dist[v], clamped */
+    int64_t processed;  /* vertex visits, descent + ascent */
+    int64_t scanned;    /* adjacency entries scanned */
+    int64_t affected;   /* distinct vertices touched */
+    int64_t moved;      /* vertices whose distance changed */
+} sbcItemRecord;
+
+/* Batched persistent form: inserts edges (u[i], v[i]), i = 0..count-1, into
+ * the graph and the forest.  The batch is cut into windows of `window`
+ * insertions; each window is one persistent kernel launch (captured in a HIP
+ * graph and replayed) in which every root applies the window's insertions in
+ * stream order, each insertion seeing only edges inserted at or before it.
+ * rootOrder (optional, K entries): the order in which roots are taken from
+ * the device queue; heavyRoots: how many of them (from the front) are run by
+ * a whole workgroup, the rest by one wavefront each.  Results do not depend
+ * on rootOrder or heavyRoots.
+ * records (optional): K * count entries, [insertion][root].
+ * Duplicate edges (already in the graph or twice in the batch) and self loops
+ * are rejected with SBC_ERR_EXISTS before anything is modified. */
+typedef struct {
+    int32_t window;         /* insertions per window (>= 1) */
+    int32_t block_threads;  /* 64..1024, multiple of 64 (0: default 256) */
+    int32_t grid_blocks;    /* persistent grid size (0: one team per root, capped) */
+    int32_t use_graph;      /* capture each window launch in a hipGraph */
+    int32_t recompute_bc;   /* 1: BC = sum over roots after each window (deterministic) */
+    int32_t heavy_roots;    /* roots run in whole-workgroup mode (-1: all) */
+    const int32_t *root_order; /* NULL: root-array order */
+} sbcBatchConfig;
+
+sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,
+                                        const int64_t *u, const int64_t *v, int64_t count,
+                                        const sbcBatchConfig *cfg, sbcItemRecord *records);
+
+/* Records of the last insertEdgeStreamingBC call (K entries). */
+sbcStatus streamingBCLastRecords(extraArraysPerThread **eAPT, sbcItemRecord *out, int64_t K);
+
+/* Kernel timing, accumulated since the last reset, from HIP events recorded
+ * on the context's stream around each launch.  kernel: 0 update (plain and
+ * windowed), 1 static pass, 2 BC recompute, 3 edge append. */
+sbcStatus streamingBCKernelStats(extraArraysPerThread **eAPT, int32_t kernel, int64_t *launches,
+                                 double *total_ms);
+sbcStatus streamingBCResetStats(extraArraysPerThread **eAPT, int32_t enable_timing);
+
+/* Bytes of device memory a forest of NK roots over NV vertices occupies
+ * (state, per-root scratch and lists). */
+int64_t streamingBCForestFootprint(int64_t NV, int64_t NK);
+
+/* Blocks until all work queued on the context's stream has finished. */
+sbcStatus streamingBCSynchronize(extraArraysPerThread **eAPT);
+
+int sbcLastError(void);
+const char *sbcLastErrorString(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STREAMINGBC_HIP_H */

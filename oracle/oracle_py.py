@@ -1,0 +1,235 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the reference
+built from its sources (oracle/_ref/libsbcref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product package never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libsbcref.so")
+
+i64 = C.c_int64
+P64 = C.POINTER(C.c_int64)
+PD = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+PUSH, PULL = 0, 1
+COARSE, FINE = 0, 1
+
+ITEM_DTYPE = np.dtype([("kase", np.int32), ("gap", np.int32), ("processed", np.int64), ("scanned", np.int64),
+                       ("affected", np.int64), ("moved", np.int64), ("descent", np.int64)])
+
+_olib = None
+_rlib = None
+
+
+def _p64(a):
+    return a.ctypes.data_as(P64)
+
+
+def _pd(a):
+    return a.ctypes.data_as(PD)
+
+
+def oracle_lib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        L = C.CDLL(ORACLE_LIB)
+        for name, res, args in [
+            ("orc_graph_new", VP, [i64, P64, P64]),
+            ("orc_graph_free", None, [VP]),
+            ("orc_graph_insert", C.c_int, [VP, i64, i64]),
+            ("orc_graph_degree", i64, [VP, i64]),
+            ("orc_forest_new", VP, [i64, i64, P64]),
+            ("orc_forest_free", None, [VP]),
+            ("orc_static", None, [VP, VP, C.c_int]),
+            ("orc_insert", None, [VP, VP, i64, i64, C.c_int, C.c_void_p]),
+            ("orc_get_tree", None, [VP, i64, P64, P64, PD]),
+            ("orc_set_tree", None, [VP, i64, P64, P64, PD]),
+            ("orc_get_bc", None, [VP, PD]),
+            ("orc_set_bc", None, [VP, PD]),
+            ("orc_sigma_overflowed", C.c_int, [VP]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _olib = L
+    return _olib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib():
+    global _rlib
+    if _rlib is None:
+        if not ref_available():
+            raise ImportError(f"{REF_LIB} missing: run oracle/build_ref.sh where the reference is mounted")
+        L = C.CDLL(REF_LIB)
+        for name, res, args in [
+            ("ref_new", VP, [i64, i64, P64, P64, i64, P64, i64]),
+            ("ref_static", C.c_double, [VP]),
+            ("ref_insert", C.c_double, [VP, i64, i64, i64, i64]),
+            ("ref_get_tree", None, [VP, i64, P64, P64, PD]),
+            ("ref_get_bc", None, [VP, PD]),
+            ("ref_counters", None, [VP, P64]),
+            ("ref_free", None, [VP]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _rlib = L
+    return _rlib
+
+
+class Oracle:
+    """Sequential CPU restatement (see oracle/sbc_oracle.h)."""
+
+    def __init__(self, rowptr, col, roots):
+        L = oracle_lib()
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int64)
+        self.roots = np.ascontiguousarray(roots, dtype=np.int64)
+        self.nv = self.rowptr.shape[0] - 1
+        self.k = self.roots.shape[0]
+        self.g = L.orc_graph_new(self.nv, _p64(self.rowptr), _p64(self.col))
+        self.f = L.orc_forest_new(self.nv, self.k, _p64(self.roots))
+
+    def close(self):
+        L = oracle_lib()
+        if self.f:
+            L.orc_forest_free(self.f)
+            self.f = None
+        if self.g:
+            L.orc_graph_free(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def static(self, order=PUSH):
+        oracle_lib().orc_static(self.f, self.g, order)
+
+    def add_edge(self, u, v) -> int:
+        return oracle_lib().orc_graph_insert(self.g, u, v)
+
+    def update(self, u, v, mode=COARSE, items=False):
+        rec = np.zeros(self.k, dtype=ITEM_DTYPE) if items else None
+        oracle_lib().orc_insert(self.f, self.g, u, v, mode, rec.ctypes.data if items else None)
+        return rec
+
+    def insert(self, u, v, mode=COARSE, items=False):
+        r = self.add_edge(u, v)
+        if r != 1:
+            raise ValueError(f"edge ({u},{v}) not insertable: {r}")
+        return self.update(u, v, mode, items)
+
+    def tree(self, ri):
+        lv = np.zeros(self.nv, dtype=np.int64)
+        sg = np.zeros(self.nv, dtype=np.int64)
+        dl = np.zeros(self.nv, dtype=np.float64)
+        oracle_lib().orc_get_tree(self.f, ri, _p64(lv), _p64(sg), _pd(dl))
+        return lv, sg, dl
+
+    def trees(self):
+        lv = np.zeros((self.k, self.nv), dtype=np.int64)
+        sg = np.zeros((self.k, self.nv), dtype=np.int64)
+        dl = np.zeros((self.k, self.nv), dtype=np.float64)
+        L = oracle_lib()
+        for ri in range(self.k):
+            L.orc_get_tree(self.f, ri, _p64(lv[ri]), _p64(sg[ri]), _pd(dl[ri]))
+        return lv, sg, dl
+
+    def set_trees(self, lv, sg, dl):
+        L = oracle_lib()
+        lv = np.ascontiguousarray(lv, dtype=np.int64)
+        sg = np.ascontiguousarray(sg, dtype=np.int64)
+        dl = np.ascontiguousarray(dl, dtype=np.float64)
+        for ri in range(self.k):
+            L.orc_set_tree(self.f, ri, _p64(lv[ri]), _p64(sg[ri]), _pd(dl[ri]))
+
+    def bc(self):
+        out = np.zeros(self.nv, dtype=np.float64)
+        oracle_lib().orc_get_bc(self.f, _pd(out))
+        return out
+
+    def set_bc(self, bc):
+        bc = np.ascontiguousarray(bc, dtype=np.float64)
+        oracle_lib().orc_set_bc(self.f, _pd(bc))
+
+    def sigma_overflowed(self) -> bool:
+        return bool(oracle_lib().orc_sigma_overflowed(self.f))
+
+
+class Reference:
+    """The reference library compiled from its own sources (oracle/_ref)."""
+
+    def __init__(self, rowptr, col, roots, nt=1):
+        L = ref_lib()
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int64)
+        self.roots = np.ascontiguousarray(roots, dtype=np.int64)
+        self.nv = self.rowptr.shape[0] - 1
+        self.k = self.roots.shape[0]
+        self.h = L.ref_new(self.nv, self.col.shape[0], _p64(self.rowptr), _p64(self.col), self.k,
+                           _p64(self.roots), nt)
+
+    def close(self):
+        if self.h:
+            ref_lib().ref_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def static(self) -> float:
+        return ref_lib().ref_static(self.h)
+
+    def insert(self, u, v, lb=0, gran=COARSE) -> float:
+        t = ref_lib().ref_insert(self.h, u, v, lb, gran)
+        if t < 0:
+            raise ValueError(f"edge ({u},{v}) already present")
+        return t
+
+    def tree(self, ri):
+        lv = np.zeros(self.nv, dtype=np.int64)
+        sg = np.zeros(self.nv, dtype=np.int64)
+        dl = np.zeros(self.nv, dtype=np.float64)
+        ref_lib().ref_get_tree(self.h, ri, _p64(lv), _p64(sg), _pd(dl))
+        return lv, sg, dl
+
+    def trees(self):
+        lv = np.zeros((self.k, self.nv), dtype=np.int64)
+        sg = np.zeros((self.k, self.nv), dtype=np.int64)
+        dl = np.zeros((self.k, self.nv), dtype=np.float64)
+        L = ref_lib()
+        for ri in range(self.k):
+            L.ref_get_tree(self.h, ri, _p64(lv[ri]), _p64(sg[ri]), _pd(dl[ri]))
+        return lv, sg, dl
+
+    def bc(self):
+        out = np.zeros(self.nv, dtype=np.float64)
+        ref_lib().ref_get_bc(self.h, _pd(out))
+        return out
+
+    def counters(self):
+        out = np.zeros(4, dtype=np.int64)
+        ref_lib().ref_counters(self.h, _p64(out))
+        return out

@@ -1,0 +1,603 @@
+/*
+ * sbc_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY, never linked into the
+ * product).  See sbc_oracle.h for what each function restates and how it is
+ * pinned.  Compiled with -ffp-contract=off so every FP64 expression rounds
+ * exactly as written, like the reference built with plain gcc -O3.
+ */
+#include "sbc_oracle.h"
+
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define INF_LEVEL LONG_MAX /* INFINITY_MY, src/streamingbc_aux.h:11 */
+
+struct orc_graph {
+    int64_t nv;
+    int64_t *deg;
+    int64_t *cap;
+    int64_t **adj;
+};
+
+struct orc_forest {
+    int64_t nv, K;
+    int64_t *roots;
+    int64_t *level; /* K x nv */
+    int64_t *sigma; /* K x nv */
+    double *delta;  /* K x nv */
+    double *bc;     /* nv */
+    int ovf;
+    /* per-call scratch, reset on the touched entries only */
+    int64_t *touched, *nsig, *diff, *mdelta, *imoved, *nlev;
+    double *ndel, *bcpart;
+    int64_t *qd, *qu, *qs;
+    int64_t *lvl_d, *lvl_s; /* per-frontier offsets (fine mode) */
+};
+
+/* ------------------------------------------------------------------ graph */
+
+orc_graph *orc_graph_new(int64_t nv, const int64_t *rowptr, const int64_t *col)
+{
+    orc_graph *g = (orc_graph *)calloc(1, sizeof(orc_graph));
+    g->nv = nv;
+    g->deg = (int64_t *)calloc((size_t)nv, sizeof(int64_t));
+    g->cap = (int64_t *)calloc((size_t)nv, sizeof(int64_t));
+    g->adj = (int64_t **)calloc((size_t)nv, sizeof(int64_t *));
+    for (int64_t v = 0; v < nv; v++) {
+        int64_t d = rowptr[v + 1] - rowptr[v];
+        g->deg[v] = d;
+        g->cap[v] = d + 4;
+        g->adj[v] = (int64_t *)malloc((size_t)g->cap[v] * sizeof(int64_t));
+        memcpy(g->adj[v], col + rowptr[v], (size_t)d * sizeof(int64_t));
+    }
+    return g;
+}
+
+void orc_graph_free(orc_graph *g)
+{
+    if (!g) return;
+    for (int64_t v = 0; v < g->nv; v++) free(g->adj[v]);
+    free(g->adj);
+    free(g->deg);
+    free(g->cap);
+    free(g);
+}
+
+int64_t orc_graph_degree(const orc_graph *g, int64_t v) { return g->deg[v]; }
+
+static void push_adj(orc_graph *g, int64_t u, int64_t v)
+{
+    if (g->deg[u] == g->cap[u]) {
+        g->cap[u] = 2 * g->cap[u] + 4;
+        g->adj[u] = (int64_t *)realloc(g->adj[u], (size_t)g->cap[u] * sizeof(int64_t));
+    }
+    g->adj[u][g->deg[u]++] = v;
+}
+
+int orc_graph_insert(orc_graph *g, int64_t u, int64_t v)
+{
+    if (u == v || u < 0 || v < 0 || u >= g->nv || v >= g->nv) return -1;
+    for (int64_t i = 0; i < g->deg[u]; i++)
+        if (g->adj[u][i] == v) return 0;
+    push_adj(g, u, v);
+    push_adj(g, v, u);
+    return 1;
+}
+
+/* ------------------------------------------------------------------ forest */
+
+orc_forest *orc_forest_new(int64_t nv, int64_t K, const int64_t *roots)
+{
+    orc_forest *f = (orc_forest *)calloc(1, sizeof(orc_forest));
+    size_t n = (size_t)nv, kn = (size_t)K * n;
+    f->nv = nv;
+    f->K = K;
+    f->roots = (int64_t *)malloc((size_t)K * sizeof(int64_t));
+    memcpy(f->roots, roots, (size_t)K * sizeof(int64_t));
+    f->level = (int64_t *)malloc(kn * sizeof(int64_t));
+    f->sigma = (int64_t *)malloc(kn * sizeof(int64_t));
+    f->delta = (double *)calloc(kn, sizeof(double));
+    f->bc = (double *)calloc(n, sizeof(double));
+    for (size_t i = 0; i < kn; i++) { f->level[i] = INF_LEVEL; f->sigma[i] = INF_LEVEL; }
+    f->touched = (int64_t *)calloc(n, sizeof(int64_t));
+    f->nsig = (int64_t *)calloc(n, sizeof(int64_t));
+    f->diff = (int64_t *)calloc(n, sizeof(int64_t));
+    f->mdelta = (int64_t *)calloc(n, sizeof(int64_t));
+    f->imoved = (int64_t *)malloc(n * sizeof(int64_t));
+    f->nlev = (int64_t *)calloc(n, sizeof(int64_t));
+    for (size_t i = 0; i < n; i++) f->imoved[i] = -1;
+    f->ndel = (double *)calloc(n, sizeof(double));
+    f->bcpart = (double *)calloc(n, sizeof(double));
+    f->qd = (int64_t *)malloc(n * sizeof(int64_t));
+    f->qu = (int64_t *)malloc(n * sizeof(int64_t));
+    f->qs = (int64_t *)malloc(n * sizeof(int64_t));
+    f->lvl_d = (int64_t *)malloc((n + 2) * sizeof(int64_t));
+    f->lvl_s = (int64_t *)malloc((n + 2) * sizeof(int64_t));
+    return f;
+}
+
+void orc_forest_free(orc_forest *f)
+{
+    if (!f) return;
+    free(f->roots); free(f->level); free(f->sigma); free(f->delta); free(f->bc);
+    free(f->touched); free(f->nsig); free(f->diff); free(f->mdelta); free(f->imoved);
+    free(f->nlev); free(f->ndel); free(f->bcpart); free(f->qd); free(f->qu); free(f->qs);
+    free(f->lvl_d); free(f->lvl_s);
+    free(f);
+}
+
+void orc_get_tree(const orc_forest *f, int64_t ri, int64_t *level, int64_t *sigma, double *delta)
+{
+    size_t o = (size_t)ri * (size_t)f->nv, n = (size_t)f->nv;
+    if (level) memcpy(level, f->level + o, n * sizeof(int64_t));
+    if (sigma) memcpy(sigma, f->sigma + o, n * sizeof(int64_t));
+    if (delta) memcpy(delta, f->delta + o, n * sizeof(double));
+}
+
+void orc_set_tree(orc_forest *f, int64_t ri, const int64_t *level, const int64_t *sigma, const double *delta)
+{
+    size_t o = (size_t)ri * (size_t)f->nv, n = (size_t)f->nv;
+    memcpy(f->level + o, level, n * sizeof(int64_t));
+    memcpy(f->sigma + o, sigma, n * sizeof(int64_t));
+    memcpy(f->delta + o, delta, n * sizeof(double));
+}
+
+void orc_get_bc(const orc_forest *f, double *bc) { memcpy(bc, f->bc, (size_t)f->nv * sizeof(double)); }
+void orc_set_bc(orc_forest *f, const double *bc) { memcpy(f->bc, bc, (size_t)f->nv * sizeof(double)); }
+int orc_sigma_overflowed(const orc_forest *f) { return f->ovf; }
+
+/* sigma additions wrap like two's complement hardware; overflow is recorded */
+static inline int64_t sig_add(orc_forest *f, int64_t a, int64_t b)
+{
+    int64_t r;
+    if (__builtin_add_overflow(a, b, &r)) f->ovf = 1;
+    return (int64_t)((uint64_t)a + (uint64_t)b);
+}
+
+/* (sa / sb) * (d + 1), the dependency term of src/streamingbc_init.c:159-161 */
+static inline double dep(int64_t sa, int64_t sb, double d)
+{
+    return ((double)sa / (double)sb) * (double)(d + 1);
+}
+
+/* fold the per-insertion partials into BC: src/streamingbc.c:211-218 (NT = 1) */
+static void fold_bc(orc_forest *f)
+{
+    for (int64_t v = 0; v < f->nv; v++) {
+        f->bc[v] += f->bcpart[v];
+        f->bcpart[v] = 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------ static */
+
+/* BFS of src/streamingbc_init.c:92-148.  Writes the visit order to q and
+ * returns its length. */
+static int64_t static_bfs(orc_forest *f, const orc_graph *g, int64_t ri, int64_t *q)
+{
+    int64_t nv = f->nv, r = f->roots[ri];
+    int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+    double *D = f->delta + (size_t)ri * nv;
+    for (int64_t j = 0; j < nv; j++) { L[j] = INF_LEVEL; S[j] = INF_LEVEL; D[j] = 0; }
+    L[r] = 0;
+    S[r] = 1;
+    int64_t qh = 0, qt = 0;
+    q[qt++] = r;
+    while (qh < qt) {
+        int64_t c = q[qh++];
+        for (int64_t e = 0; e < g->deg[c]; e++) {
+            int64_t k = g->adj[c][e];
+            if (L[k] > L[c]) {
+                if (L[k] == INF_LEVEL) {
+                    L[k] = L[c] + 1;
+                    q[qt++] = k;
+                }
+                if (S[k] == INF_LEVEL) S[k] = S[c];
+                else S[k] = sig_add(f, S[k], S[c]);
+            }
+        }
+    }
+    return qt;
+}
+
+void orc_static(orc_forest *f, const orc_graph *g, int order)
+{
+    int64_t nv = f->nv;
+    for (int64_t v = 0; v < nv; v++) f->bc[v] = 0.0;
+    for (int64_t ri = 0; ri < f->K; ri++) {
+        int64_t r = f->roots[ri];
+        int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+        double *D = f->delta + (size_t)ri * nv;
+        int64_t n = static_bfs(f, g, ri, f->qd);
+        if (order == ORC_PUSH) {
+            /* stack ascent, src/streamingbc_init.c:155-180 */
+            for (int64_t i = n - 1; i >= 0; i--) {
+                int64_t c = f->qd[i];
+                for (int64_t e = 0; e < g->deg[c]; e++) {
+                    int64_t k = g->adj[c][e];
+                    if (L[k] == L[c] - 1) D[k] += dep(S[k], S[c], D[c]);
+                }
+                if (c != r) f->bcpart[c] += D[c];
+            }
+        } else {
+            /* pull: each vertex sums its children in adjacency order */
+            for (int64_t i = n - 1; i >= 0; i--) {
+                int64_t k = f->qd[i];
+                double acc = 0.0;
+                for (int64_t e = 0; e < g->deg[k]; e++) {
+                    int64_t c = g->adj[k][e];
+                    if (L[c] == L[k] + 1) acc += dep(S[k], S[c], D[c]);
+                }
+                D[k] = acc;
+            }
+            for (int64_t i = n - 1; i >= 0; i--) {
+                int64_t c = f->qd[i];
+                if (c != r) f->bcpart[c] += D[c];
+            }
+        }
+    }
+    fold_bc(f);
+}
+
+/* ------------------------------------------------------------------ coarse */
+
+static void reset_scratch(orc_forest *f, int64_t k)
+{
+    f->diff[k] = 0;
+    f->touched[k] = 0;
+    f->ndel[k] = 0.0;
+    f->mdelta[k] = 0;
+    f->imoved[k] = -1;
+    f->nsig[k] = 0;
+    f->nlev[k] = 0;
+}
+
+/* |gap| == 1: src/streamingbc_internal.c:11-248 */
+static void coarse_adjacent(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s, int64_t p,
+                            int64_t added, orc_item *it)
+{
+    int64_t nv = f->nv, r = f->roots[ri];
+    int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+    double *D = f->delta + (size_t)ri * nv;
+    int64_t *touched = f->touched, *nsig = f->nsig, *diff = f->diff;
+    double *ndel = f->ndel;
+    int64_t *qd = f->qd, *qu = f->qu;
+
+    nsig[s] = sig_add(f, S[s], added);
+    touched[s] = 1;
+    diff[s] = added;
+    int64_t nd = 0;
+    qd[nd++] = s;
+    /* descent (53-119): the frontier bookkeeping of the reference is a FIFO */
+    for (int64_t h = 0; h < nd; h++) {
+        int64_t c = qd[h];
+        int64_t tc1 = touched[c] + 1;
+        for (int64_t e = 0; e < g->deg[c]; e++) {
+            int64_t k = g->adj[c][e];
+            if (L[k] == L[c] + 1) {
+                if (touched[k] == 0) {
+                    nsig[k] = sig_add(f, nsig[k], S[k]);
+                    qd[nd++] = k;
+                    touched[k] += tc1;
+                }
+                nsig[k] = sig_add(f, nsig[k], diff[c]);
+                diff[k] = sig_add(f, diff[k], diff[c]);
+            }
+        }
+        it->processed++;
+        it->scanned += g->deg[c];
+    }
+    /* ascent (138-211): QueueDown read backwards merged with QueueUp */
+    int64_t di = nd - 1, uh = 0, un = 0;
+    for (;;) {
+        int64_t c;
+        if (di < 0 && uh >= un) break;
+        if (di < 0) c = qu[uh++];
+        else if (uh < un && !(L[qu[uh]] < L[qd[di]])) c = qu[uh++];
+        else c = qd[di--];
+        for (int64_t e = 0; e < g->deg[c]; e++) {
+            int64_t k = g->adj[c][e];
+            if (L[k] == L[c] - 1) {
+                if (touched[k] == 0) {
+                    ndel[k] += D[k];
+                    touched[k] = -1;
+                    qu[un++] = k;
+                    nsig[k] = sig_add(f, nsig[k], S[k]);
+                }
+                ndel[k] += dep(nsig[k], nsig[c], ndel[c]);
+                if (touched[k] < 0 && (k != p || c != s)) ndel[k] -= dep(S[k], S[c], D[c]);
+            }
+        }
+        it->processed++;
+        it->scanned += g->deg[c];
+        if (c != r) f->bcpart[c] += ndel[c] - D[c];
+    }
+    /* commit (213-242) */
+    for (int64_t i = 0; i < nd; i++) {
+        int64_t k = qd[i];
+        D[k] = ndel[k];
+        S[k] = nsig[k];
+        reset_scratch(f, k);
+    }
+    for (int64_t i = 0; i < un; i++) {
+        int64_t k = qu[i];
+        D[k] = ndel[k];
+        reset_scratch(f, k);
+    }
+    it->affected = nd + un;
+    it->descent = nd;
+    it->moved = 0;
+}
+
+/* |gap| >= 2: src/streamingbc_internal.c:251-603 */
+static void coarse_move(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s, int64_t p,
+                        int64_t prevDist, orc_item *it)
+{
+    int64_t nv = f->nv, r = f->roots[ri];
+    int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+    double *D = f->delta + (size_t)ri * nv;
+    int64_t *touched = f->touched, *nsig = f->nsig, *diff = f->diff, *md = f->mdelta, *im = f->imoved;
+    double *ndel = f->ndel;
+    int64_t *qd = f->qd, *qu = f->qu, *qs = f->qs;
+
+    int64_t nd = 0, nmoved = 0;
+    qd[nd++] = s;
+    touched[s] = 1;
+    nsig[s] = S[p];
+    diff[s] = S[p];
+    md[s] = prevDist;
+    im[s] = 1;
+    /* descent (294-386); levels are rewritten in place as each vertex retires */
+    for (int64_t h = 0; h < nd; h++) {
+        int64_t c = qd[h];
+        int64_t tc1 = touched[c] + 1;
+        for (int64_t e = 0; e < g->deg[c]; e++) {
+            int64_t k = g->adj[c][e];
+            if (touched[k] == 0) {
+                int64_t cd = md[c] - (L[c] - L[k] + 1);
+                touched[k] = tc1;
+                if (cd > 0) {
+                    nsig[k] = diff[c];
+                    diff[k] = diff[c];
+                    md[k] = cd;
+                    im[k] = 1;
+                    qd[nd++] = k;
+                } else if (cd == 0) {
+                    nsig[k] = sig_add(f, S[k], diff[c]);
+                    diff[k] = sig_add(f, diff[k], diff[c]);
+                    md[k] = 0;
+                    im[k] = 0;
+                    qd[nd++] = k;
+                } else {
+                    touched[k] = 0;
+                }
+            } else if (touched[k] == tc1) {
+                int64_t cd = md[c] - (L[c] - L[k] + 1);
+                if (cd >= 0) {
+                    nsig[k] = sig_add(f, nsig[k], diff[c]);
+                    diff[k] = sig_add(f, diff[k], diff[c]);
+                }
+            }
+        }
+        it->processed++;
+        it->scanned += g->deg[c];
+        L[c] -= md[c];
+        if (im[c] == 1) nmoved++;
+    }
+    /* The level-ordered drain (392-400) reproduces the descent order: every
+     * frontier is one new level and each level list is FIFO. */
+    int64_t di = nd - 1, uh = 0, un = 0, sh = 0, sn = 0;
+    int64_t depthDown = L[qd[di]], depthUp = -1, depthSame = -1;
+    for (;;) {
+        int64_t c;
+        if (depthDown == -1 && depthSame == -1 && depthUp == -1) break;
+        if (depthDown >= depthSame && depthDown >= depthUp) {
+            c = qd[di--];
+            depthDown = di < 0 ? -1 : L[qd[di]];
+        } else if (depthUp >= depthSame && depthUp >= depthDown) {
+            c = qu[uh++];
+            depthUp = uh >= un ? -1 : L[qu[uh]];
+        } else {
+            c = qs[sh++];
+            depthSame = sh >= sn ? -1 : L[qs[sh]];
+        }
+        for (int64_t e = 0; e < g->deg[c]; e++) {
+            int64_t k = g->adj[c][e];
+            if (L[k] == L[c] - 1) {
+                if (touched[k] == 0) {
+                    ndel[k] = D[k];
+                    touched[k] = -1;
+                    qu[un] = k;
+                    if (depthUp == -1) depthUp = L[qu[uh]];
+                    un++;
+                    nsig[k] = S[k];
+                }
+                ndel[k] += dep(nsig[k], nsig[c], ndel[c]);
+                if (touched[k] < 0 && (k != p || c != s)) ndel[k] -= dep(S[k], S[c], D[c]);
+            } else if (L[k] == L[c] && im[c] == 1 && im[k] < 0) {
+                if (touched[k] == 0) {
+                    ndel[k] = D[k];
+                    touched[k] = -2;
+                    qs[sn] = k;
+                    if (depthSame == -1) depthSame = L[qs[sh]];
+                    sn++;
+                    nsig[k] = S[k];
+                }
+                ndel[k] -= dep(S[k], S[c], D[c]);
+            }
+        }
+        it->processed++;
+        it->scanned += g->deg[c];
+        if (c != r) f->bcpart[c] += ndel[c] - D[c];
+    }
+    /* commit (547-591) */
+    for (int64_t i = 0; i < nd; i++) {
+        int64_t k = qd[i];
+        D[k] = ndel[k];
+        S[k] = nsig[k];
+        reset_scratch(f, k);
+    }
+    for (int64_t i = 0; i < sn; i++) { int64_t k = qs[i]; D[k] = ndel[k]; reset_scratch(f, k); }
+    for (int64_t i = 0; i < un; i++) { int64_t k = qu[i]; D[k] = ndel[k]; reset_scratch(f, k); }
+    it->affected = nd + un + sn;
+    it->descent = nd;
+    it->moved = nmoved;
+}
+
+/* ------------------------------------------------------------------ fine */
+
+/* Sequential statement of the FINE update: level-synchronous descent, then a
+ * level-by-level ascent in which every vertex pulls from its processed
+ * children in adjacency order (src/streamingbc_internal_fine.c:160-299 and
+ * 560-999).  s = deeper endpoint, p = shallower, moved = |gap| >= 2.
+ * touched[k]: >0 descent (frontier index + 1), -1 ascent-only (U/S). */
+static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s, int64_t p,
+                        int moved, orc_item *it)
+{
+    int64_t nv = f->nv, r = f->roots[ri];
+    int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+    double *D = f->delta + (size_t)ri * nv;
+    int64_t *touched = f->touched, *nsig = f->nsig, *diff = f->diff, *im = f->imoved, *nlev = f->nlev;
+    double *ndel = f->ndel;
+    int64_t *qd = f->qd, *qu = f->qu, *qs = f->qs;
+    int64_t *ld = f->lvl_d, *ls = f->lvl_s;
+
+    int64_t base = L[p] + 1; /* new level of s */
+    int64_t nd = 0, ns = 0, nmoved = 0;
+    qd[nd++] = s;
+    touched[s] = 1;
+    im[s] = moved ? 1 : 0;
+    nlev[s] = base;
+    diff[s] = S[p];
+    nsig[s] = moved ? S[p] : sig_add(f, S[s], S[p]);
+    int64_t nf = 0; /* frontiers: qd[ld[j] .. ld[j+1]) */
+    ld[0] = 0;
+    ls[0] = 0;
+    int64_t fs = 0, fe = 1;
+    while (fs < fe) {
+        int64_t lc = base + nf; /* new level of this frontier */
+        for (int64_t h = fs; h < fe; h++) {
+            int64_t c = qd[h];
+            for (int64_t e = 0; e < g->deg[c]; e++) {
+                int64_t k = g->adj[c][e];
+                if (touched[k] == 0) {
+                    int64_t cd = (L[k] == INF_LEVEL) ? 1 : L[k] - (lc + 1);
+                    if (cd >= 0) {
+                        touched[k] = nf + 2;
+                        im[k] = cd > 0 ? 1 : 0;
+                        nlev[k] = lc + 1;
+                        diff[k] = diff[c];
+                        qd[nd++] = k;
+                    } else if (cd == -1 && im[c] == 1) {
+                        touched[k] = -1; /* same level as a vertex that moved up */
+                        qs[ns++] = k;
+                    }
+                } else if (touched[k] == nf + 2) {
+                    diff[k] = sig_add(f, diff[k], diff[c]);
+                }
+            }
+            it->processed++;
+            it->scanned += g->deg[c];
+        }
+        for (int64_t h = fe; h < nd; h++) {
+            int64_t k = qd[h];
+            nsig[k] = im[k] == 1 ? diff[k] : sig_add(f, S[k], diff[k]);
+        }
+        nf++;
+        ld[nf] = fe;
+        ls[nf] = ns;
+        fs = fe;
+        fe = nd;
+    }
+    /* frontier j (0..nf-1) = qd[ld[j]..ld[j+1]), S vertices found while
+     * expanding frontier j = qs[ls[j]..ls[j+1]) and sit at level base + j */
+    for (int64_t i = 0; i < nd; i++) if (im[qd[i]] == 1) nmoved++;
+
+    int64_t un = 0, uprev = 0; /* U vertices of the current level: qu[uprev..un) */
+    for (int64_t lev = base + nf - 1; lev >= 0; lev--) {
+        int64_t j = lev - base;
+        int64_t d0 = 0, d1 = 0, s0 = 0, s1 = 0;
+        if (j >= 0) { d0 = ld[j]; d1 = ld[j + 1]; s0 = ls[j]; s1 = ls[j + 1]; }
+        int64_t u0 = uprev, u1 = un;
+        if (d1 - d0 + s1 - s0 + u1 - u0 == 0) break;
+        uprev = un;
+        for (int64_t w = 0; w < (d1 - d0) + (s1 - s0) + (u1 - u0); w++) {
+            int64_t k;
+            if (w < d1 - d0) k = qd[d0 + w];
+            else if (w < d1 - d0 + s1 - s0) k = qs[s0 + w - (d1 - d0)];
+            else k = qu[u0 + w - (d1 - d0) - (s1 - s0)];
+            int isD = touched[k] > 0;
+            int64_t sk = isD ? nsig[k] : S[k];
+            double acc = isD ? 0.0 : D[k];
+            for (int64_t e = 0; e < g->deg[k]; e++) {
+                int64_t c = g->adj[k][e];
+                int64_t tcv = touched[c];
+                int64_t lcv = tcv > 0 ? nlev[c] : L[c];
+                if (tcv != 0 && lcv == lev + 1) {
+                    int64_t sc = tcv > 0 ? nsig[c] : S[c];
+                    acc += dep(sk, sc, ndel[c]);
+                    if (!isD && (k != p || c != s)) acc -= dep(S[k], S[c], D[c]);
+                } else if (!isD && tcv > 0 && lcv == lev && im[c] == 1) {
+                    acc -= dep(S[k], S[c], D[c]);
+                } else if (tcv == 0 && lcv == lev - 1) {
+                    touched[c] = -1;
+                    qu[un++] = c;
+                }
+            }
+            ndel[k] = acc;
+            it->processed++;
+            it->scanned += g->deg[k];
+            if (k != r) f->bcpart[k] += ndel[k] - D[k];
+        }
+    }
+    for (int64_t i = 0; i < nd; i++) {
+        int64_t k = qd[i];
+        L[k] = nlev[k];
+        S[k] = nsig[k];
+        D[k] = ndel[k];
+        reset_scratch(f, k);
+    }
+    for (int64_t i = 0; i < ns; i++) { int64_t k = qs[i]; D[k] = ndel[k]; reset_scratch(f, k); }
+    for (int64_t i = 0; i < un; i++) { int64_t k = qu[i]; D[k] = ndel[k]; reset_scratch(f, k); }
+    it->affected = nd + ns + un;
+    it->descent = nd;
+    it->moved = nmoved;
+}
+
+/* ------------------------------------------------------------------ insert */
+
+void orc_insert(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int mode, orc_item *items)
+{
+    int64_t nv = f->nv;
+    for (int64_t ri = 0; ri < f->K; ri++) {
+        int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+        orc_item tmp, *it = items ? &items[ri] : &tmp;
+        memset(it, 0, sizeof(*it));
+        int64_t lu = L[u], lv = L[v];
+        int64_t diff;
+        /* both unreachable: gap 0; one unreachable: it moves up (the
+         * reference's LONG_MAX arithmetic gives the same classification) */
+        if (lu == INF_LEVEL && lv == INF_LEVEL) diff = 0;
+        else if (lu == INF_LEVEL) diff = 2 + (nv + 2);
+        else if (lv == INF_LEVEL) diff = -(2 + (nv + 2));
+        else diff = lu - lv;
+        it->gap = (int32_t)(diff > INT32_MAX ? INT32_MAX : (diff < -INT32_MAX ? -INT32_MAX : diff));
+        if (diff == 0) { it->kase = 0; continue; }
+        int64_t s = diff > 0 ? u : v, p = diff > 0 ? v : u;
+        int64_t ad = diff > 0 ? diff : -diff;
+        if (ad == 1) {
+            it->kase = 1;
+            if (mode == ORC_COARSE) coarse_adjacent(f, g, ri, s, p, S[p], it);
+            else fine_update(f, g, ri, s, p, 0, it);
+        } else {
+            it->kase = 2;
+            if (mode == ORC_COARSE) {
+                int64_t prev = (L[s] == INF_LEVEL) ? (INF_LEVEL - L[p] - 1) : ad - 1;
+                coarse_move(f, g, ri, s, p, prev, it);
+            } else {
+                fine_update(f, g, ri, s, p, 1, it);
+            }
+        }
+    }
+    fold_bc(f);
+}

@@ -1,0 +1,1107 @@
+// sbc_engine.cpp -- host runtime behind include/streamingbc_hip.h.
+//
+// Owns the device graph (CSR + append slack, with a host mirror used for
+// duplicate checks, slot assignment and relayout), the per-root forest in HBM,
+// and the work context (stream, timing events, window staging, cached HIP
+// graphs).  All compute runs in the kernels of sbc_kernels.hip; there is no
+// host fallback: every entry point fails if the device calls fail.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "sbc_graphgen.h"
+#include "sbc_kernels.h"
+#include "streamingbc_hip.h"
+
+// ------------------------------------------------------------ errors
+
+static thread_local int g_last_err = 0;
+static thread_local char g_last_msg[512] = "";
+
+static sbcStatus set_err(sbcStatus st, const char* fmt, ...)
+{
+    g_last_err = st;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_msg, sizeof(g_last_msg), fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+extern "C" int sbcLastError(void) { return g_last_err; }
+extern "C" const char* sbcLastErrorString(void) { return g_last_msg; }
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_err(SBC_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                         \
+    } while (0)
+
+[[noreturn]] static void legacy_abort(const char* where)
+{
+    fprintf(stderr, "streamingbc_hip: %s failed: %s\n", where, g_last_msg);
+    abort();
+}
+
+// ------------------------------------------------------------ graph
+
+static const int32_t kNoSeq = 0;
+
+struct stinger {
+    int64_t nv = 0;
+    // host mirror
+    std::vector<int32_t> start, deg, deg0, cap;
+    std::vector<int32_t> col, seq;
+    int64_t total_cap = 0;
+    int32_t seq_next = 1;
+    // device image
+    SbcVInfo* d_vinfo = nullptr;
+    int32_t* d_col = nullptr;
+    int32_t* d_seq = nullptr;
+    int64_t d_cap = 0;   // slots allocated on the device
+    bool full_dirty = true;
+    uint64_t generation = 0; // bumps when device arrays are reallocated
+    // pending per-slot writes and vertex-info updates (legacy path)
+    std::vector<int32_t> p_slot, p_col, p_seq, p_v;
+};
+
+static int32_t slack_for(int32_t d) { return 4 + d / 16; }
+
+static sbcStatus graph_init(stinger* G, int64_t nv, const int64_t* rowptr, const int64_t* col)
+{
+    if (nv < 1 || nv >= INT32_MAX) return set_err(SBC_ERR_ARG, "NV %ld out of range", (long)nv);
+    if (rowptr[0] != 0) return set_err(SBC_ERR_ARG, "vertexPointerArray[0] must be 0");
+    G->nv = nv;
+    G->start.resize(nv);
+    G->deg.resize(nv);
+    G->deg0.resize(nv);
+    G->cap.resize(nv);
+    int64_t tot = 0;
+    for (int64_t v = 0; v < nv; v++) {
+        int64_t d = rowptr[v + 1] - rowptr[v];
+        if (d < 0 || d > INT32_MAX / 2) return set_err(SBC_ERR_ARG, "bad degree at vertex %ld", (long)v);
+        G->start[v] = (int32_t)tot;
+        G->deg[v] = (int32_t)d;
+        G->deg0[v] = (int32_t)d;
+        G->cap[v] = (int32_t)d + slack_for((int32_t)d);
+        tot += G->cap[v];
+        if (tot >= INT32_MAX) return set_err(SBC_ERR_ARG, "graph too large for int32 slot ids");
+    }
+    G->total_cap = tot;
+    G->col.assign(tot, 0);
+    G->seq.assign(tot, kNoSeq);
+    for (int64_t v = 0; v < nv; v++) {
+        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; e++) {
+            int64_t x = col[e];
+            if (x < 0 || x >= nv) return set_err(SBC_ERR_ARG, "edge %ld -> %ld out of range", (long)v, (long)x);
+            G->col[G->start[v] + (e - rowptr[v])] = (int32_t)x;
+        }
+    }
+    G->full_dirty = true;
+    return SBC_OK;
+}
+
+// Rebuilds the slot layout with fresh slack for every vertex (when a vertex
+// runs out of append room); the device image is re-uploaded on next flush.
+static void graph_relayout(stinger* G)
+{
+    std::vector<int32_t> nstart(G->nv), ncap(G->nv);
+    int64_t tot = 0;
+    for (int64_t v = 0; v < G->nv; v++) {
+        nstart[v] = (int32_t)tot;
+        int32_t want = G->deg[v] + slack_for(G->deg[v]);
+        ncap[v] = G->cap[v] > want ? G->cap[v] : want;
+        tot += ncap[v];
+    }
+    std::vector<int32_t> ncol(tot, 0), nseq(tot, kNoSeq);
+    for (int64_t v = 0; v < G->nv; v++) {
+        memcpy(&ncol[nstart[v]], &G->col[G->start[v]], sizeof(int32_t) * G->deg[v]);
+        memcpy(&nseq[nstart[v]], &G->seq[G->start[v]], sizeof(int32_t) * G->deg[v]);
+    }
+    G->start.swap(nstart);
+    G->cap.swap(ncap);
+    G->col.swap(ncol);
+    G->seq.swap(nseq);
+    G->total_cap = tot;
+    G->full_dirty = true;
+    G->p_slot.clear(); G->p_col.clear(); G->p_seq.clear(); G->p_v.clear();
+}
+
+static bool graph_has_edge(const stinger* G, int32_t u, int32_t v)
+{
+    if (G->deg[v] < G->deg[u]) { int32_t t = u; u = v; v = t; }
+    const int32_t* a = &G->col[G->start[u]];
+    for (int32_t i = 0; i < G->deg[u]; i++)
+        if (a[i] == v) return true;
+    return false;
+}
+
+// Appends v to u's list in the host mirror; returns the slot.
+static int32_t graph_push(stinger* G, int32_t u, int32_t v, int32_t seq)
+{
+    if (G->deg[u] == G->cap[u]) graph_relayout(G);
+    int32_t slot = G->start[u] + G->deg[u];
+    G->col[slot] = v;
+    G->seq[slot] = seq;
+    G->deg[u]++;
+    return slot;
+}
+
+static SbcVInfo vinfo_of(const stinger* G, int32_t v)
+{
+    SbcVInfo x;
+    x.start = G->start[v];
+    x.deg = G->deg[v];
+    x.deg0 = G->deg0[v];
+    x.pad = 0;
+    return x;
+}
+
+static sbcStatus graph_upload_full(stinger* G, hipStream_t st)
+{
+    HIPCHK(hipStreamSynchronize(st));
+    if (G->d_cap < G->total_cap || !G->d_vinfo) {
+        if (G->d_col) HIPCHK(hipFree(G->d_col));
+        if (G->d_seq) HIPCHK(hipFree(G->d_seq));
+        if (!G->d_vinfo) HIPCHK(hipMalloc(&G->d_vinfo, sizeof(SbcVInfo) * G->nv));
+        int64_t cap = G->total_cap + G->total_cap / 8 + 64;
+        HIPCHK(hipMalloc(&G->d_col, sizeof(int32_t) * cap));
+        HIPCHK(hipMalloc(&G->d_seq, sizeof(int32_t) * cap));
+        G->d_cap = cap;
+        G->generation++;
+    }
+    std::vector<SbcVInfo> vi(G->nv);
+    for (int64_t v = 0; v < G->nv; v++) vi[v] = vinfo_of(G, (int32_t)v);
+    HIPCHK(hipMemcpy(G->d_vinfo, vi.data(), sizeof(SbcVInfo) * G->nv, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G->d_col, G->col.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G->d_seq, G->seq.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice));
+    G->full_dirty = false;
+    G->p_slot.clear(); G->p_col.clear(); G->p_seq.clear(); G->p_v.clear();
+    return SBC_OK;
+}
+
+// Sends pending appends (legacy stinger_insert_edge path) to the device.
+static sbcStatus graph_flush(stinger* G, hipStream_t st)
+{
+    if (G->full_dirty) return graph_upload_full(G, st);
+    size_t ns = G->p_slot.size(), nvu = G->p_v.size();
+    size_t i = 0, j = 0;
+    while (i < ns || j < nvu) {
+        SbcAppend A;
+        memset(&A, 0, sizeof(A));
+        A.vinfo = G->d_vinfo;
+        A.col = G->d_col;
+        A.seq = G->d_seq;
+        while (i < ns && A.nslots < SBC_APPEND_INLINE) {
+            A.slot[A.nslots] = G->p_slot[i];
+            A.scol[A.nslots] = G->p_col[i];
+            A.sseq[A.nslots] = G->p_seq[i];
+            A.nslots++;
+            i++;
+        }
+        while (j < nvu && A.nvupd < SBC_APPEND_INLINE) {
+            A.vv[A.nvupd] = G->p_v[j];
+            A.vi[A.nvupd] = vinfo_of(G, G->p_v[j]);
+            A.nvupd++;
+            j++;
+        }
+        HIPCHK(sbc_launch_append(&A, SBC_APPEND_INLINE, st));
+    }
+    G->p_slot.clear(); G->p_col.clear(); G->p_seq.clear(); G->p_v.clear();
+    return SBC_OK;
+}
+
+static void graph_free(stinger* G)
+{
+    if (!G) return;
+    if (G->d_vinfo) (void)hipFree(G->d_vinfo);
+    if (G->d_col) (void)hipFree(G->d_col);
+    if (G->d_seq) (void)hipFree(G->d_seq);
+    delete G;
+}
+
+extern "C" void CreateStingerFromCSR(csrGraph* csr, struct stinger** stingerGraph)
+{
+    stinger* G = new stinger();
+    if (!csr || graph_init(G, csr->NV, csr->vertexPointerArray, csr->edgeArray) != SBC_OK) {
+        delete G;
+        legacy_abort("CreateStingerFromCSR");
+    }
+    *stingerGraph = G;
+}
+
+extern "C" void ParseDimacsGraph(char initial_graph_name[1024], struct stinger** stingerGraph, int64_t* NV,
+                                 int64_t* NE)
+{
+    sbcHostGraph* h = sbcReadDimacs(initial_graph_name);
+    if (!h) {
+        set_err(SBC_ERR_ARG, "cannot parse DIMACS file %s", initial_graph_name);
+        legacy_abort("ParseDimacsGraph");
+    }
+    stinger* G = new stinger();
+    if (graph_init(G, h->nv, h->rowptr, h->col) != SBC_OK) legacy_abort("ParseDimacsGraph");
+    *NV = h->nv;
+    *NE = h->ne;
+    sbcHostGraphFree(h);
+    *stingerGraph = G;
+}
+
+extern "C" int stinger_insert_edge(struct stinger* G, int64_t type, int64_t from, int64_t to, int64_t weight,
+                                   int64_t timestamp)
+{
+    (void)type; (void)weight; (void)timestamp;
+    if (!G || from == to || from < 0 || to < 0 || from >= G->nv || to >= G->nv) return -1;
+    const int32_t* a = &G->col[G->start[from]];
+    for (int32_t i = 0; i < G->deg[from]; i++)
+        if (a[i] == to) return 0;
+    bool relaid = G->deg[from] == G->cap[from];
+    int32_t slot = graph_push(G, (int32_t)from, (int32_t)to, G->seq_next++);
+    G->deg0[from] = G->deg[from]; // visible to every later update
+    if (!relaid && !G->full_dirty) {
+        G->p_slot.push_back(slot);
+        G->p_col.push_back((int32_t)to);
+        G->p_seq.push_back(G->seq[slot]);
+        G->p_v.push_back((int32_t)from);
+    }
+    return 1;
+}
+
+extern "C" int64_t stinger_outdegree(struct stinger* G, int64_t v)
+{
+    if (!G || v < 0 || v >= G->nv) return 0;
+    return G->deg[v];
+}
+
+extern "C" struct stinger* stinger_free_all(struct stinger* S)
+{
+    graph_free(S);
+    return nullptr;
+}
+
+// ------------------------------------------------------------ forest
+
+struct sbcForestDev {
+    int64_t nv = 0, K = 0;
+    int32_t lcap = 0;
+    bool exact = false;
+    bool initialized = false;
+    std::vector<int64_t> roots;
+    int32_t* d_roots = nullptr;
+    int32_t* d_dist = nullptr;
+    int64_t* d_sigma = nullptr;
+    double* d_delta = nullptr;
+    double* d_bc = nullptr;
+    int32_t* d_mark = nullptr;
+    int64_t* d_dpath = nullptr;
+    double* d_ndel = nullptr;
+    int32_t* d_list = nullptr;
+    int32_t* d_lvl = nullptr;
+    int32_t* d_err = nullptr;
+};
+
+static int32_t lcap_for(int64_t nv) { return (int32_t)(nv + 2 < 4096 ? nv + 2 : 4096); }
+
+extern "C" int64_t streamingBCForestFootprint(int64_t NV, int64_t NK)
+{
+    int64_t per = 4 + 8 + 8 + 4 + 8 + 8 + 4;
+    return NK * NV * per + NK * 2 * (int64_t)lcap_for(NV) * 4 + NV * 8 + NK * 4 + 64;
+}
+
+static void forest_free_dev(sbcForestDev* F)
+{
+    if (!F) return;
+    void* ptrs[] = {F->d_roots, F->d_dist, F->d_sigma, F->d_delta, F->d_bc, F->d_mark,
+                    F->d_dpath, F->d_ndel, F->d_list, F->d_lvl, F->d_err};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete F;
+}
+
+template <class T>
+static sbcStatus dalloc(T** p, size_t n)
+{
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(SBC_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", sizeof(T) * n, hipGetErrorString(e));
+    }
+    return SBC_OK;
+}
+
+static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, bool exact, bcForest** out)
+{
+    if (NV < 1 || NV >= INT32_MAX / 4 || NK < 1 || NK > NV) return set_err(SBC_ERR_ARG, "bad NV/NK");
+    sbcForestDev* F = new sbcForestDev();
+    F->nv = NV;
+    F->K = NK;
+    F->lcap = lcap_for(NV);
+    F->exact = exact;
+    F->roots.resize(NK);
+    std::vector<int32_t> r32(NK);
+    for (int64_t i = 0; i < NK; i++) {
+        int64_t r = exact ? i : roots[i];
+        if (r < 0 || r >= NV) {
+            forest_free_dev(F);
+            return set_err(SBC_ERR_ARG, "root %ld out of range", (long)r);
+        }
+        F->roots[i] = r;
+        r32[i] = (int32_t)r;
+    }
+    size_t kn = (size_t)NK * (size_t)NV;
+    sbcStatus st = SBC_OK;
+    if ((st = dalloc(&F->d_roots, NK)) || (st = dalloc(&F->d_dist, kn)) || (st = dalloc(&F->d_sigma, kn)) ||
+        (st = dalloc(&F->d_delta, kn)) || (st = dalloc(&F->d_bc, NV)) || (st = dalloc(&F->d_mark, kn)) ||
+        (st = dalloc(&F->d_dpath, kn)) || (st = dalloc(&F->d_ndel, kn)) || (st = dalloc(&F->d_list, kn)) ||
+        (st = dalloc(&F->d_lvl, (size_t)NK * 2 * F->lcap)) || (st = dalloc(&F->d_err, 4))) {
+        forest_free_dev(F);
+        return st;
+    }
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMemcpy(F->d_roots, r32.data(), sizeof(int32_t) * NK, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(F->d_mark, 0, sizeof(int32_t) * kn);
+    if (e == hipSuccess) e = hipMemset(F->d_dpath, 0, sizeof(int64_t) * kn);
+    if (e == hipSuccess) e = hipMemset(F->d_ndel, 0, sizeof(double) * kn);
+    if (e == hipSuccess) e = hipMemset(F->d_bc, 0, sizeof(double) * NV);
+    if (e == hipSuccess) e = hipMemset(F->d_err, 0, sizeof(int32_t) * 4);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        forest_free_dev(F);
+        return set_err(SBC_ERR_HIP, "forest init: %s", hipGetErrorString(e));
+    }
+    bcForest* f = (bcForest*)calloc(1, sizeof(bcForest));
+    f->dev = F;
+    f->NV = NV;
+    f->totalBC = (double*)calloc((size_t)NV, sizeof(double));
+    *out = f;
+    return SBC_OK;
+}
+
+extern "C" bcForest* streamingBCCreateForestExact(int64_t NV)
+{
+    bcForest* f = nullptr;
+    if (forest_create(NV, NV, nullptr, true, &f) != SBC_OK) legacy_abort("streamingBCCreateForestExact");
+    return f;
+}
+
+extern "C" bcForest* streamingBCCreateForestApproximate(int64_t NV, int64_t NK, int64_t* roots)
+{
+    bcForest* f = nullptr;
+    if (!roots) {
+        set_err(SBC_ERR_ARG, "root array is NULL");
+        legacy_abort("streamingBCCreateForestApproximate");
+    }
+    if (forest_create(NV, NK, roots, false, &f) != SBC_OK)
+        legacy_abort("streamingBCCreateForestApproximate");
+    return f;
+}
+
+static void forest_destroy(bcForestPtr* p)
+{
+    if (!p || !*p) return;
+    forest_free_dev((*p)->dev);
+    free((*p)->totalBC);
+    free(*p);
+    *p = nullptr;
+}
+
+extern "C" void streamingBCDeleteForestExact(bcForestPtr* deadForest) { forest_destroy(deadForest); }
+extern "C" void streamingBCDeleteForestApproximate(bcForestPtr* deadForest, int64_t rootArraySize, int64_t* rootArray)
+{
+    (void)rootArraySize; (void)rootArray;
+    forest_destroy(deadForest);
+}
+
+// ------------------------------------------------------------ context
+
+enum { KSTAT_UPDATE = 0, KSTAT_STATIC = 1, KSTAT_BC = 2, KSTAT_APPEND = 3, KSTAT_N = 4 };
+
+struct WinStage {
+    // device window descriptor and its pinned host twin
+    SbcWinDesc* d_desc = nullptr;
+    int32_t* d_ints = nullptr;     // wu, wv, slot, scol, sseq, vv  (6 * cap)
+    SbcVInfo* d_vi = nullptr;      // cap2
+    void* h_buf[2] = {nullptr, nullptr};
+    hipEvent_t h_done[2] = {nullptr, nullptr};
+    int h_cur = 0;
+    int cap = 0;                   // max insertions per window
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    const void* forest = nullptr;
+    const void* g = nullptr;
+    uint64_t generation = 0;
+    int nwb = 0, grid = 0, heavy = 0, recompute = 0, W = 0, has_order = 0, with_rec = 0;
+};
+
+struct extraArraysPerThread {
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[KSTAT_N];
+    int64_t launches[KSTAT_N] = {0, 0, 0, 0};
+    double total_ms[KSTAT_N] = {0, 0, 0, 0};
+    int32_t* d_qctr = nullptr;
+    SbcItemRec* d_rec = nullptr;
+    int64_t rec_cap = 0;       // entries
+    int64_t last_rec_n = 0;    // entries written by the last legacy update
+    int32_t* d_order = nullptr;
+    int64_t order_cap = 0;
+    int num_cu = 0;
+    WinStage win;
+    GraphCache gc;
+    int refs = 1;
+};
+
+static sbcStatus ctx_init(extraArraysPerThread* C)
+{
+    HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&C->d_qctr, sizeof(int32_t) * 4));
+    HIPCHK(hipMemset(C->d_qctr, 0, sizeof(int32_t) * 4));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    C->num_cu = prop.multiProcessorCount;
+    return SBC_OK;
+}
+
+static void ctx_free(extraArraysPerThread* C)
+{
+    if (!C) return;
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    for (auto& v : C->ev)
+        for (auto& pr : v) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    if (C->gc.exec) (void)hipGraphExecDestroy(C->gc.exec);
+    if (C->gc.graph) (void)hipGraphDestroy(C->gc.graph);
+    if (C->d_qctr) (void)hipFree(C->d_qctr);
+    if (C->d_rec) (void)hipFree(C->d_rec);
+    if (C->d_order) (void)hipFree(C->d_order);
+    if (C->win.d_desc) (void)hipFree(C->win.d_desc);
+    if (C->win.d_ints) (void)hipFree(C->win.d_ints);
+    if (C->win.d_vi) (void)hipFree(C->win.d_vi);
+    for (int i = 0; i < 2; i++) {
+        if (C->win.h_buf[i]) (void)hipHostFree(C->win.h_buf[i]);
+        if (C->win.h_done[i]) (void)hipEventDestroy(C->win.h_done[i]);
+    }
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+}
+
+extern "C" extraArraysPerThread** streamingBCCreateAuxilary(int64_t threadCount, int64_t NV)
+{
+    (void)NV;
+    if (threadCount < 1) threadCount = 1;
+    extraArraysPerThread* C = new extraArraysPerThread();
+    if (ctx_init(C) != SBC_OK) {
+        ctx_free(C);
+        legacy_abort("streamingBCCreateAuxilary");
+    }
+    extraArraysPerThread** arr = (extraArraysPerThread**)calloc((size_t)threadCount, sizeof(*arr));
+    for (int64_t i = 0; i < threadCount; i++) arr[i] = C;
+    return arr;
+}
+
+extern "C" void streamingBCDeleteAuxilary(extraArraysPerThread** parallelExtra, int64_t threadCount, int64_t NV)
+{
+    (void)threadCount; (void)NV;
+    if (!parallelExtra) return;
+    ctx_free(parallelExtra[0]);
+    free(parallelExtra);
+}
+
+static sbcStatus ev_begin(extraArraysPerThread* C, int k, hipEvent_t* e0)
+{
+    *e0 = nullptr;
+    if (!C->timing) return SBC_OK;
+    HIPCHK(hipEventCreate(e0));
+    HIPCHK(hipEventRecord(*e0, C->stream));
+    return SBC_OK;
+}
+
+static sbcStatus ev_end(extraArraysPerThread* C, int k, hipEvent_t e0)
+{
+    C->launches[k]++;
+    if (!C->timing || !e0) return SBC_OK;
+    hipEvent_t e1;
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e1, C->stream));
+    C->ev[k].push_back({e0, e1});
+    return SBC_OK;
+}
+
+static sbcStatus resolve_events(extraArraysPerThread* C)
+{
+    HIPCHK(hipStreamSynchronize(C->stream));
+    for (int k = 0; k < KSTAT_N; k++) {
+        for (auto& pr : C->ev[k]) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            C->total_ms[k] += ms;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        C->ev[k].clear();
+    }
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCKernelStats(extraArraysPerThread** eAPT, int32_t kernel, int64_t* launches,
+                                           double* total_ms)
+{
+    if (!eAPT || !eAPT[0] || kernel < 0 || kernel >= KSTAT_N) return set_err(SBC_ERR_ARG, "bad stats query");
+    extraArraysPerThread* C = eAPT[0];
+    sbcStatus st = resolve_events(C);
+    if (st) return st;
+    if (launches) *launches = C->launches[kernel];
+    if (total_ms) *total_ms = C->total_ms[kernel];
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCResetStats(extraArraysPerThread** eAPT, int32_t enable_timing)
+{
+    if (!eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null context");
+    extraArraysPerThread* C = eAPT[0];
+    sbcStatus st = resolve_events(C);
+    if (st) return st;
+    for (int k = 0; k < KSTAT_N; k++) { C->launches[k] = 0; C->total_ms[k] = 0; }
+    C->timing = enable_timing != 0;
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCSynchronize(extraArraysPerThread** eAPT)
+{
+    if (!eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(eAPT[0]->stream));
+    return SBC_OK;
+}
+
+static sbcStatus ensure_rec(extraArraysPerThread* C, int64_t n)
+{
+    if (C->rec_cap >= n) return SBC_OK;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (C->d_rec) HIPCHK(hipFree(C->d_rec));
+    C->d_rec = nullptr;
+    sbcStatus st = dalloc(&C->d_rec, (size_t)n);
+    if (st) return st;
+    C->rec_cap = n;
+    C->gc.with_rec = -1; // invalidate cached graph
+    return SBC_OK;
+}
+
+static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArraysPerThread* C)
+{
+    SbcParams P;
+    memset(&P, 0, sizeof(P));
+    P.vinfo = G->d_vinfo;
+    P.col = G->d_col;
+    P.seq = G->d_seq;
+    P.nv = (int32_t)F->nv;
+    P.K = (int32_t)F->K;
+    P.lcap = F->lcap;
+    P.dist = F->d_dist;
+    P.sigma = F->d_sigma;
+    P.delta = F->d_delta;
+    P.bc = F->d_bc;
+    P.mark = F->d_mark;
+    P.dpath = F->d_dpath;
+    P.ndel = F->d_ndel;
+    P.list = F->d_list;
+    P.lvl = F->d_lvl;
+    P.roots = F->d_roots;
+    P.qctr = C->d_qctr;
+    P.err = F->d_err;
+    P.bc_atomic = 1;
+    return P;
+}
+
+static sbcStatus check_kernel_err(sbcForestDev* F, extraArraysPerThread* C)
+{
+    int32_t err = 0;
+    HIPCHK(hipMemcpyAsync(&err, F->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (err & SBC_KERR_LEVELS) {
+        int32_t z = 0;
+        HIPCHK(hipMemcpy(F->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice));
+        return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", F->lcap);
+    }
+    return SBC_OK;
+}
+
+// ------------------------------------------------------------ static pass
+
+static sbcStatus run_static(bcForest* forest, stinger* G, extraArraysPerThread* C)
+{
+    if (!forest || !G || !C) return set_err(SBC_ERR_ARG, "null argument");
+    sbcForestDev* F = forest->dev;
+    if (F->nv != G->nv) return set_err(SBC_ERR_ARG, "forest NV %ld != graph NV %ld", (long)F->nv, (long)G->nv);
+    sbcStatus st = graph_flush(G, C->stream);
+    if (st) return st;
+    SbcParams P = base_params(F, G, C);
+    HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+    hipEvent_t e0;
+    if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
+    int grid = (int)(F->K < 4LL * C->num_cu ? F->K : 4LL * C->num_cu);
+    HIPCHK(sbc_launch_static(&P, 16, grid, C->stream));
+    if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+    if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
+    HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
+    if ((st = ev_end(C, KSTAT_BC, e0))) return st;
+    if ((st = check_kernel_err(F, C))) return st;
+    F->initialized = true;
+    return SBC_OK;
+}
+
+extern "C" void streamingBCInitStaticExact(bcForest* forest, struct stinger* stingerGraph, int64_t NT,
+                                           extraArraysPerThread** auxilary)
+{
+    (void)NT;
+    if (!auxilary || run_static(forest, stingerGraph, auxilary[0]) != SBC_OK) legacy_abort("streamingBCInitStaticExact");
+}
+
+extern "C" void streamingBCInitStaticApproximate(bcForest* forest, struct stinger* stingerGraph, int64_t NT,
+                                                 extraArraysPerThread** auxilary, int64_t NK, int64_t* roots)
+{
+    (void)NT;
+    if (!forest || NK != forest->dev->K) {
+        set_err(SBC_ERR_ARG, "NK does not match the forest");
+        legacy_abort("streamingBCInitStaticApproximate");
+    }
+    for (int64_t i = 0; roots && i < NK; i++)
+        if (roots[i] != forest->dev->roots[i]) {
+            set_err(SBC_ERR_ARG, "root array differs from the one given at forest creation");
+            legacy_abort("streamingBCInitStaticApproximate");
+        }
+    if (!auxilary || run_static(forest, stingerGraph, auxilary[0]) != SBC_OK)
+        legacy_abort("streamingBCInitStaticApproximate");
+}
+
+// ------------------------------------------------------------ plain update
+
+static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerThread* C, int64_t u, int64_t v,
+                                  uint32_t granularity)
+{
+    if (!forest || !G || !C) return set_err(SBC_ERR_ARG, "null argument");
+    sbcForestDev* F = forest->dev;
+    if (!F->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (u < 0 || v < 0 || u >= F->nv || v >= F->nv || u == v) return set_err(SBC_ERR_ARG, "bad edge (%ld,%ld)", (long)u, (long)v);
+    sbcStatus st = graph_flush(G, C->stream);
+    if (st) return st;
+    if ((st = ensure_rec(C, F->K))) return st;
+    SbcParams P = base_params(F, G, C);
+    P.u0 = (int32_t)u;
+    P.v0 = (int32_t)v;
+    P.rec = C->d_rec;
+    const int nwb = 4;
+    int grid;
+    if (granularity == FINE) {
+        P.heavy = (int32_t)F->K;
+        grid = (int)(F->K < 2LL * C->num_cu ? F->K : 2LL * C->num_cu);
+    } else {
+        P.heavy = 0;
+        int64_t g = (F->K + nwb - 1) / nwb;
+        grid = (int)(g < 4LL * C->num_cu ? g : 4LL * C->num_cu);
+    }
+    HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+    hipEvent_t e0;
+    if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
+    HIPCHK(sbc_launch_window(&P, nwb, grid, C->stream));
+    if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+    C->last_rec_n = F->K;
+    return check_kernel_err(F, C);
+}
+
+extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct stinger* sStinger, int64_t newU,
+                                                    int64_t newV, int64_t* rootArrayForApproximation, int64_t NK,
+                                                    int64_t NV, int64_t NT, extraArraysPerThread** eAPT,
+                                                    uint32_t loadBalancing, uint32_t granularity)
+{
+    (void)rootArrayForApproximation; (void)NT; (void)loadBalancing;
+    StreamingExtraInfo r = {0, 0, 0, 0};
+    if (!forest || NK != forest->dev->K || NV != forest->dev->nv) {
+        set_err(SBC_ERR_ARG, "NK/NV do not match the forest");
+        legacy_abort("insertEdgeStreamingBC");
+    }
+    if (!eAPT || run_insert_plain(forest, sStinger, eAPT[0], newU, newV, granularity) != SBC_OK)
+        legacy_abort("insertEdgeStreamingBC");
+    return r;
+}
+
+extern "C" sbcStatus streamingBCLastRecords(extraArraysPerThread** eAPT, sbcItemRecord* out, int64_t K)
+{
+    if (!eAPT || !eAPT[0] || !out) return set_err(SBC_ERR_ARG, "null argument");
+    extraArraysPerThread* C = eAPT[0];
+    if (K != C->last_rec_n || !C->d_rec) return set_err(SBC_ERR_STATE, "no records of that size");
+    static_assert(sizeof(sbcItemRecord) == sizeof(SbcItemRec), "record layout");
+    HIPCHK(hipMemcpyAsync(out, C->d_rec, sizeof(SbcItemRec) * K, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return SBC_OK;
+}
+
+// ------------------------------------------------------------ readouts
+
+extern "C" sbcStatus streamingBCGetBC(bcForest* forest, double* out)
+{
+    if (!forest) return set_err(SBC_ERR_ARG, "null forest");
+    sbcForestDev* F = forest->dev;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(forest->totalBC, F->d_bc, sizeof(double) * F->nv, hipMemcpyDeviceToHost));
+    if (out) memcpy(out, forest->totalBC, sizeof(double) * F->nv);
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCGetTree(bcForest* forest, int64_t ri, int64_t* level, int64_t* sigma, double* delta)
+{
+    if (!forest) return set_err(SBC_ERR_ARG, "null forest");
+    sbcForestDev* F = forest->dev;
+    if (ri < 0 || ri >= F->K) return set_err(SBC_ERR_ARG, "root index out of range");
+    HIPCHK(hipDeviceSynchronize());
+    size_t off = (size_t)ri * (size_t)F->nv;
+    if (level) {
+        std::vector<int32_t> d(F->nv);
+        HIPCHK(hipMemcpy(d.data(), F->d_dist + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
+        for (int64_t x = 0; x < F->nv; x++) level[x] = d[x] == INT32_MAX ? LONG_MAX : d[x];
+    }
+    if (sigma) HIPCHK(hipMemcpy(sigma, F->d_sigma + off, sizeof(int64_t) * F->nv, hipMemcpyDeviceToHost));
+    if (delta) HIPCHK(hipMemcpy(delta, F->d_delta + off, sizeof(double) * F->nv, hipMemcpyDeviceToHost));
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCRecomputeBC(bcForest* forest, extraArraysPerThread** eAPT)
+{
+    if (!forest || !eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null argument");
+    extraArraysPerThread* C = eAPT[0];
+    sbcForestDev* F = forest->dev;
+    SbcParams P;
+    memset(&P, 0, sizeof(P));
+    P.nv = (int32_t)F->nv;
+    P.K = (int32_t)F->K;
+    P.delta = F->d_delta;
+    P.bc = F->d_bc;
+    P.roots = F->d_roots;
+    hipEvent_t e0;
+    sbcStatus st;
+    if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
+    HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
+    if ((st = ev_end(C, KSTAT_BC, e0))) return st;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return SBC_OK;
+}
+
+// ------------------------------------------------------------ windowed form
+
+static size_t stage_bytes(int cap)
+{
+    // desc + 6 int arrays of 2*cap + vinfo array of 4*cap (touched this and last window)
+    return sizeof(SbcWinDesc) + sizeof(int32_t) * 6 * 4 * (size_t)cap + sizeof(SbcVInfo) * 4 * (size_t)cap + 64;
+}
+
+static sbcStatus ensure_stage(extraArraysPerThread* C, int W)
+{
+    WinStage& S = C->win;
+    if (S.cap >= W) return SBC_OK;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (S.d_desc) HIPCHK(hipFree(S.d_desc));
+    if (S.d_ints) HIPCHK(hipFree(S.d_ints));
+    if (S.d_vi) HIPCHK(hipFree(S.d_vi));
+    for (int i = 0; i < 2; i++) {
+        if (S.h_buf[i]) HIPCHK(hipHostFree(S.h_buf[i]));
+        S.h_buf[i] = nullptr;
+        if (!S.h_done[i]) HIPCHK(hipEventCreateWithFlags(&S.h_done[i], hipEventDisableTiming));
+    }
+    S.cap = W;
+    HIPCHK(hipMalloc(&S.d_desc, sizeof(SbcWinDesc)));
+    HIPCHK(hipMalloc(&S.d_ints, sizeof(int32_t) * 6 * 4 * (size_t)W));
+    HIPCHK(hipMalloc(&S.d_vi, sizeof(SbcVInfo) * 4 * (size_t)W));
+    for (int i = 0; i < 2; i++) {
+        HIPCHK(hipHostMalloc(&S.h_buf[i], stage_bytes(W)));
+        HIPCHK(hipEventRecord(S.h_done[i], C->stream));
+    }
+    C->gc.W = -1; // invalidate cached graph
+    return SBC_OK;
+}
+
+static sbcStatus ensure_order(extraArraysPerThread* C, int64_t K)
+{
+    if (C->order_cap >= K) return SBC_OK;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (C->d_order) HIPCHK(hipFree(C->d_order));
+    C->d_order = nullptr;
+    sbcStatus st = dalloc(&C->d_order, (size_t)K);
+    if (st) return st;
+    C->order_cap = K;
+    C->gc.has_order = -1;
+    return SBC_OK;
+}
+
+// Enqueues the launches of one window (append, queue reset, window kernel,
+// optional BC reduction) on the context stream.
+static sbcStatus enqueue_window(const SbcParams& P, const SbcAppend& A, int nwb, int grid, int W,
+                                int recompute, extraArraysPerThread* C, bool timed)
+{
+    sbcStatus st;
+    hipEvent_t e0 = nullptr;
+    HIPCHK(sbc_launch_append(&A, 4 * W, C->stream));
+    HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+    if (timed && (st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
+    HIPCHK(sbc_launch_window(&P, nwb, grid, C->stream));
+    if (timed && (st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+    if (recompute) {
+        if (timed && (st = ev_begin(C, KSTAT_BC, &e0))) return st;
+        HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
+        if (timed && (st = ev_end(C, KSTAT_BC, e0))) return st;
+    }
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT,
+                                                   const int64_t* u, const int64_t* v, int64_t count,
+                                                   const sbcBatchConfig* cfg, sbcItemRecord* records)
+{
+    if (!forest || !G || !eAPT || !eAPT[0] || (count > 0 && (!u || !v)) || !cfg)
+        return set_err(SBC_ERR_ARG, "null argument");
+    extraArraysPerThread* C = eAPT[0];
+    sbcForestDev* F = forest->dev;
+    if (!F->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (F->nv != G->nv) return set_err(SBC_ERR_ARG, "forest/graph NV mismatch");
+    const int W = cfg->window;
+    if (W < 1 || W > (1 << 16)) return set_err(SBC_ERR_ARG, "window %d out of range", W);
+    int bt = cfg->block_threads ? cfg->block_threads : 256;
+    if (bt != 64 && bt != 128 && bt != 256 && bt != 512 && bt != 1024)
+        return set_err(SBC_ERR_ARG, "block_threads must be 64, 128, 256, 512 or 1024");
+    const int nwb = bt / 64;
+    // validate the whole batch before touching anything
+    {
+        std::unordered_set<uint64_t> seen;
+        seen.reserve((size_t)count * 2);
+        for (int64_t i = 0; i < count; i++) {
+            int64_t a = u[i], b = v[i];
+            if (a < 0 || b < 0 || a >= G->nv || b >= G->nv) return set_err(SBC_ERR_ARG, "insertion %ld out of range", (long)i);
+            if (a == b) return set_err(SBC_ERR_EXISTS, "insertion %ld is a self loop", (long)i);
+            uint64_t key = ((uint64_t)(a < b ? a : b) << 32) | (uint64_t)(a < b ? b : a);
+            if (!seen.insert(key).second || graph_has_edge(G, (int32_t)a, (int32_t)b))
+                return set_err(SBC_ERR_EXISTS, "insertion %ld (%ld,%ld) duplicates an edge", (long)i, (long)a, (long)b);
+        }
+        if ((int64_t)G->seq_next + count >= INT32_MAX) return set_err(SBC_ERR_ARG, "sequence space exhausted");
+    }
+    sbcStatus st;
+    if ((st = graph_flush(G, C->stream))) return st;
+    if ((st = ensure_stage(C, W))) return st;
+    if ((st = ensure_rec(C, (int64_t)W * F->K))) return st;
+    if ((st = ensure_order(C, F->K))) return st;
+    int heavy = cfg->heavy_roots < 0 || cfg->heavy_roots > F->K ? (int)F->K : cfg->heavy_roots;
+    const int has_order = cfg->root_order != nullptr;
+    if (has_order) {
+        std::vector<char> chk(F->K, 0);
+        for (int64_t i = 0; i < F->K; i++) {
+            int32_t x = cfg->root_order[i];
+            if (x < 0 || x >= F->K || chk[x]) return set_err(SBC_ERR_ARG, "root_order is not a permutation");
+            chk[x] = 1;
+        }
+        HIPCHK(hipMemcpyAsync(C->d_order, cfg->root_order, sizeof(int32_t) * F->K, hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+    }
+    int grid = cfg->grid_blocks;
+    if (grid <= 0) {
+        int64_t gh = heavy;
+        int64_t gl = (F->K - heavy + nwb - 1) / nwb;
+        int64_t g = gh > gl ? gh : gl;
+        int64_t cap = 2LL * C->num_cu;
+        grid = (int)(g < cap ? (g > 0 ? g : 1) : cap);
+    }
+    // capacity: relayout once up front if any endpoint lacks room for the whole batch
+    {
+        std::vector<int32_t> extra(G->nv, 0);
+        bool short_room = false;
+        for (int64_t i = 0; i < count; i++) { extra[u[i]]++; extra[v[i]]++; }
+        for (int64_t x = 0; x < G->nv; x++)
+            if (extra[x] && G->deg[x] + extra[x] > G->cap[x]) {
+                G->cap[x] = G->deg[x] + extra[x] + slack_for(G->deg[x]);
+                short_room = true;
+            }
+        if (short_room) graph_relayout(G);
+        if ((st = graph_flush(G, C->stream))) return st;
+    }
+    SbcParams P = base_params(F, G, C);
+    P.heavy = heavy;
+    P.order = has_order ? C->d_order : nullptr;
+    P.win = C->win.d_desc;
+    int32_t* di = C->win.d_ints;
+    P.wu = di;
+    P.wv = di + 4 * W;
+    P.rec = C->d_rec;
+    P.bc_atomic = cfg->recompute_bc ? 0 : 1;
+    SbcAppend A;
+    memset(&A, 0, sizeof(A));
+    A.vinfo = G->d_vinfo;
+    A.col = G->d_col;
+    A.seq = G->d_seq;
+    A.desc = C->win.d_desc;
+    A.d_slot = di + 8 * W;
+    A.d_scol = di + 12 * W;
+    A.d_sseq = di + 16 * W;
+    A.d_vv = di + 20 * W;
+    A.d_vi = C->win.d_vi;
+
+    const bool use_graph = cfg->use_graph != 0;
+    GraphCache& gc = C->gc;
+    bool need_capture = use_graph &&
+        (!gc.exec || gc.forest != F || gc.g != G || gc.generation != G->generation || gc.nwb != nwb ||
+         gc.grid != grid || gc.heavy != heavy || gc.recompute != cfg->recompute_bc || gc.W != W ||
+         gc.has_order != has_order || gc.with_rec != 1);
+    if (need_capture) {
+        if (gc.exec) { (void)hipGraphExecDestroy(gc.exec); gc.exec = nullptr; }
+        if (gc.graph) { (void)hipGraphDestroy(gc.graph); gc.graph = nullptr; }
+        HIPCHK(hipStreamSynchronize(C->stream));
+        HIPCHK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal));
+        sbcStatus cst = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, false);
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(C->stream, &g);
+        if (cst) return cst;
+        if (ee != hipSuccess) return set_err(SBC_ERR_HIP, "graph capture: %s", hipGetErrorString(ee));
+        gc.graph = g;
+        HIPCHK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+        gc.forest = F; gc.g = G; gc.generation = G->generation; gc.nwb = nwb; gc.grid = grid; gc.heavy = heavy;
+        gc.recompute = cfg->recompute_bc; gc.W = W; gc.has_order = has_order; gc.with_rec = 1;
+    }
+
+    std::vector<int32_t> prev_touched;
+    std::vector<int32_t> wu(W), wv(W), slot(2 * W), scol(2 * W), sseq(2 * W), vv, touched;
+    std::vector<SbcVInfo> vi;
+    std::vector<char> mark(G->nv, 0);
+    for (int64_t w0 = 0; w0 < count; w0 += W) {
+        const int n = (int)((count - w0) < W ? (count - w0) : W);
+        const int32_t seq0 = G->seq_next;
+        // deg before this window = always-visible prefix for this window
+        touched.clear();
+        for (int i = 0; i < n; i++) {
+            int32_t a = (int32_t)u[w0 + i], b = (int32_t)v[w0 + i];
+            wu[i] = a;
+            wv[i] = b;
+            for (int32_t x : {a, b})
+                if (!mark[x]) { mark[x] = 1; touched.push_back(x); G->deg0[x] = G->deg[x]; }
+        }
+        for (int32_t x : prev_touched)
+            if (!mark[x]) G->deg0[x] = G->deg[x];
+        int ns = 0;
+        for (int i = 0; i < n; i++) {
+            int32_t q = seq0 + i;
+            int32_t sa = graph_push(G, wu[i], wv[i], q);
+            int32_t sb = graph_push(G, wv[i], wu[i], q);
+            slot[ns] = sa; scol[ns] = wv[i]; sseq[ns] = q; ns++;
+            slot[ns] = sb; scol[ns] = wu[i]; sseq[ns] = q; ns++;
+        }
+        G->seq_next += n;
+        vv.clear();
+        vi.clear();
+        for (int32_t x : touched) { vv.push_back(x); vi.push_back(vinfo_of(G, x)); }
+        for (int32_t x : prev_touched)
+            if (!mark[x]) { vv.push_back(x); vi.push_back(vinfo_of(G, x)); }
+        for (int32_t x : touched) mark[x] = 0;
+        prev_touched = touched;
+        // stage: desc | ints (wu, wv, slot, scol, sseq, vv at stride 4W) | vinfo
+        WinStage& S = C->win;
+        int b = S.h_cur;
+        S.h_cur ^= 1;
+        HIPCHK(hipEventSynchronize(S.h_done[b]));
+        char* hb = (char*)S.h_buf[b];
+        SbcWinDesc* hd = (SbcWinDesc*)hb;
+        hd->W = n;
+        hd->seq0 = seq0;
+        hd->nslots = ns;
+        hd->nvupd = (int32_t)vv.size();
+        int32_t* hi = (int32_t*)(hb + 64);
+        memcpy(hi + 0 * 4 * W, wu.data(), sizeof(int32_t) * n);
+        memcpy(hi + 1 * 4 * W, wv.data(), sizeof(int32_t) * n);
+        memcpy(hi + 2 * 4 * W, slot.data(), sizeof(int32_t) * ns);
+        memcpy(hi + 3 * 4 * W, scol.data(), sizeof(int32_t) * ns);
+        memcpy(hi + 4 * 4 * W, sseq.data(), sizeof(int32_t) * ns);
+        memcpy(hi + 5 * 4 * W, vv.data(), sizeof(int32_t) * vv.size());
+        SbcVInfo* hv = (SbcVInfo*)(hb + 64 + sizeof(int32_t) * 6 * 4 * (size_t)W);
+        memcpy(hv, vi.data(), sizeof(SbcVInfo) * vi.size());
+        HIPCHK(hipMemcpyAsync(S.d_desc, hd, sizeof(SbcWinDesc), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(S.d_ints, hi, sizeof(int32_t) * 6 * 4 * (size_t)W, hipMemcpyHostToDevice, C->stream));
+        if (!vi.empty())
+            HIPCHK(hipMemcpyAsync(S.d_vi, hv, sizeof(SbcVInfo) * vi.size(), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipEventRecord(S.h_done[b], C->stream));
+        if (use_graph) {
+            hipEvent_t e0;
+            if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
+            HIPCHK(hipGraphLaunch(gc.exec, C->stream));
+            if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+        } else if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) {
+            return st;
+        }
+        if (records) {
+            // [insertion][root] for this window
+            HIPCHK(hipMemcpyAsync(records + (size_t)w0 * F->K, C->d_rec, sizeof(SbcItemRec) * (size_t)n * F->K,
+                                  hipMemcpyDeviceToHost, C->stream));
+        }
+    }
+    // make the last window's appends visible to everyone
+    for (int32_t x : prev_touched) G->deg0[x] = G->deg[x];
+    for (size_t i = 0; i < prev_touched.size(); i += SBC_APPEND_INLINE) {
+        SbcAppend B;
+        memset(&B, 0, sizeof(B));
+        B.vinfo = G->d_vinfo;
+        B.col = G->d_col;
+        B.seq = G->d_seq;
+        for (size_t j = i; j < prev_touched.size() && B.nvupd < SBC_APPEND_INLINE; j++) {
+            B.vv[B.nvupd] = prev_touched[j];
+            B.vi[B.nvupd] = vinfo_of(G, prev_touched[j]);
+            B.nvupd++;
+        }
+        HIPCHK(sbc_launch_append(&B, SBC_APPEND_INLINE, C->stream));
+    }
+    return check_kernel_err(F, C);
+}
+
+// ------------------------------------------------------------ status forms
+
+extern "C" sbcStatus sbcGraphCreate(int64_t NV, const int64_t* rowptr, const int64_t* col, struct stinger** out)
+{
+    if (!rowptr || !out || (!col && NV > 0 && rowptr[NV] > 0)) return set_err(SBC_ERR_ARG, "null argument");
+    stinger* G = new stinger();
+    sbcStatus st = graph_init(G, NV, rowptr, col);
+    if (st) {
+        delete G;
+        return st;
+    }
+    *out = G;
+    return SBC_OK;
+}
+
+extern "C" sbcStatus sbcForestCreate(int64_t NV, int64_t NK, const int64_t* roots, bcForest** out)
+{
+    if (!out) return set_err(SBC_ERR_ARG, "null argument");
+    if (!roots && NK != NV) return set_err(SBC_ERR_ARG, "exact forest needs NK == NV");
+    return forest_create(NV, NK, roots, roots == nullptr, out);
+}
+
+extern "C" sbcStatus sbcInitStatic(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT)
+{
+    if (!eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null context");
+    return run_static(forest, G, eAPT[0]);
+}
+
+extern "C" sbcStatus sbcInsertEdge(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT, int64_t u,
+                                   int64_t v, uint32_t granularity)
+{
+    if (!forest || !G || !eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null argument");
+    if (!forest->dev->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (u < 0 || v < 0 || u >= G->nv || v >= G->nv) return set_err(SBC_ERR_ARG, "vertex id out of range");
+    if (u == v) return set_err(SBC_ERR_EXISTS, "self loop");
+    if (graph_has_edge(G, (int32_t)u, (int32_t)v)) return set_err(SBC_ERR_EXISTS, "edge (%ld,%ld) exists", (long)u, (long)v);
+    stinger_insert_edge(G, 0, u, v, 0, 0);
+    stinger_insert_edge(G, 0, v, u, 0, 0);
+    return run_insert_plain(forest, G, eAPT[0], u, v, granularity);
+}

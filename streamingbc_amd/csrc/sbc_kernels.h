@@ -1,0 +1,99 @@
+// sbc_kernels.h -- device-side data layout shared by sbc_kernels.hip and the
+// host runtime sbc_engine.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Per-vertex adjacency descriptor: slots [start, start + deg) of col/seq;
+// the first deg0 slots are visible to every update (the rest carry the
+// sequence number of the insertion that appended them).
+struct SbcVInfo {
+    int32_t start;
+    int32_t deg;
+    int32_t deg0;
+    int32_t pad;
+};
+
+// Same layout as sbcItemRecord in include/streamingbc_hip.h.
+struct SbcItemRec {
+    int32_t root_index;
+    int32_t insertion;
+    int32_t kase;
+    int32_t gap;
+    int64_t processed;
+    int64_t scanned;
+    int64_t affected;
+    int64_t moved;
+};
+
+// Device-resident descriptor of one insertion window (read by the kernels so
+// a captured graph can be replayed with new contents).
+struct SbcWinDesc {
+    int32_t W;       // insertions in the window
+    int32_t seq0;    // sequence number of insertion 0
+    int32_t nslots;  // appended slots
+    int32_t nvupd;   // vertex-info updates
+};
+
+enum { SBC_KERR_LEVELS = 1 };
+
+struct SbcParams {
+    const SbcVInfo* vinfo;
+    const int32_t* col;
+    const int32_t* seq;
+    int32_t nv;
+    int32_t K;
+    int32_t lcap;
+    int32_t heavy;       // queue slots [0, heavy) run by whole-workgroup teams
+    int32_t* dist;
+    int64_t* sigma;
+    double* delta;
+    double* bc;
+    int32_t* mark;
+    int64_t* dpath;
+    double* ndel;
+    int32_t* list;
+    int32_t* lvl;
+    const int32_t* roots;
+    const int32_t* order; // queue order (nullptr: identity)
+    int32_t* qctr;        // [0] heavy queue head, [1] light queue head
+    const SbcWinDesc* win; // nullptr: single insertion (u0, v0), all edges visible
+    const int32_t* wu;
+    const int32_t* wv;
+    int32_t u0;
+    int32_t v0;
+    SbcItemRec* rec;      // [W][K] or nullptr
+    int32_t* err;
+    int32_t bc_atomic;
+};
+
+#define SBC_APPEND_INLINE 8
+
+// Appended adjacency slots and vertex-info updates: either a few entries
+// passed by value (single insertions) or a device-resident window
+// descriptor (captured window launches).
+struct SbcAppend {
+    SbcVInfo* vinfo;
+    int32_t* col;
+    int32_t* seq;
+    int32_t nslots, nvupd;
+    int32_t slot[SBC_APPEND_INLINE];
+    int32_t scol[SBC_APPEND_INLINE];
+    int32_t sseq[SBC_APPEND_INLINE];
+    int32_t vv[SBC_APPEND_INLINE];
+    SbcVInfo vi[SBC_APPEND_INLINE];
+    const SbcWinDesc* desc;
+    const int32_t* d_slot;
+    const int32_t* d_scol;
+    const int32_t* d_sseq;
+    const int32_t* d_vv;
+    const SbcVInfo* d_vi;
+};
+
+extern "C" {
+hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);
+hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
+hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
+hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
+hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);
+}

@@ -1,0 +1,618 @@
+// sbc_kernels.hip -- gfx950 kernels of the streaming-BC engine.
+//
+// Per root r the forest holds, in HBM, structure-of-arrays slices of NV
+// entries: dist (int32, INT32_MAX = unreachable), sigma (int64, LONG_MAX =
+// unreachable), delta (FP64), and per-root scratch used only while an update
+// of that root runs: mark (int32), dpath (int64, the added-path count the
+// reference calls diffPath), ndel (FP64, the new delta) and list (int32, the
+// processed vertices).  Scratch is cleared on the touched entries only, at
+// commit, as src/streamingbc_internal.c:213-242 does.
+//
+// mark encoding: 0 = untouched, otherwise ((newLevel + 1) << 2) | type with
+//   type 1 = descent vertex whose level is unchanged ("stay"),
+//   type 2 = descent vertex that moved up,
+//   type 3 = ascent-only vertex (a new parent of a processed vertex, or an
+//            unmoved vertex now level with a moved neighbour).
+//
+// One update item = (root, inserted edge).  A team (one wavefront, or the
+// whole workgroup) runs it:
+//   classify  gap = dist[u] - dist[v]  (src/streamingbc.c:98-118)
+//   descent   level-synchronous frontier expansion from the deeper endpoint s;
+//             frontier j holds exactly the vertices whose new level is
+//             dist[p] + 1 + j, so no level re-sort is needed
+//             (src/streamingbc_internal.c:294-400); sigma gains are
+//             accumulated with int64 atomics (exact in any order)
+//   ascent    levels from the deepest descent level up to the root; each
+//             vertex pulls from its processed children in adjacency order
+//             (src/streamingbc_internal_fine.c:560-999), with the terms
+//             summed in that order by walking the wavefront's lanes, so the
+//             new delta is bit-identical to the sequential FINE order
+//   commit    dist/sigma/delta written back, BC updated, scratch cleared.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sbc_kernels.h"
+
+#define INF32 0x7fffffff
+#define SIGMA_INF 0x7fffffffffffffffll
+
+// ------------------------------------------------------------ primitives
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Loads of scratch written by atomics from other wavefronts go to L2.
+__device__ __forceinline__ int32_t ld_acq32(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int64_t ld_acq64(const int64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_accd(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel32(int32_t* p, int32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel64(int64_t* p, int64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_reld(double* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+    int2 b = __builtin_bit_cast(int2, x);
+    b.x = __builtin_amdgcn_readlane(b.x, l);
+    b.y = __builtin_amdgcn_readlane(b.y, l);
+    return __builtin_bit_cast(double, b);
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int prefix_in_wave(uint64_t m)
+{
+    const int l = lane_id();
+    return __popcll(l == 0 ? 0ull : (m & (~0ull >> (64 - l))));
+}
+
+// Appends the lanes with `flag` set to a list whose length lives in LDS.
+// Must be called by the whole wavefront.  Returns the slot or -1.
+__device__ __forceinline__ int wave_append(bool flag, int* ctr)
+{
+    uint64_t m = ballot(flag);
+    if (m == 0) return -1;
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, __popcll(m));
+    base = __shfl(base, 0);
+    return flag ? base + prefix_in_wave(m) : -1;
+}
+
+template <int TW>
+__device__ __forceinline__ void team_sync()
+{
+    if constexpr (TW == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int32_t mark_enc(int32_t level, int32_t type) { return ((level + 1) << 2) | type; }
+__device__ __forceinline__ int32_t mark_level(int32_t m) { return (m >> 2) - 1; }
+
+// Number of leading slots of a vertex visible to an item with sequence q:
+// slots below deg0 are always visible; appended slots carry increasing
+// sequence numbers, so the visible ones form a prefix.
+__device__ __forceinline__ int visible_len(const SbcParams& P, const SbcVInfo& vi, int32_t q)
+{
+    if (q == INF32 || vi.deg == vi.deg0) return vi.deg;
+    int len = vi.deg0;
+    for (int e0 = vi.deg0; e0 < vi.deg; e0 += 64) {
+        int e = e0 + lane_id();
+        bool ok = e < vi.deg && P.seq[vi.start + e] <= q;
+        int n = __popcll(ballot(ok));
+        len += n;
+        if (n < 64) break;
+    }
+    return len;
+}
+
+// (sa / sb) * (d + 1) in FP64 -- the dependency term of
+// src/streamingbc_init.c:159-161 and src/streamingbc_internal.c:185-198
+__device__ __forceinline__ double dep(int64_t sa, int64_t sb, double d)
+{
+    return ((double)sa / (double)sb) * (d + 1.0);
+}
+
+// Per-team LDS block.
+struct TeamShared {
+    int cnt[4];                 // 0: |D|, 1: |S|, 2: end of U list, 3: pop result
+    unsigned long long stat[4]; // processed, scanned, moved, (unused)
+};
+
+// ------------------------------------------------------------ update item
+
+template <int TW>
+__device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, int twave)
+{
+    const int lane = lane_id();
+    const bool leader = (twave == 0 && lane == 0);
+    const int nv = P.nv;
+    const size_t off = (size_t)ri * (size_t)nv;
+    int32_t* dist = P.dist + off;
+    int64_t* sigma = P.sigma + off;
+    double* delta = P.delta + off;
+    int32_t* mark = P.mark + off;
+    int64_t* dpath = P.dpath + off;
+    double* ndel = P.ndel + off;
+    int32_t* list = P.list + off;
+    int32_t* lvlD = P.lvl + (size_t)ri * 2 * P.lcap;
+    int32_t* lvlS = lvlD + P.lcap;
+    const int32_t r = P.roots[ri];
+    const int32_t u = P.win ? P.wu[item] : P.u0;
+    const int32_t v = P.win ? P.wv[item] : P.v0;
+    const int32_t q = (P.win == nullptr) ? INF32 : P.win->seq0 + item;
+
+    const int32_t du = dist[u], dv = dist[v];
+    int64_t gap;
+    if (du == INF32 && dv == INF32) gap = 0;
+    else gap = (int64_t)du - (int64_t)dv;
+    SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
+    if (gap == 0) {
+        if (rec && leader) {
+            rec->root_index = ri; rec->insertion = item; rec->kase = 0; rec->gap = 0;
+            rec->processed = 0; rec->scanned = 0; rec->affected = 0; rec->moved = 0;
+        }
+        return;
+    }
+    const int32_t s = gap > 0 ? u : v;
+    const int32_t p = gap > 0 ? v : u;
+    const int64_t agap = gap > 0 ? gap : -gap;
+    const bool smoved = agap >= 2;
+    const int32_t base = dist[p] + 1; // new level of s
+
+    if (leader) {
+        st_rel32(&mark[s], mark_enc(base, smoved ? 2 : 1));
+        st_rel64(&dpath[s], sigma[p]);
+        list[0] = s;
+        lvlD[0] = 0;
+        lvlS[0] = 0;
+        ts.cnt[0] = 1;
+        ts.cnt[1] = 0;
+        ts.cnt[2] = 0;
+        ts.stat[0] = 0; ts.stat[1] = 0; ts.stat[2] = 0;
+    }
+    team_sync<TW>();
+
+    unsigned long long my_proc = 0, my_scan = 0, my_moved = 0;
+
+    // ---------------- descent
+    int fs = 0, fe = 1, j = 0;
+    bool overflow = false;
+    while (fs < fe) {
+        const int32_t lc = base + j; // new level of frontier j
+        const int32_t wantStay = mark_enc(lc + 1, 1), wantMove = mark_enc(lc + 1, 2);
+        for (int idx = fs + twave; idx < fe; idx += TW) {
+            const int32_t c = list[idx];
+            const int32_t mc = ld_acq32(&mark[c]);
+            const bool cmoved = (mc & 3) == 2;
+            const int64_t dc = ld_acq64(&dpath[c]);
+            const SbcVInfo vi = P.vinfo[c];
+            const int len = visible_len(P, vi, q);
+            for (int e0 = 0; e0 < len; e0 += 64) {
+                const int e = e0 + lane;
+                const bool ok = e < len;
+                bool appD = false, appS = false;
+                int32_t k = 0;
+                if (ok) {
+                    k = P.col[vi.start + e];
+                    const int32_t mk = ld_acq32(&mark[k]);
+                    if (mk == 0) {
+                        const int64_t cd = (int64_t)dist[k] - (int64_t)(lc + 1);
+                        if (cd >= 0) {
+                            const int32_t want = cd > 0 ? wantMove : wantStay;
+                            const int32_t old = atomicCAS(&mark[k], 0, want);
+                            appD = (old == 0);
+                            if (old == 0 || old == want)
+                                atomicAdd((unsigned long long*)&dpath[k], (unsigned long long)dc);
+                        } else if (cd == -1 && cmoved) {
+                            const int32_t old = atomicCAS(&mark[k], 0, mark_enc(lc, 3));
+                            appS = (old == 0);
+                        }
+                    } else if (mk == wantStay || mk == wantMove) {
+                        atomicAdd((unsigned long long*)&dpath[k], (unsigned long long)dc);
+                    }
+                }
+                const int pd = wave_append(appD, &ts.cnt[0]);
+                if (pd >= 0) list[pd] = k;
+                const int psl = wave_append(appS, &ts.cnt[1]);
+                if (psl >= 0) list[nv - 1 - psl] = k;
+            }
+            my_proc += 1;
+            my_scan += (unsigned long long)len;
+            my_moved += cmoved ? 1 : 0;
+        }
+        team_sync<TW>();
+        const int nfe = ts.cnt[0];
+        const int nS = ts.cnt[1];
+        team_sync<TW>();
+        j++;
+        if (j + 1 >= P.lcap) { overflow = true; break; }
+        // frontier j-1 = list[lvlD[j-1] .. lvlD[j]); S vertices found while
+        // expanding it = S list [lvlS[j-1] .. lvlS[j])
+        if (leader) { lvlD[j] = fe; lvlS[j] = nS; }
+        fs = fe;
+        fe = nfe;
+    }
+    if (overflow) {
+        if (leader) atomicOr(P.err, SBC_KERR_LEVELS);
+        return;
+    }
+    team_sync<TW>();
+    const int nf = j; // frontiers 0..nf-1
+    const int nD = ts.cnt[0];
+    if (leader) ts.cnt[2] = nD;
+    team_sync<TW>();
+
+    // ---------------- ascent
+    int uA = nD, uB = nD;
+    for (int lev = base + nf - 1; lev >= 0; --lev) {
+        const int jj = lev - base;
+        int dA = 0, dB = 0, sA = 0, sB = 0;
+        if (jj >= 0) { dA = lvlD[jj]; dB = lvlD[jj + 1]; sA = lvlS[jj]; sB = lvlS[jj + 1]; }
+        const int nd = dB - dA, ns = sB - sA, nu = uB - uA;
+        const int tot = nd + ns + nu;
+        if (tot == 0) break;
+        for (int w = twave; w < tot; w += TW) {
+            int32_t k;
+            if (w < nd) k = list[dA + w];
+            else if (w < nd + ns) k = list[nv - 1 - (sA + w - nd)];
+            else k = list[uA + w - nd - ns];
+            const int32_t mk = ld_acq32(&mark[k]);
+            const int ktype = mk & 3;
+            const bool isD = ktype != 3;
+            const int64_t sk_old = sigma[k];
+            const int64_t sk_new = isD ? (int64_t)((uint64_t)ld_acq64(&dpath[k]) + (uint64_t)(ktype == 1 ? sk_old : 0))
+                                       : sk_old;
+            double acc = isD ? 0.0 : delta[k];
+            const SbcVInfo vi = P.vinfo[k];
+            const int len = visible_len(P, vi, q);
+            for (int e0 = 0; e0 < len; e0 += 64) {
+                const int e = e0 + lane;
+                const bool ok = e < len;
+                bool child = false, hsub = false, appU = false;
+                double add = 0.0, sub = 0.0;
+                int32_t c = 0;
+                if (ok) {
+                    c = P.col[vi.start + e];
+                    const int32_t mc = ld_acq32(&mark[c]);
+                    if (mc != 0) {
+                        const int32_t lvc = mark_level(mc);
+                        const int ctype = mc & 3;
+                        if (lvc == lev + 1) {
+                            child = true;
+                            const int64_t sc_old = sigma[c];
+                            const int64_t sc_new = ctype == 3 ? sc_old
+                                : (int64_t)((uint64_t)ld_acq64(&dpath[c]) + (uint64_t)(ctype == 1 ? sc_old : 0));
+                            add = dep(sk_new, sc_new, ld_accd(&ndel[c]));
+                            if (!isD && !(k == p && c == s)) {
+                                hsub = true;
+                                sub = dep(sk_old, sc_old, delta[c]);
+                            }
+                        } else if (!isD && lvc == lev && ctype == 2) {
+                            hsub = true;
+                            sub = dep(sk_old, sigma[c], delta[c]);
+                        }
+                    } else if (lev >= 1 && dist[c] == lev - 1) {
+                        appU = (atomicCAS(&mark[c], 0, mark_enc(lev - 1, 3)) == 0);
+                    }
+                }
+                const int pu = wave_append(appU, &ts.cnt[2]);
+                if (pu >= 0) list[pu] = c;
+                // sum the terms in adjacency order
+                const uint64_t cm = ballot(child), sm = ballot(hsub);
+                uint64_t m = cm | sm;
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if ((cm >> l) & 1) acc = acc + readlane_d(add, l);
+                    if ((sm >> l) & 1) acc = acc - readlane_d(sub, l);
+                }
+            }
+            if (lane == 0) st_reld(&ndel[k], acc);
+            my_proc += 1;
+            my_scan += (unsigned long long)len;
+        }
+        team_sync<TW>();
+        uA = uB;
+        uB = ts.cnt[2];
+        team_sync<TW>();
+    }
+
+    // ---------------- commit
+    const int nUend = ts.cnt[2];
+    const int nS = ts.cnt[1];
+    const int total = nUend + nS;
+    for (int w = twave * 64 + lane; w < total; w += TW * 64) {
+        const int32_t k = w < nUend ? list[w] : list[nv - 1 - (w - nUend)];
+        const int32_t mk = ld_acq32(&mark[k]);
+        const int ktype = mk & 3;
+        const double nd = ld_accd(&ndel[k]);
+        const double od = delta[k];
+        if (ktype != 3) {
+            const int64_t so = sigma[k];
+            dist[k] = mark_level(mk);
+            sigma[k] = (int64_t)((uint64_t)ld_acq64(&dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
+        }
+        delta[k] = nd;
+        if (P.bc_atomic && k != r) atomicAdd(&P.bc[k], nd - od);
+        st_rel32(&mark[k], 0);
+        st_rel64(&dpath[k], 0);
+        st_reld(&ndel[k], 0.0);
+    }
+    if (lane == 0) {
+        atomicAdd(&ts.stat[0], my_proc);
+        atomicAdd(&ts.stat[1], my_scan);
+        atomicAdd(&ts.stat[2], my_moved);
+    }
+    team_sync<TW>();
+    if (rec && leader) {
+        rec->root_index = ri;
+        rec->insertion = item;
+        rec->kase = smoved ? 2 : 1;
+        rec->gap = (int32_t)(gap > INF32 ? INF32 : (gap < -INF32 ? -INF32 : gap));
+        rec->processed = (int64_t)ts.stat[0];
+        rec->scanned = (int64_t)ts.stat[1];
+        rec->affected = total;
+        rec->moved = (int64_t)ts.stat[2];
+    }
+    team_sync<TW>();
+}
+
+// Persistent window kernel.  Phase 1: whole-workgroup teams pop root slots
+// [0, heavy) of the queue order; phase 2: each wavefront pops slots
+// [heavy, K).  A team applies every insertion of the window to its root in
+// stream order before popping the next root.
+template <int NWB>
+__global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
+{
+    __shared__ TeamShared ts[NWB];
+    const int wave = threadIdx.x >> 6;
+    const int W = P.win ? P.win->W : 1;
+    // phase 1: heavy
+    for (;;) {
+        if (threadIdx.x == 0) ts[0].cnt[3] = atomicAdd(&P.qctr[0], 1);
+        __syncthreads();
+        const int t = ts[0].cnt[3];
+        __syncthreads();
+        if (t >= P.heavy) break;
+        const int ri = P.order ? P.order[t] : t;
+        for (int i = 0; i < W; ++i) process_item<NWB>(P, ri, i, ts[0], wave);
+    }
+    // phase 2: light, one wavefront per root
+    for (;;) {
+        if (lane_id() == 0) ts[wave].cnt[3] = atomicAdd(&P.qctr[1], 1);
+        team_sync<1>();
+        const int t = P.heavy + ts[wave].cnt[3];
+        team_sync<1>();
+        if (t >= P.K) break;
+        const int ri = P.order ? P.order[t] : t;
+        for (int i = 0; i < W; ++i) process_item<1>(P, ri, i, ts[wave], 0);
+    }
+}
+
+// ------------------------------------------------------------ static pass
+
+// One workgroup per root: BFS computing dist and sigma level by level, then
+// each vertex pulls delta from its children in adjacency order.
+// (src/streamingbc_init.c:87-183 with the ascent in pull form)
+template <int NWB>
+__global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
+{
+    __shared__ TeamShared tsh;
+    TeamShared& ts = tsh;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const bool leader = threadIdx.x == 0;
+    const int nv = P.nv;
+    for (;;) {
+        if (leader) ts.cnt[3] = atomicAdd(&P.qctr[0], 1);
+        __syncthreads();
+        const int ri = ts.cnt[3];
+        __syncthreads();
+        if (ri >= P.K) break;
+        const size_t off = (size_t)ri * (size_t)nv;
+        int32_t* dist = P.dist + off;
+        int64_t* sigma = P.sigma + off;
+        double* delta = P.delta + off;
+        int64_t* acc_s = P.dpath + off;
+        int32_t* list = P.list + off;
+        int32_t* lvl = P.lvl + (size_t)ri * 2 * P.lcap;
+        const int32_t r = P.roots[ri];
+        for (int x = threadIdx.x; x < nv; x += NWB * 64) {
+            st_rel32(&dist[x], INF32);
+            sigma[x] = SIGMA_INF;
+            delta[x] = 0.0;
+        }
+        __syncthreads();
+        if (leader) {
+            st_rel32(&dist[r], 0);
+            sigma[r] = 1;
+            list[0] = r;
+            lvl[0] = 0;
+            lvl[1] = 1;
+            ts.cnt[0] = 1;
+        }
+        __syncthreads();
+        int fs = 0, fe = 1, j = 0;
+        bool overflow = false;
+        while (fs < fe) {
+            for (int idx = fs + wave; idx < fe; idx += NWB) {
+                const int32_t c = list[idx];
+                const int64_t sc = sigma[c];
+                const SbcVInfo vi = P.vinfo[c];
+                const int len = vi.deg; // static pass sees the whole graph
+                for (int e0 = 0; e0 < len; e0 += 64) {
+                    const int e = e0 + lane;
+                    bool app = false;
+                    int32_t k = 0;
+                    if (e < len) {
+                        k = P.col[vi.start + e];
+                        int32_t dk = ld_acq32(&dist[k]);
+                        if (dk == INF32) {
+                            const int32_t old = atomicCAS(&dist[k], INF32, j + 1);
+                            app = (old == INF32);
+                            dk = app ? j + 1 : old;
+                        }
+                        if (dk == j + 1) atomicAdd((unsigned long long*)&acc_s[k], (unsigned long long)sc);
+                    }
+                    const int pos = wave_append(app, &ts.cnt[0]);
+                    if (pos >= 0) list[pos] = k;
+                }
+            }
+            __syncthreads();
+            const int nfe = ts.cnt[0];
+            __syncthreads();
+            for (int x = fe + threadIdx.x; x < nfe; x += NWB * 64) {
+                const int32_t k = list[x];
+                sigma[k] = ld_acq64(&acc_s[k]);
+                st_rel64(&acc_s[k], 0);
+            }
+            j++;
+            if (j + 1 >= P.lcap) { overflow = true; break; }
+            if (leader) lvl[j + 1] = nfe;
+            fs = fe;
+            fe = nfe;
+            __syncthreads();
+        }
+        if (overflow) {
+            if (leader) atomicOr(P.err, SBC_KERR_LEVELS);
+            continue;
+        }
+        // levels 0..j-1; level L = list[lvl[L] .. lvl[L+1])
+        __syncthreads();
+        for (int L = j - 1; L >= 0; --L) {
+            const int a = lvl[L], b = lvl[L + 1];
+            for (int idx = a + wave; idx < b; idx += NWB) {
+                const int32_t k = list[idx];
+                const int64_t sk = sigma[k];
+                const SbcVInfo vi = P.vinfo[k];
+                double acc = 0.0;
+                for (int e0 = 0; e0 < vi.deg; e0 += 64) {
+                    const int e = e0 + lane;
+                    bool child = false;
+                    double add = 0.0;
+                    if (e < vi.deg) {
+                        const int32_t c = P.col[vi.start + e];
+                        if (ld_acq32(&dist[c]) == L + 1) {
+                            child = true;
+                            add = dep(sk, sigma[c], delta[c]);
+                        }
+                    }
+                    uint64_t m = ballot(child);
+                    while (m) {
+                        const int l = __builtin_ctzll(m);
+                        m &= m - 1;
+                        acc = acc + readlane_d(add, l);
+                    }
+                }
+                if (lane == 0) delta[k] = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// BC[v] = sum over roots, in root-array order, of delta_r[v] (r != v).
+__global__ void sbc_bc_reduce_kernel(SbcParams P)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    double acc = 0.0;
+    for (int ri = 0; ri < P.K; ++ri) {
+        if (P.roots[ri] == v) continue;
+        acc += P.delta[(size_t)ri * (size_t)P.nv + v];
+    }
+    P.bc[v] = 0.0 + acc;
+}
+
+// Clears the work-queue counters (a kernel, not a memset, so it can sit in
+// a captured graph).
+__global__ void sbc_reset_kernel(int32_t* q, int n)
+{
+    if ((int)threadIdx.x < n) q[threadIdx.x] = 0;
+}
+
+// Applies a batch of appended slots and vertex-info updates.
+__global__ void sbc_append_kernel(SbcAppend A)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A.desc) {
+        if (t < A.desc->nslots) {
+            A.col[A.d_slot[t]] = A.d_scol[t];
+            A.seq[A.d_slot[t]] = A.d_sseq[t];
+        }
+        if (t < A.desc->nvupd) A.vinfo[A.d_vv[t]] = A.d_vi[t];
+    } else {
+        if (t < A.nslots) {
+            A.col[A.slot[t]] = A.scol[t];
+            A.seq[A.slot[t]] = A.sseq[t];
+        }
+        if (t < A.nvupd) A.vinfo[A.vv[t]] = A.vi[t];
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+extern "C" hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st)
+{
+    switch (nwb) {
+    case 1: hipLaunchKernelGGL(sbc_window_kernel<1>, dim3(grid), dim3(64), 0, st, *P); break;
+    case 2: hipLaunchKernelGGL(sbc_window_kernel<2>, dim3(grid), dim3(128), 0, st, *P); break;
+    case 4: hipLaunchKernelGGL(sbc_window_kernel<4>, dim3(grid), dim3(256), 0, st, *P); break;
+    case 8: hipLaunchKernelGGL(sbc_window_kernel<8>, dim3(grid), dim3(512), 0, st, *P); break;
+    case 16: hipLaunchKernelGGL(sbc_window_kernel<16>, dim3(grid), dim3(1024), 0, st, *P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st)
+{
+    switch (nwb) {
+    case 4: hipLaunchKernelGGL(sbc_static_kernel<4>, dim3(grid), dim3(256), 0, st, *P); break;
+    case 8: hipLaunchKernelGGL(sbc_static_kernel<8>, dim3(grid), dim3(512), 0, st, *P); break;
+    case 16: hipLaunchKernelGGL(sbc_static_kernel<16>, dim3(grid), dim3(1024), 0, st, *P); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st)
+{
+    const int bs = 256;
+    hipLaunchKernelGGL(sbc_bc_reduce_kernel, dim3((P->nv + bs - 1) / bs), dim3(bs), 0, st, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_reset_kernel, dim3(1), dim3(64), 0, st, q, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st)
+{
+    if (maxn <= 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(sbc_append_kernel, dim3((maxn + bs - 1) / bs), dim3(bs), 0, st, *A);
+    return hipGetLastError();
+}

@@ -1,0 +1,173 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded inputs.
+
+Bar: dist (level) and sigma bit-exact; per-root delta bit-exact against the
+oracle's FINE-order restatement (the engine sums each vertex's terms in
+adjacency order, with FP contraction off); BC within 1e-9 relative (floor
+1e-6 x max|BC|) because the device adds per-root BC changes with atomics in
+arbitrary order.
+"""
+import numpy as np
+import pytest
+
+from oracle_py import COARSE as O_COARSE, FINE as O_FINE, PULL, PUSH, Oracle
+from sbc_testutil import LONG_MAX, assert_bc_close, karate_path
+
+pytestmark = pytest.mark.gpu
+
+sbc = pytest.importorskip("streamingbc_amd")
+
+
+def _engine(hg, roots):
+    ctx = sbc.Context(hg.nv)
+    g = sbc.Graph(hg)
+    f = sbc.Forest(hg.nv, roots)
+    f.init_static(g, ctx)
+    return ctx, g, f
+
+
+def _compare_trees(f, o, what, delta_exact=True):
+    lv, sg, dl = f.trees()
+    lo, so, do = o.trees()
+    assert np.array_equal(lv, lo), f"{what}: level mismatch at {np.argwhere(lv != lo)[:5]}"
+    assert np.array_equal(sg, so), f"{what}: sigma mismatch at {np.argwhere(sg != so)[:5]}"
+    if delta_exact:
+        bad = np.argwhere(dl != do)
+        assert bad.size == 0, f"{what}: delta differs at {bad[:5]} gpu={dl[tuple(bad[0])]!r} orc={do[tuple(bad[0])]!r}"
+    else:
+        assert_bc_close(dl, do, what=f"{what} delta")
+
+
+def _rmat_c1():
+    hg = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
+    u, v = hg.insertions(256, 1)
+    return hg, np.arange(hg.nv), u, v
+
+
+@pytest.mark.parametrize("name", ["karate", "rmat10"])
+def test_static_matches_oracle(name):
+    if name == "karate":
+        hg = sbc.read_dimacs(karate_path())
+        roots = np.arange(hg.nv)
+    else:
+        hg, roots, _, _ = _rmat_c1()
+    print(f"[static] {name} nv={hg.nv} k={len(roots)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    _compare_trees(f, o, f"static {name}")
+    # same fold order as the oracle's (root order): BC is bit-identical
+    assert np.array_equal(f.bc(), o.bc())
+    # and the reference's push order agrees within tolerance
+    op = Oracle(hg.rowptr, hg.col, roots)
+    op.static(PUSH)
+    assert_bc_close(f.bc(), op.bc())
+
+
+@pytest.mark.parametrize("gran", [sbc.FINE, sbc.COARSE])
+def test_plain_insert_karate(gran):
+    hg = sbc.read_dimacs(karate_path())
+    roots = np.arange(hg.nv)
+    u, v = hg.insertions(20, 5)
+    print(f"[plain] karate gran={gran} n={len(u)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    for a, b in zip(u, v):
+        f.insert_edge(g, ctx, int(a), int(b), gran)
+        o.insert(int(a), int(b), O_FINE)
+    _compare_trees(f, o, "karate plain")
+    assert_bc_close(f.bc(), o.bc())
+
+
+def test_plain_insert_rmat10_exact():
+    hg, roots, u, v = _rmat_c1()
+    print(f"[plain] rmat10 exact nv={hg.nv} n={len(u)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    for i, (a, b) in enumerate(zip(u, v)):
+        f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+        rec_o = o.insert(int(a), int(b), O_FINE, items=True)
+        if i % 64 == 0:
+            rec_g = ctx.last_records(f.k)
+            assert np.array_equal(rec_g["kase"], rec_o["kase"])
+            assert np.array_equal(rec_g["affected"], rec_o["affected"])
+            assert np.array_equal(rec_g["moved"], rec_o["moved"])
+            assert np.array_equal(rec_g["processed"], rec_o["processed"])
+            assert np.array_equal(rec_g["scanned"], rec_o["scanned"])
+    _compare_trees(f, o, "rmat10 plain")
+    assert_bc_close(f.bc(), o.bc())
+
+
+def test_update_equals_static_recompute_chunglu():
+    hg = sbc.gen_chung_lu(4096, 2.3, 8.0, seed=3, keep_lcc=True)
+    roots = hg.pick_roots(64, 7)
+    u, v = hg.insertions(200, 9)
+    print(f"[recompute] chunglu nv={hg.nv} k=64 n={len(u)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    for a, b in zip(u, v):
+        f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+    # static pass of the oracle on the post-insertion graph
+    o = Oracle(hg.rowptr, hg.col, roots)
+    for a, b in zip(u, v):
+        o.add_edge(int(a), int(b))
+    o.static(PULL)
+    lv, sg, dl = f.trees()
+    lo, so, do = o.trees()
+    assert np.array_equal(lv, lo)
+    assert np.array_equal(sg, so)
+    assert_bc_close(dl, do, what="delta vs recompute")
+    assert_bc_close(f.bc(), o.bc(), what="bc vs recompute")
+
+
+@pytest.mark.parametrize("window,block,heavy", [(8, 256, -1), (16, 64, 0), (5, 512, 10)])
+def test_batched_matches_plain(window, block, heavy):
+    hg = sbc.gen_chung_lu(4096, 2.3, 8.0, seed=5, keep_lcc=True)
+    roots = hg.pick_roots(48, 7)
+    u, v = hg.insertions(64, 11)
+    print(f"[batched] w={window} bt={block} heavy={heavy} nv={hg.nv}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    cfg = sbc.BatchConfig(window=window, block_threads=block, heavy_roots=heavy, use_graph=True)
+    rec = f.insert_batched(g, ctx, u, v, cfg, records=True)
+    for i, (a, b) in enumerate(zip(u, v)):
+        ro = o.insert(int(a), int(b), O_FINE, items=True)
+        assert np.array_equal(rec[i]["kase"], ro["kase"]), f"insertion {i}"
+        assert np.array_equal(rec[i]["affected"], ro["affected"]), f"insertion {i}"
+    _compare_trees(f, o, "batched")
+    assert_bc_close(f.bc(), o.bc())
+    # deterministic BC readout equals the oracle's root-order sum within tolerance
+    f.recompute_bc(ctx)
+    assert_bc_close(f.bc(), o.bc())
+
+
+def test_batched_schedule_invariance():
+    hg = sbc.gen_chung_lu(4096, 2.3, 8.0, seed=6, keep_lcc=True)
+    roots = hg.pick_roots(40, 2)
+    u, v = hg.insertions(48, 4)
+    print("[invariance] two schedules", flush=True)
+    outs = []
+    for order, heavy, bt in [(None, -1, 256), (np.arange(40)[::-1].copy(), 7, 1024)]:
+        ctx, g, f = _engine(hg, roots)
+        cfg = sbc.BatchConfig(window=12, block_threads=bt, heavy_roots=heavy, root_order=order,
+                              recompute_bc=True)
+        f.insert_batched(g, ctx, u, v, cfg)
+        outs.append((f.trees(), f.bc()))
+    (t1, b1), (t2, b2) = outs
+    for a, b in zip(t1, t2):
+        assert np.array_equal(a, b)
+    assert np.array_equal(b1, b2)  # deterministic BC: bit-identical across schedules
+
+
+def test_errors_are_reported():
+    hg = sbc.read_dimacs(karate_path())
+    ctx, g, f = _engine(hg, np.arange(hg.nv))
+    with pytest.raises(sbc.SbcError) as e:
+        f.insert_edge(g, ctx, 1, 2, sbc.FINE)  # already an edge
+    assert e.value.status == sbc.SBC_ERR_EXISTS
+    with pytest.raises(sbc.SbcError) as e:
+        f.insert_batched(g, ctx, [3, 5], [3, 6], sbc.BatchConfig(window=2))
+    assert e.value.status == sbc.SBC_ERR_EXISTS
+    with pytest.raises(sbc.SbcError):
+        f.insert_edge(g, ctx, 1, 10_000, sbc.FINE)
