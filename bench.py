@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark: edge insertions per second with BC updated for all K roots.
+
+Default workload (the headline configuration): undirected R-MAT scale 20
+(2^20 vertices, 16 edge draws per vertex, a/b/c/d = .57/.19/.19/.05,
+deduplicated, seed 1), K = 256 roots drawn from the largest component (seed
+7), random non-adjacent insertions inside it (seed 11), applied in windows of
+128 by the batched persistent kernel.  One step = one window.
+
+Printed JSON line (rank 0):
+  value         insertions/s of the batched form over the timed steps (inputs
+                resident in HBM; the window's edge list is copied in per step)
+  plain_gpu     insertions/s of the plain form (one launch sequence per insertion)
+  roofline      algorithmic bytes of the update kernel (per-item formula below)
+                / its average launch time from HIP events on the context stream
+  cpu_baseline  the reference's own OpenMP COARSE update (compiled from its
+                sources into oracle/_ref) timed on a bounded prefix of the
+                same stream on this host's cores
+
+Algorithmic bytes per (root, insertion) item:
+  8 (classify: dist[u], dist[v])
+  + 8 * processed (row begin/end per vertex visit, descent and ascent)
+  + 8 * scanned   (4 B neighbour id + 4 B dist per adjacency entry)
+  + 48 * affected (read sigma,delta 16 + write sigma,delta 16 + BC RMW 16)
+  + 4 * moved     (dist write)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "edge insertions/sec with BC updated (K roots) vs plain GPU; p99 update latency"
+
+WORKLOADS = {
+    # name: (generator, args, K, window, description)
+    "rmat20_k256_w128": ("rmat", dict(scale=20, draws=16 << 20, seed=1, keep_lcc=False), 256, 128,
+                         "undirected R-MAT scale 20, 16 draws/vertex, deduplicated, K=256, windows of 128"),
+    "rmat18_k512_w64": ("rmat", dict(scale=18, draws=16 << 18, seed=1, keep_lcc=False), 512, 64,
+                        "undirected R-MAT scale 18, 16 draws/vertex, deduplicated, K=512, windows of 64"),
+    "chunglu15_k256": ("chunglu", dict(n=1 << 15, gamma=2.3, mean_deg=8.0, seed=1, keep_lcc=True), 256, 64,
+                       "Chung-Lu 32768 vertices, exponent 2.3, mean degree 8, LCC, K=256"),
+}
+
+
+def item_bytes(rec) -> float:
+    r = rec
+    return float(8 * r.size + 8 * r["processed"].sum() + 8 * r["scanned"].sum() + 48 * r["affected"].sum()
+                 + 4 * r["moved"].sum())
+
+
+def make_workload(name, n_ins):
+    import streamingbc_amd as sbc
+    kind, kw, K, W, desc = WORKLOADS[name]
+    t0 = time.time()
+    if kind == "rmat":
+        hg = sbc.gen_rmat(kw["scale"], kw["draws"], seed=kw["seed"], keep_lcc=kw["keep_lcc"])
+    else:
+        hg = sbc.gen_chung_lu(kw["n"], kw["gamma"], kw["mean_deg"], seed=kw["seed"], keep_lcc=kw["keep_lcc"])
+    roots = hg.pick_roots(K, 7)
+    u, v = hg.insertions(n_ins, 11)
+    return hg, roots, u, v, K, W, desc, time.time() - t0
+
+
+def cpu_baseline(hg, roots, u, v, budget_s, threads, gran):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_py import Reference, ref_available
+    if not ref_available():
+        return None
+    ref = Reference(hg.rowptr, hg.col, roots, nt=threads)
+    t_static = ref.static()
+    spent, n = 0.0, 0
+    while n < len(u) and (spent < budget_s or n == 0):
+        spent += ref.insert(int(u[n]), int(v[n]), 1, gran)  # BALANCE, as every reference run script
+        n += 1
+    ref.close()
+    return {"value": n / spent, "unit": "insertions/s", "cores": threads, "kind": "reference",
+            "granularity": "COARSE" if gran == 0 else "FINE",
+            "sample": f"first {n} insertions of the same stream, {spent:.1f} s inside insertEdgeStreamingBC "
+                      f"(static pass {t_static:.1f} s untimed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat20_k256_w128", choices=sorted(WORKLOADS))
+    ap.add_argument("--window", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--heavy", type=int, default=-2, help="-2: autotuned/default, -1: all heavy")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain", type=int, default=64, help="insertions timed in the plain form")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference update time")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(roots would be sharded across GPUs with one all-reduce of BC per readout)", file=sys.stderr)
+        sys.exit(2)
+
+    import streamingbc_amd as sbc
+
+    kind, kw, K, W, desc = WORKLOADS[args.workload]
+    if args.window:
+        W = args.window
+    n_ins = (args.warmup + args.steps) * W + args.plain
+    hg, roots, u, v, K, _, desc, t_gen = make_workload(args.workload, n_ins)
+    print(f"[bench] {args.workload}: nv={hg.nv} directed slots={hg.ne} K={K} W={W} "
+          f"insertions={len(u)} gen={t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    ctx = sbc.Context(hg.nv)
+    g = sbc.Graph(hg)
+    f = sbc.Forest(hg.nv, roots)
+    t0 = time.time()
+    f.init_static(g, ctx)
+    t_static = time.time() - t0
+    print(f"[bench] static pass {t_static:.2f}s", file=sys.stderr, flush=True)
+
+    cfg = sbc.BatchConfig(window=W, block_threads=args.block or 1024, use_graph=not args.no_graph)
+    if args.heavy >= -1:
+        cfg.heavy_roots = args.heavy
+
+    # warmup windows
+    pos = 0
+    for _ in range(args.warmup):
+        f.insert_batched(g, ctx, u[pos:pos + W], v[pos:pos + W], cfg)
+        pos += W
+    ctx.synchronize()
+
+    # timed windows
+    ctx.reset_stats(timing=True)
+    lat = []
+    recs = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        recs.append(f.insert_batched(g, ctx, u[pos:pos + W], v[pos:pos + W], cfg, records=True))
+        lat.append(time.perf_counter() - ts)
+        pos += W
+    ctx.synchronize()
+    t_batched = time.perf_counter() - t0
+    n_launch, k_ms = ctx.kernel_stats(sbc.KSTAT_UPDATE)
+    rec = np.concatenate([r.reshape(-1) for r in recs])
+    alg_bytes = item_bytes(rec)
+    per_root = np.bincount(rec["root_index"], weights=rec["scanned"], minlength=K)
+    value = args.steps * W / t_batched
+    print(f"[bench] batched: {value:.1f} ins/s, window {1e3 * t_batched / args.steps:.2f} ms, "
+          f"kernel avg {k_ms / max(n_launch, 1):.2f} ms", file=sys.stderr, flush=True)
+
+    # plain form on the continuation of the stream
+    plain = None
+    if args.plain > 0:
+        ctx.reset_stats(timing=True)
+        pr = []
+        ctx.synchronize()
+        tp = time.perf_counter()
+        for i in range(args.plain):
+            f.insert_edge(g, ctx, int(u[pos + i]), int(v[pos + i]), sbc.FINE)
+            pr.append(ctx.last_records(K))
+        ctx.synchronize()
+        t_plain = time.perf_counter() - tp
+        pn, pms = ctx.kernel_stats(sbc.KSTAT_UPDATE)
+        prec = np.concatenate(pr)
+        plain = {"value": args.plain / t_plain, "unit": "insertions/s", "insertions": args.plain,
+                 "kernel_ms_avg": pms / max(pn, 1),
+                 "achieved_GBs": item_bytes(prec) / (pms * 1e-3) / 1e9 if pms > 0 else None}
+        print(f"[bench] plain: {plain['value']:.1f} ins/s", file=sys.stderr, flush=True)
+
+    launch_ms = k_ms / max(n_launch, 1)
+    achieved = alg_bytes / max(n_launch, 1) / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "sbc_window_kernel", "launch_ms": round(launch_ms, 4),
+                "alg_bytes_per_launch": alg_bytes / max(n_launch, 1)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "insertions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_batched / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "nv": hg.nv, "directed_slots": hg.ne,
+                   "roots": K, "window": W, "block_threads": cfg.block_threads, "hip_graph": cfg.use_graph,
+                   "parallelism": "single GPU, roots across workgroups"},
+        "latency_ms": {"window_p50": round(1e3 * float(np.percentile(lat, 50)), 4),
+                       "window_p99": round(1e3 * float(np.percentile(lat, 99)), 4)},
+        "plain_gpu": plain,
+        "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
+        "roofline": roofline,
+        "items": {"count": int(rec.size), "nonzero_gap": int((rec["kase"] > 0).sum()),
+                  "root_scan_max_over_mean": float(per_root.max() / max(per_root.mean(), 1)),
+                  "mean_affected": float(rec["affected"].mean()), "max_affected": int(rec["affected"].max()),
+                  "mean_scanned": float(rec["scanned"].mean())},
+        "static_pass_s": round(t_static, 3),
+    }
+
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        f.close()
+        g.close()
+        cb = cpu_baseline(hg, roots, u, v, args.cpu_budget, threads, 0)
+        if cb:
+            out["cpu_baseline"] = cb
+            out["speedup_vs_cpu"] = round(value / cb["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -296,11 +296,10 @@ struct sbcForestDev {
     bool initialized = false;
     std::vector<int64_t> roots;
     int32_t* d_roots = nullptr;
-    int32_t* d_dist = nullptr;
+    int32_t* d_st = nullptr;     // packed level << 2 | type
     int64_t* d_sigma = nullptr;
     double* d_delta = nullptr;
     double* d_bc = nullptr;
-    int32_t* d_mark = nullptr;
     int64_t* d_dpath = nullptr;
     double* d_ndel = nullptr;
     int32_t* d_list = nullptr;
@@ -312,14 +311,14 @@ static int32_t lcap_for(int64_t nv) { return (int32_t)(nv + 2 < 4096 ? nv + 2 : 
 
 extern "C" int64_t streamingBCForestFootprint(int64_t NV, int64_t NK)
 {
-    int64_t per = 4 + 8 + 8 + 4 + 8 + 8 + 4;
+    int64_t per = 4 + 8 + 8 + 8 + 8 + 4; // st, sigma, delta, dpath, ndel, list
     return NK * NV * per + NK * 2 * (int64_t)lcap_for(NV) * 4 + NV * 8 + NK * 4 + 64;
 }
 
 static void forest_free_dev(sbcForestDev* F)
 {
     if (!F) return;
-    void* ptrs[] = {F->d_roots, F->d_dist, F->d_sigma, F->d_delta, F->d_bc, F->d_mark,
+    void* ptrs[] = {F->d_roots, F->d_st, F->d_sigma, F->d_delta, F->d_bc,
                     F->d_dpath, F->d_ndel, F->d_list, F->d_lvl, F->d_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -358,8 +357,8 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
     }
     size_t kn = (size_t)NK * (size_t)NV;
     sbcStatus st = SBC_OK;
-    if ((st = dalloc(&F->d_roots, NK)) || (st = dalloc(&F->d_dist, kn)) || (st = dalloc(&F->d_sigma, kn)) ||
-        (st = dalloc(&F->d_delta, kn)) || (st = dalloc(&F->d_bc, NV)) || (st = dalloc(&F->d_mark, kn)) ||
+    if ((st = dalloc(&F->d_roots, NK)) || (st = dalloc(&F->d_st, kn)) || (st = dalloc(&F->d_sigma, kn)) ||
+        (st = dalloc(&F->d_delta, kn)) || (st = dalloc(&F->d_bc, NV)) ||
         (st = dalloc(&F->d_dpath, kn)) || (st = dalloc(&F->d_ndel, kn)) || (st = dalloc(&F->d_list, kn)) ||
         (st = dalloc(&F->d_lvl, (size_t)NK * 2 * F->lcap)) || (st = dalloc(&F->d_err, 4))) {
         forest_free_dev(F);
@@ -367,7 +366,7 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
     }
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMemcpy(F->d_roots, r32.data(), sizeof(int32_t) * NK, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(F->d_mark, 0, sizeof(int32_t) * kn);
+    if (e == hipSuccess) e = hipMemset(F->d_st, 0, sizeof(int32_t) * kn);
     if (e == hipSuccess) e = hipMemset(F->d_dpath, 0, sizeof(int64_t) * kn);
     if (e == hipSuccess) e = hipMemset(F->d_ndel, 0, sizeof(double) * kn);
     if (e == hipSuccess) e = hipMemset(F->d_bc, 0, sizeof(double) * NV);
@@ -608,11 +607,10 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.nv = (int32_t)F->nv;
     P.K = (int32_t)F->K;
     P.lcap = F->lcap;
-    P.dist = F->d_dist;
+    P.st = F->d_st;
     P.sigma = F->d_sigma;
     P.delta = F->d_delta;
     P.bc = F->d_bc;
-    P.mark = F->d_mark;
     P.dpath = F->d_dpath;
     P.ndel = F->d_ndel;
     P.list = F->d_list;
@@ -701,9 +699,10 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
     P.u0 = (int32_t)u;
     P.v0 = (int32_t)v;
     P.rec = C->d_rec;
-    const int nwb = 4;
+    int nwb = 4;
     int grid;
     if (granularity == FINE) {
+        nwb = 16;
         P.heavy = (int32_t)F->K;
         grid = (int)(F->K < 2LL * C->num_cu ? F->K : 2LL * C->num_cu);
     } else {
@@ -768,8 +767,9 @@ extern "C" sbcStatus streamingBCGetTree(bcForest* forest, int64_t ri, int64_t* l
     size_t off = (size_t)ri * (size_t)F->nv;
     if (level) {
         std::vector<int32_t> d(F->nv);
-        HIPCHK(hipMemcpy(d.data(), F->d_dist + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
-        for (int64_t x = 0; x < F->nv; x++) level[x] = d[x] == INT32_MAX ? LONG_MAX : d[x];
+        HIPCHK(hipMemcpy(d.data(), F->d_st + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
+        const int32_t lv_inf = (1 << 29) - 1;
+        for (int64_t x = 0; x < F->nv; x++) level[x] = (d[x] >> 2) == lv_inf ? LONG_MAX : (d[x] >> 2);
     }
     if (sigma) HIPCHK(hipMemcpy(sigma, F->d_sigma + off, sizeof(int64_t) * F->nv, hipMemcpyDeviceToHost));
     if (delta) HIPCHK(hipMemcpy(delta, F->d_delta + off, sizeof(double) * F->nv, hipMemcpyDeviceToHost));

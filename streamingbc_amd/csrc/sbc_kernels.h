@@ -45,11 +45,10 @@ struct SbcParams {
     int32_t K;
     int32_t lcap;
     int32_t heavy;       // queue slots [0, heavy) run by whole-workgroup teams
-    int32_t* dist;
+    int32_t* st;          // packed level << 2 | type
     int64_t* sigma;
     double* delta;
     double* bc;
-    int32_t* mark;
     int64_t* dpath;
     double* ndel;
     int32_t* list;

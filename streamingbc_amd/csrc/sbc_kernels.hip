@@ -1,18 +1,12 @@
 // sbc_kernels.hip -- gfx950 kernels of the streaming-BC engine.
 //
 // Per root r the forest holds, in HBM, structure-of-arrays slices of NV
-// entries: dist (int32, INT32_MAX = unreachable), sigma (int64, LONG_MAX =
-// unreachable), delta (FP64), and per-root scratch used only while an update
-// of that root runs: mark (int32), dpath (int64, the added-path count the
+// entries: st (int32 packed level/type word, see below), sigma (int64,
+// LONG_MAX = unreachable), delta (FP64), and per-root scratch used only while
+// an update of that root runs: dpath (int64, the added-path count the
 // reference calls diffPath), ndel (FP64, the new delta) and list (int32, the
 // processed vertices).  Scratch is cleared on the touched entries only, at
 // commit, as src/streamingbc_internal.c:213-242 does.
-//
-// mark encoding: 0 = untouched, otherwise ((newLevel + 1) << 2) | type with
-//   type 1 = descent vertex whose level is unchanged ("stay"),
-//   type 2 = descent vertex that moved up,
-//   type 3 = ascent-only vertex (a new parent of a processed vertex, or an
-//            unmoved vertex now level with a moved neighbour).
 //
 // One update item = (root, inserted edge).  A team (one wavefront, or the
 // whole workgroup) runs it:
@@ -27,7 +21,10 @@
 //             (src/streamingbc_internal_fine.c:560-999), with the terms
 //             summed in that order by walking the wavefront's lanes, so the
 //             new delta is bit-identical to the sequential FINE order
-//   commit    dist/sigma/delta written back, BC updated, scratch cleared.
+//   commit    level/sigma/delta written back, BC updated, scratch cleared.
+// Vertices with many neighbours are scanned by every wavefront of a
+// multi-wave team; their ascent terms are staged per wave in LDS and summed
+// in wave order, which is again adjacency order.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -106,8 +103,16 @@ __device__ __forceinline__ void team_sync()
     }
 }
 
-__device__ __forceinline__ int32_t mark_enc(int32_t level, int32_t type) { return ((level + 1) << 2) | type; }
-__device__ __forceinline__ int32_t mark_level(int32_t m) { return (m >> 2) - 1; }
+// Packed per-(root, vertex) state word: level << 2 | type, with
+//   type 0 = untouched (level = current distance, LV_INF = unreachable),
+//   type 1 = descent vertex whose level is unchanged,
+//   type 2 = descent vertex that moved up (level = its new level),
+//   type 3 = ascent-only vertex (a new parent of a processed vertex, or an
+//            unmoved vertex now level with a moved neighbour).
+// Between updates every word has type 0, so the word is the distance.
+#define LV_INF ((1 << 29) - 1)
+__device__ __forceinline__ int32_t st_enc(int32_t level, int32_t type) { return (level << 2) | type; }
+__device__ __forceinline__ int32_t st_level(int32_t w) { return w >> 2; }
 
 // Number of leading slots of a vertex visible to an item with sequence q:
 // slots below deg0 are always visible; appended slots carry increasing
@@ -141,30 +146,284 @@ struct TeamShared {
 
 // ------------------------------------------------------------ update item
 
+// Vertices with more visible neighbours than this are scanned by every
+// wavefront of a multi-wave team (contiguous chunk ranges per wave).
+constexpr int HUB_MIN = 512;
+constexpr int HUB_CAP = 256;   // hubs queued per level pass (more: scanned by one wave)
+constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
+
+struct alignas(16) HubShared {
+    int n;
+    int overflow;
+    int list[HUB_CAP];
+    int runlen[16];
+    double radd[16][RUN_CAP];
+    double rsub[16][RUN_CAP];
+    uint8_t rflag[16][RUN_CAP];
+};
+
+// Per-item constants.
+struct Item {
+    int32_t* st;      // packed state words of this root
+    int64_t* sigma;
+    double* delta;
+    int64_t* dpath;
+    double* ndel;
+    int32_t* list;
+    int nv;
+    int32_t q;        // visibility sequence
+    int32_t s, p, r;  // deeper endpoint, shallower endpoint, root
+};
+
+// Descent over chunks [cb, ce) of vertex c (new level lc, added paths dc).
+// Marks children (state CAS), accumulates their added paths, appends them to
+// the next frontier and records unmoved same-level neighbours of a moved c.
+__device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X, const SbcVInfo& vi, int len,
+                                               int cb, int ce, int32_t lc, int64_t dc, bool cmoved,
+                                               TeamShared& ts)
+{
+    const int lane = lane_id();
+    const int32_t wantStay = st_enc(lc + 1, 1), wantMove = st_enc(lc + 1, 2);
+    for (int ch = cb; ch < ce; ch += 2) {
+        int32_t kk[2], ww[2];
+        bool ok[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = (ch + u) * 64 + lane;
+            ok[u] = (ch + u) < ce && e < len;
+            kk[u] = ok[u] ? P.col[vi.start + e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ww[u] = ok[u] ? ld_acq32(&X.st[kk[u]]) : 0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            bool appD = false, appS = false;
+            const int32_t k = kk[u], w = ww[u];
+            if (ok[u]) {
+                if ((w & 3) == 0) {
+                    const int64_t cd = (int64_t)st_level(w) - (int64_t)(lc + 1);
+                    if (cd >= 0) {
+                        const int32_t want = cd > 0 ? wantMove : wantStay;
+                        const int32_t old = atomicCAS(&X.st[k], w, want);
+                        appD = (old == w);
+                        if (old == w || old == want)
+                            atomicAdd((unsigned long long*)&X.dpath[k], (unsigned long long)dc);
+                    } else if (cd == -1 && cmoved) {
+                        appS = (atomicCAS(&X.st[k], w, st_enc(lc, 3)) == w);
+                    }
+                } else if (w == wantStay || w == wantMove) {
+                    atomicAdd((unsigned long long*)&X.dpath[k], (unsigned long long)dc);
+                }
+            }
+            const int pd = wave_append(appD, &ts.cnt[0]);
+            if (pd >= 0) X.list[pd] = k;
+            const int psl = wave_append(appS, &ts.cnt[1]);
+            if (psl >= 0) X.list[X.nv - 1 - psl] = k;
+        }
+    }
+}
+
+// Ascent terms of one 64-neighbour chunk of vertex k at level lev: marks
+// unmarked parents (level lev-1) as ascent vertices and returns this lane's
+// term: child (add = new dependency of a processed child) and/or hsub (sub =
+// old dependency to remove).
+struct Term {
+    bool child, hsub;
+    double add, sub;
+};
+
+__device__ __forceinline__ Term ascent_lane(const SbcParams& P, const Item& X, int32_t k, bool isD,
+                                            int64_t sk_old, int64_t sk_new, int lev, bool ok, int32_t c,
+                                            int32_t wc, bool& appU)
+{
+    Term t{false, false, 0.0, 0.0};
+    appU = false;
+    if (!ok) return t;
+    const int ctype = wc & 3;
+    const int32_t lvc = st_level(wc);
+    if (ctype != 0) {
+        if (lvc == lev + 1) {
+            t.child = true;
+            const int64_t sc_old = X.sigma[c];
+            const int64_t sc_new = ctype == 3 ? sc_old
+                : (int64_t)((uint64_t)ld_acq64(&X.dpath[c]) + (uint64_t)(ctype == 1 ? sc_old : 0));
+            t.add = dep(sk_new, sc_new, ld_accd(&X.ndel[c]));
+            if (!isD && !(k == X.p && c == X.s)) {
+                t.hsub = true;
+                t.sub = dep(sk_old, sc_old, X.delta[c]);
+            }
+        } else if (!isD && lvc == lev && ctype == 2) {
+            t.hsub = true;
+            t.sub = dep(sk_old, X.sigma[c], X.delta[c]);
+        }
+    } else if (lev >= 1 && lvc == lev - 1) {
+        appU = (atomicCAS(&X.st[c], wc, st_enc(lev - 1, 3)) == wc);
+    }
+    return t;
+}
+
+struct KState {
+    bool isD;
+    int64_t sk_old, sk_new;
+    double base;
+};
+
+__device__ __forceinline__ KState kstate(const Item& X, int32_t k)
+{
+    KState s;
+    const int32_t wk = ld_acq32(&X.st[k]);
+    const int kt = wk & 3;
+    s.isD = kt != 3;
+    s.sk_old = X.sigma[k];
+    s.sk_new = s.isD ? (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(kt == 1 ? s.sk_old : 0)) : s.sk_old;
+    s.base = s.isD ? 0.0 : X.delta[k];
+    return s;
+}
+
+// One wavefront computes the new delta of k, summing the terms in adjacency
+// order lane by lane.
+__device__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
+                                   int lev, TeamShared& ts)
+{
+    const int lane = lane_id();
+    const KState ks = kstate(X, k);
+    double acc = ks.base;
+    const int nch = (len + 63) >> 6;
+    for (int ch = 0; ch < nch; ch += 2) {
+        int32_t cc[2], ww[2];
+        bool ok[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = (ch + u) * 64 + lane;
+            ok[u] = (ch + u) < nch && e < len;
+            cc[u] = ok[u] ? P.col[vi.start + e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ww[u] = ok[u] ? ld_acq32(&X.st[cc[u]]) : 0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            bool appU;
+            const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
+            const int pu = wave_append(appU, &ts.cnt[2]);
+            if (pu >= 0) X.list[pu] = cc[u];
+            const uint64_t cm = ballot(t.child), sm = ballot(t.hsub);
+            uint64_t m = cm | sm;
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                if ((cm >> l) & 1) acc = acc + readlane_d(t.add, l);
+                if ((sm >> l) & 1) acc = acc - readlane_d(t.sub, l);
+            }
+        }
+    }
+    if (lane == 0) st_reld(&X.ndel[k], acc);
+}
+
+// The whole team computes the new delta of hub k: wave w scans a contiguous
+// range of chunks and keeps its terms in order in LDS; the runs are then
+// summed in wave order, which is adjacency order.  A run that outgrows
+// RUN_CAP sends the vertex back to the single-wave path.
 template <int TW>
-__device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, int twave)
+__device__ void ascent_vertex_team(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
+                                   int lev, TeamShared& ts, HubShared& hs, int twave)
+{
+    const int lane = lane_id();
+    const KState ks = kstate(X, k);
+    const int nch = (len + 63) >> 6;
+    const int cb = (int)(((long)twave * nch) / TW), ce = (int)(((long)(twave + 1) * nch) / TW);
+    int rl = 0;
+    for (int ch = cb; ch < ce; ch += 2) {
+        int32_t cc[2], ww[2];
+        bool ok[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = (ch + u) * 64 + lane;
+            ok[u] = (ch + u) < ce && e < len;
+            cc[u] = ok[u] ? P.col[vi.start + e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ww[u] = ok[u] ? ld_acq32(&X.st[cc[u]]) : 0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            bool appU;
+            const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
+            const int pu = wave_append(appU, &ts.cnt[2]);
+            if (pu >= 0) X.list[pu] = cc[u];
+            const uint64_t m = ballot(t.child || t.hsub);
+            const int n = __popcll(m);
+            if (rl + n <= RUN_CAP && (t.child || t.hsub)) {
+                const int idx = rl + prefix_in_wave(m);
+                hs.radd[twave][idx] = t.add;
+                hs.rsub[twave][idx] = t.sub;
+                hs.rflag[twave][idx] = (uint8_t)((t.child ? 1 : 0) | (t.hsub ? 2 : 0));
+            }
+            rl += n;
+        }
+    }
+    if (lane == 0) {
+        hs.runlen[twave] = rl;
+        if (rl > RUN_CAP) atomicOr(&hs.overflow, 1);
+    }
+    __syncthreads();
+    if (twave == 0) {
+        if (hs.overflow) {
+            ascent_vertex_wave(P, X, k, vi, len, lev, ts);
+        } else if (lane == 0) {
+            double acc = ks.base;
+            for (int w = 0; w < TW; ++w)
+                for (int i = 0; i < hs.runlen[w]; ++i) {
+                    const uint8_t fl = hs.rflag[w][i];
+                    if (fl & 1) acc = acc + hs.radd[w][i];
+                    if (fl & 2) acc = acc - hs.rsub[w][i];
+                }
+            st_reld(&X.ndel[k], acc);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) hs.overflow = 0;
+    __syncthreads();
+}
+
+// Routes vertex k of the current pass: multi-wave teams queue hubs in LDS
+// (processed afterwards by the whole team); everything else runs on the
+// calling wavefront.  Returns true if k was queued.
+template <int TW>
+__device__ __forceinline__ bool queue_hub(HubShared* hs, int32_t k, int len)
+{
+    if (TW == 1 || len <= HUB_MIN) return false;
+    int pos = 0;
+    if (lane_id() == 0) pos = atomicAdd(&hs->n, 1);
+    pos = __shfl(pos, 0);
+    if (pos >= HUB_CAP) return false;
+    if (lane_id() == 0) hs->list[pos] = k;
+    return true;
+}
+
+template <int TW>
+__device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs, int twave)
 {
     const int lane = lane_id();
     const bool leader = (twave == 0 && lane == 0);
     const int nv = P.nv;
     const size_t off = (size_t)ri * (size_t)nv;
-    int32_t* dist = P.dist + off;
-    int64_t* sigma = P.sigma + off;
-    double* delta = P.delta + off;
-    int32_t* mark = P.mark + off;
-    int64_t* dpath = P.dpath + off;
-    double* ndel = P.ndel + off;
-    int32_t* list = P.list + off;
+    Item X;
+    X.st = P.st + off;
+    X.sigma = P.sigma + off;
+    X.delta = P.delta + off;
+    X.dpath = P.dpath + off;
+    X.ndel = P.ndel + off;
+    X.list = P.list + off;
+    X.nv = nv;
+    X.r = P.roots[ri];
     int32_t* lvlD = P.lvl + (size_t)ri * 2 * P.lcap;
     int32_t* lvlS = lvlD + P.lcap;
-    const int32_t r = P.roots[ri];
     const int32_t u = P.win ? P.wu[item] : P.u0;
     const int32_t v = P.win ? P.wv[item] : P.v0;
-    const int32_t q = (P.win == nullptr) ? INF32 : P.win->seq0 + item;
+    X.q = (P.win == nullptr) ? INF32 : P.win->seq0 + item;
 
-    const int32_t du = dist[u], dv = dist[v];
+    const int32_t du = st_level(X.st[u]), dv = st_level(X.st[v]);
     int64_t gap;
-    if (du == INF32 && dv == INF32) gap = 0;
+    if (du == LV_INF && dv == LV_INF) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
     SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
     if (gap == 0) {
@@ -174,22 +433,23 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
         }
         return;
     }
-    const int32_t s = gap > 0 ? u : v;
-    const int32_t p = gap > 0 ? v : u;
+    X.s = gap > 0 ? u : v;
+    X.p = gap > 0 ? v : u;
     const int64_t agap = gap > 0 ? gap : -gap;
     const bool smoved = agap >= 2;
-    const int32_t base = dist[p] + 1; // new level of s
+    const int32_t base = st_level(X.st[X.p]) + 1; // new level of s
 
     if (leader) {
-        st_rel32(&mark[s], mark_enc(base, smoved ? 2 : 1));
-        st_rel64(&dpath[s], sigma[p]);
-        list[0] = s;
+        st_rel32(&X.st[X.s], st_enc(base, smoved ? 2 : 1));
+        st_rel64(&X.dpath[X.s], X.sigma[X.p]);
+        X.list[0] = X.s;
         lvlD[0] = 0;
         lvlS[0] = 0;
         ts.cnt[0] = 1;
         ts.cnt[1] = 0;
         ts.cnt[2] = 0;
         ts.stat[0] = 0; ts.stat[1] = 0; ts.stat[2] = 0;
+        if (hs) { hs->n = 0; hs->overflow = 0; }
     }
     team_sync<TW>();
 
@@ -200,46 +460,38 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
     bool overflow = false;
     while (fs < fe) {
         const int32_t lc = base + j; // new level of frontier j
-        const int32_t wantStay = mark_enc(lc + 1, 1), wantMove = mark_enc(lc + 1, 2);
         for (int idx = fs + twave; idx < fe; idx += TW) {
-            const int32_t c = list[idx];
-            const int32_t mc = ld_acq32(&mark[c]);
-            const bool cmoved = (mc & 3) == 2;
-            const int64_t dc = ld_acq64(&dpath[c]);
+            const int32_t c = X.list[idx];
             const SbcVInfo vi = P.vinfo[c];
-            const int len = visible_len(P, vi, q);
-            for (int e0 = 0; e0 < len; e0 += 64) {
-                const int e = e0 + lane;
-                const bool ok = e < len;
-                bool appD = false, appS = false;
-                int32_t k = 0;
-                if (ok) {
-                    k = P.col[vi.start + e];
-                    const int32_t mk = ld_acq32(&mark[k]);
-                    if (mk == 0) {
-                        const int64_t cd = (int64_t)dist[k] - (int64_t)(lc + 1);
-                        if (cd >= 0) {
-                            const int32_t want = cd > 0 ? wantMove : wantStay;
-                            const int32_t old = atomicCAS(&mark[k], 0, want);
-                            appD = (old == 0);
-                            if (old == 0 || old == want)
-                                atomicAdd((unsigned long long*)&dpath[k], (unsigned long long)dc);
-                        } else if (cd == -1 && cmoved) {
-                            const int32_t old = atomicCAS(&mark[k], 0, mark_enc(lc, 3));
-                            appS = (old == 0);
-                        }
-                    } else if (mk == wantStay || mk == wantMove) {
-                        atomicAdd((unsigned long long*)&dpath[k], (unsigned long long)dc);
-                    }
-                }
-                const int pd = wave_append(appD, &ts.cnt[0]);
-                if (pd >= 0) list[pd] = k;
-                const int psl = wave_append(appS, &ts.cnt[1]);
-                if (psl >= 0) list[nv - 1 - psl] = k;
-            }
+            const int len = visible_len(P, vi, X.q);
+            if (queue_hub<TW>(hs, c, len)) continue;
+            const int32_t wc = ld_acq32(&X.st[c]);
+            const int64_t dc = ld_acq64(&X.dpath[c]);
+            descent_chunks(P, X, vi, len, 0, (len + 63) >> 6, lc, dc, (wc & 3) == 2, ts);
             my_proc += 1;
             my_scan += (unsigned long long)len;
-            my_moved += cmoved ? 1 : 0;
+            my_moved += ((wc & 3) == 2) ? 1 : 0;
+        }
+        if constexpr (TW > 1) {
+            __syncthreads();
+            const int nh = min(hs->n, HUB_CAP);
+            for (int h = 0; h < nh; ++h) {
+                const int32_t c = hs->list[h];
+                const SbcVInfo vi = P.vinfo[c];
+                const int len = visible_len(P, vi, X.q);
+                const int32_t wc = ld_acq32(&X.st[c]);
+                const int64_t dc = ld_acq64(&X.dpath[c]);
+                const int nch = (len + 63) >> 6;
+                descent_chunks(P, X, vi, len, (int)(((long)twave * nch) / TW),
+                               (int)(((long)(twave + 1) * nch) / TW), lc, dc, (wc & 3) == 2, ts);
+                if (twave == 0) {
+                    my_proc += 1;
+                    my_scan += (unsigned long long)len;
+                    my_moved += ((wc & 3) == 2) ? 1 : 0;
+                }
+            }
+            __syncthreads();
+            if (leader) hs->n = 0;
         }
         team_sync<TW>();
         const int nfe = ts.cnt[0];
@@ -274,63 +526,31 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
         if (tot == 0) break;
         for (int w = twave; w < tot; w += TW) {
             int32_t k;
-            if (w < nd) k = list[dA + w];
-            else if (w < nd + ns) k = list[nv - 1 - (sA + w - nd)];
-            else k = list[uA + w - nd - ns];
-            const int32_t mk = ld_acq32(&mark[k]);
-            const int ktype = mk & 3;
-            const bool isD = ktype != 3;
-            const int64_t sk_old = sigma[k];
-            const int64_t sk_new = isD ? (int64_t)((uint64_t)ld_acq64(&dpath[k]) + (uint64_t)(ktype == 1 ? sk_old : 0))
-                                       : sk_old;
-            double acc = isD ? 0.0 : delta[k];
+            if (w < nd) k = X.list[dA + w];
+            else if (w < nd + ns) k = X.list[nv - 1 - (sA + w - nd)];
+            else k = X.list[uA + w - nd - ns];
             const SbcVInfo vi = P.vinfo[k];
-            const int len = visible_len(P, vi, q);
-            for (int e0 = 0; e0 < len; e0 += 64) {
-                const int e = e0 + lane;
-                const bool ok = e < len;
-                bool child = false, hsub = false, appU = false;
-                double add = 0.0, sub = 0.0;
-                int32_t c = 0;
-                if (ok) {
-                    c = P.col[vi.start + e];
-                    const int32_t mc = ld_acq32(&mark[c]);
-                    if (mc != 0) {
-                        const int32_t lvc = mark_level(mc);
-                        const int ctype = mc & 3;
-                        if (lvc == lev + 1) {
-                            child = true;
-                            const int64_t sc_old = sigma[c];
-                            const int64_t sc_new = ctype == 3 ? sc_old
-                                : (int64_t)((uint64_t)ld_acq64(&dpath[c]) + (uint64_t)(ctype == 1 ? sc_old : 0));
-                            add = dep(sk_new, sc_new, ld_accd(&ndel[c]));
-                            if (!isD && !(k == p && c == s)) {
-                                hsub = true;
-                                sub = dep(sk_old, sc_old, delta[c]);
-                            }
-                        } else if (!isD && lvc == lev && ctype == 2) {
-                            hsub = true;
-                            sub = dep(sk_old, sigma[c], delta[c]);
-                        }
-                    } else if (lev >= 1 && dist[c] == lev - 1) {
-                        appU = (atomicCAS(&mark[c], 0, mark_enc(lev - 1, 3)) == 0);
-                    }
-                }
-                const int pu = wave_append(appU, &ts.cnt[2]);
-                if (pu >= 0) list[pu] = c;
-                // sum the terms in adjacency order
-                const uint64_t cm = ballot(child), sm = ballot(hsub);
-                uint64_t m = cm | sm;
-                while (m) {
-                    const int l = __builtin_ctzll(m);
-                    m &= m - 1;
-                    if ((cm >> l) & 1) acc = acc + readlane_d(add, l);
-                    if ((sm >> l) & 1) acc = acc - readlane_d(sub, l);
-                }
-            }
-            if (lane == 0) st_reld(&ndel[k], acc);
+            const int len = visible_len(P, vi, X.q);
+            if (queue_hub<TW>(hs, k, len)) continue;
+            ascent_vertex_wave(P, X, k, vi, len, lev, ts);
             my_proc += 1;
             my_scan += (unsigned long long)len;
+        }
+        if constexpr (TW > 1) {
+            __syncthreads();
+            const int nh = min(hs->n, HUB_CAP);
+            for (int h = 0; h < nh; ++h) {
+                const int32_t k = hs->list[h];
+                const SbcVInfo vi = P.vinfo[k];
+                const int len = visible_len(P, vi, X.q);
+                ascent_vertex_team<TW>(P, X, k, vi, len, lev, ts, *hs, twave);
+                if (twave == 0) {
+                    my_proc += 1;
+                    my_scan += (unsigned long long)len;
+                }
+            }
+            __syncthreads();
+            if (leader) hs->n = 0;
         }
         team_sync<TW>();
         uA = uB;
@@ -343,21 +563,20 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
     const int nS = ts.cnt[1];
     const int total = nUend + nS;
     for (int w = twave * 64 + lane; w < total; w += TW * 64) {
-        const int32_t k = w < nUend ? list[w] : list[nv - 1 - (w - nUend)];
-        const int32_t mk = ld_acq32(&mark[k]);
-        const int ktype = mk & 3;
-        const double nd = ld_accd(&ndel[k]);
-        const double od = delta[k];
+        const int32_t k = w < nUend ? X.list[w] : X.list[nv - 1 - (w - nUend)];
+        const int32_t wk = ld_acq32(&X.st[k]);
+        const int ktype = wk & 3;
+        const double nd = ld_accd(&X.ndel[k]);
+        const double od = X.delta[k];
         if (ktype != 3) {
-            const int64_t so = sigma[k];
-            dist[k] = mark_level(mk);
-            sigma[k] = (int64_t)((uint64_t)ld_acq64(&dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
+            const int64_t so = X.sigma[k];
+            X.sigma[k] = (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
         }
-        delta[k] = nd;
-        if (P.bc_atomic && k != r) atomicAdd(&P.bc[k], nd - od);
-        st_rel32(&mark[k], 0);
-        st_rel64(&dpath[k], 0);
-        st_reld(&ndel[k], 0.0);
+        X.delta[k] = nd;
+        if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
+        st_rel32(&X.st[k], wk & ~3); // the level field already holds the new level
+        st_rel64(&X.dpath[k], 0);
+        st_reld(&X.ndel[k], 0.0);
     }
     if (lane == 0) {
         atomicAdd(&ts.stat[0], my_proc);
@@ -386,6 +605,7 @@ template <int NWB>
 __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
 {
     __shared__ TeamShared ts[NWB];
+    __shared__ HubShared hs;
     const int wave = threadIdx.x >> 6;
     const int W = P.win ? P.win->W : 1;
     // phase 1: heavy
@@ -396,7 +616,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         __syncthreads();
         if (t >= P.heavy) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = 0; i < W; ++i) process_item<NWB>(P, ri, i, ts[0], wave);
+        for (int i = 0; i < W; ++i) process_item<NWB>(P, ri, i, ts[0], &hs, wave);
     }
     // phase 2: light, one wavefront per root
     for (;;) {
@@ -406,13 +626,13 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         team_sync<1>();
         if (t >= P.K) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = 0; i < W; ++i) process_item<1>(P, ri, i, ts[wave], 0);
+        for (int i = 0; i < W; ++i) process_item<1>(P, ri, i, ts[wave], nullptr, 0);
     }
 }
 
 // ------------------------------------------------------------ static pass
 
-// One workgroup per root: BFS computing dist and sigma level by level, then
+// One workgroup per root: BFS computing levels and sigma level by level, then
 // each vertex pulls delta from its children in adjacency order.
 // (src/streamingbc_init.c:87-183 with the ascent in pull form)
 template <int NWB>
@@ -430,7 +650,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         __syncthreads();
         if (ri >= P.K) break;
         const size_t off = (size_t)ri * (size_t)nv;
-        int32_t* dist = P.dist + off;
+        int32_t* st = P.st + off;
         int64_t* sigma = P.sigma + off;
         double* delta = P.delta + off;
         int64_t* acc_s = P.dpath + off;
@@ -438,13 +658,13 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         int32_t* lvl = P.lvl + (size_t)ri * 2 * P.lcap;
         const int32_t r = P.roots[ri];
         for (int x = threadIdx.x; x < nv; x += NWB * 64) {
-            st_rel32(&dist[x], INF32);
+            st_rel32(&st[x], st_enc(LV_INF, 0));
             sigma[x] = SIGMA_INF;
             delta[x] = 0.0;
         }
         __syncthreads();
         if (leader) {
-            st_rel32(&dist[r], 0);
+            st_rel32(&st[r], st_enc(0, 0));
             sigma[r] = 1;
             list[0] = r;
             lvl[0] = 0;
@@ -454,25 +674,27 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         __syncthreads();
         int fs = 0, fe = 1, j = 0;
         bool overflow = false;
+        const int32_t unseen = st_enc(LV_INF, 0);
         while (fs < fe) {
+            const int32_t next = st_enc(j + 1, 0);
             for (int idx = fs + wave; idx < fe; idx += NWB) {
                 const int32_t c = list[idx];
                 const int64_t sc = sigma[c];
                 const SbcVInfo vi = P.vinfo[c];
-                const int len = vi.deg; // static pass sees the whole graph
+                const int len = vi.deg; // the static pass sees the whole graph
                 for (int e0 = 0; e0 < len; e0 += 64) {
                     const int e = e0 + lane;
                     bool app = false;
                     int32_t k = 0;
                     if (e < len) {
                         k = P.col[vi.start + e];
-                        int32_t dk = ld_acq32(&dist[k]);
-                        if (dk == INF32) {
-                            const int32_t old = atomicCAS(&dist[k], INF32, j + 1);
-                            app = (old == INF32);
-                            dk = app ? j + 1 : old;
+                        int32_t wk = ld_acq32(&st[k]);
+                        if (wk == unseen) {
+                            const int32_t old = atomicCAS(&st[k], unseen, next);
+                            app = (old == unseen);
+                            wk = app ? next : old;
                         }
-                        if (dk == j + 1) atomicAdd((unsigned long long*)&acc_s[k], (unsigned long long)sc);
+                        if (wk == next) atomicAdd((unsigned long long*)&acc_s[k], (unsigned long long)sc);
                     }
                     const int pos = wave_append(app, &ts.cnt[0]);
                     if (pos >= 0) list[pos] = k;
@@ -501,6 +723,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         __syncthreads();
         for (int L = j - 1; L >= 0; --L) {
             const int a = lvl[L], b = lvl[L + 1];
+            const int32_t childw = st_enc(L + 1, 0);
             for (int idx = a + wave; idx < b; idx += NWB) {
                 const int32_t k = list[idx];
                 const int64_t sk = sigma[k];
@@ -512,7 +735,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
                     double add = 0.0;
                     if (e < vi.deg) {
                         const int32_t c = P.col[vi.start + e];
-                        if (ld_acq32(&dist[c]) == L + 1) {
+                        if (ld_acq32(&st[c]) == childw) {
                             child = true;
                             add = dep(sk, sigma[c], delta[c]);
                         }
