@@ -296,7 +296,9 @@ struct sbcForestDev {
     bool initialized = false;
     std::vector<int64_t> roots;
     int32_t* d_roots = nullptr;
-    int32_t* d_st = nullptr;     // packed level << 2 | type
+    int32_t* d_dist = nullptr;
+    uint8_t* d_sb = nullptr;     // state bytes, sb_stride per root
+    int64_t sb_stride = 0;
     int64_t* d_sigma = nullptr;
     double* d_delta = nullptr;
     double* d_bc = nullptr;
@@ -311,14 +313,14 @@ static int32_t lcap_for(int64_t nv) { return (int32_t)(nv + 2 < 4096 ? nv + 2 : 
 
 extern "C" int64_t streamingBCForestFootprint(int64_t NV, int64_t NK)
 {
-    int64_t per = 4 + 8 + 8 + 8 + 8 + 4; // st, sigma, delta, dpath, ndel, list
+    int64_t per = 4 + 1 + 8 + 8 + 8 + 8 + 4; // dist, state byte, sigma, delta, dpath, ndel, list
     return NK * NV * per + NK * 2 * (int64_t)lcap_for(NV) * 4 + NV * 8 + NK * 4 + 64;
 }
 
 static void forest_free_dev(sbcForestDev* F)
 {
     if (!F) return;
-    void* ptrs[] = {F->d_roots, F->d_st, F->d_sigma, F->d_delta, F->d_bc,
+    void* ptrs[] = {F->d_roots, F->d_dist, F->d_sb, F->d_sigma, F->d_delta, F->d_bc,
                     F->d_dpath, F->d_ndel, F->d_list, F->d_lvl, F->d_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -357,7 +359,9 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
     }
     size_t kn = (size_t)NK * (size_t)NV;
     sbcStatus st = SBC_OK;
-    if ((st = dalloc(&F->d_roots, NK)) || (st = dalloc(&F->d_st, kn)) || (st = dalloc(&F->d_sigma, kn)) ||
+    F->sb_stride = (NV + 63) / 64 * 64;
+    if ((st = dalloc(&F->d_roots, NK)) || (st = dalloc(&F->d_dist, kn)) ||
+        (st = dalloc(&F->d_sb, (size_t)NK * (size_t)F->sb_stride)) || (st = dalloc(&F->d_sigma, kn)) ||
         (st = dalloc(&F->d_delta, kn)) || (st = dalloc(&F->d_bc, NV)) ||
         (st = dalloc(&F->d_dpath, kn)) || (st = dalloc(&F->d_ndel, kn)) || (st = dalloc(&F->d_list, kn)) ||
         (st = dalloc(&F->d_lvl, (size_t)NK * 2 * F->lcap)) || (st = dalloc(&F->d_err, 4))) {
@@ -366,7 +370,7 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
     }
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMemcpy(F->d_roots, r32.data(), sizeof(int32_t) * NK, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(F->d_st, 0, sizeof(int32_t) * kn);
+    if (e == hipSuccess) e = hipMemset(F->d_sb, 0, (size_t)NK * (size_t)F->sb_stride);
     if (e == hipSuccess) e = hipMemset(F->d_dpath, 0, sizeof(int64_t) * kn);
     if (e == hipSuccess) e = hipMemset(F->d_ndel, 0, sizeof(double) * kn);
     if (e == hipSuccess) e = hipMemset(F->d_bc, 0, sizeof(double) * NV);
@@ -607,7 +611,9 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.nv = (int32_t)F->nv;
     P.K = (int32_t)F->K;
     P.lcap = F->lcap;
-    P.st = F->d_st;
+    P.dist = F->d_dist;
+    P.sb = F->d_sb;
+    P.sb_stride = F->sb_stride;
     P.sigma = F->d_sigma;
     P.delta = F->d_delta;
     P.bc = F->d_bc;
@@ -767,9 +773,8 @@ extern "C" sbcStatus streamingBCGetTree(bcForest* forest, int64_t ri, int64_t* l
     size_t off = (size_t)ri * (size_t)F->nv;
     if (level) {
         std::vector<int32_t> d(F->nv);
-        HIPCHK(hipMemcpy(d.data(), F->d_st + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
-        const int32_t lv_inf = (1 << 29) - 1;
-        for (int64_t x = 0; x < F->nv; x++) level[x] = (d[x] >> 2) == lv_inf ? LONG_MAX : (d[x] >> 2);
+        HIPCHK(hipMemcpy(d.data(), F->d_dist + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
+        for (int64_t x = 0; x < F->nv; x++) level[x] = d[x] == INT32_MAX ? LONG_MAX : d[x];
     }
     if (sigma) HIPCHK(hipMemcpy(sigma, F->d_sigma + off, sizeof(int64_t) * F->nv, hipMemcpyDeviceToHost));
     if (delta) HIPCHK(hipMemcpy(delta, F->d_delta + off, sizeof(double) * F->nv, hipMemcpyDeviceToHost));

@@ -45,7 +45,9 @@ struct SbcParams {
     int32_t K;
     int32_t lcap;
     int32_t heavy;       // queue slots [0, heavy) run by whole-workgroup teams
-    int32_t* st;          // packed level << 2 | type
+    int32_t* dist;        // levels, INT32_MAX = unreachable
+    uint8_t* sb;          // state bytes, sb_stride bytes per root
+    int64_t sb_stride;
     int64_t* sigma;
     double* delta;
     double* bc;

@@ -37,30 +37,25 @@
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-// Loads of scratch written by atomics from other wavefronts go to L2.
-__device__ __forceinline__ int32_t ld_acq32(const int32_t* p)
+// Per-root arrays are only ever touched by the one workgroup that runs the
+// root (kernel boundaries order them across launches), so their atomics and
+// coherent loads use workgroup scope and stay in the XCD's L2.  Only data
+// shared between workgroups (BC, queue heads, error flag) uses device scope.
+#define WG_SCOPE __HIP_MEMORY_SCOPE_WORKGROUP
+__device__ __forceinline__ int32_t ld_acq32(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, WG_SCOPE); }
+__device__ __forceinline__ int64_t ld_acq64(const int64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, WG_SCOPE); }
+__device__ __forceinline__ double ld_accd(const double* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, WG_SCOPE); }
+__device__ __forceinline__ void st_rel32(int32_t* p, int32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, WG_SCOPE); }
+__device__ __forceinline__ void st_rel64(int64_t* p, int64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, WG_SCOPE); }
+__device__ __forceinline__ void st_reld(double* p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, WG_SCOPE); }
+__device__ __forceinline__ void add64_wg(int64_t* p, int64_t v)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED, WG_SCOPE);
 }
-__device__ __forceinline__ int64_t ld_acq64(const int64_t* p)
+__device__ __forceinline__ int32_t cas32_wg(int32_t* p, int32_t expect, int32_t want)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double ld_accd(const double* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel32(int32_t* p, int32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel64(int64_t* p, int64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_reld(double* p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED, WG_SCOPE);
+    return expect; // the value seen
 }
 
 __device__ __forceinline__ double readlane_d(double x, int l)
@@ -103,16 +98,59 @@ __device__ __forceinline__ void team_sync()
     }
 }
 
-// Packed per-(root, vertex) state word: level << 2 | type, with
-//   type 0 = untouched (level = current distance, LV_INF = unreachable),
-//   type 1 = descent vertex whose level is unchanged,
-//   type 2 = descent vertex that moved up (level = its new level),
-//   type 3 = ascent-only vertex (a new parent of a processed vertex, or an
-//            unmoved vertex now level with a moved neighbour).
-// Between updates every word has type 0, so the word is the distance.
-#define LV_INF ((1 << 29) - 1)
-__device__ __forceinline__ int32_t st_enc(int32_t level, int32_t type) { return (level << 2) | type; }
-__device__ __forceinline__ int32_t st_level(int32_t w) { return w >> 2; }
+// Per-(root, vertex) state byte, the only per-root array gathered while
+// scanning adjacency lists:
+//   bit 7    unreachable (level infinite)
+//   bits 6:2 level mod 32 (the new level once the vertex is marked)
+//   bits 1:0 type: 0 untouched, 1 descent vertex whose level is unchanged,
+//            2 descent vertex that moved up, 3 ascent-only vertex (a new
+//            parent of a processed vertex, or an unmoved vertex now level
+//            with a moved neighbour).
+// The full level lives in the int32 dist array, read only for the vertex
+// being processed.  Neighbours in a BFS tree differ by at most one level, so
+// level mod 32 determines a neighbour's level relative to a known one; the
+// one exception, the edge being inserted, is handled explicitly.
+#define SB_INF 0x80
+__device__ __forceinline__ uint8_t sb_make(int32_t level, int type)
+{
+    return (uint8_t)(((level & 31) << 2) | type);
+}
+__device__ __forceinline__ int sb_type(uint8_t b) { return b & 3; }
+// level of a finite byte relative to `ref`, valid when it lies in [ref-16, ref+15]
+__device__ __forceinline__ int sb_rel(uint8_t b, int32_t ref)
+{
+    const int d = (((b >> 2) & 31) - (ref & 31)) & 31;
+    return d >= 16 ? d - 32 : d;
+}
+
+__device__ __forceinline__ uint8_t ld_sb(const uint8_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, WG_SCOPE);
+}
+__device__ __forceinline__ void st_sb(uint8_t* p, uint8_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, WG_SCOPE);
+}
+
+// Compare-and-swap of one state byte through its aligned 32-bit word.
+// Returns the byte seen; the swap happened iff that equals `expect`.  Bytes
+// only move from type 0 to a marked type once per update, so the retry loop
+// ends after at most four rounds.
+__device__ __forceinline__ uint8_t cas_sb(uint8_t* p, uint8_t expect, uint8_t want)
+{
+    unsigned int* w = (unsigned int*)((uintptr_t)p & ~(uintptr_t)3);
+    const int sh = (int)((uintptr_t)p & 3) * 8;
+    unsigned int old = __hip_atomic_load(w, __ATOMIC_RELAXED, WG_SCOPE);
+    for (;;) {
+        const uint8_t cur = (uint8_t)(old >> sh);
+        if (cur != expect) return cur;
+        const unsigned int nw = (old & ~(0xffu << sh)) | ((unsigned int)want << sh);
+        unsigned int prev = old;
+        __hip_atomic_compare_exchange_strong(w, &prev, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED, WG_SCOPE);
+        if (prev == old) return expect;
+        old = prev;
+    }
+}
 
 // Number of leading slots of a vertex visible to an item with sequence q:
 // slots below deg0 are always visible; appended slots carry increasing
@@ -149,6 +187,11 @@ struct TeamShared {
 // Vertices with more visible neighbours than this are scanned by every
 // wavefront of a multi-wave team (contiguous chunk ranges per wave).
 constexpr int HUB_MIN = 512;
+// 64-neighbour chunks whose loads are issued together by one wavefront.
+#ifndef SBC_SCAN_UNROLL
+#define SBC_SCAN_UNROLL 4
+#endif
+constexpr int SCAN_UNROLL = SBC_SCAN_UNROLL;
 constexpr int HUB_CAP = 256;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
@@ -164,7 +207,8 @@ struct alignas(16) HubShared {
 
 // Per-item constants.
 struct Item {
-    int32_t* st;      // packed state words of this root
+    int32_t* dist;    // levels (unchanged until commit)
+    uint8_t* sb;      // state bytes
     int64_t* sigma;
     double* delta;
     int64_t* dpath;
@@ -175,44 +219,53 @@ struct Item {
     int32_t s, p, r;  // deeper endpoint, shallower endpoint, root
 };
 
-// Descent over chunks [cb, ce) of vertex c (new level lc, added paths dc).
-// Marks children (state CAS), accumulates their added paths, appends them to
-// the next frontier and records unmoved same-level neighbours of a moved c.
-__device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X, const SbcVInfo& vi, int len,
-                                               int cb, int ce, int32_t lc, int64_t dc, bool cmoved,
-                                               TeamShared& ts)
+// Descent over chunks [cb, ce) of vertex c (old level oc, new level lc,
+// added paths dc).  Marks children, accumulates their added paths, appends
+// them to the next frontier and records unmoved same-level neighbours of a
+// moved c.
+__device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X, int32_t c, const SbcVInfo& vi,
+                                               int len, int cb, int ce, int32_t oc, int32_t lc, int64_t dc,
+                                               bool cmoved, TeamShared& ts)
 {
     const int lane = lane_id();
-    const int32_t wantStay = st_enc(lc + 1, 1), wantMove = st_enc(lc + 1, 2);
-    for (int ch = cb; ch < ce; ch += 2) {
-        int32_t kk[2], ww[2];
-        bool ok[2];
+    const uint8_t wantStay = sb_make(lc + 1, 1), wantMove = sb_make(lc + 1, 2);
+    const bool c_is_s = (c == X.s);
+    for (int ch = cb; ch < ce; ch += SCAN_UNROLL) {
+        int32_t kk[SCAN_UNROLL];
+        uint8_t bb[SCAN_UNROLL];
+        bool ok[SCAN_UNROLL];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
             const int e = (ch + u) * 64 + lane;
             ok[u] = (ch + u) < ce && e < len;
             kk[u] = ok[u] ? P.col[vi.start + e] : 0;
+            // the inserted edge is the one edge whose ends may be more than
+            // one level apart; it never adds a child of s
+            if (ok[u] && c_is_s && kk[u] == X.p) ok[u] = false;
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ww[u] = ok[u] ? ld_acq32(&X.st[kk[u]]) : 0;
+        for (int u = 0; u < SCAN_UNROLL; ++u) bb[u] = ok[u] ? ld_sb(&X.sb[kk[u]]) : (uint8_t)0;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
             bool appD = false, appS = false;
-            const int32_t k = kk[u], w = ww[u];
+            const int32_t k = kk[u];
+            const uint8_t b = bb[u];
             if (ok[u]) {
-                if ((w & 3) == 0) {
-                    const int64_t cd = (int64_t)st_level(w) - (int64_t)(lc + 1);
+                if (sb_type(b) == 0) {
+                    int64_t cd;
+                    if (b & SB_INF) cd = 1;  // unreachable until now: moves up
+                    else cd = (int64_t)oc + sb_rel(b, oc) - (int64_t)(lc + 1);
                     if (cd >= 0) {
-                        const int32_t want = cd > 0 ? wantMove : wantStay;
-                        const int32_t old = atomicCAS(&X.st[k], w, want);
-                        appD = (old == w);
-                        if (old == w || old == want)
-                            atomicAdd((unsigned long long*)&X.dpath[k], (unsigned long long)dc);
+                        const uint8_t want = cd > 0 ? wantMove : wantStay;
+                        const uint8_t seen = cas_sb(&X.sb[k], b, want);
+                        appD = (seen == b);
+                        if (seen == b || seen == want)
+                            add64_wg(&X.dpath[k], dc);
                     } else if (cd == -1 && cmoved) {
-                        appS = (atomicCAS(&X.st[k], w, st_enc(lc, 3)) == w);
+                        appS = (cas_sb(&X.sb[k], b, sb_make(lc, 3)) == b);
                     }
-                } else if (w == wantStay || w == wantMove) {
-                    atomicAdd((unsigned long long*)&X.dpath[k], (unsigned long long)dc);
+                } else if (b == wantStay || b == wantMove) {
+                    add64_wg(&X.dpath[k], dc);
                 }
             }
             const int pd = wave_append(appD, &ts.cnt[0]);
@@ -234,15 +287,15 @@ struct Term {
 
 __device__ __forceinline__ Term ascent_lane(const SbcParams& P, const Item& X, int32_t k, bool isD,
                                             int64_t sk_old, int64_t sk_new, int lev, bool ok, int32_t c,
-                                            int32_t wc, bool& appU)
+                                            uint8_t bc, bool& appU)
 {
     Term t{false, false, 0.0, 0.0};
     appU = false;
-    if (!ok) return t;
-    const int ctype = wc & 3;
-    const int32_t lvc = st_level(wc);
+    if (!ok || (bc & SB_INF)) return t;
+    const int ctype = sb_type(bc);
+    const int rel = sb_rel(bc, lev);
     if (ctype != 0) {
-        if (lvc == lev + 1) {
+        if (rel == 1) {
             t.child = true;
             const int64_t sc_old = X.sigma[c];
             const int64_t sc_new = ctype == 3 ? sc_old
@@ -252,12 +305,12 @@ __device__ __forceinline__ Term ascent_lane(const SbcParams& P, const Item& X, i
                 t.hsub = true;
                 t.sub = dep(sk_old, sc_old, X.delta[c]);
             }
-        } else if (!isD && lvc == lev && ctype == 2) {
+        } else if (!isD && rel == 0 && ctype == 2) {
             t.hsub = true;
             t.sub = dep(sk_old, X.sigma[c], X.delta[c]);
         }
-    } else if (lev >= 1 && lvc == lev - 1) {
-        appU = (atomicCAS(&X.st[c], wc, st_enc(lev - 1, 3)) == wc);
+    } else if (lev >= 1 && rel == -1) {
+        appU = (cas_sb(&X.sb[c], bc, sb_make(lev - 1, 3)) == bc);
     }
     return t;
 }
@@ -271,8 +324,7 @@ struct KState {
 __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
 {
     KState s;
-    const int32_t wk = ld_acq32(&X.st[k]);
-    const int kt = wk & 3;
+    const int kt = sb_type(ld_sb(&X.sb[k]));
     s.isD = kt != 3;
     s.sk_old = X.sigma[k];
     s.sk_new = s.isD ? (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(kt == 1 ? s.sk_old : 0)) : s.sk_old;
@@ -282,26 +334,27 @@ __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
 
 // One wavefront computes the new delta of k, summing the terms in adjacency
 // order lane by lane.
-__device__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
+__device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
                                    int lev, TeamShared& ts)
 {
     const int lane = lane_id();
     const KState ks = kstate(X, k);
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
-    for (int ch = 0; ch < nch; ch += 2) {
-        int32_t cc[2], ww[2];
-        bool ok[2];
+    for (int ch = 0; ch < nch; ch += SCAN_UNROLL) {
+        int32_t cc[SCAN_UNROLL];
+        uint8_t ww[SCAN_UNROLL];
+        bool ok[SCAN_UNROLL];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
             const int e = (ch + u) * 64 + lane;
             ok[u] = (ch + u) < nch && e < len;
             cc[u] = ok[u] ? P.col[vi.start + e] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ww[u] = ok[u] ? ld_acq32(&X.st[cc[u]]) : 0;
+        for (int u = 0; u < SCAN_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
             bool appU;
             const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
             const int pu = wave_append(appU, &ts.cnt[2]);
@@ -324,7 +377,7 @@ __device__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k,
 // summed in wave order, which is adjacency order.  A run that outgrows
 // RUN_CAP sends the vertex back to the single-wave path.
 template <int TW>
-__device__ void ascent_vertex_team(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
+__device__ __forceinline__ void ascent_vertex_team(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
                                    int lev, TeamShared& ts, HubShared& hs, int twave)
 {
     const int lane = lane_id();
@@ -332,19 +385,20 @@ __device__ void ascent_vertex_team(const SbcParams& P, const Item& X, int32_t k,
     const int nch = (len + 63) >> 6;
     const int cb = (int)(((long)twave * nch) / TW), ce = (int)(((long)(twave + 1) * nch) / TW);
     int rl = 0;
-    for (int ch = cb; ch < ce; ch += 2) {
-        int32_t cc[2], ww[2];
-        bool ok[2];
+    for (int ch = cb; ch < ce; ch += SCAN_UNROLL) {
+        int32_t cc[SCAN_UNROLL];
+        uint8_t ww[SCAN_UNROLL];
+        bool ok[SCAN_UNROLL];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
             const int e = (ch + u) * 64 + lane;
             ok[u] = (ch + u) < ce && e < len;
             cc[u] = ok[u] ? P.col[vi.start + e] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ww[u] = ok[u] ? ld_acq32(&X.st[cc[u]]) : 0;
+        for (int u = 0; u < SCAN_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
             bool appU;
             const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
             const int pu = wave_append(appU, &ts.cnt[2]);
@@ -400,14 +454,15 @@ __device__ __forceinline__ bool queue_hub(HubShared* hs, int32_t k, int len)
 }
 
 template <int TW>
-__device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs, int twave)
+__device__ __forceinline__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs, int twave)
 {
     const int lane = lane_id();
     const bool leader = (twave == 0 && lane == 0);
     const int nv = P.nv;
     const size_t off = (size_t)ri * (size_t)nv;
     Item X;
-    X.st = P.st + off;
+    X.dist = P.dist + off;
+    X.sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
     X.sigma = P.sigma + off;
     X.delta = P.delta + off;
     X.dpath = P.dpath + off;
@@ -421,9 +476,9 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
     const int32_t v = P.win ? P.wv[item] : P.v0;
     X.q = (P.win == nullptr) ? INF32 : P.win->seq0 + item;
 
-    const int32_t du = st_level(X.st[u]), dv = st_level(X.st[v]);
+    const int32_t du = X.dist[u], dv = X.dist[v];
     int64_t gap;
-    if (du == LV_INF && dv == LV_INF) gap = 0;
+    if (du == INF32 && dv == INF32) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
     SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
     if (gap == 0) {
@@ -437,10 +492,10 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
     X.p = gap > 0 ? v : u;
     const int64_t agap = gap > 0 ? gap : -gap;
     const bool smoved = agap >= 2;
-    const int32_t base = st_level(X.st[X.p]) + 1; // new level of s
+    const int32_t base = X.dist[X.p] + 1; // new level of s
 
     if (leader) {
-        st_rel32(&X.st[X.s], st_enc(base, smoved ? 2 : 1));
+        st_sb(&X.sb[X.s], sb_make(base, smoved ? 2 : 1));
         st_rel64(&X.dpath[X.s], X.sigma[X.p]);
         X.list[0] = X.s;
         lvlD[0] = 0;
@@ -465,12 +520,12 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
             const SbcVInfo vi = P.vinfo[c];
             const int len = visible_len(P, vi, X.q);
             if (queue_hub<TW>(hs, c, len)) continue;
-            const int32_t wc = ld_acq32(&X.st[c]);
+            const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
             const int64_t dc = ld_acq64(&X.dpath[c]);
-            descent_chunks(P, X, vi, len, 0, (len + 63) >> 6, lc, dc, (wc & 3) == 2, ts);
+            descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, ts);
             my_proc += 1;
             my_scan += (unsigned long long)len;
-            my_moved += ((wc & 3) == 2) ? 1 : 0;
+            my_moved += cmoved ? 1 : 0;
         }
         if constexpr (TW > 1) {
             __syncthreads();
@@ -479,15 +534,15 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
                 const int32_t c = hs->list[h];
                 const SbcVInfo vi = P.vinfo[c];
                 const int len = visible_len(P, vi, X.q);
-                const int32_t wc = ld_acq32(&X.st[c]);
+                const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
                 const int64_t dc = ld_acq64(&X.dpath[c]);
                 const int nch = (len + 63) >> 6;
-                descent_chunks(P, X, vi, len, (int)(((long)twave * nch) / TW),
-                               (int)(((long)(twave + 1) * nch) / TW), lc, dc, (wc & 3) == 2, ts);
+                descent_chunks(P, X, c, vi, len, (int)(((long)twave * nch) / TW),
+                               (int)(((long)(twave + 1) * nch) / TW), X.dist[c], lc, dc, cmoved, ts);
                 if (twave == 0) {
                     my_proc += 1;
                     my_scan += (unsigned long long)len;
-                    my_moved += ((wc & 3) == 2) ? 1 : 0;
+                    my_moved += cmoved ? 1 : 0;
                 }
             }
             __syncthreads();
@@ -562,20 +617,33 @@ __device__ void process_item(const SbcParams& P, int ri, int item, TeamShared& t
     const int nUend = ts.cnt[2];
     const int nS = ts.cnt[1];
     const int total = nUend + nS;
-    for (int w = twave * 64 + lane; w < total; w += TW * 64) {
-        const int32_t k = w < nUend ? X.list[w] : X.list[nv - 1 - (w - nUend)];
-        const int32_t wk = ld_acq32(&X.st[k]);
-        const int ktype = wk & 3;
-        const double nd = ld_accd(&X.ndel[k]);
-        const double od = X.delta[k];
-        if (ktype != 3) {
+    // descent vertices, frontier by frontier (new level base + jf)
+    for (int jf = 0; jf < nf; ++jf) {
+        const int32_t nl = base + jf;
+        for (int w = lvlD[jf] + twave * 64 + lane; w < lvlD[jf + 1]; w += TW * 64) {
+            const int32_t k = X.list[w];
+            const int ktype = sb_type(ld_sb(&X.sb[k]));
+            const double nd = ld_accd(&X.ndel[k]);
+            const double od = X.delta[k];
             const int64_t so = X.sigma[k];
             X.sigma[k] = (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
+            X.delta[k] = nd;
+            X.dist[k] = nl;
+            st_sb(&X.sb[k], sb_make(nl, 0));
+            if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
+            st_rel64(&X.dpath[k], 0);
+            st_reld(&X.ndel[k], 0.0);
         }
+    }
+    // ascent-only vertices keep their level
+    for (int w = twave * 64 + lane; w < total - nD; w += TW * 64) {
+        const int32_t k = w < nUend - nD ? X.list[nD + w] : X.list[nv - 1 - (w - (nUend - nD))];
+        const uint8_t b = ld_sb(&X.sb[k]);
+        const double nd = ld_accd(&X.ndel[k]);
+        const double od = X.delta[k];
         X.delta[k] = nd;
+        st_sb(&X.sb[k], (uint8_t)(b & ~3));
         if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
-        st_rel32(&X.st[k], wk & ~3); // the level field already holds the new level
-        st_rel64(&X.dpath[k], 0);
         st_reld(&X.ndel[k], 0.0);
     }
     if (lane == 0) {
@@ -650,7 +718,8 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         __syncthreads();
         if (ri >= P.K) break;
         const size_t off = (size_t)ri * (size_t)nv;
-        int32_t* st = P.st + off;
+        int32_t* dist = P.dist + off;
+        uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
         int64_t* sigma = P.sigma + off;
         double* delta = P.delta + off;
         int64_t* acc_s = P.dpath + off;
@@ -658,13 +727,15 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         int32_t* lvl = P.lvl + (size_t)ri * 2 * P.lcap;
         const int32_t r = P.roots[ri];
         for (int x = threadIdx.x; x < nv; x += NWB * 64) {
-            st_rel32(&st[x], st_enc(LV_INF, 0));
+            st_rel32(&dist[x], INF32);
+            sb[x] = SB_INF;
             sigma[x] = SIGMA_INF;
             delta[x] = 0.0;
         }
         __syncthreads();
         if (leader) {
-            st_rel32(&st[r], st_enc(0, 0));
+            st_rel32(&dist[r], 0);
+            sb[r] = sb_make(0, 0);
             sigma[r] = 1;
             list[0] = r;
             lvl[0] = 0;
@@ -674,9 +745,9 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         __syncthreads();
         int fs = 0, fe = 1, j = 0;
         bool overflow = false;
-        const int32_t unseen = st_enc(LV_INF, 0);
+        const int32_t unseen = INF32;
         while (fs < fe) {
-            const int32_t next = st_enc(j + 1, 0);
+            const int32_t next = j + 1;
             for (int idx = fs + wave; idx < fe; idx += NWB) {
                 const int32_t c = list[idx];
                 const int64_t sc = sigma[c];
@@ -688,13 +759,13 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
                     int32_t k = 0;
                     if (e < len) {
                         k = P.col[vi.start + e];
-                        int32_t wk = ld_acq32(&st[k]);
+                        int32_t wk = ld_acq32(&dist[k]);
                         if (wk == unseen) {
-                            const int32_t old = atomicCAS(&st[k], unseen, next);
+                            const int32_t old = cas32_wg(&dist[k], unseen, next);
                             app = (old == unseen);
                             wk = app ? next : old;
                         }
-                        if (wk == next) atomicAdd((unsigned long long*)&acc_s[k], (unsigned long long)sc);
+                        if (wk == next) add64_wg(&acc_s[k], sc);
                     }
                     const int pos = wave_append(app, &ts.cnt[0]);
                     if (pos >= 0) list[pos] = k;
@@ -706,6 +777,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
             for (int x = fe + threadIdx.x; x < nfe; x += NWB * 64) {
                 const int32_t k = list[x];
                 sigma[k] = ld_acq64(&acc_s[k]);
+                sb[k] = sb_make(j + 1, 0);
                 st_rel64(&acc_s[k], 0);
             }
             j++;
@@ -723,7 +795,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         __syncthreads();
         for (int L = j - 1; L >= 0; --L) {
             const int a = lvl[L], b = lvl[L + 1];
-            const int32_t childw = st_enc(L + 1, 0);
+            const int32_t childw = L + 1;
             for (int idx = a + wave; idx < b; idx += NWB) {
                 const int32_t k = list[idx];
                 const int64_t sk = sigma[k];
@@ -735,7 +807,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
                     double add = 0.0;
                     if (e < vi.deg) {
                         const int32_t c = P.col[vi.start + e];
-                        if (ld_acq32(&st[c]) == childw) {
+                        if (ld_acq32(&dist[c]) == childw) {
                             child = true;
                             add = dep(sk, sigma[c], delta[c]);
                         }
