@@ -460,6 +460,10 @@ struct extraArraysPerThread {
     int32_t* d_order = nullptr;
     int64_t order_cap = 0;
     int num_cu = 0;
+    // optional per-item device timestamps, appended to the file named by
+    // SBC_ITEM_TRACE after each batched call (diagnostics)
+    int64_t* d_tprof = nullptr;
+    int64_t tprof_cap = 0;
     WinStage win;
     GraphCache gc;
     int refs = 1;
@@ -489,6 +493,7 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_qctr) (void)hipFree(C->d_qctr);
     if (C->d_rec) (void)hipFree(C->d_rec);
     if (C->d_order) (void)hipFree(C->d_order);
+    if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->win.d_desc) (void)hipFree(C->win.d_desc);
     if (C->win.d_ints) (void)hipFree(C->win.d_ints);
     if (C->win.d_vi) (void)hipFree(C->win.d_vi);
@@ -936,7 +941,17 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         if (short_room) graph_relayout(G);
         if ((st = graph_flush(G, C->stream))) return st;
     }
+    const char* trace_path = getenv("SBC_ITEM_TRACE");
+    if (trace_path && C->tprof_cap < (int64_t)W * F->K * 4) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (C->d_tprof) HIPCHK(hipFree(C->d_tprof));
+        C->d_tprof = nullptr;
+        if ((st = dalloc(&C->d_tprof, (size_t)W * F->K * 4))) return st;
+        C->tprof_cap = (int64_t)W * F->K * 4;
+        C->gc.with_rec = -1;
+    }
     SbcParams P = base_params(F, G, C);
+    P.tprof = trace_path ? C->d_tprof : nullptr;
     P.heavy = heavy;
     P.order = has_order ? C->d_order : nullptr;
     P.win = C->win.d_desc;
@@ -1045,6 +1060,16 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
         } else if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) {
             return st;
+        }
+        if (trace_path) {
+            std::vector<int64_t> tb((size_t)n * F->K * 4);
+            HIPCHK(hipMemcpyAsync(tb.data(), C->d_tprof, sizeof(int64_t) * tb.size(), hipMemcpyDeviceToHost, C->stream));
+            HIPCHK(hipStreamSynchronize(C->stream));
+            FILE* fp = fopen(trace_path, "ab");
+            if (fp) {
+                fwrite(tb.data(), sizeof(int64_t), tb.size(), fp);
+                fclose(fp);
+            }
         }
         if (records) {
             // [insertion][root] for this window

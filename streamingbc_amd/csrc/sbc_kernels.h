@@ -64,6 +64,7 @@ struct SbcParams {
     int32_t u0;
     int32_t v0;
     SbcItemRec* rec;      // [W][K] or nullptr
+    int64_t* tprof;       // [W][K][4] device timestamps (start, descent, ascent, commit) or nullptr
     int32_t* err;
     int32_t bc_atomic;
 };

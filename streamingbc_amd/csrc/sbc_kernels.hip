@@ -481,6 +481,8 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
     if (du == INF32 && dv == INF32) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
     SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
+    int64_t* tp = P.tprof ? &P.tprof[((size_t)item * P.K + ri) * 4] : nullptr;
+    if (tp && leader) { const int64_t t0 = (int64_t)wall_clock64(); tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0; }
     if (gap == 0) {
         if (rec && leader) {
             rec->root_index = ri; rec->insertion = item; rec->kase = 0; rec->gap = 0;
@@ -565,6 +567,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         return;
     }
     team_sync<TW>();
+    if (tp && leader) tp[1] = (int64_t)wall_clock64();
     const int nf = j; // frontiers 0..nf-1
     const int nD = ts.cnt[0];
     if (leader) ts.cnt[2] = nD;
@@ -614,6 +617,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
     }
 
     // ---------------- commit
+    if (tp && leader) tp[2] = (int64_t)wall_clock64();
     const int nUend = ts.cnt[2];
     const int nS = ts.cnt[1];
     const int total = nUend + nS;
@@ -652,6 +656,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         atomicAdd(&ts.stat[2], my_moved);
     }
     team_sync<TW>();
+    if (tp && leader) tp[3] = (int64_t)wall_clock64();
     if (rec && leader) {
         rec->root_index = ri;
         rec->insertion = item;

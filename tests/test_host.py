@@ -1,0 +1,115 @@
+"""CPU tests of the host side: generators, DIMACS IO, streams, and the C ABI
+of libstreamingbc_hip.so (loads and exports every declared symbol; no device
+calls are made)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import streamingbc_amd as sbc
+from sbc_testutil import GOLDEN, ROOT
+
+
+def _check_simple(hg):
+    rp, col = hg.rowptr, hg.col
+    assert rp[0] == 0 and rp[1] == 0, "phantom vertex 0 must be isolated"
+    assert np.all(np.diff(rp) >= 0)
+    seen = set()
+    for x in range(hg.nv):
+        nb = col[rp[x]:rp[x + 1]]
+        assert len(set(nb.tolist())) == len(nb), "duplicate neighbour"
+        assert x not in set(nb.tolist()), "self loop"
+        for y in nb:
+            seen.add((x, int(y)))
+    for x, y in seen:
+        assert (y, x) in seen, "graph not symmetric"
+
+
+def test_rmat_deterministic_and_simple():
+    a = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
+    b = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
+    c = sbc.gen_rmat(10, 8192, seed=2, keep_lcc=True)
+    assert np.array_equal(a.rowptr, b.rowptr) and np.array_equal(a.col, b.col)
+    assert not (a.ne == c.ne and np.array_equal(a.col, c.col))
+    _check_simple(a)
+    lcc = a.largest_component()
+    assert lcc[1:].all() and not lcc[0], "keep_lcc leaves one component 1..n'"
+
+
+def test_rmat_full_keeps_all_ids():
+    g = sbc.gen_rmat(12, 16 << 12, seed=1, keep_lcc=False)
+    assert g.nv == (1 << 12) + 1
+    _check_simple(g)
+
+
+def test_chung_lu_degrees():
+    g = sbc.gen_chung_lu(1 << 12, 2.3, 8.0, seed=1, keep_lcc=True)
+    _check_simple(g)
+    d = g.degree()[1:]
+    assert 5.0 < d.mean() < 9.0
+    assert d.max() > 20 * np.median(d), "power-law tail expected"
+
+
+def test_dimacs_roundtrip(tmp_path):
+    g = sbc.read_dimacs(os.path.join(GOLDEN, "graphs", "karate.graph"))
+    assert g.nv == 35 and g.ne == 156
+    p = str(tmp_path / "k.graph")
+    g.write_dimacs(p)
+    h = sbc.read_dimacs(p)
+    assert np.array_equal(g.rowptr, h.rowptr) and np.array_equal(g.col, h.col)
+
+
+def test_dimacs_rejects_bad_ids(tmp_path):
+    p = tmp_path / "bad.graph"
+    p.write_text("3 2 0\n2\n1 7\n\n")
+    with pytest.raises(ValueError):
+        sbc.read_dimacs(str(p))
+
+
+def test_insertion_stream_properties():
+    g = sbc.gen_chung_lu(2000, 2.3, 8.0, seed=3, keep_lcc=True)
+    u, v = g.insertions(500, 9)
+    assert len(u) == 500
+    lcc = g.largest_component()
+    pairs = set()
+    for a, b in zip(u, v):
+        assert a != b and lcc[a] and lcc[b]
+        nb = set(g.col[g.rowptr[a]:g.rowptr[a + 1]].tolist())
+        assert int(b) not in nb, "insertion of an existing edge"
+        key = (min(a, b), max(a, b))
+        assert key not in pairs
+        pairs.add(key)
+    u2, v2 = g.insertions(500, 9)
+    assert np.array_equal(u, u2) and np.array_equal(v, v2)
+
+
+def test_roots():
+    g = sbc.gen_chung_lu(2000, 2.3, 8.0, seed=3, keep_lcc=True)
+    r = g.pick_roots(64, 7)
+    assert len(set(r.tolist())) == 64
+    assert g.largest_component()[r].all()
+    assert np.array_equal(g.pick_roots(g.nv, 1), np.arange(g.nv))
+
+
+def _declared_functions(header):
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text)
+    return [n for n in names if n not in ("if", "sizeof")]
+
+
+@pytest.mark.parametrize("header", ["streamingbc_hip.h", "sbc_graphgen.h"])
+def test_library_exports_declared_symbols(header):
+    lib = ctypes.CDLL(sbc.LIB_PATH)
+    names = _declared_functions(os.path.join(ROOT, "include", header))
+    assert len(names) > 5
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"{header}: not exported: {missing}"
+
+
+def test_footprint_formula():
+    # 41 B per (root, vertex) + level offsets + BC
+    fp = sbc.forest_footprint(1 << 20, 256)
+    assert 256 * (1 << 20) * 41 <= fp < 256 * (1 << 20) * 41 + 64 << 20

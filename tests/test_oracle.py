@@ -1,0 +1,274 @@
+"""CPU tests that pin the oracle (no GPU needed).
+
+Pins, per oracle function (see oracle/sbc_oracle.h):
+  static push  -> golden vectors of the reference (bit-exact), networkx x2,
+                  closed forms (path, star, cycle, complete), brute force
+  static pull  -> push (dist/sigma equal, delta/BC within 1e-12), closed forms
+  insert COARSE / FINE -> golden vectors of the reference's COARSE / FINE
+                  update at NT=1 (bit-exact, incl. BC), static recompute of the
+                  post-insertion graph, the reference's traversal counters
+"""
+import hashlib
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import streamingbc_amd as sbc
+from oracle_py import COARSE, FINE, PULL, PUSH, Oracle
+from sbc_testutil import GOLDEN, LONG_MAX, assert_bc_close
+
+GRAPH_DIR = os.path.join(GOLDEN, "graphs")
+
+
+def load_graph(name):
+    return sbc.read_dimacs(os.path.join(GRAPH_DIR, f"{name}.graph"))
+
+
+def from_edges(n, edges):
+    eu = np.array([a for a, b in edges], dtype=np.int64)
+    ev = np.array([b for a, b in edges], dtype=np.int64)
+    return sbc.graph_from_edges(n, eu, ev)
+
+
+# ------------------------------------------------------------ golden vectors
+
+@pytest.mark.parametrize("name", ["karate", "dolphins", "jazz"])
+@pytest.mark.parametrize("mode,key", [(COARSE, "coarse"), (FINE, "fine")])
+def test_oracle_matches_reference_golden(name, mode, key):
+    g = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    hg = load_graph(name)
+    roots = np.arange(hg.nv)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PUSH)
+    assert np.array_equal(o.bc(), g[f"{key}_bc"][0]), "static BC differs from the reference"
+    for i, (a, b) in enumerate(zip(g["u"], g["v"])):
+        items = o.insert(int(a), int(b), mode, items=True)
+        assert np.array_equal(o.bc(), g[f"{key}_bc"][i + 1]), f"BC after insertion {i}"
+        # the reference's own traversal counters (vertex visits, edges scanned,
+        # adjacent-case roots, movement-case roots) for this insertion
+        ref_c = g[f"{key}_counters"][i]
+        assert items["processed"].sum() == ref_c[0]
+        assert items["scanned"].sum() == ref_c[1]
+        assert (items["kase"] == 1).sum() == ref_c[2]
+        assert (items["kase"] == 2).sum() == ref_c[3]
+    lv, sg, dl = o.trees()
+    assert np.array_equal(lv, g[f"{key}_level"])
+    assert np.array_equal(sg, g[f"{key}_sigma"])
+    assert np.array_equal(dl, g[f"{key}_delta"])
+
+
+def test_oracle_matches_reference_golden_rmat10():
+    g = np.load(os.path.join(GOLDEN, "ref_rmat10.npz"))
+    hg = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
+    assert hg.nv == int(g["nv"]) and hg.ne == int(g["ne"]), "generator changed: fixtures are stale"
+    u, v = hg.insertions(256, 1)
+    assert np.array_equal(u, g["u"]) and np.array_equal(v, g["v"])
+    roots = np.arange(hg.nv)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PUSH)
+    assert np.array_equal(o.bc(), g["coarse_bc"][0])
+    for i, (a, b) in enumerate(zip(u, v)):
+        o.insert(int(a), int(b), COARSE)
+        if i % 32 == 0 or i == len(u) - 1:
+            lv, sg, _ = o.trees()
+            h = hashlib.sha256(lv.tobytes() + sg.tobytes()).digest()
+            assert h == g["coarse_tree_sha256"][i].tobytes(), f"trees differ after insertion {i}"
+        if (i + 1) % 16 == 0:
+            assert np.array_equal(o.bc(), g["coarse_bc"][(i + 1) // 16])
+    lv, sg, dl = o.trees()
+    assert np.array_equal(lv, g["coarse_level"])
+    assert np.array_equal(sg, g["coarse_sigma"])
+    assert np.array_equal(dl[:64], g["coarse_delta"])
+
+
+@pytest.mark.parametrize("name", ["karate", "dolphins", "jazz"])
+def test_static_matches_networkx(name):
+    g = np.load(os.path.join(GOLDEN, f"nx_{name}.npz"))
+    hg = load_graph(name)
+    for order in (PUSH, PULL):
+        o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+        o.static(order)
+        assert_bc_close(o.bc(), g["bc2"], rel=1e-12)
+
+
+# ------------------------------------------------------------ closed forms
+
+def _exact_bc(hg, order=PUSH):
+    o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+    o.static(order)
+    return o.bc()
+
+
+@pytest.mark.parametrize("order", [PUSH, PULL])
+def test_closed_forms(order):
+    n = 9
+    path = from_edges(n, [(i, i + 1) for i in range(1, n)])
+    bc = _exact_bc(path, order)
+    for k in range(1, n + 1):  # ordered pairs: 2 (k-1)(n-k)
+        assert bc[k] == 2 * (k - 1) * (n - k)
+    assert bc[0] == 0
+    star = from_edges(n, [(1, i) for i in range(2, n + 1)])
+    bc = _exact_bc(star, order)
+    assert bc[1] == (n - 1) * (n - 2)
+    assert np.all(bc[2:] == 0)
+    comp = from_edges(6, list(itertools.combinations(range(1, 7), 2)))
+    assert np.all(_exact_bc(comp, order) == 0)
+    cyc = from_edges(8, [(i, i % 8 + 1) for i in range(1, 9)])
+    bc = _exact_bc(cyc, order)
+    assert np.allclose(bc[1:], bc[1]) and bc[1] > 0
+
+
+def _brute_bc(n, edges):
+    """BC by enumerating all shortest paths on a tiny graph (ordered pairs)."""
+    adj = {x: set() for x in range(1, n + 1)}
+    for a, b in edges:
+        adj[a].add(b)
+        adj[b].add(a)
+
+    def paths(s, t):
+        # all shortest s-t paths by BFS layering
+        dist = {s: 0}
+        layer = [s]
+        while layer and t not in dist:
+            nxt = []
+            for x in layer:
+                for y in adj[x]:
+                    if y not in dist:
+                        dist[y] = dist[x] + 1
+                        nxt.append(y)
+            layer = nxt
+        if t not in dist:
+            return []
+        out = []
+
+        def walk(path):
+            x = path[-1]
+            if x == t:
+                out.append(path)
+                return
+            for y in adj[x]:
+                if dist.get(y) == dist[x] + 1 and (y == t or dist[y] < dist[t]):
+                    walk(path + [y])
+        walk([s])
+        return out
+
+    bc = np.zeros(n + 1)
+    for s in range(1, n + 1):
+        for t in range(1, n + 1):
+            if s == t:
+                continue
+            ps = paths(s, t)
+            for p in ps:
+                for x in p[1:-1]:
+                    bc[x] += 1.0 / len(ps)
+    return bc
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_brute_force_small(seed):
+    rng = np.random.default_rng(seed)
+    n = 10
+    edges = set()
+    while len(edges) < 16:
+        a, b = rng.integers(1, n + 1, 2)
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    edges = sorted(edges)
+    hg = from_edges(n, edges)
+    want = _brute_bc(n, edges)
+    for order in (PUSH, PULL):
+        assert_bc_close(_exact_bc(hg, order), want, rel=1e-12)
+
+
+# ------------------------------------------------------------ invariants
+
+@pytest.mark.parametrize("mode", [COARSE, FINE])
+def test_incremental_equals_static_recompute(mode):
+    hg = sbc.gen_chung_lu(1500, 2.3, 8.0, seed=2, keep_lcc=True)
+    roots = hg.pick_roots(40, 3)
+    u, v = hg.insertions(60, 4)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PUSH)
+    for a, b in zip(u, v):
+        o.insert(int(a), int(b), mode)
+    s = Oracle(hg.rowptr, hg.col, roots)
+    for a, b in zip(u, v):
+        s.add_edge(int(a), int(b))
+    s.static(PUSH)
+    lv, sg, dl = o.trees()
+    ls, ss, ds = s.trees()
+    assert np.array_equal(lv, ls)
+    assert np.array_equal(sg, ss)
+    assert_bc_close(dl, ds, what="delta")
+    assert_bc_close(o.bc(), s.bc())
+
+
+def test_connecting_components_matches_recompute():
+    # two components joined by the insertion: one endpoint unreachable from
+    # some roots (the reference's LONG_MAX arithmetic path)
+    a = [(i, i + 1) for i in range(1, 6)]
+    b = [(i, i + 1) for i in range(7, 11)] + [(7, 9)]
+    hg = from_edges(11, a + b)
+    roots = np.arange(hg.nv)
+    for mode in (COARSE, FINE):
+        o = Oracle(hg.rowptr, hg.col, roots)
+        o.static(PUSH)
+        o.insert(3, 8, mode)
+        s = Oracle(hg.rowptr, hg.col, roots)
+        s.add_edge(3, 8)
+        s.static(PUSH)
+        lv, sg, dl = o.trees()
+        ls, ss, ds = s.trees()
+        assert np.array_equal(lv, ls) and np.array_equal(sg, ss)
+        assert_bc_close(dl, ds)
+        assert_bc_close(o.bc(), s.bc())
+
+
+def test_fine_and_coarse_agree_and_pull_static_close():
+    hg = load_graph("jazz")
+    roots = np.arange(hg.nv)
+    u, v = hg.insertions(30, 8)
+    oc = Oracle(hg.rowptr, hg.col, roots)
+    of = Oracle(hg.rowptr, hg.col, roots)
+    op = Oracle(hg.rowptr, hg.col, roots)
+    oc.static(PUSH)
+    of.static(PUSH)
+    op.static(PULL)
+    lc, sc, dc = oc.trees()
+    lp, sp_, dp = op.trees()
+    assert np.array_equal(lc, lp) and np.array_equal(sc, sp_)
+    assert_bc_close(dp, dc, rel=1e-12, what="pull vs push delta")
+    for a, b in zip(u, v):
+        oc.insert(int(a), int(b), COARSE)
+        of.insert(int(a), int(b), FINE)
+    l1, s1, d1 = oc.trees()
+    l2, s2, d2 = of.trees()
+    assert np.array_equal(l1, l2) and np.array_equal(s1, s2)
+    assert_bc_close(d2, d1, rel=1e-10)  # summation order differs (push vs pull)
+
+
+def test_planted_error_is_caught():
+    """The tolerance check rejects a delta perturbed well above rounding."""
+    hg = load_graph("karate")
+    o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+    o.static(PUSH)
+    lv, sg, dl = o.trees()
+    bad = dl.copy()
+    bad[5, 3] *= (1 + 1e-6)
+    from sbc_testutil import bc_close
+    assert not bc_close(bad, dl)[0]
+    lv2 = lv.copy()
+    lv2[1, 2] += 1
+    assert not np.array_equal(lv2, lv)
+
+
+def test_sigma_sentinels():
+    hg = load_graph("karate")
+    o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+    o.static(PUSH)
+    lv, sg, dl = o.tree(0)  # phantom root: everything else unreachable
+    assert lv[0] == 0 and sg[0] == 1
+    assert np.all(lv[1:] == LONG_MAX) and np.all(sg[1:] == LONG_MAX) and np.all(dl == 0)
+    assert not o.sigma_overflowed()
