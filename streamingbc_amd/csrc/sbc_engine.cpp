@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <unordered_set>
 #include <vector>
 
@@ -72,7 +73,17 @@ struct stinger {
     uint64_t generation = 0; // bumps when device arrays are reallocated
     // pending per-slot writes and vertex-info updates (legacy path)
     std::vector<int32_t> p_slot, p_col, p_seq, p_v;
+    // open-addressing tables over the initial adjacency of high-degree
+    // vertices: hub[v] = {offset, count, log2 size, 0}, htab = {neighbour + 1, position}
+    std::vector<int4> hub;
+    std::vector<int2> htab;
+    int4* d_hub = nullptr;
+    int2* d_htab = nullptr;
 };
+
+// Vertices with more initial neighbours than this get a sorted index (the
+// update kernel looks neighbours up in it instead of scanning hub lists).
+static const int32_t kHubIndexMin = 512;
 
 static int32_t slack_for(int32_t d) { return 4 + d / 16; }
 
@@ -105,6 +116,26 @@ static sbcStatus graph_init(stinger* G, int64_t nv, const int64_t* rowptr, const
             if (x < 0 || x >= nv) return set_err(SBC_ERR_ARG, "edge %ld -> %ld out of range", (long)v, (long)x);
             G->col[G->start[v] + (e - rowptr[v])] = (int32_t)x;
         }
+    }
+    // lookup tables of high-degree vertices (neighbour -> position in the list)
+    G->hub.assign(nv, int4{0, 0, 0, 0});
+    G->htab.clear();
+    for (int64_t v = 0; v < nv; v++) {
+        int32_t d = G->deg[v];
+        if (d <= kHubIndexMin || d >= (1 << 22)) continue;
+        int logT = 1;
+        while ((1 << logT) < 2 * d) logT++;
+        const uint32_t mask = (1u << logT) - 1u;
+        int64_t off = (int64_t)G->htab.size();
+        if (off + (1 << logT) >= INT32_MAX) break;
+        G->htab.resize(off + (1 << logT), int2{0, 0});
+        for (int32_t i = 0; i < d; i++) {
+            const int32_t w = G->col[G->start[v] + i];
+            uint32_t h = ((uint32_t)w * 0x9E3779B1u) >> (32 - logT);
+            while (G->htab[off + h].x != 0) h = (h + 1) & mask;
+            G->htab[off + h] = int2{w + 1, i};
+        }
+        G->hub[v] = int4{(int32_t)off, d, logT, 0};
     }
     G->full_dirty = true;
     return SBC_OK;
@@ -179,6 +210,13 @@ static sbcStatus graph_upload_full(stinger* G, hipStream_t st)
         G->d_cap = cap;
         G->generation++;
     }
+    if (!G->d_hub) {
+        HIPCHK(hipMalloc(&G->d_hub, sizeof(int4) * G->nv));
+        HIPCHK(hipMalloc(&G->d_htab, sizeof(int2) * (G->htab.size() ? G->htab.size() : 1)));
+        HIPCHK(hipMemcpy(G->d_hub, G->hub.data(), sizeof(int4) * G->nv, hipMemcpyHostToDevice));
+        if (!G->htab.empty())
+            HIPCHK(hipMemcpy(G->d_htab, G->htab.data(), sizeof(int2) * G->htab.size(), hipMemcpyHostToDevice));
+    }
     std::vector<SbcVInfo> vi(G->nv);
     for (int64_t v = 0; v < G->nv; v++) vi[v] = vinfo_of(G, (int32_t)v);
     HIPCHK(hipMemcpy(G->d_vinfo, vi.data(), sizeof(SbcVInfo) * G->nv, hipMemcpyHostToDevice));
@@ -226,6 +264,8 @@ static void graph_free(stinger* G)
     if (G->d_vinfo) (void)hipFree(G->d_vinfo);
     if (G->d_col) (void)hipFree(G->d_col);
     if (G->d_seq) (void)hipFree(G->d_seq);
+    if (G->d_hub) (void)hipFree(G->d_hub);
+    if (G->d_htab) (void)hipFree(G->d_htab);
     delete G;
 }
 
@@ -613,6 +653,8 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.vinfo = G->d_vinfo;
     P.col = G->d_col;
     P.seq = G->d_seq;
+    P.hub = G->d_hub;
+    P.htab = G->d_htab;
     P.nv = (int32_t)F->nv;
     P.K = (int32_t)F->K;
     P.lcap = F->lcap;
@@ -942,12 +984,12 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         if ((st = graph_flush(G, C->stream))) return st;
     }
     const char* trace_path = getenv("SBC_ITEM_TRACE");
-    if (trace_path && C->tprof_cap < (int64_t)W * F->K * 4) {
+    if (trace_path && C->tprof_cap < (int64_t)W * F->K * 8) {
         HIPCHK(hipStreamSynchronize(C->stream));
         if (C->d_tprof) HIPCHK(hipFree(C->d_tprof));
         C->d_tprof = nullptr;
-        if ((st = dalloc(&C->d_tprof, (size_t)W * F->K * 4))) return st;
-        C->tprof_cap = (int64_t)W * F->K * 4;
+        if ((st = dalloc(&C->d_tprof, (size_t)W * F->K * 8))) return st;
+        C->tprof_cap = (int64_t)W * F->K * 8;
         C->gc.with_rec = -1;
     }
     SbcParams P = base_params(F, G, C);
@@ -1062,7 +1104,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             return st;
         }
         if (trace_path) {
-            std::vector<int64_t> tb((size_t)n * F->K * 4);
+            std::vector<int64_t> tb((size_t)n * F->K * 8);
             HIPCHK(hipMemcpyAsync(tb.data(), C->d_tprof, sizeof(int64_t) * tb.size(), hipMemcpyDeviceToHost, C->stream));
             HIPCHK(hipStreamSynchronize(C->stream));
             FILE* fp = fopen(trace_path, "ab");

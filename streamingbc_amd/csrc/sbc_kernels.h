@@ -41,6 +41,8 @@ struct SbcParams {
     const SbcVInfo* vinfo;
     const int32_t* col;
     const int32_t* seq;
+    const int4* hub;      // per vertex {offset into htab, indexed count, log2 table size, 0} (count 0: none)
+    const int2* htab;     // open-addressing tables {neighbour + 1, position} of indexed vertices
     int32_t nv;
     int32_t K;
     int32_t lcap;
@@ -64,7 +66,8 @@ struct SbcParams {
     int32_t u0;
     int32_t v0;
     SbcItemRec* rec;      // [W][K] or nullptr
-    int64_t* tprof;       // [W][K][4] device timestamps (start, descent, ascent, commit) or nullptr
+    int64_t* tprof;       // [W][K][8]: timestamps (start, descent, ascent, commit), hubs probed, hubs scanned,
+                          // neighbours of probed hubs, neighbours of scanned hubs; or nullptr
     int32_t* err;
     int32_t bc_atomic;
 };

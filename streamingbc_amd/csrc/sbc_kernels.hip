@@ -189,20 +189,28 @@ struct TeamShared {
 constexpr int HUB_MIN = 512;
 // 64-neighbour chunks whose loads are issued together by one wavefront.
 #ifndef SBC_SCAN_UNROLL
-#define SBC_SCAN_UNROLL 4
+#define SBC_SCAN_UNROLL 2
 #endif
 constexpr int SCAN_UNROLL = SBC_SCAN_UNROLL;
 constexpr int HUB_CAP = 256;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
+constexpr int PROBE_CAP = 16 * RUN_CAP; // terms of one probed hub (shares the run buffers)
+
 struct alignas(16) HubShared {
     int n;
     int overflow;
+    int pn;
+    int np;
     int list[HUB_CAP];
+    int plist[HUB_CAP];   // ascent-only hubs of the level handled by table lookups
+    uint8_t kind[HUB_CAP];
     int runlen[16];
     double radd[16][RUN_CAP];
     double rsub[16][RUN_CAP];
     uint8_t rflag[16][RUN_CAP];
+    int pkey[PROBE_CAP];
+    int pperm[PROBE_CAP];
 };
 
 // Per-item constants.
@@ -438,6 +446,172 @@ __device__ __forceinline__ void ascent_vertex_team(const SbcParams& P, const Ite
     __syncthreads();
 }
 
+
+// Position of w in the indexed (initial) part of a vertex's adjacency list,
+// looked up in the vertex's open-addressing table, or -1.
+__device__ __forceinline__ int hub_find(const int2* __restrict__ tab, int logT, int32_t w)
+{
+    const uint32_t mask = (1u << logT) - 1u;
+    uint32_t h = ((uint32_t)w * 0x9E3779B1u) >> (32 - logT);
+    for (;;) {
+        const int2 e = tab[h];
+        if (e.x == w + 1) return e.y;
+        if (e.x == 0) return -1;
+        h = (h + 1) & mask;
+    }
+}
+
+// Processed-vertex ranges of one ascent level: descent frontier list[dA..dB),
+// same-level list (stored backwards) sl[sA..sB), ascent list list[uA..uB).
+struct LevelSets {
+    int dA, dB, sA, sB, uA, uB;
+    __device__ int size() const { return (dB - dA) + (sB - sA) + (uB - uA); }
+    __device__ int32_t at(const Item& X, int i) const
+    {
+        const int nd = dB - dA, ns = sB - sA;
+        if (i < nd) return X.list[dA + i];
+        if (i < nd + ns) return X.list[X.nv - 1 - (sA + i - nd)];
+        return X.list[uA + i - nd - ns];
+    }
+};
+
+// Term of processed neighbour w of ascent-only hub k (number hi in the
+// level's probe list) at position pos of k's list, appended to the LDS term
+// buffer under key (hi, pos).
+__device__ __forceinline__ void probe_term(const SbcParams& P, const Item& X, HubShared& hs, int32_t k, int hi,
+                                           int lev, int32_t w, int pos)
+{
+    bool appU;
+    const int64_t sk = X.sigma[k];
+    const Term t = ascent_lane(P, X, k, false, sk, sk, lev, true, w, ld_sb(&X.sb[w]), appU);
+    if (t.child || t.hsub) {
+        const int slot = atomicAdd(&hs.pn, 1);
+        if (slot < PROBE_CAP) {
+            hs.pkey[slot] = (hi << 22) | pos;
+            (&hs.radd[0][0])[slot] = t.add;
+            (&hs.rsub[0][0])[slot] = t.sub;
+            (&hs.rflag[0][0])[slot] = (uint8_t)((t.child ? 1 : 0) | (t.hsub ? 2 : 0));
+        } else {
+            hs.overflow = 1;
+        }
+    }
+}
+
+__device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32_t x, int lev_parent)
+{
+    const uint8_t b = ld_sb(&X.sb[x]);
+    if (sb_type(b) == 0 && cas_sb(&X.sb[x], b, sb_make(lev_parent, 3)) == b)
+        X.list[atomicAdd(&ts.cnt[2], 1)] = x;
+}
+
+// New deltas of the ascent-only hubs hs.plist[0..np) of level lev (1 or 2)
+// without scanning their adjacency lists.  A hub's terms can only come from
+// processed vertices one level down (children) and moved descent vertices on
+// its own level, so every (hub, candidate) pair is looked up in the hub's
+// table; the appended (inserted) part of each hub's list is checked slot by
+// slot.  The terms are sorted by (hub, position in the hub's list) and each
+// hub sums its own in that order -- the order of the full pull scan, so the
+// result is bit-identical to it.  Parents: on level 1 only the root; on
+// level 2 exactly the root's neighbours adjacent to the hub.
+// Returns false if the term buffer overflowed (nothing written then; the
+// caller scans those hubs, whose parent marks and terms are the same).
+template <int TW>
+__device__ __forceinline__ bool ascent_hubs_probe(const SbcParams& P, const Item& X, int np, int lev,
+                                                  TeamShared& ts, HubShared& hs, const LevelSets& below,
+                                                  const LevelSets& same, const SbcVInfo& vr, int len_r)
+{
+    const int tid = threadIdx.x;
+    const int nthr = TW * 64;
+    if (tid == 0) { hs.pn = 0; hs.overflow = 0; }
+    __syncthreads();
+    const int nb = below.size(), nsd = same.dB - same.dA, nc = nb + nsd;
+    for (int t = tid; t < np * nc; t += nthr) {
+        const int hi = t / nc, ci = t - hi * nc;
+        const int32_t k = hs.plist[hi];
+        const int4 hb = P.hub[k];
+        const int32_t w = ci < nb ? below.at(X, ci) : X.list[same.dA + (ci - nb)];
+        const int pos = hub_find(P.htab + hb.x, hb.z, w);
+        if (pos >= 0) probe_term(P, X, hs, k, hi, lev, w, pos);
+    }
+    for (int hi = 0; hi < np; ++hi) {
+        const int32_t k = hs.plist[hi];
+        const SbcVInfo vi = P.vinfo[k];
+        const int len = visible_len(P, vi, X.q);
+        for (int e = P.hub[k].y + tid; e < len; e += nthr) {
+            const int32_t c = P.col[vi.start + e];
+            const uint8_t b = ld_sb(&X.sb[c]);
+            if (sb_type(b) == 0) {
+                if (!(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
+            } else {
+                probe_term(P, X, hs, k, hi, lev, c, e);
+            }
+        }
+    }
+    if (lev == 1) {
+        if (tid == 0) mark_parent(X, ts, X.r, 0);
+    } else {
+        for (int t = tid; t < np * len_r; t += nthr) {
+            const int hi = t / len_r, i = t - hi * len_r;
+            const int4 hb = P.hub[hs.plist[hi]];
+            const int32_t x = P.col[vr.start + i];
+            if (hub_find(P.htab + hb.x, hb.z, x) >= 0) mark_parent(X, ts, x, 1);
+        }
+    }
+    __syncthreads();
+    const int n = hs.pn;
+    const bool ovf = hs.overflow != 0;
+    if (!ovf) {
+        int N = 1;
+        while (N < n) N <<= 1;
+        for (int i = tid; i < N; i += nthr) {
+            if (i >= n) hs.pkey[i] = INF32;
+            hs.pperm[i] = i;
+        }
+        __syncthreads();
+        for (int k2 = 2; k2 <= N; k2 <<= 1) {
+            for (int j = k2 >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < N; i += nthr) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & k2) == 0;
+                        const int ki = hs.pkey[i], kj = hs.pkey[ixj];
+                        if ((ki > kj) == up) {
+                            hs.pkey[i] = kj;
+                            hs.pkey[ixj] = ki;
+                            const int tmp = hs.pperm[i];
+                            hs.pperm[i] = hs.pperm[ixj];
+                            hs.pperm[ixj] = tmp;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // segmented, ordered sums: thread hi owns hub hi
+        for (int hi = tid; hi < np; hi += nthr) {
+            const int32_t k = hs.plist[hi];
+            int lo = 0, hi2 = n;
+            while (lo < hi2) {
+                const int mid = (lo + hi2) >> 1;
+                if ((hs.pkey[mid] >> 22) < hi) lo = mid + 1;
+                else hi2 = mid;
+            }
+            double acc = X.delta[k];
+            for (int i = lo; i < n && (hs.pkey[i] >> 22) == hi; ++i) {
+                const int j = hs.pperm[i];
+                const uint8_t fl = (&hs.rflag[0][0])[j];
+                if (fl & 1) acc = acc + (&hs.radd[0][0])[j];
+                if (fl & 2) acc = acc - (&hs.rsub[0][0])[j];
+            }
+            st_reld(&X.ndel[k], acc);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) hs.overflow = 0;
+    __syncthreads();
+    return !ovf;
+}
+
 // Routes vertex k of the current pass: multi-wave teams queue hubs in LDS
 // (processed afterwards by the whole team); everything else runs on the
 // calling wavefront.  Returns true if k was queued.
@@ -481,8 +655,8 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
     if (du == INF32 && dv == INF32) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
     SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
-    int64_t* tp = P.tprof ? &P.tprof[((size_t)item * P.K + ri) * 4] : nullptr;
-    if (tp && leader) { const int64_t t0 = (int64_t)wall_clock64(); tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0; }
+    int64_t* tp = P.tprof ? &P.tprof[((size_t)item * P.K + ri) * 8] : nullptr;
+    if (tp && leader) { const int64_t t0 = (int64_t)wall_clock64(); tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0; tp[4] = 0; tp[5] = 0; tp[6] = 0; tp[7] = 0; }
     if (gap == 0) {
         if (rec && leader) {
             rec->root_index = ri; rec->insertion = item; rec->kase = 0; rec->gap = 0;
@@ -575,10 +749,14 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
 
     // ---------------- ascent
     int uA = nD, uB = nD;
+    LevelSets below{0, 0, 0, 0, 0, 0};
+    SbcVInfo vr = P.vinfo[X.r];
+    const int len_r = visible_len(P, vr, X.q);
     for (int lev = base + nf - 1; lev >= 0; --lev) {
         const int jj = lev - base;
         int dA = 0, dB = 0, sA = 0, sB = 0;
         if (jj >= 0) { dA = lvlD[jj]; dB = lvlD[jj + 1]; sA = lvlS[jj]; sB = lvlS[jj + 1]; }
+        const LevelSets here{dA, dB, sA, sB, uA, uB};
         const int nd = dB - dA, ns = sB - sA, nu = uB - uA;
         const int tot = nd + ns + nu;
         if (tot == 0) break;
@@ -597,11 +775,33 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         if constexpr (TW > 1) {
             __syncthreads();
             const int nh = min(hs->n, HUB_CAP);
+            // split the queued hubs: ascent-only indexed hubs on level 1 or 2
+            // whose lookups are much cheaper than their lists are handled
+            // together by table lookups; the rest are scanned one by one
+            if (threadIdx.x == 0) hs->np = 0;
+            __syncthreads();
+            const long cand = (long)below.size() + (here.dB - here.dA) + (lev == 2 ? len_r : 0);
+            for (int h = threadIdx.x; h < nh; h += TW * 64) {
+                const int32_t k = hs->list[h];
+                bool pk = false;
+                if (P.hub && lev >= 1 && lev <= 2 && P.hub[k].y > 0 && sb_type(ld_sb(&X.sb[k])) == 3) {
+                    const int4 hb = P.hub[k];
+                    pk = cand * 4 < (long)hb.y;
+                }
+                hs->kind[h] = pk ? 1 : 0;
+                if (pk) hs->plist[atomicAdd(&hs->np, 1)] = k;
+            }
+            __syncthreads();
+            const int np = hs->np;
+            bool probed = false;
+            if (np > 0) probed = ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, below, here, vr, len_r);
             for (int h = 0; h < nh; ++h) {
                 const int32_t k = hs->list[h];
                 const SbcVInfo vi = P.vinfo[k];
                 const int len = visible_len(P, vi, X.q);
-                ascent_vertex_team<TW>(P, X, k, vi, len, lev, ts, *hs, twave);
+                const bool by_lookup = probed && hs->kind[h];
+                if (!by_lookup) ascent_vertex_team<TW>(P, X, k, vi, len, lev, ts, *hs, twave);
+                if (tp && leader) { tp[by_lookup ? 4 : 5] += 1; tp[by_lookup ? 6 : 7] += len; }
                 if (twave == 0) {
                     my_proc += 1;
                     my_scan += (unsigned long long)len;
@@ -611,6 +811,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
             if (leader) hs->n = 0;
         }
         team_sync<TW>();
+        below = here;
         uA = uB;
         uB = ts.cnt[2];
         team_sync<TW>();

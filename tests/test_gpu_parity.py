@@ -197,3 +197,50 @@ def test_gpu_matches_reference_golden(name):
     assert np.array_equal(sg, g["fine_sigma"])
     assert_bc_close(dl, g["fine_delta"], what="delta")
     assert_bc_close(f.bc(), g["fine_bc"][-1])
+
+
+def _hub_stream(hg, n, seed):
+    """Random insertions plus some at the highest-degree vertices, so the
+    device's hub lookup path (tables over the initial lists, appended slots
+    checked one by one) is exercised."""
+    u, v = hg.insertions(n, seed)
+    deg = hg.degree()
+    hubs = np.argsort(deg)[::-1][:4]
+    rng = np.random.default_rng(seed)
+    lcc = np.where(hg.largest_component())[0]
+    extra_u, extra_v = [], []
+    seen = set(zip(u.tolist(), v.tolist())) | set(zip(v.tolist(), u.tolist()))
+    for h in hubs:
+        nb = set(hg.col[hg.rowptr[h]:hg.rowptr[h + 1]].tolist())
+        for _ in range(3):
+            x = int(rng.choice(lcc))
+            if x != h and x not in nb and (h, x) not in seen:
+                extra_u.append(int(h)); extra_v.append(x); seen.add((h, x)); seen.add((x, int(h)))
+    uu = np.concatenate([u[: n // 2], extra_u, u[n // 2:]]).astype(np.int64)
+    vv = np.concatenate([v[: n // 2], extra_v, v[n // 2:]]).astype(np.int64)
+    return uu, vv
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_hub_lookup_path_rmat14(batched):
+    hg = sbc.gen_rmat(14, 16 << 14, seed=3, keep_lcc=True)
+    assert hg.degree().max() > 1000, "need vertices above the lookup-table threshold"
+    roots = hg.pick_roots(32, 5)
+    u, v = _hub_stream(hg, 48, 6)
+    print(f"[hubs] rmat14 nv={hg.nv} maxdeg={hg.degree().max()} n={len(u)} batched={batched}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    if batched:
+        rec = f.insert_batched(g, ctx, u, v, sbc.BatchConfig(window=16, block_threads=1024), records=True)
+    for i, (a, b) in enumerate(zip(u, v)):
+        if not batched:
+            f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+            rec_g = ctx.last_records(f.k)
+        else:
+            rec_g = rec[i]
+        rec_o = o.insert(int(a), int(b), O_FINE, items=True)
+        assert np.array_equal(rec_g["affected"], rec_o["affected"]), f"insertion {i}"
+        assert np.array_equal(rec_g["scanned"], rec_o["scanned"]), f"insertion {i}"
+    _compare_trees(f, o, "rmat14 hubs")
+    assert_bc_close(f.bc(), o.bc())
