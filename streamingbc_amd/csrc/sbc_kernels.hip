@@ -169,6 +169,44 @@ __device__ __forceinline__ int visible_len(const SbcParams& P, const SbcVInfo& v
     return len;
 }
 
+// Per-thread form of visible_len (no wavefront collectives).
+__device__ __forceinline__ int visible_len_thread(const SbcParams& P, const SbcVInfo& vi, int32_t q)
+{
+    if (q == INF32 || vi.deg == vi.deg0) return vi.deg;
+    int len = vi.deg0;
+    while (len < vi.deg && P.seq[vi.start + len] <= q) ++len;
+    return len;
+}
+
+// Exclusive prefix sum over a workgroup of NW wavefronts; `total` receives
+// the sum.  tmp: NW + 1 ints of LDS.
+template <int NW>
+__device__ __forceinline__ int block_exscan(int x, int* tmp, int& total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int v = x;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(v, d);
+        if (lane >= d) v += y;
+    }
+    if (lane == 63) tmp[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int t = tmp[w];
+            tmp[w] = acc;
+            acc += t;
+        }
+        tmp[NW] = acc;
+    }
+    __syncthreads();
+    const int res = tmp[wave] + v - x;
+    total = tmp[NW];
+    __syncthreads();
+    return res;
+}
+
 // (sa / sb) * (d + 1) in FP64 -- the dependency term of
 // src/streamingbc_init.c:159-161 and src/streamingbc_internal.c:185-198
 __device__ __forceinline__ double dep(int64_t sa, int64_t sb, double d)
@@ -211,6 +249,30 @@ struct alignas(16) HubShared {
     uint8_t rflag[16][RUN_CAP];
     int pkey[PROBE_CAP];
     int pperm[PROBE_CAP];
+};
+
+// Light (non-hub) vertices of one ascent level, batched: every edge of
+// every batched vertex is handled by one lane, its term kept at the edge's
+// index, then each vertex sums its edges' terms in list order.
+constexpr int LIGHT_CAP = 1024;  // vertices per batch (one per thread of a 1024-thread team)
+constexpr int EDGE_CAP = 2048;   // edges per batch
+
+struct alignas(16) LightShared {
+    int nb;                       // batched vertices
+    int novf;                     // vertices left to the one-wave-per-vertex path
+    int scan[17];
+    int32_t k[LIGHT_CAP];
+    int32_t start[LIGHT_CAP];
+    int32_t off[LIGHT_CAP];       // first edge index of the vertex in the batch
+    int32_t len[LIGHT_CAP];
+    uint8_t isD[LIGHT_CAP];
+    int64_t sk_old[LIGHT_CAP];
+    int64_t sk_new[LIGHT_CAP];
+    double base[LIGHT_CAP];
+    int32_t ovf[LIGHT_CAP];
+    uint8_t eflag[EDGE_CAP];
+    double eadd[EDGE_CAP];
+    double esub[EDGE_CAP];
 };
 
 // Per-item constants.
@@ -612,6 +674,117 @@ __device__ __forceinline__ bool ascent_hubs_probe(const SbcParams& P, const Item
     return !ovf;
 }
 
+
+// All vertices of ascent level `here`: hubs are queued in hs.list for the
+// hub passes; light vertices are batched (see LightShared); vertices that do
+// not fit a batch run one wavefront per vertex.
+template <int TW>
+__device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Item& X, const LevelSets& here, int tot,
+                                                   int lev, TeamShared& ts, HubShared& hs, LightShared& ls,
+                                                   int twave, unsigned long long& my_proc,
+                                                   unsigned long long& my_scan)
+{
+    const int tid = threadIdx.x, lane = lane_id();
+    constexpr int NT = TW * 64;
+    for (int r0 = 0; r0 < tot; r0 += NT) {
+        const int t = r0 + tid;
+        int32_t k = 0;
+        int len = 0;
+        SbcVInfo vi{0, 0, 0, 0};
+        bool light = false;
+        if (t < tot) {
+            k = here.at(X, t);
+            vi = P.vinfo[k];
+            len = visible_len_thread(P, vi, X.q);
+            if (len > HUB_MIN) {
+                const int pos = atomicAdd(&hs.n, 1);
+                if (pos < HUB_CAP) hs.list[pos] = k;
+                else ls.ovf[atomicAdd(&ls.novf, 1)] = k;
+            } else {
+                light = true;
+            }
+        }
+        int nl_round, ne_round;
+        const int slot = block_exscan<TW>(light ? 1 : 0, ls.scan, nl_round);
+        const int eoff = block_exscan<TW>(light ? len : 0, ls.scan, ne_round);
+        const bool in_batch = light && eoff + len <= EDGE_CAP;
+        if (light && !in_batch) ls.ovf[atomicAdd(&ls.novf, 1)] = k;
+        if (in_batch) {
+            ls.k[slot] = k;
+            ls.start[slot] = vi.start;
+            ls.off[slot] = eoff;
+            ls.len[slot] = len;
+            const KState kst = kstate(X, k);
+            ls.isD[slot] = kst.isD ? 1 : 0;
+            ls.sk_old[slot] = kst.sk_old;
+            ls.sk_new[slot] = kst.sk_new;
+            ls.base[slot] = kst.base;
+            atomicMax(&ls.nb, slot + 1);
+        }
+        __syncthreads();
+        const int nb = ls.nb;
+        const int ne = nb > 0 ? ls.off[nb - 1] + ls.len[nb - 1] : 0;
+        // one lane per edge
+        for (int e = tid; e < ne; e += NT) {
+            int lo = 0, hi = nb;  // last slot with off <= e
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (ls.off[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            const int p = e - ls.off[lo];
+            const int32_t kk = ls.k[lo];
+            const int32_t c = P.col[ls.start[lo] + p];
+            bool appU;
+            const Term tm = ascent_lane(P, X, kk, ls.isD[lo] != 0, ls.sk_old[lo], ls.sk_new[lo], lev, true, c,
+                                        ld_sb(&X.sb[c]), appU);
+            if (appU) X.list[atomicAdd(&ts.cnt[2], 1)] = c;
+            ls.eflag[e] = (uint8_t)((tm.child ? 1 : 0) | (tm.hsub ? 2 : 0));
+            ls.eadd[e] = tm.add;
+            ls.esub[e] = tm.sub;
+        }
+        __syncthreads();
+        // one wavefront per vertex: sum its edges' terms in list order
+        for (int sl = twave; sl < nb; sl += TW) {
+            const int a = ls.off[sl], l = ls.len[sl];
+            double acc = ls.base[sl];
+            for (int e0 = 0; e0 < l; e0 += 64) {
+                const int e = e0 + lane;
+                const uint8_t fl = e < l ? ls.eflag[a + e] : (uint8_t)0;
+                const double ad = (fl & 1) ? ls.eadd[a + e] : 0.0;
+                const double sb2 = (fl & 2) ? ls.esub[a + e] : 0.0;
+                const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
+                uint64_t m = cm | sm;
+                while (m) {
+                    const int l2 = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
+                    if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
+                }
+            }
+            if (lane == 0) st_reld(&X.ndel[ls.k[sl]], acc);
+        }
+        if (tid == 0) {
+            my_proc += (unsigned long long)nb;
+            my_scan += (unsigned long long)ne;
+        }
+        __syncthreads();
+        // leftovers, one wavefront each
+        const int no = ls.novf;
+        for (int i = twave; i < no; i += TW) {
+            const int32_t kk = ls.ovf[i];
+            const SbcVInfo v2 = P.vinfo[kk];
+            const int l2 = visible_len(P, v2, X.q);
+            ascent_vertex_wave(P, X, kk, v2, l2, lev, ts);
+            my_proc += 1;
+            my_scan += (unsigned long long)l2;
+        }
+        __syncthreads();
+        if (tid == 0) { ls.nb = 0; ls.novf = 0; }
+        __syncthreads();
+    }
+}
+
 // Routes vertex k of the current pass: multi-wave teams queue hubs in LDS
 // (processed afterwards by the whole team); everything else runs on the
 // calling wavefront.  Returns true if k was queued.
@@ -628,7 +801,8 @@ __device__ __forceinline__ bool queue_hub(HubShared* hs, int32_t k, int len)
 }
 
 template <int TW>
-__device__ __forceinline__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs, int twave)
+__device__ __forceinline__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs,
+                                             LightShared* ls, int twave)
 {
     const int lane = lane_id();
     const bool leader = (twave == 0 && lane == 0);
@@ -681,6 +855,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         ts.cnt[2] = 0;
         ts.stat[0] = 0; ts.stat[1] = 0; ts.stat[2] = 0;
         if (hs) { hs->n = 0; hs->overflow = 0; }
+        if (ls) { ls->nb = 0; ls->novf = 0; }
     }
     team_sync<TW>();
 
@@ -760,17 +935,17 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         const int nd = dB - dA, ns = sB - sA, nu = uB - uA;
         const int tot = nd + ns + nu;
         if (tot == 0) break;
-        for (int w = twave; w < tot; w += TW) {
-            int32_t k;
-            if (w < nd) k = X.list[dA + w];
-            else if (w < nd + ns) k = X.list[nv - 1 - (sA + w - nd)];
-            else k = X.list[uA + w - nd - ns];
-            const SbcVInfo vi = P.vinfo[k];
-            const int len = visible_len(P, vi, X.q);
-            if (queue_hub<TW>(hs, k, len)) continue;
-            ascent_vertex_wave(P, X, k, vi, len, lev, ts);
-            my_proc += 1;
-            my_scan += (unsigned long long)len;
+        if constexpr (TW > 1) {
+            ascent_light_level<TW>(P, X, here, tot, lev, ts, *hs, *ls, twave, my_proc, my_scan);
+        } else {
+            for (int w = 0; w < tot; ++w) {
+                const int32_t k = here.at(X, w);
+                const SbcVInfo vi = P.vinfo[k];
+                const int len = visible_len(P, vi, X.q);
+                ascent_vertex_wave(P, X, k, vi, len, lev, ts);
+                my_proc += 1;
+                my_scan += (unsigned long long)len;
+            }
         }
         if constexpr (TW > 1) {
             __syncthreads();
@@ -880,6 +1055,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
 {
     __shared__ TeamShared ts[NWB];
     __shared__ HubShared hs;
+    __shared__ LightShared ls;
     const int wave = threadIdx.x >> 6;
     const int W = P.win ? P.win->W : 1;
     // phase 1: heavy
@@ -890,7 +1066,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         __syncthreads();
         if (t >= P.heavy) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = 0; i < W; ++i) process_item<NWB>(P, ri, i, ts[0], &hs, wave);
+        for (int i = 0; i < W; ++i) process_item<NWB>(P, ri, i, ts[0], &hs, &ls, wave);
     }
     // phase 2: light, one wavefront per root
     for (;;) {
@@ -900,7 +1076,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         team_sync<1>();
         if (t >= P.K) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = 0; i < W; ++i) process_item<1>(P, ri, i, ts[wave], nullptr, 0);
+        for (int i = 0; i < W; ++i) process_item<1>(P, ri, i, ts[wave], nullptr, nullptr, 0);
     }
 }
 
