@@ -504,6 +504,9 @@ struct extraArraysPerThread {
     // SBC_ITEM_TRACE after each batched call (diagnostics)
     int64_t* d_tprof = nullptr;
     int64_t tprof_cap = 0;
+    // per-workgroup term buckets of the hub lookup pass
+    SbcBucketEnt* d_bucket = nullptr;
+    int64_t bucket_blocks = 0;
     WinStage win;
     GraphCache gc;
     int refs = 1;
@@ -534,6 +537,7 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_rec) (void)hipFree(C->d_rec);
     if (C->d_order) (void)hipFree(C->d_order);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
+    if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->win.d_desc) (void)hipFree(C->win.d_desc);
     if (C->win.d_ints) (void)hipFree(C->win.d_ints);
     if (C->win.d_vi) (void)hipFree(C->win.d_vi);
@@ -646,6 +650,21 @@ static sbcStatus ensure_rec(extraArraysPerThread* C, int64_t n)
     return SBC_OK;
 }
 
+// Hub lookup scratch for `grid` workgroups (only graphs with lookup tables).
+static const int64_t kBucketHubs = 512; // HUB_CAP of the kernels
+static sbcStatus ensure_bucket(extraArraysPerThread* C, const stinger* G, int64_t grid)
+{
+    if (G->htab.empty() || C->bucket_blocks >= grid) return SBC_OK;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (C->d_bucket) HIPCHK(hipFree(C->d_bucket));
+    C->d_bucket = nullptr;
+    sbcStatus st = dalloc(&C->d_bucket, (size_t)grid * kBucketHubs * SBC_BUCKET);
+    if (st) return st;
+    C->bucket_blocks = grid;
+    C->gc.with_rec = -1; // cached graph holds the old pointer
+    return SBC_OK;
+}
+
 static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArraysPerThread* C)
 {
     SbcParams P;
@@ -655,6 +674,7 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.seq = G->d_seq;
     P.hub = G->d_hub;
     P.htab = G->d_htab;
+    P.bucket = G->htab.empty() ? nullptr : C->d_bucket;
     P.nv = (int32_t)F->nv;
     P.K = (int32_t)F->K;
     P.lcap = F->lcap;
@@ -748,6 +768,7 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
     sbcStatus st = graph_flush(G, C->stream);
     if (st) return st;
     if ((st = ensure_rec(C, F->K))) return st;
+    if ((st = ensure_bucket(C, G, 4LL * C->num_cu))) return st;
     SbcParams P = base_params(F, G, C);
     P.u0 = (int32_t)u;
     P.v0 = (int32_t)v;
@@ -970,6 +991,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         int64_t cap = 2LL * C->num_cu;
         grid = (int)(g < cap ? (g > 0 ? g : 1) : cap);
     }
+    if ((st = ensure_bucket(C, G, grid))) return st;
     // capacity: relayout once up front if any endpoint lacks room for the whole batch
     {
         std::vector<int32_t> extra(G->nv, 0);

@@ -230,10 +230,10 @@ constexpr int HUB_MIN = 512;
 #define SBC_SCAN_UNROLL 2
 #endif
 constexpr int SCAN_UNROLL = SBC_SCAN_UNROLL;
-constexpr int HUB_CAP = 256;   // hubs queued per level pass (more: scanned by one wave)
+constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
-constexpr int PROBE_CAP = 16 * RUN_CAP; // terms of one probed hub (shares the run buffers)
+constexpr int PHASH = 2 * HUB_CAP;      // LDS hash: vertex -> number in the level's lookup list
 
 struct alignas(16) HubShared {
     int n;
@@ -242,13 +242,15 @@ struct alignas(16) HubShared {
     int np;
     int list[HUB_CAP];
     int plist[HUB_CAP];   // ascent-only hubs of the level handled by table lookups
+    int bcnt[HUB_CAP];    // terms collected per lookup hub
+    int phk[PHASH];       // hash keys (vertex + 1, 0 = empty)
+    int phv[PHASH];       // hash values (number in plist)
+    int pidx[HUB_CAP];    // queued hub -> number in plist, or -1
     uint8_t kind[HUB_CAP];
     int runlen[16];
     double radd[16][RUN_CAP];
     double rsub[16][RUN_CAP];
     uint8_t rflag[16][RUN_CAP];
-    int pkey[PROBE_CAP];
-    int pperm[PROBE_CAP];
 };
 
 // Light (non-hub) vertices of one ascent level, batched: every edge of
@@ -538,23 +540,21 @@ struct LevelSets {
 };
 
 // Term of processed neighbour w of ascent-only hub k (number hi in the
-// level's probe list) at position pos of k's list, appended to the LDS term
-// buffer under key (hi, pos).
-__device__ __forceinline__ void probe_term(const SbcParams& P, const Item& X, HubShared& hs, int32_t k, int hi,
-                                           int lev, int32_t w, int pos)
+// level's lookup list) at position pos of k's list, stored in k's bucket.
+__device__ __forceinline__ void probe_term(const SbcParams& P, const Item& X, HubShared& hs, SbcBucketEnt* bk,
+                                           int32_t k, int hi, int lev, int32_t w, int pos)
 {
     bool appU;
     const int64_t sk = X.sigma[k];
     const Term t = ascent_lane(P, X, k, false, sk, sk, lev, true, w, ld_sb(&X.sb[w]), appU);
     if (t.child || t.hsub) {
-        const int slot = atomicAdd(&hs.pn, 1);
-        if (slot < PROBE_CAP) {
-            hs.pkey[slot] = (hi << 22) | pos;
-            (&hs.radd[0][0])[slot] = t.add;
-            (&hs.rsub[0][0])[slot] = t.sub;
-            (&hs.rflag[0][0])[slot] = (uint8_t)((t.child ? 1 : 0) | (t.hsub ? 2 : 0));
-        } else {
-            hs.overflow = 1;
+        const int slot = atomicAdd(&hs.bcnt[hi], 1);
+        if (slot < SBC_BUCKET) {
+            SbcBucketEnt& e = bk[(size_t)hi * SBC_BUCKET + slot];
+            e.key = pos;
+            e.flag = (t.child ? 1 : 0) | (t.hsub ? 2 : 0);
+            e.add = t.add;
+            e.sub = t.sub;
         }
     }
 }
@@ -566,114 +566,192 @@ __device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32
         X.list[atomicAdd(&ts.cnt[2], 1)] = x;
 }
 
-// New deltas of the ascent-only hubs hs.plist[0..np) of level lev (1 or 2)
-// without scanning their adjacency lists.  A hub's terms can only come from
-// processed vertices one level down (children) and moved descent vertices on
-// its own level, so every (hub, candidate) pair is looked up in the hub's
-// table; the appended (inserted) part of each hub's list is checked slot by
-// slot.  The terms are sorted by (hub, position in the hub's list) and each
-// hub sums its own in that order -- the order of the full pull scan, so the
-// result is bit-identical to it.  Parents: on level 1 only the root; on
-// level 2 exactly the root's neighbours adjacent to the hub.
-// Returns false if the term buffer overflowed (nothing written then; the
-// caller scans those hubs, whose parent marks and terms are the same).
-template <int TW>
-__device__ __forceinline__ bool ascent_hubs_probe(const SbcParams& P, const Item& X, int np, int lev,
-                                                  TeamShared& ts, HubShared& hs, const LevelSets& below,
-                                                  const LevelSets& same, const SbcVInfo& vr, int len_r)
+__device__ __forceinline__ int phash_slot(int32_t v) { return (int)(((uint32_t)v * 0x9E3779B1u) >> 21) & (PHASH - 1); }
+
+__device__ __forceinline__ int phash_find(const HubShared& hs, int32_t v)
 {
-    const int tid = threadIdx.x;
-    const int nthr = TW * 64;
-    if (tid == 0) { hs.pn = 0; hs.overflow = 0; }
-    __syncthreads();
-    const int nb = below.size(), nsd = same.dB - same.dA, nc = nb + nsd;
-    for (int t = tid; t < np * nc; t += nthr) {
-        const int hi = t / nc, ci = t - hi * nc;
-        const int32_t k = hs.plist[hi];
-        const int4 hb = P.hub[k];
-        const int32_t w = ci < nb ? below.at(X, ci) : X.list[same.dA + (ci - nb)];
-        const int pos = hub_find(P.htab + hb.x, hb.z, w);
-        if (pos >= 0) probe_term(P, X, hs, k, hi, lev, w, pos);
+    int h = phash_slot(v);
+    for (;;) {
+        const int key = hs.phk[h];
+        if (key == v + 1) return hs.phv[h];
+        if (key == 0) return -1;
+        h = (h + 1) & (PHASH - 1);
     }
+}
+
+// Source vertices of one lookup pass: the processed vertices one level
+// down (candidate children), the descent vertices on the level (candidate
+// same-level neighbours) and, on level 2, the root's neighbours (candidate
+// parents).
+struct ProbeSources {
+    LevelSets below;
+    int dA, dB;          // descent vertices on the level
+    int32_t rstart;      // root adjacency
+    int nr;              // visible root neighbours (0 unless level 2)
+    __device__ int size() const { return below.size() + (dB - dA) + nr; }
+};
+
+// New deltas of the ascent-only hubs hs.plist[0..np) of level lev (1 or 2)
+// without scanning their adjacency lists.  Their terms can only come from
+// processed vertices one level down (children) and moved descent vertices on
+// their own level, and on level 2 their parents are exactly the root's
+// neighbours adjacent to them.  Every source vertex w either scans its own
+// list for lookup hubs (LDS hash) when that list is short, or is looked up in
+// each hub's table; each (w, hub) incidence gives the position of w in the
+// hub's list.  Inserted edges of the hubs are checked slot by slot.  Each
+// hub's terms are sorted by position and summed in that order, the order of
+// the full pull scan, so the result is bit-identical to it.  Hubs with more
+// than SBC_BUCKET terms are marked kind 2 and left to the caller, which scans
+// them (parent marks are idempotent).
+template <int TW>
+__device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item& X, int np, int lev,
+                                                  TeamShared& ts, HubShared& hs, LightShared& ls,
+                                                  const ProbeSources& src)
+{
+    const int tid = threadIdx.x, lane = lane_id(), twave = tid >> 6;
+    constexpr int NT = TW * 64;
+    SbcBucketEnt* bk = P.bucket + (size_t)blockIdx.x * HUB_CAP * SBC_BUCKET;
+    for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
+    for (int i = tid; i < np; i += NT) hs.bcnt[i] = 0;
+    __syncthreads();
+    for (int hi = tid; hi < np; hi += NT) {
+        const int32_t k = hs.plist[hi];
+        int h = phash_slot(k);
+        while (atomicCAS(&hs.phk[h], 0, k + 1) != 0) h = (h + 1) & (PHASH - 1);
+        hs.phv[h] = hi;
+    }
+    __syncthreads();
+    const int nb = src.below.size(), ndl = src.dB - src.dA;
+    const int nsrc = src.size();
+    for (int r0 = 0; r0 < nsrc; r0 += NT) {
+        // one source per thread; short lists are scanned, long ones looked up
+        const int i = r0 + tid;
+        int32_t w = 0;
+        int len = 0;
+        int start = 0;
+        bool scan_mode = false, table_mode = false;
+        if (i < nsrc) {
+            if (i < nb) w = src.below.at(X, i);
+            else if (i < nb + ndl) w = X.list[src.dA + (i - nb)];
+            else w = P.col[src.rstart + (i - nb - ndl)];
+            const SbcVInfo vi = P.vinfo[w];
+            len = visible_len_thread(P, vi, X.q);
+            start = vi.start;
+            scan_mode = len <= 2 * np;
+            table_mode = !scan_mode;
+        }
+        int ne, nt;
+        const int eoff = block_exscan<TW>(scan_mode ? len : 0, ls.scan, ne);
+        const int toff = block_exscan<TW>(table_mode ? 1 : 0, ls.scan, nt);
+        if (scan_mode) {
+            ls.k[tid] = w;
+            ls.start[tid] = start;
+            ls.off[tid] = eoff;
+            ls.isD[tid] = (i >= nb + ndl) ? 1 : 0;  // 1: parent candidate
+        }
+        if (table_mode) {
+            ls.ovf[toff] = w;
+            ls.eflag[toff] = (i >= nb + ndl) ? 1 : 0;
+        }
+        // compact scan-mode slots in thread order: slot = number of scan-mode threads before
+        int nsc;
+        const int sslot = block_exscan<TW>(scan_mode ? 1 : 0, ls.scan, nsc);
+        if (scan_mode) {
+            ls.len[sslot] = tid;  // owner thread of slot
+        }
+        __syncthreads();
+        // edges of scan-mode sources
+        for (int e = tid; e < ne; e += NT) {
+            int lo = 0, hi = nsc;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (ls.off[ls.len[mid]] <= e) lo = mid;
+                else hi = mid;
+            }
+            const int own = ls.len[lo];
+            const int32_t ws = ls.k[own];
+            const int32_t c = P.col[ls.start[own] + (e - ls.off[own])];
+            const int hi2 = phash_find(hs, c);
+            if (hi2 < 0) continue;
+            if (ls.isD[own]) {
+                mark_parent(X, ts, ws, 1);
+            } else {
+                const int32_t k = hs.plist[hi2];
+                const int4 hb = P.hub[k];
+                const int pos = hub_find(P.htab + hb.x, hb.z, ws);
+                if (pos >= 0) probe_term(P, X, hs, bk, k, hi2, lev, ws, pos);
+            }
+        }
+        // table-mode sources against every hub
+        for (int t = tid; t < nt * np; t += NT) {
+            const int si = t / np, hi2 = t - si * np;
+            const int32_t ws = ls.ovf[si];
+            const int32_t k = hs.plist[hi2];
+            const int4 hb = P.hub[k];
+            const int pos = hub_find(P.htab + hb.x, hb.z, ws);
+            if (pos < 0) continue;
+            if (ls.eflag[si]) mark_parent(X, ts, ws, 1);
+            else probe_term(P, X, hs, bk, k, hi2, lev, ws, pos);
+        }
+        __syncthreads();
+    }
+    // inserted edges of each hub, slot by slot
     for (int hi = 0; hi < np; ++hi) {
         const int32_t k = hs.plist[hi];
         const SbcVInfo vi = P.vinfo[k];
         const int len = visible_len(P, vi, X.q);
-        for (int e = P.hub[k].y + tid; e < len; e += nthr) {
+        for (int e = P.hub[k].y + tid; e < len; e += NT) {
             const int32_t c = P.col[vi.start + e];
             const uint8_t b = ld_sb(&X.sb[c]);
             if (sb_type(b) == 0) {
                 if (!(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
             } else {
-                probe_term(P, X, hs, k, hi, lev, c, e);
+                probe_term(P, X, hs, bk, k, hi, lev, c, e);
             }
         }
     }
-    if (lev == 1) {
-        if (tid == 0) mark_parent(X, ts, X.r, 0);
-    } else {
-        for (int t = tid; t < np * len_r; t += nthr) {
-            const int hi = t / len_r, i = t - hi * len_r;
-            const int4 hb = P.hub[hs.plist[hi]];
-            const int32_t x = P.col[vr.start + i];
-            if (hub_find(P.htab + hb.x, hb.z, x) >= 0) mark_parent(X, ts, x, 1);
-        }
-    }
+    if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
     __syncthreads();
-    const int n = hs.pn;
-    const bool ovf = hs.overflow != 0;
-    if (!ovf) {
-        int N = 1;
-        while (N < n) N <<= 1;
-        for (int i = tid; i < N; i += nthr) {
-            if (i >= n) hs.pkey[i] = INF32;
-            hs.pperm[i] = i;
+    // one wavefront per hub: sort its bucket by position, sum in that order
+    for (int hi = twave; hi < np; hi += TW) {
+        const int n = hs.bcnt[hi];
+        if (n > SBC_BUCKET) {
+            if (lane == 0) hs.kind[hi] = 2;
+            continue;
         }
-        __syncthreads();
-        for (int k2 = 2; k2 <= N; k2 <<= 1) {
+        const int32_t k = hs.plist[hi];
+        const SbcBucketEnt* b = bk + (size_t)hi * SBC_BUCKET;
+        int key = lane < n ? b[lane].key : INF32;
+        int idx = lane;
+        for (int k2 = 2; k2 <= 64; k2 <<= 1) {
             for (int j = k2 >> 1; j > 0; j >>= 1) {
-                for (int i = tid; i < N; i += nthr) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const bool up = (i & k2) == 0;
-                        const int ki = hs.pkey[i], kj = hs.pkey[ixj];
-                        if ((ki > kj) == up) {
-                            hs.pkey[i] = kj;
-                            hs.pkey[ixj] = ki;
-                            const int tmp = hs.pperm[i];
-                            hs.pperm[i] = hs.pperm[ixj];
-                            hs.pperm[ixj] = tmp;
-                        }
-                    }
-                }
-                __syncthreads();
+                const int pk = __shfl_xor(key, j), pi = __shfl_xor(idx, j);
+                const bool up = (lane & k2) == 0;
+                const bool lower = (lane & j) == 0;
+                const bool take = lower ? ((pk < key) == up) : ((pk > key) == up);
+                if (take) { key = pk; idx = pi; }
             }
         }
-        // segmented, ordered sums: thread hi owns hub hi
-        for (int hi = tid; hi < np; hi += nthr) {
-            const int32_t k = hs.plist[hi];
-            int lo = 0, hi2 = n;
-            while (lo < hi2) {
-                const int mid = (lo + hi2) >> 1;
-                if ((hs.pkey[mid] >> 22) < hi) lo = mid + 1;
-                else hi2 = mid;
-            }
-            double acc = X.delta[k];
-            for (int i = lo; i < n && (hs.pkey[i] >> 22) == hi; ++i) {
-                const int j = hs.pperm[i];
-                const uint8_t fl = (&hs.rflag[0][0])[j];
-                if (fl & 1) acc = acc + (&hs.radd[0][0])[j];
-                if (fl & 2) acc = acc - (&hs.rsub[0][0])[j];
-            }
-            st_reld(&X.ndel[k], acc);
+        uint8_t fl = 0;
+        double ad = 0.0, sb2 = 0.0;
+        if (lane < n) {
+            fl = b[idx].flag;
+            ad = b[idx].add;
+            sb2 = b[idx].sub;
         }
+        const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
+        uint64_t m = cm | sm;
+        double acc = X.delta[k];
+        while (m) {
+            const int l2 = __builtin_ctzll(m);
+            m &= m - 1;
+            if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
+            if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
+        }
+        if (lane == 0) st_reld(&X.ndel[k], acc);
     }
     __syncthreads();
-    if (tid == 0) hs.overflow = 0;
-    __syncthreads();
-    return !ovf;
 }
-
 
 // All vertices of ascent level `here`: hubs are queued in hs.list for the
 // hub passes; light vertices are batched (see LightShared); vertices that do
@@ -951,32 +1029,43 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
             __syncthreads();
             const int nh = min(hs->n, HUB_CAP);
             // split the queued hubs: ascent-only indexed hubs on level 1 or 2
-            // whose lookups are much cheaper than their lists are handled
+            // whose lists are much longer than the lookup work are handled
             // together by table lookups; the rest are scanned one by one
             if (threadIdx.x == 0) hs->np = 0;
             __syncthreads();
-            const long cand = (long)below.size() + (here.dB - here.dA) + (lev == 2 ? len_r : 0);
+            ProbeSources src;
+            src.below = below;
+            src.dA = here.dA;
+            src.dB = here.dB;
+            src.rstart = vr.start;
+            src.nr = lev == 2 ? len_r : 0;
+            const long cand = (long)src.size();
             for (int h = threadIdx.x; h < nh; h += TW * 64) {
                 const int32_t k = hs->list[h];
                 bool pk = false;
-                if (P.hub && lev >= 1 && lev <= 2 && P.hub[k].y > 0 && sb_type(ld_sb(&X.sb[k])) == 3) {
-                    const int4 hb = P.hub[k];
-                    pk = cand * 4 < (long)hb.y;
+                if (P.hub && P.bucket && lev >= 1 && lev <= 2 && P.hub[k].y > 0 && sb_type(ld_sb(&X.sb[k])) == 3)
+                    pk = cand < (long)P.hub[k].y;
+                int hi = -1;
+                if (pk) {
+                    hi = atomicAdd(&hs->np, 1);
+                    hs->plist[hi] = k;
+                    hs->kind[hi] = 0;
                 }
-                hs->kind[h] = pk ? 1 : 0;
-                if (pk) hs->plist[atomicAdd(&hs->np, 1)] = k;
+                hs->pidx[h] = hi;
             }
             __syncthreads();
             const int np = hs->np;
-            bool probed = false;
-            if (np > 0) probed = ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, below, here, vr, len_r);
+            if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src);
+            // hubs not resolved by lookups: every queued hub that is not in
+            // plist, and plist hubs whose bucket overflowed (kind 2)
             for (int h = 0; h < nh; ++h) {
                 const int32_t k = hs->list[h];
+                const int hi = hs->pidx[h];
+                const bool looked_up = hi >= 0 && hs->kind[hi] != 2;
                 const SbcVInfo vi = P.vinfo[k];
                 const int len = visible_len(P, vi, X.q);
-                const bool by_lookup = probed && hs->kind[h];
-                if (!by_lookup) ascent_vertex_team<TW>(P, X, k, vi, len, lev, ts, *hs, twave);
-                if (tp && leader) { tp[by_lookup ? 4 : 5] += 1; tp[by_lookup ? 6 : 7] += len; }
+                if (!looked_up) ascent_vertex_team<TW>(P, X, k, vi, len, lev, ts, *hs, twave);
+                if (tp && leader) { tp[looked_up ? 4 : 5] += 1; tp[looked_up ? 6 : 7] += len; }
                 if (twave == 0) {
                     my_proc += 1;
                     my_scan += (unsigned long long)len;
