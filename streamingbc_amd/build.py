@@ -73,6 +73,12 @@ def build_library() -> str:
     tmp = out + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, out)
+    # the C driver (a reference-style caller of the C ABI)
+    exe = os.path.join(LIBDIR, "sbc_stream")
+    _run(["gcc", "-std=gnu99", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "sbc_stream_main.c"), "-o", exe + ".tmp",
+          "-L" + LIBDIR, "-lstreamingbc_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -244,3 +244,29 @@ def test_hub_lookup_path_rmat14(batched):
         assert np.array_equal(rec_g["scanned"], rec_o["scanned"]), f"insertion {i}"
     _compare_trees(f, o, "rmat14 hubs")
     assert_bc_close(f.bc(), o.bc())
+
+
+@pytest.mark.parametrize("window", [0, 4])
+def test_c_driver_matches_oracle(window):
+    """The C driver (a reference-style caller of include/streamingbc_hip.h)
+    gives the oracle's BC on the same graph, roots and stream."""
+    import json
+    import os
+    import subprocess
+    from sbc_testutil import GOLDEN, ROOT
+    exe = os.path.join(ROOT, "streamingbc_amd", "lib", "sbc_stream")
+    path = os.path.join(GOLDEN, "graphs", "jazz.graph")
+    out = subprocess.run([exe, "-g", path, "-e", "24", "-s", "3", "-w", str(window)], check=True,
+                         capture_output=True, text=True, timeout=120)
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    hg = sbc.read_dimacs(path)
+    roots = np.arange(hg.nv)
+    u, v = hg.insertions(24, 13)  # the driver draws its stream with seed + 10
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    for a, b in zip(u, v):
+        o.insert(int(a), int(b), O_FINE)
+    bo = o.bc()
+    assert res["insertions"] == 24
+    assert abs(res["bc_sum"] - bo.sum()) <= 1e-9 * abs(bo.sum())
+    assert abs(res["bc_max"] - bo.max()) <= 1e-9 * abs(bo.max())
