@@ -121,6 +121,18 @@ _sig("streamingBCKernelStats", C.c_int, [VPP, i32, P64, PD])
 _sig("streamingBCResetStats", C.c_int, [VPP, i32])
 _sig("streamingBCForestFootprint", i64, [i64, i64])
 _sig("streamingBCSynchronize", C.c_int, [VPP])
+# cost model (include/sbc_costmodel.h)
+PF = C.POINTER(C.c_float)
+_sig("sbcCostModelNumParams", i32, [])
+_sig("sbcCostModelCreate", C.c_int, [C.c_uint64, VPP])
+_sig("sbcCostModelFree", None, [VP])
+_sig("sbcCostModelGetParams", C.c_int, [VP, PF])
+_sig("sbcCostModelSetParams", C.c_int, [VP, PF])
+_sig("sbcCostModelTrain", C.c_int, [VP, PF, PF, i64, i32, i32, C.c_float, C.c_uint64, PF])
+_sig("sbcCostModelGradients", C.c_int, [VP, PF, PF, i64, PF, PF])
+_sig("sbcCostModelPredict", C.c_int, [VP, PF, i64, PF])
+_sig("sbcCostModelPredictFP32", C.c_int, [VP, PF, i64, PF])
+_sig("streamingBCWindowFeatures", C.c_int, [FP, VP, VPP, P64, P64, i64, PF, PF])
 
 
 def _check(st):
@@ -375,3 +387,97 @@ class Forest:
         for ri in range(self.k):
             _check(lib.streamingBCGetTree(self._p, ri, _p64(lv[ri]), _p64(sg[ri]), _pd(dl[ri])))
         return lv, sg, dl
+
+
+# ----------------------------------------------------------------- cost model
+
+COST_FEATURES = 12
+
+
+def _pf(a):
+    return a.ctypes.data_as(PF)
+
+
+def _f32(a, shape_tail=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape_tail is not None and a.shape[-1] != shape_tail:
+        raise ValueError(f"expected trailing dimension {shape_tail}, got {a.shape}")
+    return a
+
+
+class CostModel:
+    """The 12-64-64-1 MLP cost model (sbcCostModel): FP32 training kernels,
+    BF16-MFMA inference.  Predicts log1p(affected vertices) of an item."""
+
+    def __init__(self, seed: int = 1):
+        p = C.c_void_p()
+        _check(lib.sbcCostModelCreate(seed, C.byref(p)))
+        self._p = p
+        self.n_params = int(lib.sbcCostModelNumParams())
+
+    def close(self):
+        if self._p:
+            lib.sbcCostModelFree(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib.sbcCostModelGetParams(self._p, _pf(out)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p)
+        if p.shape != (self.n_params,):
+            raise ValueError(f"expected {self.n_params} parameters")
+        _check(lib.sbcCostModelSetParams(self._p, _pf(p)))
+
+    def train(self, X, y, epochs: int = 10, batch: int = 512, lr: float = 1e-3, seed: int = 1) -> float:
+        X = _f32(X, COST_FEATURES).reshape(-1, COST_FEATURES)
+        y = _f32(y).reshape(-1)
+        if X.shape[0] != y.shape[0]:
+            raise ValueError("X and y disagree in length")
+        loss = C.c_float(0.0)
+        _check(lib.sbcCostModelTrain(self._p, _pf(X), _pf(y), X.shape[0], epochs, batch, lr, seed, C.byref(loss)))
+        return float(loss.value)
+
+    def gradients(self, X, y):
+        X = _f32(X, COST_FEATURES).reshape(-1, COST_FEATURES)
+        y = _f32(y).reshape(-1)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_float(0.0)
+        _check(lib.sbcCostModelGradients(self._p, _pf(X), _pf(y), X.shape[0], _pf(g), C.byref(loss)))
+        return g, float(loss.value)
+
+    def predict(self, X, mfma: bool = True) -> np.ndarray:
+        X = _f32(X, COST_FEATURES)
+        shape = X.shape[:-1]
+        X = X.reshape(-1, COST_FEATURES)
+        out = np.zeros(X.shape[0], dtype=np.float32)
+        f = lib.sbcCostModelPredict if mfma else lib.sbcCostModelPredictFP32
+        _check(f(self._p, _pf(X), X.shape[0], _pf(out)))
+        return out.reshape(shape)
+
+
+def window_features(forest: Forest, g: Graph, ctx: Context, u, v, root_hist=None) -> np.ndarray:
+    """Cost-model features of every (insertion, root) item of a window, from
+    the forest's current state: array (n, K, 12)."""
+    u = np.ascontiguousarray(u, dtype=np.int64)
+    v = np.ascontiguousarray(v, dtype=np.int64)
+    n = u.shape[0]
+    X = np.zeros((n, forest.k, COST_FEATURES), dtype=np.float32)
+    hp = None
+    if root_hist is not None:
+        h = _f32(root_hist)
+        if h.shape != (forest.k,):
+            raise ValueError("root_hist needs one value per root")
+        hp = _pf(h)
+    _check(lib.streamingBCWindowFeatures(forest._p, g._p, ctx._p, _p64(u), _p64(v), n, hp, _pf(X)))
+    return X

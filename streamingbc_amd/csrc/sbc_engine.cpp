@@ -17,6 +17,7 @@
 #include <unordered_set>
 #include <vector>
 
+#include "sbc_costmodel.h"
 #include "sbc_graphgen.h"
 #include "sbc_kernels.h"
 #include "streamingbc_hip.h"
@@ -1198,4 +1199,52 @@ extern "C" sbcStatus sbcInsertEdge(bcForest* forest, struct stinger* G, extraArr
     stinger_insert_edge(G, 0, u, v, 0, 0);
     stinger_insert_edge(G, 0, v, u, 0, 0);
     return run_insert_plain(forest, G, eAPT[0], u, v, granularity);
+}
+
+// ------------------------------------------------------------ cost-model features
+
+extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT,
+                                               const int64_t* u, const int64_t* v, int64_t count,
+                                               const float* root_hist, float* X)
+{
+    if (!forest || !G || !eAPT || !eAPT[0] || !X || (count > 0 && (!u || !v)) || count < 0 || count > (1 << 20))
+        return set_err(SBC_ERR_ARG, "bad argument");
+    extraArraysPerThread* C = eAPT[0];
+    sbcForestDev* F = forest->dev;
+    if (!F->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (count == 0) return SBC_OK;
+    sbcStatus st = graph_flush(G, C->stream);
+    if (st) return st;
+    std::vector<int32_t> uv(2 * count);
+    for (int64_t i = 0; i < count; i++) {
+        if (u[i] < 0 || v[i] < 0 || u[i] >= F->nv || v[i] >= F->nv) return set_err(SBC_ERR_ARG, "vertex out of range");
+        uv[i] = (int32_t)u[i];
+        uv[count + i] = (int32_t)v[i];
+    }
+    int32_t* d_uv = nullptr;
+    float *d_hist = nullptr, *d_X = nullptr;
+    const size_t nx = (size_t)count * F->K * SBC_COST_FEATURES;
+    if ((st = dalloc(&d_uv, 2 * count)) || (st = dalloc(&d_X, nx))) return st;
+    if (root_hist && (st = dalloc(&d_hist, F->K))) return st;
+    HIPCHK(hipMemcpyAsync(d_uv, uv.data(), sizeof(int32_t) * 2 * count, hipMemcpyHostToDevice, C->stream));
+    if (root_hist) HIPCHK(hipMemcpyAsync(d_hist, root_hist, sizeof(float) * F->K, hipMemcpyHostToDevice, C->stream));
+    SbcFeatArgs A;
+    A.vinfo = G->d_vinfo;
+    A.dist = F->d_dist;
+    A.sigma = F->d_sigma;
+    A.delta = F->d_delta;
+    A.u = d_uv;
+    A.v = d_uv + count;
+    A.hist = root_hist ? d_hist : nullptr;
+    A.X = d_X;
+    A.nv = (int32_t)F->nv;
+    A.K = (int32_t)F->K;
+    A.count = (int32_t)count;
+    HIPCHK(sbc_launch_features(&A, C->stream));
+    HIPCHK(hipMemcpyAsync(X, d_X, sizeof(float) * nx, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    (void)hipFree(d_uv);
+    (void)hipFree(d_X);
+    if (d_hist) (void)hipFree(d_hist);
+    return SBC_OK;
 }

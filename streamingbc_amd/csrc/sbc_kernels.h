@@ -105,7 +105,23 @@ struct SbcAppend {
     const SbcVInfo* d_vi;
 };
 
+// Arguments of the cost-model feature kernel (sbc_mlp.hip).
+struct SbcFeatArgs {
+    const SbcVInfo* vinfo;
+    const int32_t* dist;
+    const int64_t* sigma;
+    const double* delta;
+    const int32_t* u;
+    const int32_t* v;
+    const float* hist;
+    float* X;
+    int32_t nv;
+    int32_t K;
+    int32_t count;
+};
+
 extern "C" {
+hipError_t sbc_launch_features(const SbcFeatArgs* A, hipStream_t st);
 hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);

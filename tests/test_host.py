@@ -100,7 +100,7 @@ def _declared_functions(header):
     return [n for n in names if n not in ("if", "sizeof")]
 
 
-@pytest.mark.parametrize("header", ["streamingbc_hip.h", "sbc_graphgen.h"])
+@pytest.mark.parametrize("header", ["streamingbc_hip.h", "sbc_graphgen.h", "sbc_costmodel.h"])
 def test_library_exports_declared_symbols(header):
     lib = ctypes.CDLL(sbc.LIB_PATH)
     names = _declared_functions(os.path.join(ROOT, "include", header))
