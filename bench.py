@@ -101,6 +101,12 @@ def main():
     ap.add_argument("--plain", type=int, default=64, help="insertions timed in the plain form")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference update time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", default="static", choices=["static", "mlp"],
+                    help="root order per window: identity, or by cost-model prediction (trained on --train-windows)")
+    ap.add_argument("--train-windows", type=int, default=8, help="windows of training data for --schedule mlp")
+    ap.add_argument("--light-max", type=float, default=0.0,
+                    help="--schedule mlp: roots predicted below this many affected vertices per window run light")
+    ap.add_argument("--autotune", action="store_true", help="tune block size on extra warmup windows first")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (PMC traffic calibration)")
     args = ap.parse_args()
@@ -127,7 +133,10 @@ def main():
     kind, kw, K, W, desc = WORKLOADS[args.workload]
     if args.window:
         W = args.window
-    n_ins = (args.warmup + args.steps) * W + args.plain
+    from streamingbc_amd.autotune import DEFAULT_CANDIDATES
+    n_tune = 2 * len(DEFAULT_CANDIDATES) if args.autotune else 0
+    n_train = args.train_windows if args.schedule == "mlp" else 0
+    n_ins = (n_tune + n_train + args.warmup + args.steps) * W + args.plain
     hg, roots, u, v, K, _, desc, t_gen = make_workload(args.workload, n_ins)
     print(f"[bench] {args.workload}: nv={hg.nv} directed slots={hg.ne} K={K} W={W} "
           f"insertions={len(u)} gen={t_gen:.1f}s", file=sys.stderr, flush=True)
@@ -143,11 +152,34 @@ def main():
     cfg = sbc.BatchConfig(window=W, block_threads=args.block or 1024, use_graph=not args.no_graph)
     if args.heavy >= -1:
         cfg.heavy_roots = args.heavy
-
-    # warmup windows
     pos = 0
-    for _ in range(args.warmup):
-        f.insert_batched(g, ctx, u[pos:pos + W], v[pos:pos + W], cfg)
+    tune = None
+    if args.autotune:
+        from streamingbc_amd.autotune import autotune
+        res, used = autotune(f, g, ctx, u, v, W, hg.ne, base=cfg)
+        pos += used
+        cfg.block_threads, cfg.grid_blocks = res.block_threads, res.grid_blocks
+        tune = {"block_threads": res.block_threads, "grid_blocks": res.grid_blocks, "table": res.table}
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
+
+    from streamingbc_amd.schedule import LearnedSchedule, StaticSchedule
+    if args.schedule == "mlp":
+        sched = LearnedSchedule(train_after=max(1, n_train), light_max=args.light_max)
+    else:
+        sched = StaticSchedule()
+
+    def window(a, b, records=False):
+        plan, X = sched.plan(f, g, ctx, u[a:b], v[a:b])
+        c = sbc.BatchConfig(window=W, block_threads=cfg.block_threads, grid_blocks=cfg.grid_blocks,
+                            use_graph=cfg.use_graph, heavy_roots=cfg.heavy_roots if plan.heavy_roots < 0
+                            else plan.heavy_roots, root_order=plan.root_order)
+        rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or X is not None)
+        sched.observe(plan, X, rec)
+        return rec
+
+    # training windows (cost model) and warmup windows, untimed
+    for _ in range(n_train + args.warmup):
+        window(pos, pos + W)
         pos += W
     ctx.synchronize()
 
@@ -159,7 +191,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        recs.append(f.insert_batched(g, ctx, u[pos:pos + W], v[pos:pos + W], cfg, records=True))
+        recs.append(window(pos, pos + W, records=True))
         lat.append(time.perf_counter() - ts)
         pos += W
     ctx.synchronize()
@@ -224,6 +256,8 @@ def main():
                   "mean_affected": float(rec["affected"].mean()), "max_affected": int(rec["affected"].max()),
                   "mean_scanned": float(rec["scanned"].mean())},
         "static_pass_s": round(t_static, 3),
+        "schedule": {"name": sched.name, **(sched.report() if args.schedule == "mlp" else {})},
+        "autotune": tune,
     }
 
     if not args.no_cpu_baseline:

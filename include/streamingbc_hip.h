@@ -213,6 +213,12 @@ sbcStatus streamingBCResetStats(extraArraysPerThread **eAPT, int32_t enable_timi
  * (state, per-root scratch and lists). */
 int64_t streamingBCForestFootprint(int64_t NV, int64_t NK);
 
+/* Bytes of device memory one resident graph needs: its forest, the device
+ * CSR with append slack and hub hash tables, and the windowed update's
+ * scratch (num_cu <= 0: 256 compute units).  Used for admission control
+ * (SBC_ERR_BUDGET).  No counterpart in the reference. */
+int64_t streamingBCResidentFootprint(const int64_t *rowptr, int64_t NV, int64_t NK, int32_t num_cu);
+
 /* Blocks until all work queued on the context's stream has finished. */
 sbcStatus streamingBCSynchronize(extraArraysPerThread **eAPT);
 

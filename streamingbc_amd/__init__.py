@@ -3,7 +3,7 @@
 Python bindings (ctypes) over the C ABI of ``lib/libstreamingbc_hip.so``
 (declared in ``include/streamingbc_hip.h`` and ``include/sbc_graphgen.h``).
 The library is the only compute path: importing this package fails loudly if
-it has not been built (``python -m streamingbc_amd.build``), and every device
+it has not been built (``python streamingbc_amd/build.py``), and every device
 call raises :class:`SbcError` if the HIP runtime reports an error.
 
 Names follow the reference's C interface (src/streamingbc.h:40-64):
@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libstreamingbc_hip.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        f"{LIB_PATH} is missing: build it with `python -m streamingbc_amd.build` "
+        f"{LIB_PATH} is missing: build it with `python streamingbc_amd/build.py` "
         "(the HIP library is the only compute path)")
 
 lib = C.CDLL(LIB_PATH)
@@ -120,6 +120,7 @@ _sig("streamingBCLastRecords", C.c_int, [VPP, C.POINTER(ItemRecord), i64])
 _sig("streamingBCKernelStats", C.c_int, [VPP, i32, P64, PD])
 _sig("streamingBCResetStats", C.c_int, [VPP, i32])
 _sig("streamingBCForestFootprint", i64, [i64, i64])
+_sig("streamingBCResidentFootprint", i64, [P64, i64, i64, i32])
 _sig("streamingBCSynchronize", C.c_int, [VPP])
 # cost model (include/sbc_costmodel.h)
 PF = C.POINTER(C.c_float)
@@ -299,6 +300,11 @@ class Graph:
 
 def forest_footprint(nv: int, k: int) -> int:
     return int(lib.streamingBCForestFootprint(nv, k))
+
+
+def resident_footprint(hg: HostGraph, k: int, num_cu: int = 0) -> int:
+    """Device bytes for a resident graph: forest + device CSR + scratch."""
+    return int(lib.streamingBCResidentFootprint(_p64(hg.rowptr), hg.nv, k, num_cu))
 
 
 @dataclass

@@ -1,6 +1,7 @@
 """Build recipe for libstreamingbc_hip.so (gfx950) and the CPU oracle.
 
-Run as ``python -m streamingbc_amd.build`` or through ``__graft_entry__.build()``.
+Run as ``python streamingbc_amd/build.py`` or through ``__graft_entry__.build()`` (both
+work before the library exists; importing the package needs it).
 Always recompiles.  Outputs:
 
 * ``streamingbc_amd/lib/libstreamingbc_hip.so`` -- kernels, host runtime and

@@ -85,6 +85,7 @@ struct stinger {
 // Vertices with more initial neighbours than this get a sorted index (the
 // update kernel looks neighbours up in it instead of scanning hub lists).
 static const int32_t kHubIndexMin = 512;
+static const int64_t kBucketHubs = 512; // HUB_CAP of the kernels
 
 static int32_t slack_for(int32_t d) { return 4 + d / 16; }
 
@@ -356,6 +357,26 @@ extern "C" int64_t streamingBCForestFootprint(int64_t NV, int64_t NK)
 {
     int64_t per = 4 + 1 + 8 + 8 + 8 + 8 + 4; // dist, state byte, sigma, delta, dpath, ndel, list
     return NK * NV * per + NK * 2 * (int64_t)lcap_for(NV) * 4 + NV * 8 + NK * 4 + 64;
+}
+
+extern "C" int64_t streamingBCResidentFootprint(const int64_t* rowptr, int64_t NV, int64_t NK, int32_t num_cu)
+{
+    if (!rowptr || NV <= 0 || NK <= 0) return -1;
+    if (num_cu <= 0) num_cu = 256;
+    int64_t slots = 0, htab = 0;
+    for (int64_t v = 0; v < NV; v++) {
+        const int64_t d = rowptr[v + 1] - rowptr[v];
+        slots += d + slack_for((int32_t)d);
+        if (d > kHubIndexMin && d < (1 << 22)) {
+            int64_t t = 2;
+            while (t < 2 * d) t <<= 1;
+            htab += t;
+        }
+    }
+    int64_t graph = slots * 8 + NV * (int64_t)sizeof(SbcVInfo) + NV * 16 + htab * 8;
+    // windowed-update scratch: hub term buckets for up to 4 workgroups per CU
+    int64_t scratch = htab ? 4LL * num_cu * kBucketHubs * SBC_BUCKET * (int64_t)sizeof(SbcBucketEnt) : 0;
+    return streamingBCForestFootprint(NV, NK) + graph + scratch;
 }
 
 static void forest_free_dev(sbcForestDev* F)
@@ -652,7 +673,6 @@ static sbcStatus ensure_rec(extraArraysPerThread* C, int64_t n)
 }
 
 // Hub lookup scratch for `grid` workgroups (only graphs with lookup tables).
-static const int64_t kBucketHubs = 512; // HUB_CAP of the kernels
 static sbcStatus ensure_bucket(extraArraysPerThread* C, const stinger* G, int64_t grid)
 {
     if (G->htab.empty() || C->bucket_blocks >= grid) return SBC_OK;

@@ -1,0 +1,67 @@
+"""Autotuner: the cache (host logic, no GPU) and, on the GPU, that tuning
+windows are real work whose results match an untuned run bit for bit."""
+import json
+
+import numpy as np
+import pytest
+
+sbc = pytest.importorskip("streamingbc_amd")
+from streamingbc_amd.autotune import DEFAULT_CANDIDATES, _store, autotune, cache_key  # noqa: E402
+
+
+class _NoDevice:
+    """Stands in for a forest: a cache hit must not touch the device."""
+    nv, k = 1000, 64
+
+    def insert_batched(self, *a, **k):
+        raise AssertionError("device touched on a cache hit")
+
+
+def test_cache_hit_skips_the_sweep(tmp_path):
+    path = str(tmp_path / "tune.json")
+    key = cache_key(1000, 5000, 64, 32)
+    _store(path, key, {"block_threads": 512, "grid_blocks": 0, "table": []})
+    _store(path, cache_key(1, 2, 3, 4), {"block_threads": 256, "grid_blocks": 7})
+    data = json.load(open(path))
+    assert set(data) == {key, cache_key(1, 2, 3, 4)}
+    res, used = autotune(_NoDevice(), None, None, np.zeros(64), np.zeros(64), 32, 5000, cache=path)
+    assert used == 0 and res.from_cache and res.block_threads == 512
+    cfg = res.config(32, recompute_bc=True)
+    assert cfg.window == 32 and cfg.block_threads == 512 and cfg.recompute_bc
+
+
+def test_candidates_are_valid_block_sizes():
+    for c in DEFAULT_CANDIDATES:
+        assert c["block_threads"] in (64, 128, 256, 512, 1024) and c["grid_blocks"] >= 0
+
+
+@pytest.mark.gpu
+def test_tuning_windows_are_real_work(tmp_path):
+    hg = sbc.gen_rmat(12, 16 << 12, seed=3)
+    roots = hg.pick_roots(64, 2)
+    W = 8
+    u, v = hg.insertions(W * 10, 5)
+
+    def fresh():
+        ctx = sbc.Context(hg.nv)
+        g = sbc.Graph(hg)
+        f = sbc.Forest(hg.nv, roots)
+        f.init_static(g, ctx)
+        return ctx, g, f
+
+    ctx, g, f = fresh()
+    path = str(tmp_path / "tune.json")
+    res, used = autotune(f, g, ctx, u, v, W, hg.ne, reps=2, cache=path)
+    assert used == 2 * len(DEFAULT_CANDIDATES) * W and not res.from_cache
+    assert len(res.table) == len(DEFAULT_CANDIDATES) and all(r["ms_per_window"] > 0 for r in res.table)
+    f.insert_batched(g, ctx, u[used:], v[used:], res.config(W))
+    f.recompute_bc(ctx)
+    ctx2, g2, f2 = fresh()
+    f2.insert_batched(g2, ctx2, u, v, sbc.BatchConfig(window=W, block_threads=256))
+    f2.recompute_bc(ctx2)
+    for a, b in zip(f.trees(), f2.trees()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(f.bc(), f2.bc())
+    # second call: served from the cache
+    res2, used2 = autotune(f, g, ctx, u, v, W, hg.ne, cache=path)
+    assert res2.from_cache and used2 == 0 and res2.block_threads == res.block_threads
