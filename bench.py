@@ -5,7 +5,10 @@ Default workload (the headline configuration): undirected R-MAT scale 20
 (2^20 vertices, 16 edge draws per vertex, a/b/c/d = .57/.19/.19/.05,
 deduplicated, seed 1), K = 256 roots drawn from the largest component (seed
 7), random non-adjacent insertions inside it (seed 11), applied in windows of
-128 by the batched persistent kernel.  One step = one window.
+128 by the batched persistent kernel.  One step = one window.  The stream is
+16,384 insertions: 30 untimed leading windows (the cost model's training
+data under --schedule mlp), 2 warmup windows, 96 timed windows (12,288
+insertions), then 64 insertions in the plain form.
 
 Printed JSON line (rank 0):
   value         insertions/s of the batched form over the timed steps (inputs
@@ -91,7 +94,7 @@ def cpu_baseline(hg, roots, u, v, budget_s, threads, gran):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=96)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat20_k256_w128", choices=sorted(WORKLOADS))
     ap.add_argument("--window", type=int, default=0)
@@ -103,7 +106,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", default="static", choices=["static", "mlp"],
                     help="root order per window: identity, or by cost-model prediction (trained on --train-windows)")
-    ap.add_argument("--train-windows", type=int, default=8, help="windows of training data for --schedule mlp")
+    ap.add_argument("--train-windows", type=int, default=30,
+                    help="untimed leading windows (the cost model's training data under --schedule mlp)")
     ap.add_argument("--light-max", type=float, default=0.0,
                     help="--schedule mlp: roots predicted below this many affected vertices per window run light")
     ap.add_argument("--autotune", action="store_true", help="tune block size on extra warmup windows first")
@@ -135,7 +139,7 @@ def main():
         W = args.window
     from streamingbc_amd.autotune import DEFAULT_CANDIDATES
     n_tune = 2 * len(DEFAULT_CANDIDATES) if args.autotune else 0
-    n_train = args.train_windows if args.schedule == "mlp" else 0
+    n_train = args.train_windows
     n_ins = (n_tune + n_train + args.warmup + args.steps) * W + args.plain
     hg, roots, u, v, K, _, desc, t_gen = make_workload(args.workload, n_ins)
     print(f"[bench] {args.workload}: nv={hg.nv} directed slots={hg.ne} K={K} W={W} "

@@ -713,6 +713,9 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.qctr = C->d_qctr;
     P.err = F->d_err;
     P.bc_atomic = 1;
+    // test hook: a small run capacity forces the flat hub scan's overflow path
+    const char* rc = getenv("SBC_HUB_RUN_CAP");
+    P.run_cap = rc ? atoi(rc) : 0;
     return P;
 }
 

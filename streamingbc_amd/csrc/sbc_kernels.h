@@ -80,6 +80,7 @@ struct SbcParams {
                           // neighbours of probed hubs, neighbours of scanned hubs; or nullptr
     int32_t* err;
     int32_t bc_atomic;
+    int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)
 };
 
 #define SBC_APPEND_INLINE 8

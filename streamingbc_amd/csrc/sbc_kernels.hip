@@ -241,6 +241,7 @@ struct alignas(16) HubShared {
     int overflow;
     int pn;
     int np;
+    int ns;               // hubs staged for the team scan
     int list[HUB_CAP];
     int plist[HUB_CAP];   // ascent-only hubs of the level handled by table lookups
     int bcnt[HUB_CAP];    // terms collected per lookup hub
@@ -248,6 +249,13 @@ struct alignas(16) HubShared {
     int phv[PHASH];       // hash values (number in plist)
     int pidx[HUB_CAP];    // queued hub -> number in plist, or -1
     uint8_t kind[HUB_CAP];
+    // flat scan of the staged hubs: segment i = j + w holds the terms of hub
+    // j found by wavefront w, entries [seg_b, seg_e) of w's run
+    int seg_j[HUB_CAP + 16];
+    int seg_b[HUB_CAP + 16];
+    int seg_e[HUB_CAP + 16];
+    int wcnt[16];         // terms per wavefront run (above capacity: overflow)
+    uint8_t sovf[HUB_CAP];
     int runlen[16];
     double radd[16][RUN_CAP];
     double rsub[16][RUN_CAP];
@@ -345,6 +353,67 @@ __device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X
             if (pd >= 0) X.list[pd] = k;
             const int psl = wave_append(appS, &ts.cnt[1]);
             if (psl >= 0) X.list[X.nv - 1 - psl] = k;
+        }
+    }
+}
+
+// Descent over the hubs queued while expanding one frontier (new level lc of
+// their children), as one flat range of 64-neighbour chunks split evenly
+// between the wavefronts: per-hub state is read once, in parallel, into LDS
+// (the light-vertex arrays are free during the descent), and a wavefront's
+// range may span several hubs.  Marks and added paths are exact in any order.
+template <int TW>
+__device__ __forceinline__ void descent_hubs_flat(const SbcParams& P, const Item& X, int nh, int32_t lc,
+                                                  TeamShared& ts, HubShared& hs, LightShared& ls, int twave)
+{
+    constexpr int NT = TW * 64;
+    const int tid = threadIdx.x;
+    // thread t stages hubs [t*per, (t+1)*per)
+    const int per = (nh + NT - 1) / NT;
+    const int h0 = tid * per, h1 = min(h0 + per, nh);
+    int mine = 0;
+    for (int h = h0; h < h1; ++h) {
+        const int32_t c = hs.list[h];
+        const SbcVInfo vi = P.vinfo[c];
+        const int len = visible_len_thread(P, vi, X.q);
+        const bool cm = sb_type(ld_sb(&X.sb[c])) == 2;
+        ls.k[h] = c;
+        ls.start[h] = vi.start;
+        ls.len[h] = len;
+        ls.isD[h] = cm ? 1 : 0;
+        ls.sk_old[h] = ld_acq64(&X.dpath[c]);  // added paths of c
+        ls.sk_new[h] = X.dist[c];              // old level of c
+        mine += (len + 63) >> 6;
+        atomicAdd(&ts.stat[0], 1ull);
+        atomicAdd(&ts.stat[1], (unsigned long long)len);
+        if (cm) atomicAdd(&ts.stat[2], 1ull);
+    }
+    int total;
+    int off = block_exscan<TW>(mine, ls.scan, total);
+    for (int h = h0; h < h1; ++h) {
+        ls.off[h] = off;
+        off += (ls.len[h] + 63) >> 6;
+    }
+    __syncthreads();
+    const int cb = (int)(((long)twave * total) / TW), ce = (int)(((long)(twave + 1) * total) / TW);
+    if (cb < ce) {
+        int lo = 0, hi = nh;  // last hub starting at or before cb
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (ls.off[mid] <= cb) lo = mid;
+            else hi = mid;
+        }
+        for (int h = lo; h < nh; ++h) {
+            const int hb = ls.off[h];
+            if (hb >= ce) break;
+            const int hn = (ls.len[h] + 63) >> 6;
+            const int a = cb > hb ? cb - hb : 0;
+            const int b = ce - hb < hn ? ce - hb : hn;
+            if (a < b) {
+                SbcVInfo vi{ls.start[h], 0, 0, 0};
+                descent_chunks(P, X, ls.k[h], vi, ls.len[h], a, b, (int32_t)ls.sk_new[h], lc, ls.sk_old[h],
+                               ls.isD[h] != 0, ts);
+            }
         }
     }
 }
@@ -451,10 +520,9 @@ __device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Ite
 // RUN_CAP sends the vertex back to the single-wave path.
 template <int TW>
 __device__ __forceinline__ void ascent_vertex_team(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
-                                   int lev, TeamShared& ts, HubShared& hs, int twave)
+                                   const KState& ks, int lev, TeamShared& ts, HubShared& hs, int twave)
 {
     const int lane = lane_id();
-    const KState ks = kstate(X, k);
     const int nch = (len + 63) >> 6;
     const int cb = (int)(((long)twave * nch) / TW), ce = (int)(((long)(twave + 1) * nch) / TW);
     int rl = 0;
@@ -511,6 +579,150 @@ __device__ __forceinline__ void ascent_vertex_team(const SbcParams& P, const Ite
     __syncthreads();
 }
 
+
+// Scan of chunks [a, b) of hub k (state ks): marks parents like the other
+// scans and appends the terms, in adjacency order, to the calling
+// wavefront's run (entries rl.. ; those past cap are dropped but counted).
+__device__ __forceinline__ int ascent_scan_run(const SbcParams& P, const Item& X, int32_t k, int32_t start, int len,
+                                               const KState& ks, int lev, int a, int b, TeamShared& ts,
+                                               SbcBucketEnt* run, int rl, int cap)
+{
+    const int lane = lane_id();
+    for (int ch = a; ch < b; ch += SCAN_UNROLL) {
+        int32_t cc[SCAN_UNROLL];
+        uint8_t ww[SCAN_UNROLL];
+        bool ok[SCAN_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
+            const int e = (ch + u) * 64 + lane;
+            ok[u] = (ch + u) < b && e < len;
+            cc[u] = ok[u] ? P.col[start + e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < SCAN_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+#pragma unroll
+        for (int u = 0; u < SCAN_UNROLL; ++u) {
+            bool appU;
+            const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
+            const int pu = wave_append(appU, &ts.cnt[2]);
+            if (pu >= 0) X.list[pu] = cc[u];
+            const bool has = t.child || t.hsub;
+            const uint64_t m = ballot(has);
+            if (has) {
+                const int idx = rl + prefix_in_wave(m);
+                if (idx < cap) {
+                    SbcBucketEnt& e = run[idx];
+                    e.key = 0;
+                    e.flag = (t.child ? 1 : 0) | (t.hsub ? 2 : 0);
+                    e.add = t.add;
+                    e.sub = t.sub;
+                }
+            }
+            rl += __popcll(m);
+        }
+    }
+    return rl;
+}
+
+// New deltas of the hubs staged in ls[0..nsh) (vertex, list start, visible
+// length, state), all scanned together: their chunks form one flat range
+// split evenly between the wavefronts, so a wavefront's range may span
+// several hubs and a hub several wavefronts.  Each wavefront appends its
+// terms in scan order to its own run in the workgroup's scratch; a hub's
+// terms are then its segments of consecutive wavefronts' runs, which is
+// adjacency order, and one wavefront per hub sums them in that order (the
+// order of the single-vertex scans, so the result is bit-identical).  Hubs
+// touched by a wavefront whose run overflowed are rescanned by the whole
+// team (parent marks are idempotent).
+template <int TW>
+__device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item& X, int nsh, int lev, TeamShared& ts,
+                                                 HubShared& hs, LightShared& ls, int twave)
+{
+    constexpr int NT = TW * 64;
+    constexpr int share = (HUB_CAP * SBC_BUCKET) / TW;  // run stride
+    const int cap = (P.run_cap > 0 && P.run_cap < share) ? P.run_cap : share;
+    const int tid = threadIdx.x, lane = lane_id();
+    SbcBucketEnt* runs = P.bucket + (size_t)blockIdx.x * HUB_CAP * SBC_BUCKET;
+    const int per = (nsh + NT - 1) / NT;
+    const int h0 = min(tid * per, nsh), h1 = min(h0 + per, nsh);
+    int mine = 0;
+    for (int h = h0; h < h1; ++h) mine += (ls.len[h] + 63) >> 6;
+    int total;
+    int off = block_exscan<TW>(mine, ls.scan, total);
+    for (int h = h0; h < h1; ++h) {
+        ls.off[h] = off;
+        off += (ls.len[h] + 63) >> 6;
+    }
+    for (int i = tid; i < nsh + TW; i += NT) hs.seg_j[i] = -1;
+    for (int h = tid; h < nsh; h += NT) hs.sovf[h] = 0;
+    __syncthreads();
+    const int cb = (int)(((long)twave * total) / TW), ce = (int)(((long)(twave + 1) * total) / TW);
+    int rl = 0;
+    if (cb < ce) {
+        int lo = 0, hi = nsh;  // last hub starting at or before cb
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (ls.off[mid] <= cb) lo = mid;
+            else hi = mid;
+        }
+        for (int h = lo; h < nsh; ++h) {
+            const int hb = ls.off[h];
+            if (hb >= ce) break;
+            const int hn = (ls.len[h] + 63) >> 6;
+            const int a = cb > hb ? cb - hb : 0;
+            const int b = ce - hb < hn ? ce - hb : hn;
+            if (a < b) {
+                const KState kst{ls.isD[h] != 0, ls.sk_old[h], ls.sk_new[h], ls.base[h]};
+                const int r0 = rl;
+                rl = ascent_scan_run(P, X, ls.k[h], ls.start[h], ls.len[h], kst, lev, a, b, ts,
+                                     runs + (size_t)twave * share, rl, cap);
+                if (lane == 0) {
+                    hs.seg_j[h + twave] = h;
+                    hs.seg_b[h + twave] = r0;
+                    hs.seg_e[h + twave] = rl;
+                }
+            }
+        }
+    }
+    if (lane == 0) hs.wcnt[twave] = rl;
+    __syncthreads();
+    for (int i = tid; i < nsh + TW - 1; i += NT) {
+        const int j = hs.seg_j[i];
+        if (j >= 0 && hs.wcnt[i - j] > cap) hs.sovf[j] = 1;
+    }
+    __syncthreads();
+    for (int j = twave; j < nsh; j += TW) {
+        if (hs.sovf[j]) continue;
+        double acc = ls.base[j];
+        for (int i = j; i < j + TW; ++i) {
+            if (hs.seg_j[i] != j) continue;
+            const SbcBucketEnt* r = runs + (size_t)(i - j) * share;
+            const int eb = hs.seg_b[i], ee = hs.seg_e[i];
+            for (int e0 = eb; e0 < ee; e0 += 64) {
+                const int e = e0 + lane;
+                const int fl = e < ee ? r[e].flag : 0;
+                const double ad = (fl & 1) ? r[e].add : 0.0;
+                const double sb2 = (fl & 2) ? r[e].sub : 0.0;
+                const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
+                uint64_t m = cm | sm;
+                while (m) {
+                    const int l2 = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
+                    if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
+                }
+            }
+        }
+        if (lane == 0) st_reld(&X.ndel[ls.k[j]], acc);
+    }
+    __syncthreads();
+    for (int j = 0; j < nsh; ++j) {
+        if (!hs.sovf[j]) continue;
+        const KState kst{ls.isD[j] != 0, ls.sk_old[j], ls.sk_new[j], ls.base[j]};
+        const SbcVInfo vi{ls.start[j], 0, 0, 0};
+        ascent_vertex_team<TW>(P, X, ls.k[j], vi, ls.len[j], kst, lev, ts, hs, twave);
+    }
+}
 
 // Position of w in the indexed (initial) part of a vertex's adjacency list,
 // looked up in the vertex's open-addressing table, or -1.
@@ -696,12 +908,12 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         }
         __syncthreads();
     }
-    // inserted edges of each hub, slot by slot
-    for (int hi = 0; hi < np; ++hi) {
+    // inserted edges of the hubs (a few each), one thread per hub
+    for (int hi = tid; hi < np; hi += NT) {
         const int32_t k = hs.plist[hi];
         const SbcVInfo vi = P.vinfo[k];
-        const int len = visible_len(P, vi, X.q);
-        for (int e = P.hub[k].y + tid; e < len; e += NT) {
+        const int len = visible_len_thread(P, vi, X.q);
+        for (int e = P.hub[k].y; e < len; ++e) {
             const int32_t c = P.col[vi.start + e];
             const uint8_t b = ld_sb(&X.sb[c]);
             if (sb_type(b) == 0) {
@@ -960,21 +1172,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         if constexpr (TW > 1) {
             __syncthreads();
             const int nh = min(hs->n, HUB_CAP);
-            for (int h = 0; h < nh; ++h) {
-                const int32_t c = hs->list[h];
-                const SbcVInfo vi = P.vinfo[c];
-                const int len = visible_len(P, vi, X.q);
-                const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
-                const int64_t dc = ld_acq64(&X.dpath[c]);
-                const int nch = (len + 63) >> 6;
-                descent_chunks(P, X, c, vi, len, (int)(((long)twave * nch) / TW),
-                               (int)(((long)(twave + 1) * nch) / TW), X.dist[c], lc, dc, cmoved, ts);
-                if (twave == 0) {
-                    my_proc += 1;
-                    my_scan += (unsigned long long)len;
-                    my_moved += cmoved ? 1 : 0;
-                }
-            }
+            if (nh > 0) descent_hubs_flat<TW>(P, X, nh, lc, ts, *hs, *ls, twave);
             __syncthreads();
             if (leader) hs->n = 0;
         }
@@ -1057,19 +1255,44 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
             __syncthreads();
             const int np = hs->np;
             if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src);
-            // hubs not resolved by lookups: every queued hub that is not in
-            // plist, and plist hubs whose bucket overflowed (kind 2)
-            for (int h = 0; h < nh; ++h) {
+            // counters of every queued hub, one thread per hub; hubs not
+            // resolved by lookups (not in plist, or their bucket overflowed:
+            // kind 2) are staged in LDS with their state for the team scans
+            if (threadIdx.x == 0) hs->ns = 0;
+            __syncthreads();
+            for (int h = threadIdx.x; h < nh; h += TW * 64) {
                 const int32_t k = hs->list[h];
                 const int hi = hs->pidx[h];
                 const bool looked_up = hi >= 0 && hs->kind[hi] != 2;
                 const SbcVInfo vi = P.vinfo[k];
-                const int len = visible_len(P, vi, X.q);
-                if (!looked_up) ascent_vertex_team<TW>(P, X, k, vi, len, lev, ts, *hs, twave);
-                if (tp && leader) { tp[looked_up ? 4 : 5] += 1; tp[looked_up ? 6 : 7] += len; }
-                if (twave == 0) {
-                    my_proc += 1;
-                    my_scan += (unsigned long long)len;
+                const int len = visible_len_thread(P, vi, X.q);
+                atomicAdd(&ts.stat[0], 1ull);
+                atomicAdd(&ts.stat[1], (unsigned long long)len);
+                if (tp) {
+                    atomicAdd((unsigned long long*)&tp[looked_up ? 4 : 5], 1ull);
+                    atomicAdd((unsigned long long*)&tp[looked_up ? 6 : 7], (unsigned long long)len);
+                }
+                if (!looked_up) {
+                    const int j = atomicAdd(&hs->ns, 1);
+                    const KState kst = kstate(X, k);
+                    ls->k[j] = k;
+                    ls->start[j] = vi.start;
+                    ls->len[j] = len;
+                    ls->isD[j] = kst.isD ? 1 : 0;
+                    ls->sk_old[j] = kst.sk_old;
+                    ls->sk_new[j] = kst.sk_new;
+                    ls->base[j] = kst.base;
+                }
+            }
+            __syncthreads();
+            const int nsh = hs->ns;
+            if (P.bucket) {
+                if (nsh > 0) ascent_hubs_flat<TW>(P, X, nsh, lev, ts, *hs, *ls, twave);
+            } else {
+                for (int j = 0; j < nsh; ++j) {
+                    const KState kst{ls->isD[j] != 0, ls->sk_old[j], ls->sk_new[j], ls->base[j]};
+                    const SbcVInfo vi{ls->start[j], 0, 0, 0};
+                    ascent_vertex_team<TW>(P, X, ls->k[j], vi, ls->len[j], kst, lev, ts, *hs, twave);
                 }
             }
             __syncthreads();
