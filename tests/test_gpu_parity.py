@@ -274,3 +274,37 @@ def test_c_driver_matches_oracle(window):
     assert res["insertions"] == 24
     assert abs(res["bc_sum"] - bo.sum()) <= 1e-9 * abs(bo.sum())
     assert abs(res["bc_max"] - bo.max()) <= 1e-9 * abs(bo.max())
+
+
+@pytest.mark.skip(reason="shared levels (idle workgroups taking chunks of another root's large BFS levels) "
+                         "faulted the GPU twice; the path is out of the library until the cause is found "
+                         "on the CPU, and SBC_ASSIST_* are ignored meanwhile")
+@pytest.mark.parametrize("knobs", [("16", "8", "32"), ("96", "64", "256")])
+def test_shared_levels_match_oracle(knobs, monkeypatch):
+    """Levels above a small size are split into chunks taken by idle
+    workgroups (the owner and helpers append to shared list counters);
+    trees, records and BC stay exactly the oracle's."""
+    mn, dc, ac = knobs
+    monkeypatch.setenv("SBC_ASSIST_MIN", mn)
+    monkeypatch.setenv("SBC_ASSIST_DCHUNK", dc)
+    monkeypatch.setenv("SBC_ASSIST_ACHUNK", ac)
+    hg = sbc.gen_rmat(14, 16 << 14, seed=3, keep_lcc=True)
+    roots = hg.pick_roots(24, 5)
+    u, v = _hub_stream(hg, 48, 6)
+    print(f"[shared] rmat14 knobs={knobs} k={len(roots)} n={len(u)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    half = len(u) // 2
+    rec = f.insert_batched(g, ctx, u[:half], v[:half], sbc.BatchConfig(window=8, block_threads=1024), records=True)
+    for i in range(len(u)):
+        if i >= half:
+            f.insert_edge(g, ctx, int(u[i]), int(v[i]), sbc.FINE)
+            rec_g = ctx.last_records(f.k)
+        else:
+            rec_g = rec[i]
+        rec_o = o.insert(int(u[i]), int(v[i]), O_FINE, items=True)
+        for key in ("affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec_g[key], rec_o[key]), f"insertion {i} {key}"
+    _compare_trees(f, o, "shared levels")
+    assert_bc_close(f.bc(), o.bc())
