@@ -233,6 +233,15 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "kernel": "sbc_window_kernel", "launch_ms": round(launch_ms, 4),
                 "alg_bytes_per_launch": alg_bytes / max(n_launch, 1)}
+    # measured HBM bytes per launch of this kernel on this workload: the
+    # calibrated FETCH_SIZE + WRITE_SIZE of the rocprofv3 PMC passes kept
+    # beside the kernel-trace summary
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath) and cfg.block_threads == 1024 and W == WORKLOADS[args.workload][3]:
+        with open(tpath) as fh:
+            t = json.load(fh)
+        roofline["traffic"] = t["hbm_bytes_per_launch"]
+        roofline["traffic_source"] = f"profiles/traffic_{args.workload}.json ({t['windows']})"
 
     out = {
         "metric": METRIC,
