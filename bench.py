@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--light-max", type=float, default=0.0,
                     help="--schedule mlp: roots predicted below this many affected vertices per window run light")
     ap.add_argument("--autotune", action="store_true", help="tune block size on extra warmup windows first")
+    ap.add_argument("--park-min", type=int, default=0,
+                    help="descent frontier size at which an item is finished by whole-GPU kernels "
+                         "(0: library default, <0: never)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (PMC traffic calibration)")
     args = ap.parse_args()
@@ -153,7 +156,8 @@ def main():
     t_static = time.time() - t0
     print(f"[bench] static pass {t_static:.2f}s", file=sys.stderr, flush=True)
 
-    cfg = sbc.BatchConfig(window=W, block_threads=args.block or 1024, use_graph=not args.no_graph)
+    cfg = sbc.BatchConfig(window=W, block_threads=args.block or 1024, use_graph=not args.no_graph,
+                          park_min=args.park_min)
     if args.heavy >= -1:
         cfg.heavy_roots = args.heavy
     pos = 0
@@ -176,11 +180,13 @@ def main():
         plan, X = sched.plan(f, g, ctx, u[a:b], v[a:b])
         c = sbc.BatchConfig(window=W, block_threads=cfg.block_threads, grid_blocks=cfg.grid_blocks,
                             use_graph=cfg.use_graph, heavy_roots=cfg.heavy_roots if plan.heavy_roots < 0
-                            else plan.heavy_roots, root_order=plan.root_order)
+                            else plan.heavy_roots, root_order=plan.root_order, park_min=cfg.park_min)
         rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or X is not None)
+        parked[0] += ctx.last_parked()
         sched.observe(plan, X, rec)
         return rec
 
+    parked = [0]
     # training windows (cost model) and warmup windows, untimed
     for _ in range(n_train + args.warmup):
         window(pos, pos + W)
@@ -188,6 +194,7 @@ def main():
     ctx.synchronize()
 
     # timed windows
+    parked[0] = 0
     ctx.reset_stats(timing=True)
     lat = []
     recs = []
@@ -267,7 +274,8 @@ def main():
         "items": {"count": int(rec.size), "nonzero_gap": int((rec["kase"] > 0).sum()),
                   "root_scan_max_over_mean": float(per_root.max() / max(per_root.mean(), 1)),
                   "mean_affected": float(rec["affected"].mean()), "max_affected": int(rec["affected"].max()),
-                  "mean_scanned": float(rec["scanned"].mean())},
+                  "mean_scanned": float(rec["scanned"].mean()),
+                  "parked": int(parked[0])},
         "static_pass_s": round(t_static, 3),
         "schedule": {"name": sched.name, **(sched.report() if args.schedule == "mlp" else {})},
         "autotune": tune,

@@ -193,11 +193,17 @@ typedef struct {
     int32_t recompute_bc;   /* 1: BC = sum over roots after each window (deterministic) */
     int32_t heavy_roots;    /* roots run in whole-workgroup mode (-1: all) */
     const int32_t *root_order; /* NULL: root-array order */
+    int32_t park_min;       /* an item whose descent frontier reaches this many vertices is finished by
+                               kernels spanning the whole GPU (0: 16384, < 0: never) */
 } sbcBatchConfig;
 
 sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,
                                         const int64_t *u, const int64_t *v, int64_t count,
                                         const sbcBatchConfig *cfg, sbcItemRecord *records);
+
+/* Number of items of the last streamingBCInsertEdgesBatched call that were
+ * finished by the whole-GPU kernels (sbcBatchConfig.park_min). */
+int64_t streamingBCLastParked(extraArraysPerThread **eAPT);
 
 /* Records of the last insertEdgeStreamingBC call (K entries). */
 sbcStatus streamingBCLastRecords(extraArraysPerThread **eAPT, sbcItemRecord *out, int64_t K);

@@ -66,7 +66,8 @@ class ExtraInfoC(C.Structure):
 
 class BatchConfigC(C.Structure):
     _fields_ = [("window", i32), ("block_threads", i32), ("grid_blocks", i32), ("use_graph", i32),
-                ("recompute_bc", i32), ("heavy_roots", i32), ("root_order", C.POINTER(C.c_int32))]
+                ("recompute_bc", i32), ("heavy_roots", i32), ("root_order", C.POINTER(C.c_int32)),
+                ("park_min", i32)]
 
 
 RECORD_DTYPE = np.dtype([("root_index", np.int32), ("insertion", np.int32), ("kase", np.int32),
@@ -122,6 +123,7 @@ _sig("streamingBCResetStats", C.c_int, [VPP, i32])
 _sig("streamingBCForestFootprint", i64, [i64, i64])
 _sig("streamingBCResidentFootprint", i64, [P64, i64, i64, i32])
 _sig("streamingBCSynchronize", C.c_int, [VPP])
+_sig("streamingBCLastParked", i64, [VPP])
 # cost model (include/sbc_costmodel.h)
 PF = C.POINTER(C.c_float)
 _sig("sbcCostModelNumParams", i32, [])
@@ -258,6 +260,10 @@ class Context:
         _check(lib.streamingBCKernelStats(self._p, kernel, C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
 
+    def last_parked(self) -> int:
+        """Items of the last batched call finished by the whole-GPU kernels."""
+        return int(lib.streamingBCLastParked(self._p))
+
     def synchronize(self):
         _check(lib.streamingBCSynchronize(self._p))
 
@@ -316,6 +322,7 @@ class BatchConfig:
     recompute_bc: bool = False
     heavy_roots: int = -1
     root_order: np.ndarray | None = None
+    park_min: int = 0  # 0: library default, < 0: never hand an item to the whole-GPU kernels
 
 
 class Forest:
@@ -362,7 +369,7 @@ class Forest:
         n = u.shape[0]
         order = None
         c = BatchConfigC(cfg.window, cfg.block_threads, cfg.grid_blocks, 1 if cfg.use_graph else 0,
-                         1 if cfg.recompute_bc else 0, cfg.heavy_roots, None)
+                         1 if cfg.recompute_bc else 0, cfg.heavy_roots, None, cfg.park_min)
         if cfg.root_order is not None:
             order = np.ascontiguousarray(cfg.root_order, dtype=np.int32)
             c.root_order = order.ctypes.data_as(C.POINTER(C.c_int32))

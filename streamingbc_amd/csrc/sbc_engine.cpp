@@ -506,7 +506,7 @@ struct GraphCache {
     const void* forest = nullptr;
     const void* g = nullptr;
     uint64_t generation = 0;
-    int nwb = 0, grid = 0, heavy = 0, recompute = 0, W = 0, has_order = 0, with_rec = 0;
+    int nwb = 0, grid = 0, heavy = 0, recompute = 0, W = 0, has_order = 0, with_rec = 0, park_min = 0;
 };
 
 struct extraArraysPerThread {
@@ -520,6 +520,14 @@ struct extraArraysPerThread {
     int64_t rec_cap = 0;       // entries
     int64_t last_rec_n = 0;    // entries written by the last legacy update
     int32_t* d_order = nullptr;
+    // parked items: records, parked-root list, resume order and start items
+    SbcWideItem* d_park = nullptr;
+    int32_t* d_parked = nullptr;
+    int32_t* d_order2 = nullptr;
+    int32_t* d_start = nullptr;
+    int64_t park_cap = 0;
+    std::vector<int32_t> h_start;
+    int64_t last_parked = 0;   // items finished by the wide kernels in the last batched call
     int64_t order_cap = 0;
     int num_cu = 0;
     // optional per-item device timestamps, appended to the file named by
@@ -558,6 +566,8 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_qctr) (void)hipFree(C->d_qctr);
     if (C->d_rec) (void)hipFree(C->d_rec);
     if (C->d_order) (void)hipFree(C->d_order);
+    for (void* q : {(void*)C->d_park, (void*)C->d_parked, (void*)C->d_order2, (void*)C->d_start})
+        if (q) (void)hipFree(q);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->win.d_desc) (void)hipFree(C->win.d_desc);
@@ -713,6 +723,11 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.qctr = C->d_qctr;
     P.err = F->d_err;
     P.bc_atomic = 1;
+    P.park = nullptr;
+    P.parked = nullptr;
+    P.park_min = 0;
+    P.start_item = nullptr;
+    P.qlen = (int32_t)F->K;
     // test hook: a small run capacity forces the flat hub scan's overflow path
     const char* rc = getenv("SBC_HUB_RUN_CAP");
     P.run_cap = rc ? atoi(rc) : 0;
@@ -942,6 +957,125 @@ static sbcStatus ensure_order(extraArraysPerThread* C, int64_t K)
 
 // Enqueues the launches of one window (append, queue reset, window kernel,
 // optional BC reduction) on the context stream.
+static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K)
+{
+    if (C->park_cap >= K) return SBC_OK;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    for (void* q : {(void*)C->d_park, (void*)C->d_parked, (void*)C->d_order2, (void*)C->d_start})
+        if (q) HIPCHK(hipFree(q));
+    C->d_park = nullptr;
+    C->d_parked = C->d_order2 = C->d_start = nullptr;
+    sbcStatus st;
+    if ((st = dalloc(&C->d_park, (size_t)K)) || (st = dalloc(&C->d_parked, (size_t)K)) ||
+        (st = dalloc(&C->d_order2, (size_t)K)) || (st = dalloc(&C->d_start, (size_t)K)))
+        return st;
+    C->h_start.assign((size_t)K, 0);
+    C->park_cap = K;
+    C->gc.with_rec = -1; // cached graph holds the old pointers
+    return SBC_OK;
+}
+
+static int wide_grid(const extraArraysPerThread* C, int64_t n)
+{
+    // one wavefront per vertex, 4 per workgroup, up to 8 workgroups per CU
+    int64_t g = (n + 3) / 4;
+    int64_t cap = 8LL * (C->num_cu > 0 ? C->num_cu : 256);
+    return (int)(g < 1 ? 1 : (g < cap ? g : cap));
+}
+
+// Finishes the parked item of root ri: the remaining descent frontiers, the
+// ascent levels and the commit, one whole-GPU launch per level; the level
+// offsets and list counters are read back between launches to drive the
+// same loop as a team's.
+static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri)
+{
+    SbcWideItem w;
+    SbcWideItem* dw = C->d_park + ri;
+    const int lcap = F->lcap;
+    std::vector<int32_t> lv(2 * (size_t)lcap);
+    int32_t* lvlD = lv.data();
+    int32_t* lvlS = lvlD + lcap;
+    int32_t* dlvl = F->d_lvl + (size_t)ri * 2 * lcap;
+    HIPCHK(hipMemcpyAsync(&w, dw, sizeof(w), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipMemcpyAsync(lv.data(), dlvl, sizeof(int32_t) * lv.size(), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (w.j < 1 || w.j >= lcap || w.fs < 0 || w.fs > w.fe || w.fe > F->nv || w.item < 0)
+        return set_err(SBC_ERR_STATE, "parked item of root %d is inconsistent", (int)ri);
+    int j = w.j, fs = w.fs, fe = w.fe;
+    int32_t c01[2] = {w.cnt[0], w.cnt[1]};
+    while (fs < fe) {
+        HIPCHK(sbc_launch_wide_descent(&P, dw, ri, fs, fe, w.base + j, wide_grid(C, fe - fs), C->stream));
+        HIPCHK(hipMemcpyAsync(c01, &dw->cnt[0], sizeof(c01), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        j++;
+        if (j + 1 >= lcap) return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", lcap);
+        if (c01[0] < fe || c01[0] > F->nv || c01[1] < 0 || c01[1] > F->nv)
+            return set_err(SBC_ERR_STATE, "parked item of root %d: list counters out of range", (int)ri);
+        lvlD[j] = fe;
+        lvlS[j] = c01[1];
+        fs = fe;
+        fe = c01[0];
+    }
+    const int nf = j, nD = c01[0], nS = c01[1];
+    int32_t u = nD;
+    HIPCHK(hipMemcpyAsync(dlvl, lv.data(), sizeof(int32_t) * lv.size(), hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipMemcpyAsync(&dw->cnt[2], &u, sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));  // lv and u are host stack data
+    int uA = nD, uB = nD;
+    for (int lev = w.base + nf - 1; lev >= 0; --lev) {
+        const int jj = lev - w.base;
+        int32_t h[6] = {0, 0, 0, 0, uA, uB};
+        if (jj >= 0) { h[0] = lvlD[jj]; h[1] = lvlD[jj + 1]; h[2] = lvlS[jj]; h[3] = lvlS[jj + 1]; }
+        const int64_t tot = (int64_t)(h[1] - h[0]) + (h[3] - h[2]) + (h[5] - h[4]);
+        if (tot == 0) break;
+        HIPCHK(sbc_launch_wide_ascent(&P, dw, ri, h, lev, wide_grid(C, tot), C->stream));
+        HIPCHK(hipMemcpyAsync(&u, &dw->cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (u < uB || (int64_t)u + nS > F->nv)
+            return set_err(SBC_ERR_STATE, "parked item of root %d: ascent list out of range", (int)ri);
+        uA = uB;
+        uB = u;
+    }
+    const int64_t ncommit = nD > uB + nS - nD ? nD : uB + nS - nD;
+    HIPCHK(sbc_launch_wide_commit(&P, dw, ri, nf, nD, uB, nS, wide_grid(C, ncommit / 64 + 1), C->stream));
+    C->h_start[ri] = w.item + 1;
+    return SBC_OK;
+}
+
+// After a window launch: finishes every parked item and resumes its root at
+// the next item, until no root parks.
+static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int64_t* total)
+{
+    for (;;) {
+        int32_t np = 0;
+        HIPCHK(hipMemcpyAsync(&np, C->d_qctr + 2, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (np == 0) return SBC_OK;
+        *total += np;
+        if (np < 0 || np > F->K) return set_err(SBC_ERR_STATE, "parked count %d out of range", (int)np);
+        std::vector<int32_t> roots(np);
+        HIPCHK(hipMemcpyAsync(roots.data(), C->d_parked, sizeof(int32_t) * np, hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        for (int32_t ri : roots) {
+            if (ri < 0 || ri >= F->K) return set_err(SBC_ERR_STATE, "parked root %d out of range", (int)ri);
+            sbcStatus st = finish_parked_item(F, C, P, ri);
+            if (st) return st;
+        }
+        HIPCHK(hipMemcpyAsync(C->d_order2, roots.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_start, C->h_start.data(), sizeof(int32_t) * F->K, hipMemcpyHostToDevice,
+                              C->stream));
+        SbcParams Q = P;
+        Q.order = C->d_order2;
+        Q.heavy = np;
+        Q.qlen = np;
+        Q.start_item = C->d_start;
+        HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+        HIPCHK(sbc_launch_window(&Q, nwb, np, C->stream));
+        // the host copy of start items must not change before the upload ran
+        HIPCHK(hipStreamSynchronize(C->stream));
+    }
+}
+
 static sbcStatus enqueue_window(const SbcParams& P, const SbcAppend& A, int nwb, int grid, int W,
                                 int recompute, extraArraysPerThread* C, bool timed)
 {
@@ -995,6 +1129,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     if ((st = ensure_stage(C, W))) return st;
     if ((st = ensure_rec(C, (int64_t)W * F->K))) return st;
     if ((st = ensure_order(C, F->K))) return st;
+    const int park_min = cfg->park_min == 0 ? 16384 : cfg->park_min;
+    if (park_min > 0 && nwb > 1 && (st = ensure_park(C, F->K))) return st;
     int heavy = cfg->heavy_roots < 0 || cfg->heavy_roots > F->K ? (int)F->K : cfg->heavy_roots;
     const int has_order = cfg->root_order != nullptr;
     if (has_order) {
@@ -1048,6 +1184,11 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     P.wv = di + 4 * W;
     P.rec = C->d_rec;
     P.bc_atomic = cfg->recompute_bc ? 0 : 1;
+    if (park_min > 0 && nwb > 1) {
+        P.park = C->d_park;
+        P.parked = C->d_parked;
+        P.park_min = park_min;
+    }
     SbcAppend A;
     memset(&A, 0, sizeof(A));
     A.vinfo = G->d_vinfo;
@@ -1065,7 +1206,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     bool need_capture = use_graph &&
         (!gc.exec || gc.forest != F || gc.g != G || gc.generation != G->generation || gc.nwb != nwb ||
          gc.grid != grid || gc.heavy != heavy || gc.recompute != cfg->recompute_bc || gc.W != W ||
-         gc.has_order != has_order || gc.with_rec != 1);
+         gc.has_order != has_order || gc.with_rec != 1 || gc.park_min != P.park_min);
     if (need_capture) {
         if (gc.exec) { (void)hipGraphExecDestroy(gc.exec); gc.exec = nullptr; }
         if (gc.graph) { (void)hipGraphDestroy(gc.graph); gc.graph = nullptr; }
@@ -1080,9 +1221,11 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         HIPCHK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
         gc.forest = F; gc.g = G; gc.generation = G->generation; gc.nwb = nwb; gc.grid = grid; gc.heavy = heavy;
         gc.recompute = cfg->recompute_bc; gc.W = W; gc.has_order = has_order; gc.with_rec = 1;
+        gc.park_min = P.park_min;
     }
 
     std::vector<int32_t> prev_touched;
+    int64_t nparked = 0, nparked_before = 0;
     std::vector<int32_t> wu(W), wv(W), slot(2 * W), scol(2 * W), sseq(2 * W), vv, touched;
     std::vector<SbcVInfo> vi;
     std::vector<char> mark(G->nv, 0);
@@ -1145,10 +1288,16 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             hipEvent_t e0;
             if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
             HIPCHK(hipGraphLaunch(gc.exec, C->stream));
+            // items parked by this window are finished inside the same timed interval
+            if (P.park && (st = finish_parked(F, C, P, nwb, &nparked))) return st;
             if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
-        } else if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) {
-            return st;
+        } else {
+            if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) return st;
+            if (P.park && (st = finish_parked(F, C, P, nwb, &nparked))) return st;
         }
+        // the captured BC reduction ran before the parked items were finished
+        if (nparked > nparked_before && cfg->recompute_bc) HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
+        nparked_before = nparked;
         if (trace_path) {
             std::vector<int64_t> tb((size_t)n * F->K * 8);
             HIPCHK(hipMemcpyAsync(tb.data(), C->d_tprof, sizeof(int64_t) * tb.size(), hipMemcpyDeviceToHost, C->stream));
@@ -1165,6 +1314,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
                                   hipMemcpyDeviceToHost, C->stream));
         }
     }
+    C->last_parked = nparked;
     // make the last window's appends visible to everyone
     for (int32_t x : prev_touched) G->deg0[x] = G->deg[x];
     for (size_t i = 0; i < prev_touched.size(); i += SBC_APPEND_INLINE) {
@@ -1270,4 +1420,9 @@ extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger*
     (void)hipFree(d_X);
     if (d_hist) (void)hipFree(d_hist);
     return SBC_OK;
+}
+
+extern "C" int64_t streamingBCLastParked(extraArraysPerThread** eAPT)
+{
+    return (eAPT && eAPT[0]) ? eAPT[0]->last_parked : -1;
 }

@@ -81,6 +81,26 @@ struct SbcParams {
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)
+    // parked items: a whole-workgroup team whose next descent frontier has at
+    // least park_min vertices stops, saves the item in park[ri] and appends
+    // ri to parked (count in qctr[2]); the host finishes it with the wide
+    // kernels and resumes the root at start_item[ri]
+    struct SbcWideItem* park;   // [K] or nullptr (never park)
+    int32_t* parked;            // [K]
+    int32_t park_min;
+    const int32_t* start_item;  // [K] first item per root, or nullptr (0)
+    int32_t qlen;               // queue slots in use (K, or the roots resumed)
+};
+
+// A parked item (one per root slot), in global memory.  cnt/stat have the
+// roles of a team's list and work counters, so the same device functions
+// serve a team (counters in LDS) and the wide kernels (counters here).
+struct SbcWideItem {
+    int32_t cnt[4];              // end of the descent list, same-level count, end of the ascent list, unused
+    unsigned long long stat[4];  // processed, scanned, moved, unused
+    int32_t item, s, p, q;       // window item, deeper and shallower endpoint, visibility sequence
+    int32_t base, j, fs, fe;     // new level of s, frontiers done, next frontier list[fs, fe)
+    int32_t gap, smoved, pad[2];
 };
 
 #define SBC_APPEND_INLINE 8
@@ -123,6 +143,12 @@ struct SbcFeatArgs {
 
 extern "C" {
 hipError_t sbc_launch_features(const SbcFeatArgs* A, hipStream_t st);
+hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc, int grid,
+                                   hipStream_t st);
+hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* here6, int lev, int grid,
+                                  hipStream_t st);
+hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, int nf, int nD, int nUend, int nS,
+                                  int grid, hipStream_t st);
 hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);

@@ -304,9 +304,12 @@ struct Item {
 // added paths dc).  Marks children, accumulates their added paths, appends
 // them to the next frontier and records unmoved same-level neighbours of a
 // moved c.
+// TS: the owner of the list counters cnt[0..1] (a team's LDS copy, or a
+// parked item's global one).
+template <typename TS>
 __device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X, int32_t c, const SbcVInfo& vi,
                                                int len, int cb, int ce, int32_t oc, int32_t lc, int64_t dc,
-                                               bool cmoved, TeamShared& ts)
+                                               bool cmoved, TS& ts)
 {
     const int lane = lane_id();
     const uint8_t wantStay = sb_make(lc + 1, 1), wantMove = sb_make(lc + 1, 2);
@@ -476,8 +479,9 @@ __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
 
 // One wavefront computes the new delta of k, summing the terms in adjacency
 // order lane by lane.
+template <typename TS>
 __device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
-                                   int lev, TeamShared& ts)
+                                   int lev, TS& ts)
 {
     const int lane = lane_id();
     const KState ks = kstate(X, k);
@@ -1091,8 +1095,10 @@ __device__ __forceinline__ bool queue_hub(HubShared* hs, int32_t k, int len)
     return true;
 }
 
+// Applies window item `item` to root ri.  Returns true if the item was
+// parked (its remaining levels are left to the wide kernels).
 template <int TW>
-__device__ __forceinline__ void process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs,
+__device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs,
                                              LightShared* ls, int twave)
 {
     const int lane = lane_id();
@@ -1127,7 +1133,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
             rec->root_index = ri; rec->insertion = item; rec->kase = 0; rec->gap = 0;
             rec->processed = 0; rec->scanned = 0; rec->affected = 0; rec->moved = 0;
         }
-        return;
+        return false;
     }
     X.s = gap > 0 ? u : v;
     X.p = gap > 0 ? v : u;
@@ -1187,10 +1193,42 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         if (leader) { lvlD[j] = fe; lvlS[j] = nS; }
         fs = fe;
         fe = nfe;
+        if constexpr (TW > 1) {
+            if (P.park && fe - fs >= P.park_min) {
+                // too wide for one workgroup: save the item, stop this root
+                if (lane == 0) {
+                    atomicAdd(&ts.stat[0], my_proc);
+                    atomicAdd(&ts.stat[1], my_scan);
+                    atomicAdd(&ts.stat[2], my_moved);
+                }
+                team_sync<TW>();
+                if (leader) {
+                    SbcWideItem& w = P.park[ri];
+                    w.cnt[0] = ts.cnt[0];
+                    w.cnt[1] = ts.cnt[1];
+                    w.cnt[2] = 0;
+                    w.cnt[3] = 0;
+                    for (int i = 0; i < 4; ++i) w.stat[i] = ts.stat[i];
+                    w.item = item;
+                    w.s = X.s;
+                    w.p = X.p;
+                    w.q = X.q;
+                    w.base = base;
+                    w.j = j;
+                    w.fs = fs;
+                    w.fe = fe;
+                    w.gap = (int32_t)(gap > INF32 ? INF32 : (gap < -INF32 ? -INF32 : gap));
+                    w.smoved = smoved ? 1 : 0;
+                    P.parked[atomicAdd(&P.qctr[2], 1)] = ri;
+                }
+                team_sync<TW>();
+                return true;
+            }
+        }
     }
     if (overflow) {
         if (leader) atomicOr(P.err, SBC_KERR_LEVELS);
-        return;
+        return false;
     }
     team_sync<TW>();
     if (tp && leader) tp[1] = (int64_t)wall_clock64();
@@ -1357,6 +1395,7 @@ __device__ __forceinline__ void process_item(const SbcParams& P, int ri, int ite
         rec->moved = (int64_t)ts.stat[2];
     }
     team_sync<TW>();
+    return false;
 }
 
 // Persistent window kernel.  Phase 1: whole-workgroup teams pop root slots
@@ -1377,9 +1416,10 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         __syncthreads();
         const int t = ts[0].cnt[3];
         __syncthreads();
-        if (t >= P.heavy) break;
+        if (t >= P.heavy || t >= P.qlen) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = 0; i < W; ++i) process_item<NWB>(P, ri, i, ts[0], &hs, &ls, wave);
+        for (int i = P.start_item ? P.start_item[ri] : 0; i < W; ++i)
+            if (process_item<NWB>(P, ri, i, ts[0], &hs, &ls, wave)) break;
     }
     // phase 2: light, one wavefront per root
     for (;;) {
@@ -1387,9 +1427,146 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         team_sync<1>();
         const int t = P.heavy + ts[wave].cnt[3];
         team_sync<1>();
-        if (t >= P.K) break;
+        if (t >= P.qlen) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = 0; i < W; ++i) process_item<1>(P, ri, i, ts[wave], nullptr, nullptr, 0);
+        for (int i = P.start_item ? P.start_item[ri] : 0; i < W; ++i)
+            process_item<1>(P, ri, i, ts[wave], nullptr, nullptr, 0);
+    }
+}
+
+// ------------------------------------------------------------ parked items
+//
+// A parked item is finished by kernels that span the whole GPU, one launch
+// per BFS level; the launch boundaries order the levels and make each
+// level's writes visible to the next.  Within a level every wavefront takes
+// vertices of the level in a grid stride and runs the same per-vertex code
+// as a team (marks by CAS, added paths by atomic adds, appends through the
+// item's global list counters, delta summed in adjacency order), so the
+// result is bit-identical to the team's.
+
+__device__ __forceinline__ Item parked_item(const SbcParams& P, const SbcWideItem& w, int ri)
+{
+    const size_t off = (size_t)ri * (size_t)P.nv;
+    Item X;
+    X.dist = P.dist + off;
+    X.sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
+    X.sigma = P.sigma + off;
+    X.delta = P.delta + off;
+    X.dpath = P.dpath + off;
+    X.ndel = P.ndel + off;
+    X.list = P.list + off;
+    X.nv = P.nv;
+    X.q = w.q;
+    X.s = w.s;
+    X.p = w.p;
+    X.r = P.roots[ri];
+    return X;
+}
+
+// Descent of frontier list[fs, fe) (new level lc of the children).
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void sbc_wide_descent_kernel(SbcParams P, SbcWideItem* wi, int ri, int fs, int fe,
+                                                                   int32_t lc)
+{
+    SbcWideItem& w = *wi;
+    const Item X = parked_item(P, w, ri);
+    const int nwaves = (int)gridDim.x * NW;
+    unsigned long long my_proc = 0, my_scan = 0, my_moved = 0;
+    for (int idx = fs + (int)blockIdx.x * NW + (int)(threadIdx.x >> 6); idx < fe; idx += nwaves) {
+        const int32_t c = X.list[idx];
+        const SbcVInfo vi = P.vinfo[c];
+        const int len = visible_len(P, vi, X.q);
+        const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
+        const int64_t dc = ld_acq64(&X.dpath[c]);
+        descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, w);
+        my_proc += 1;
+        my_scan += (unsigned long long)len;
+        my_moved += cmoved ? 1 : 0;
+    }
+    if (lane_id() == 0 && my_proc > 0) {
+        atomicAdd(&w.stat[0], my_proc);
+        atomicAdd(&w.stat[1], my_scan);
+        atomicAdd(&w.stat[2], my_moved);
+    }
+}
+
+// Ascent of level lev: the processed vertices here (descent frontier,
+// same-level set, ascent set), one wavefront per vertex.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_kernel(SbcParams P, SbcWideItem* wi, int ri, int dA, int dB,
+                                                                  int sA, int sB, int uA, int uB, int lev)
+{
+    SbcWideItem& w = *wi;
+    const Item X = parked_item(P, w, ri);
+    const LevelSets here{dA, dB, sA, sB, uA, uB};
+    const int tot = here.size();
+    const int nwaves = (int)gridDim.x * NW;
+    unsigned long long my_proc = 0, my_scan = 0;
+    for (int i = (int)blockIdx.x * NW + (int)(threadIdx.x >> 6); i < tot; i += nwaves) {
+        const int32_t k = here.at(X, i);
+        const SbcVInfo vi = P.vinfo[k];
+        const int len = visible_len(P, vi, X.q);
+        ascent_vertex_wave(P, X, k, vi, len, lev, w);
+        my_proc += 1;
+        my_scan += (unsigned long long)len;
+    }
+    if (lane_id() == 0 && my_proc > 0) {
+        atomicAdd(&w.stat[0], my_proc);
+        atomicAdd(&w.stat[1], my_scan);
+    }
+}
+
+// Commit of a parked item (the team's commit, over the whole grid), and its
+// record.  Descent frontier jf = list[lvlD[jf], lvlD[jf + 1]) gets level
+// base + jf; ascent-only vertices keep their level.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void sbc_wide_commit_kernel(SbcParams P, SbcWideItem* wi, int ri, int nf, int nD,
+                                                                  int nUend, int nS)
+{
+    SbcWideItem& w = *wi;
+    const Item X = parked_item(P, w, ri);
+    const int32_t* lvlD = P.lvl + (size_t)ri * 2 * P.lcap;
+    const int nthreads = (int)gridDim.x * NW * 64;
+    const int gt = (int)blockIdx.x * NW * 64 + (int)threadIdx.x;
+    for (int jf = 0; jf < nf; ++jf) {
+        const int32_t nl = w.base + jf;
+        const int a = lvlD[jf], b = lvlD[jf + 1];
+        for (int i = a + gt; i < b; i += nthreads) {
+            const int32_t k = X.list[i];
+            const int ktype = sb_type(ld_sb(&X.sb[k]));
+            const double nd = ld_accd(&X.ndel[k]);
+            const double od = X.delta[k];
+            const int64_t so = X.sigma[k];
+            X.sigma[k] = (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
+            X.delta[k] = nd;
+            X.dist[k] = nl;
+            st_sb(&X.sb[k], sb_make(nl, 0));
+            if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
+            st_rel64(&X.dpath[k], 0);
+            st_reld(&X.ndel[k], 0.0);
+        }
+    }
+    const int rest = nUend + nS - nD;
+    for (int i = gt; i < rest; i += nthreads) {
+        const int32_t k = i < nUend - nD ? X.list[nD + i] : X.list[X.nv - 1 - (i - (nUend - nD))];
+        const uint8_t b = ld_sb(&X.sb[k]);
+        const double nd = ld_accd(&X.ndel[k]);
+        const double od = X.delta[k];
+        X.delta[k] = nd;
+        st_sb(&X.sb[k], (uint8_t)(b & ~3));
+        if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
+        st_reld(&X.ndel[k], 0.0);
+    }
+    if (gt == 0 && P.rec) {
+        SbcItemRec* rec = &P.rec[(size_t)w.item * P.K + ri];
+        rec->root_index = ri;
+        rec->insertion = w.item;
+        rec->kase = w.smoved ? 2 : 1;
+        rec->gap = w.gap;
+        rec->processed = (int64_t)w.stat[0];
+        rec->scanned = (int64_t)w.stat[1];
+        rec->affected = nUend + nS;
+        rec->moved = (int64_t)w.stat[2];
     }
 }
 
@@ -1572,6 +1749,31 @@ extern "C" hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, h
     case 16: hipLaunchKernelGGL(sbc_window_kernel<16>, dim3(grid), dim3(1024), 0, st, *P); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+constexpr int WIDE_NW = 4;  // wavefronts per workgroup of the wide kernels
+
+extern "C" hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc,
+                                              int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_wide_descent_kernel<WIDE_NW>, dim3(grid), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, fs, fe, lc);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* h, int lev,
+                                             int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_wide_ascent_kernel<WIDE_NW>, dim3(grid), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, h[0], h[1],
+                       h[2], h[3], h[4], h[5], lev);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, int nf, int nD, int nUend,
+                                             int nS, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_wide_commit_kernel<WIDE_NW>, dim3(grid), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, nf, nD, nUend,
+                       nS);
     return hipGetLastError();
 }
 

@@ -308,3 +308,35 @@ def test_shared_levels_match_oracle(knobs, monkeypatch):
             assert np.array_equal(rec_g[key], rec_o[key]), f"insertion {i} {key}"
     _compare_trees(f, o, "shared levels")
     assert_bc_close(f.bc(), o.bc())
+
+
+@pytest.mark.parametrize("recompute", [False, True])
+def test_parked_items_match_oracle(recompute):
+    """Items whose descent frontier reaches park_min vertices are parked by
+    their team and finished by the whole-GPU level kernels, then the root
+    resumes; with a tiny threshold nearly every nonzero item takes that path.
+    Trees, records and BC stay exactly the oracle's."""
+    hg = sbc.gen_rmat(12, 16 << 12, seed=9, keep_lcc=True)
+    roots = hg.pick_roots(48, 4)
+    u, v = hg.insertions(64, 21)
+    print(f"[parked] rmat12 nv={hg.nv} k={len(roots)} n={len(u)} recompute={recompute}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    cfg = sbc.BatchConfig(window=16, block_threads=1024, park_min=4, recompute_bc=recompute)
+    rec = f.insert_batched(g, ctx, u, v, cfg, records=True)
+    assert ctx.last_parked() > 0
+    for i in range(len(u)):
+        rec_o = o.insert(int(u[i]), int(v[i]), O_FINE, items=True)
+        for key in ("kase", "affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec[i][key], rec_o[key]), f"insertion {i} {key}"
+    _compare_trees(f, o, "parked items")
+    if recompute:
+        f.recompute_bc(ctx)
+    assert_bc_close(f.bc(), o.bc())
+    # and never parking gives the same trees
+    ctx2, g2, f2 = _engine(hg, roots)
+    f2.insert_batched(g2, ctx2, u, v, sbc.BatchConfig(window=16, block_threads=1024, park_min=-1))
+    assert ctx2.last_parked() == 0
+    for a, b in zip(f.trees(), f2.trees()):
+        assert np.array_equal(a, b)
