@@ -194,7 +194,7 @@ typedef struct {
     int32_t heavy_roots;    /* roots run in whole-workgroup mode (-1: all) */
     const int32_t *root_order; /* NULL: root-array order */
     int32_t park_min;       /* an item whose descent frontier reaches this many vertices is finished by
-                               kernels spanning the whole GPU (0: 16384, < 0: never) */
+                               kernels spanning the whole GPU (0: 4096, < 0: never) */
 } sbcBatchConfig;
 
 sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,

@@ -1129,7 +1129,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     if ((st = ensure_stage(C, W))) return st;
     if ((st = ensure_rec(C, (int64_t)W * F->K))) return st;
     if ((st = ensure_order(C, F->K))) return st;
-    const int park_min = cfg->park_min == 0 ? 16384 : cfg->park_min;
+    const int park_min = cfg->park_min == 0 ? 4096 : cfg->park_min;
     if (park_min > 0 && nwb > 1 && (st = ensure_park(C, F->K))) return st;
     int heavy = cfg->heavy_roots < 0 || cfg->heavy_roots > F->K ? (int)F->K : cfg->heavy_roots;
     const int has_order = cfg->root_order != nullptr;
@@ -1166,12 +1166,12 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         if ((st = graph_flush(G, C->stream))) return st;
     }
     const char* trace_path = getenv("SBC_ITEM_TRACE");
-    if (trace_path && C->tprof_cap < (int64_t)W * F->K * 8) {
+    if (trace_path && C->tprof_cap < (int64_t)W * F->K * SBC_TRACE_SLOTS) {
         HIPCHK(hipStreamSynchronize(C->stream));
         if (C->d_tprof) HIPCHK(hipFree(C->d_tprof));
         C->d_tprof = nullptr;
-        if ((st = dalloc(&C->d_tprof, (size_t)W * F->K * 8))) return st;
-        C->tprof_cap = (int64_t)W * F->K * 8;
+        if ((st = dalloc(&C->d_tprof, (size_t)W * F->K * SBC_TRACE_SLOTS))) return st;
+        C->tprof_cap = (int64_t)W * F->K * SBC_TRACE_SLOTS;
         C->gc.with_rec = -1;
     }
     SbcParams P = base_params(F, G, C);
@@ -1299,7 +1299,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         if (nparked > nparked_before && cfg->recompute_bc) HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
         nparked_before = nparked;
         if (trace_path) {
-            std::vector<int64_t> tb((size_t)n * F->K * 8);
+            std::vector<int64_t> tb((size_t)n * F->K * SBC_TRACE_SLOTS);
             HIPCHK(hipMemcpyAsync(tb.data(), C->d_tprof, sizeof(int64_t) * tb.size(), hipMemcpyDeviceToHost, C->stream));
             HIPCHK(hipStreamSynchronize(C->stream));
             FILE* fp = fopen(trace_path, "ab");

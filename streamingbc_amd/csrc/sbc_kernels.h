@@ -46,6 +46,8 @@ struct SbcBucketEnt {
     double sub;
 };
 
+#define SBC_TRACE_SLOTS 16
+
 struct SbcParams {
     const SbcVInfo* vinfo;
     const int32_t* col;
@@ -76,8 +78,10 @@ struct SbcParams {
     int32_t u0;
     int32_t v0;
     SbcItemRec* rec;      // [W][K] or nullptr
-    int64_t* tprof;       // [W][K][8]: timestamps (start, descent, ascent, commit), hubs probed, hubs scanned,
-                          // neighbours of probed hubs, neighbours of scanned hubs; or nullptr
+    int64_t* tprof;       // [W][K][SBC_TRACE_SLOTS] or nullptr: timestamps (start, descent end, ascent end,
+                          // commit end), hubs looked up, hubs scanned, neighbours of each; ascent time in the
+                          // light-vertex passes, the hub lookups, the hub staging and scans; ascent levels,
+                          // vertices over the ascent levels
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)

@@ -977,11 +977,13 @@ template <int TW>
 __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Item& X, const LevelSets& here, int tot,
                                                    int lev, TeamShared& ts, HubShared& hs, LightShared& ls,
                                                    int twave, unsigned long long& my_proc,
-                                                   unsigned long long& my_scan)
+                                                   unsigned long long& my_scan, int64_t* tp)
 {
+    int64_t tq = 0;  // trace: batch setup (13), edge terms and sums (14), leftovers (15)
     const int tid = threadIdx.x, lane = lane_id();
     constexpr int NT = TW * 64;
     for (int r0 = 0; r0 < tot; r0 += NT) {
+        if (tp && tid == 0) tq = (int64_t)wall_clock64();
         const int t = r0 + tid;
         int32_t k = 0;
         int len = 0;
@@ -1017,6 +1019,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             atomicMax(&ls.nb, slot + 1);
         }
         __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[13] += x - tq; tq = x; }
         const int nb = ls.nb;
         const int ne = nb > 0 ? ls.off[nb - 1] + ls.len[nb - 1] : 0;
         // one lane per edge
@@ -1064,6 +1067,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             my_scan += (unsigned long long)ne;
         }
         __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[14] += x - tq; tq = x; }
         // leftovers, one wavefront each
         const int no = ls.novf;
         for (int i = twave; i < no; i += TW) {
@@ -1077,6 +1081,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
         __syncthreads();
         if (tid == 0) { ls.nb = 0; ls.novf = 0; }
         __syncthreads();
+        if (tp && tid == 0) tp[15] += (int64_t)wall_clock64() - tq;
     }
 }
 
@@ -1126,8 +1131,12 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     if (du == INF32 && dv == INF32) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
     SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
-    int64_t* tp = P.tprof ? &P.tprof[((size_t)item * P.K + ri) * 8] : nullptr;
-    if (tp && leader) { const int64_t t0 = (int64_t)wall_clock64(); tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0; tp[4] = 0; tp[5] = 0; tp[6] = 0; tp[7] = 0; }
+    int64_t* tp = P.tprof ? &P.tprof[((size_t)item * P.K + ri) * SBC_TRACE_SLOTS] : nullptr;
+    if (tp && leader) {
+        const int64_t t0 = (int64_t)wall_clock64();
+        tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0;
+        for (int i = 4; i < SBC_TRACE_SLOTS; ++i) tp[i] = 0;
+    }
     if (gap == 0) {
         if (rec && leader) {
             rec->root_index = ri; rec->insertion = item; rec->kase = 0; rec->gap = 0;
@@ -1250,8 +1259,10 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         const int nd = dB - dA, ns = sB - sA, nu = uB - uA;
         const int tot = nd + ns + nu;
         if (tot == 0) break;
+        int64_t tl0 = 0;
+        if (tp && leader) { tl0 = (int64_t)wall_clock64(); tp[11] += 1; tp[12] += tot; }
         if constexpr (TW > 1) {
-            ascent_light_level<TW>(P, X, here, tot, lev, ts, *hs, *ls, twave, my_proc, my_scan);
+            ascent_light_level<TW>(P, X, here, tot, lev, ts, *hs, *ls, twave, my_proc, my_scan, tp);
         } else {
             for (int w = 0; w < tot; ++w) {
                 const int32_t k = here.at(X, w);
@@ -1292,7 +1303,10 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             }
             __syncthreads();
             const int np = hs->np;
+            int64_t tl1 = 0;
+            if (tp && leader) { tl1 = (int64_t)wall_clock64(); tp[8] += tl1 - tl0; }
             if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src);
+            if (tp && leader) { const int64_t t = (int64_t)wall_clock64(); tp[9] += t - tl1; tl1 = t; }
             // counters of every queued hub, one thread per hub; hubs not
             // resolved by lookups (not in plist, or their bucket overflowed:
             // kind 2) are staged in LDS with their state for the team scans
@@ -1335,6 +1349,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             }
             __syncthreads();
             if (leader) hs->n = 0;
+            if (tp && leader) tp[10] += (int64_t)wall_clock64() - tl1;
         }
         team_sync<TW>();
         below = here;
