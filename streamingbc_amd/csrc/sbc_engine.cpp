@@ -507,6 +507,7 @@ struct GraphCache {
     const void* g = nullptr;
     uint64_t generation = 0;
     int nwb = 0, grid = 0, heavy = 0, recompute = 0, W = 0, has_order = 0, with_rec = 0, park_min = 0;
+    int run_cap = 0, hub_batch = 0;  // test hooks baked into the captured arguments
 };
 
 struct extraArraysPerThread {
@@ -537,6 +538,8 @@ struct extraArraysPerThread {
     // per-workgroup term buckets of the hub lookup pass
     SbcBucketEnt* d_bucket = nullptr;
     int64_t bucket_blocks = 0;
+    int32_t* d_hubq = nullptr;  // per workgroup: hubs queued beyond the LDS queue
+    int64_t hubq_blocks = 0;
     WinStage win;
     GraphCache gc;
     int refs = 1;
@@ -570,6 +573,7 @@ static void ctx_free(extraArraysPerThread* C)
         if (q) (void)hipFree(q);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
+    if (C->d_hubq) (void)hipFree(C->d_hubq);
     if (C->win.d_desc) (void)hipFree(C->win.d_desc);
     if (C->win.d_ints) (void)hipFree(C->win.d_ints);
     if (C->win.d_vi) (void)hipFree(C->win.d_vi);
@@ -696,6 +700,19 @@ static sbcStatus ensure_bucket(extraArraysPerThread* C, const stinger* G, int64_
     return SBC_OK;
 }
 
+static sbcStatus ensure_hubq(extraArraysPerThread* C, int64_t grid)
+{
+    if (C->hubq_blocks >= grid) return SBC_OK;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (C->d_hubq) HIPCHK(hipFree(C->d_hubq));
+    C->d_hubq = nullptr;
+    sbcStatus st = dalloc(&C->d_hubq, (size_t)grid * SBC_HUBQ_CAP);
+    if (st) return st;
+    C->hubq_blocks = grid;
+    C->gc.with_rec = -1; // cached graph holds the old pointer
+    return SBC_OK;
+}
+
 static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArraysPerThread* C)
 {
     SbcParams P;
@@ -723,6 +740,7 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.qctr = C->d_qctr;
     P.err = F->d_err;
     P.bc_atomic = 1;
+    P.hubq = C->d_hubq;
     P.park = nullptr;
     P.parked = nullptr;
     P.park_min = 0;
@@ -731,6 +749,9 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     // test hook: a small run capacity forces the flat hub scan's overflow path
     const char* rc = getenv("SBC_HUB_RUN_CAP");
     P.run_cap = rc ? atoi(rc) : 0;
+    // test hook: fewer hubs per pass forces the batched hub passes
+    const char* hb = getenv("SBC_HUB_BATCH");
+    P.hub_batch = hb ? atoi(hb) : 0;
     return P;
 }
 
@@ -808,6 +829,7 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
     if (st) return st;
     if ((st = ensure_rec(C, F->K))) return st;
     if ((st = ensure_bucket(C, G, 4LL * C->num_cu))) return st;
+    if ((st = ensure_hubq(C, 4LL * C->num_cu))) return st;
     SbcParams P = base_params(F, G, C);
     P.u0 = (int32_t)u;
     P.v0 = (int32_t)v;
@@ -1044,7 +1066,8 @@ static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, co
 
 // After a window launch: finishes every parked item and resumes its root at
 // the next item, until no root parks.
-static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int64_t* total)
+static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
+                              int64_t* total)
 {
     for (;;) {
         int32_t np = 0;
@@ -1070,7 +1093,8 @@ static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const S
         Q.qlen = np;
         Q.start_item = C->d_start;
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
-        HIPCHK(sbc_launch_window(&Q, nwb, np, C->stream));
+        // at most the window's grid: per-workgroup scratch is sized for it
+        HIPCHK(sbc_launch_window(&Q, nwb, np < grid ? np : grid, C->stream));
         // the host copy of start items must not change before the upload ran
         HIPCHK(hipStreamSynchronize(C->stream));
     }
@@ -1152,6 +1176,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         grid = (int)(g < cap ? (g > 0 ? g : 1) : cap);
     }
     if ((st = ensure_bucket(C, G, grid))) return st;
+    if ((st = ensure_hubq(C, grid))) return st;
     // capacity: relayout once up front if any endpoint lacks room for the whole batch
     {
         std::vector<int32_t> extra(G->nv, 0);
@@ -1206,7 +1231,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     bool need_capture = use_graph &&
         (!gc.exec || gc.forest != F || gc.g != G || gc.generation != G->generation || gc.nwb != nwb ||
          gc.grid != grid || gc.heavy != heavy || gc.recompute != cfg->recompute_bc || gc.W != W ||
-         gc.has_order != has_order || gc.with_rec != 1 || gc.park_min != P.park_min);
+         gc.has_order != has_order || gc.with_rec != 1 || gc.park_min != P.park_min ||
+         gc.run_cap != P.run_cap || gc.hub_batch != P.hub_batch);
     if (need_capture) {
         if (gc.exec) { (void)hipGraphExecDestroy(gc.exec); gc.exec = nullptr; }
         if (gc.graph) { (void)hipGraphDestroy(gc.graph); gc.graph = nullptr; }
@@ -1222,6 +1248,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         gc.forest = F; gc.g = G; gc.generation = G->generation; gc.nwb = nwb; gc.grid = grid; gc.heavy = heavy;
         gc.recompute = cfg->recompute_bc; gc.W = W; gc.has_order = has_order; gc.with_rec = 1;
         gc.park_min = P.park_min;
+        gc.run_cap = P.run_cap;
+        gc.hub_batch = P.hub_batch;
     }
 
     std::vector<int32_t> prev_touched;
@@ -1289,11 +1317,11 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
             HIPCHK(hipGraphLaunch(gc.exec, C->stream));
             // items parked by this window are finished inside the same timed interval
-            if (P.park && (st = finish_parked(F, C, P, nwb, &nparked))) return st;
+            if (P.park && (st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
             if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
         } else {
             if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) return st;
-            if (P.park && (st = finish_parked(F, C, P, nwb, &nparked))) return st;
+            if (P.park && (st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
         }
         // the captured BC reduction ran before the parked items were finished
         if (nparked > nparked_before && cfg->recompute_bc) HIPCHK(sbc_launch_bc_reduce(&P, C->stream));

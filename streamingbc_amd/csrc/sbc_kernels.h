@@ -47,6 +47,7 @@ struct SbcBucketEnt {
 };
 
 #define SBC_TRACE_SLOTS 16
+#define SBC_HUBQ_CAP 8192   // hubs per workgroup and pass beyond the LDS queue (global overflow queue)
 
 struct SbcParams {
     const SbcVInfo* vinfo;
@@ -85,6 +86,8 @@ struct SbcParams {
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)
+    int32_t* hubq;        // per workgroup: SBC_HUBQ_CAP queued hubs beyond the LDS queue, or nullptr
+    int32_t hub_batch;    // hubs per pass (<= the LDS queue; smaller only to test the batching)
     // parked items: a whole-workgroup team whose next descent frontier has at
     // least park_min vertices stops, saves the item in park[ri] and appends
     // ri to parked (count in qctr[2]); the host finishes it with the wide

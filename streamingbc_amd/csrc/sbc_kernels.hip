@@ -262,6 +262,13 @@ struct alignas(16) HubShared {
     uint8_t rflag[16][RUN_CAP];
 };
 
+// Hubs per hub pass: the LDS queue's capacity (smaller only to test the
+// batched passes).
+__device__ __forceinline__ int hub_batch(const SbcParams& P)
+{
+    return (P.hub_batch > 0 && P.hub_batch < HUB_CAP) ? P.hub_batch : HUB_CAP;
+}
+
 // Light (non-hub) vertices of one ascent level, batched: every edge of
 // every batched vertex is handled by one lane, its term kept at the edge's
 // index, then each vertex sums its edges' terms in list order.
@@ -995,7 +1002,10 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             len = visible_len_thread(P, vi, X.q);
             if (len > HUB_MIN) {
                 const int pos = atomicAdd(&hs.n, 1);
-                if (pos < HUB_CAP) hs.list[pos] = k;
+                const int hcap = hub_batch(P);
+                if (pos < hcap) hs.list[pos] = k;
+                else if (P.hubq && pos < hcap + SBC_HUBQ_CAP)
+                    P.hubq[(size_t)blockIdx.x * SBC_HUBQ_CAP + (pos - hcap)] = k;
                 else ls.ovf[atomicAdd(&ls.novf, 1)] = k;
             } else {
                 light = true;
@@ -1089,15 +1099,38 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
 // (processed afterwards by the whole team); everything else runs on the
 // calling wavefront.  Returns true if k was queued.
 template <int TW>
-__device__ __forceinline__ bool queue_hub(HubShared* hs, int32_t k, int len)
+__device__ __forceinline__ bool queue_hub(const SbcParams& P, HubShared* hs, int32_t k, int len)
 {
     if (TW == 1 || len <= HUB_MIN) return false;
     int pos = 0;
     if (lane_id() == 0) pos = atomicAdd(&hs->n, 1);
     pos = __shfl(pos, 0);
-    if (pos >= HUB_CAP) return false;
-    if (lane_id() == 0) hs->list[pos] = k;
-    return true;
+    const int hcap = hub_batch(P);
+    if (pos < hcap) {
+        if (lane_id() == 0) hs->list[pos] = k;
+        return true;
+    }
+    if (P.hubq && pos < hcap + SBC_HUBQ_CAP) {
+        if (lane_id() == 0) P.hubq[(size_t)blockIdx.x * SBC_HUBQ_CAP + (pos - hcap)] = k;
+        return true;
+    }
+    return false;
+}
+
+// Number of queued hubs of the current pass, and batch hb..hb+batch of
+// them copied into the LDS queue (batch 0 is already there).
+__device__ __forceinline__ int hubs_queued(const SbcParams& P, const HubShared& hs)
+{
+    return min(hs.n, hub_batch(P) + (P.hubq ? SBC_HUBQ_CAP : 0));
+}
+
+template <int TW>
+__device__ __forceinline__ void load_hub_batch(const SbcParams& P, HubShared& hs, int hb, int nh)
+{
+    if (hb == 0) return;
+    const int32_t* q = P.hubq + (size_t)blockIdx.x * SBC_HUBQ_CAP + (hb - hub_batch(P));
+    for (int i = threadIdx.x; i < nh; i += TW * 64) hs.list[i] = q[i];
+    __syncthreads();
 }
 
 // Applies window item `item` to root ri.  Returns true if the item was
@@ -1176,7 +1209,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             const int32_t c = X.list[idx];
             const SbcVInfo vi = P.vinfo[c];
             const int len = visible_len(P, vi, X.q);
-            if (queue_hub<TW>(hs, c, len)) continue;
+            if (queue_hub<TW>(P, hs, c, len)) continue;
             const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
             const int64_t dc = ld_acq64(&X.dpath[c]);
             descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, ts);
@@ -1186,9 +1219,13 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         }
         if constexpr (TW > 1) {
             __syncthreads();
-            const int nh = min(hs->n, HUB_CAP);
-            if (nh > 0) descent_hubs_flat<TW>(P, X, nh, lc, ts, *hs, *ls, twave);
-            __syncthreads();
+            const int nall = hubs_queued(P, *hs);
+            for (int hb = 0; hb < nall; hb += hub_batch(P)) {
+                const int nh = min(nall - hb, hub_batch(P));
+                load_hub_batch<TW>(P, *hs, hb, nh);
+                descent_hubs_flat<TW>(P, X, nh, lc, ts, *hs, *ls, twave);
+                __syncthreads();
+            }
             if (leader) hs->n = 0;
         }
         team_sync<TW>();
@@ -1275,81 +1312,87 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         }
         if constexpr (TW > 1) {
             __syncthreads();
-            const int nh = min(hs->n, HUB_CAP);
-            // split the queued hubs: ascent-only indexed hubs on level 1 or 2
-            // whose lists are much longer than the lookup work are handled
-            // together by table lookups; the rest are scanned one by one
-            if (threadIdx.x == 0) hs->np = 0;
-            __syncthreads();
-            ProbeSources src;
-            src.below = below;
-            src.dA = here.dA;
-            src.dB = here.dB;
-            src.rstart = vr.start;
-            src.nr = lev == 2 ? len_r : 0;
-            const long cand = (long)src.size();
-            for (int h = threadIdx.x; h < nh; h += TW * 64) {
-                const int32_t k = hs->list[h];
-                bool pk = false;
-                if (P.hub && P.bucket && lev >= 1 && lev <= 2 && P.hub[k].y > 0 && sb_type(ld_sb(&X.sb[k])) == 3)
-                    pk = cand < (long)P.hub[k].y;
-                int hi = -1;
-                if (pk) {
-                    hi = atomicAdd(&hs->np, 1);
-                    hs->plist[hi] = k;
-                    hs->kind[hi] = 0;
+            if (tp && leader) tp[8] += (int64_t)wall_clock64() - tl0;
+            const int nall = hubs_queued(P, *hs);
+            for (int hb = 0; hb < nall; hb += hub_batch(P)) {
+                const int nh = min(nall - hb, hub_batch(P));
+                load_hub_batch<TW>(P, *hs, hb, nh);
+                int64_t tl1 = 0;
+                if (tp && leader) tl1 = (int64_t)wall_clock64();
+                // split the batch: ascent-only indexed hubs on level 1 or 2
+                // whose lists are much longer than the lookup work are
+                // handled together by table lookups; the rest are scanned
+                if (threadIdx.x == 0) hs->np = 0;
+                __syncthreads();
+                ProbeSources src;
+                src.below = below;
+                src.dA = here.dA;
+                src.dB = here.dB;
+                src.rstart = vr.start;
+                src.nr = lev == 2 ? len_r : 0;
+                const long cand = (long)src.size();
+                for (int h = threadIdx.x; h < nh; h += TW * 64) {
+                    const int32_t k = hs->list[h];
+                    bool pk = false;
+                    if (P.hub && P.bucket && lev >= 1 && lev <= 2 && P.hub[k].y > 0 && sb_type(ld_sb(&X.sb[k])) == 3)
+                        pk = cand < (long)P.hub[k].y;
+                    int hi = -1;
+                    if (pk) {
+                        hi = atomicAdd(&hs->np, 1);
+                        hs->plist[hi] = k;
+                        hs->kind[hi] = 0;
+                    }
+                    hs->pidx[h] = hi;
                 }
-                hs->pidx[h] = hi;
+                __syncthreads();
+                const int np = hs->np;
+                if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src);
+                if (tp && leader) { const int64_t t = (int64_t)wall_clock64(); tp[9] += t - tl1; tl1 = t; }
+                // counters of every hub of the batch, one thread per hub; hubs
+                // not resolved by lookups (not in plist, or their bucket
+                // overflowed: kind 2) are staged in LDS with their state for
+                // the scans
+                if (threadIdx.x == 0) hs->ns = 0;
+                __syncthreads();
+                for (int h = threadIdx.x; h < nh; h += TW * 64) {
+                    const int32_t k = hs->list[h];
+                    const int hi = hs->pidx[h];
+                    const bool looked_up = hi >= 0 && hs->kind[hi] != 2;
+                    const SbcVInfo vi = P.vinfo[k];
+                    const int len = visible_len_thread(P, vi, X.q);
+                    atomicAdd(&ts.stat[0], 1ull);
+                    atomicAdd(&ts.stat[1], (unsigned long long)len);
+                    if (tp) {
+                        atomicAdd((unsigned long long*)&tp[looked_up ? 4 : 5], 1ull);
+                        atomicAdd((unsigned long long*)&tp[looked_up ? 6 : 7], (unsigned long long)len);
+                    }
+                    if (!looked_up) {
+                        const int j = atomicAdd(&hs->ns, 1);
+                        const KState kst = kstate(X, k);
+                        ls->k[j] = k;
+                        ls->start[j] = vi.start;
+                        ls->len[j] = len;
+                        ls->isD[j] = kst.isD ? 1 : 0;
+                        ls->sk_old[j] = kst.sk_old;
+                        ls->sk_new[j] = kst.sk_new;
+                        ls->base[j] = kst.base;
+                    }
+                }
+                __syncthreads();
+                const int nsh = hs->ns;
+                if (P.bucket) {
+                    if (nsh > 0) ascent_hubs_flat<TW>(P, X, nsh, lev, ts, *hs, *ls, twave);
+                } else {
+                    for (int j = 0; j < nsh; ++j) {
+                        const KState kst{ls->isD[j] != 0, ls->sk_old[j], ls->sk_new[j], ls->base[j]};
+                        const SbcVInfo vi{ls->start[j], 0, 0, 0};
+                        ascent_vertex_team<TW>(P, X, ls->k[j], vi, ls->len[j], kst, lev, ts, *hs, twave);
+                    }
+                }
+                __syncthreads();
+                if (tp && leader) tp[10] += (int64_t)wall_clock64() - tl1;
             }
-            __syncthreads();
-            const int np = hs->np;
-            int64_t tl1 = 0;
-            if (tp && leader) { tl1 = (int64_t)wall_clock64(); tp[8] += tl1 - tl0; }
-            if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src);
-            if (tp && leader) { const int64_t t = (int64_t)wall_clock64(); tp[9] += t - tl1; tl1 = t; }
-            // counters of every queued hub, one thread per hub; hubs not
-            // resolved by lookups (not in plist, or their bucket overflowed:
-            // kind 2) are staged in LDS with their state for the team scans
-            if (threadIdx.x == 0) hs->ns = 0;
-            __syncthreads();
-            for (int h = threadIdx.x; h < nh; h += TW * 64) {
-                const int32_t k = hs->list[h];
-                const int hi = hs->pidx[h];
-                const bool looked_up = hi >= 0 && hs->kind[hi] != 2;
-                const SbcVInfo vi = P.vinfo[k];
-                const int len = visible_len_thread(P, vi, X.q);
-                atomicAdd(&ts.stat[0], 1ull);
-                atomicAdd(&ts.stat[1], (unsigned long long)len);
-                if (tp) {
-                    atomicAdd((unsigned long long*)&tp[looked_up ? 4 : 5], 1ull);
-                    atomicAdd((unsigned long long*)&tp[looked_up ? 6 : 7], (unsigned long long)len);
-                }
-                if (!looked_up) {
-                    const int j = atomicAdd(&hs->ns, 1);
-                    const KState kst = kstate(X, k);
-                    ls->k[j] = k;
-                    ls->start[j] = vi.start;
-                    ls->len[j] = len;
-                    ls->isD[j] = kst.isD ? 1 : 0;
-                    ls->sk_old[j] = kst.sk_old;
-                    ls->sk_new[j] = kst.sk_new;
-                    ls->base[j] = kst.base;
-                }
-            }
-            __syncthreads();
-            const int nsh = hs->ns;
-            if (P.bucket) {
-                if (nsh > 0) ascent_hubs_flat<TW>(P, X, nsh, lev, ts, *hs, *ls, twave);
-            } else {
-                for (int j = 0; j < nsh; ++j) {
-                    const KState kst{ls->isD[j] != 0, ls->sk_old[j], ls->sk_new[j], ls->base[j]};
-                    const SbcVInfo vi{ls->start[j], 0, 0, 0};
-                    ascent_vertex_team<TW>(P, X, ls->k[j], vi, ls->len[j], kst, lev, ts, *hs, twave);
-                }
-            }
-            __syncthreads();
             if (leader) hs->n = 0;
-            if (tp && leader) tp[10] += (int64_t)wall_clock64() - tl1;
         }
         team_sync<TW>();
         below = here;

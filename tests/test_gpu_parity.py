@@ -221,12 +221,17 @@ def _hub_stream(hg, n, seed):
     return uu, vv
 
 
-@pytest.mark.parametrize("batched,run_cap", [(False, None), (True, None), (True, "3")])
-def test_hub_lookup_path_rmat14(batched, run_cap, monkeypatch):
+@pytest.mark.parametrize("batched,run_cap,hub_batch", [(False, None, None), (True, None, None), (True, "3", None),
+                                                       (True, None, "3"), (False, None, "2")])
+def test_hub_lookup_path_rmat14(batched, run_cap, hub_batch, monkeypatch):
     # run_cap "3": the flat hub scan keeps 3 terms per wavefront, so nearly
-    # every hub level takes the overflow path (whole-team rescans)
+    # every hub level takes the overflow path (whole-team rescans);
+    # hub_batch: hubs per pass, so levels with more hubs run several passes
+    # through the global overflow queue
     if run_cap:
         monkeypatch.setenv("SBC_HUB_RUN_CAP", run_cap)
+    if hub_batch:
+        monkeypatch.setenv("SBC_HUB_BATCH", hub_batch)
     hg = sbc.gen_rmat(14, 16 << 14, seed=3, keep_lcc=True)
     assert hg.degree().max() > 1000, "need vertices above the lookup-table threshold"
     roots = hg.pick_roots(32, 5)
