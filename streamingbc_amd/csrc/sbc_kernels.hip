@@ -594,10 +594,16 @@ __device__ __forceinline__ void ascent_vertex_team(const SbcParams& P, const Ite
 // Scan of chunks [a, b) of hub k (state ks): marks parents like the other
 // scans and appends the terms, in adjacency order, to the calling
 // wavefront's run (entries rl.. ; those past cap are dropped but counted).
+#ifndef SBC_FLAT_UNROLL
+#define SBC_FLAT_UNROLL 8
+#endif
+constexpr int FLAT_UNROLL = SBC_FLAT_UNROLL;  // chunks whose loads one wavefront issues together in hub scans
+
 __device__ __forceinline__ int ascent_scan_run(const SbcParams& P, const Item& X, int32_t k, int32_t start, int len,
                                                const KState& ks, int lev, int a, int b, TeamShared& ts,
                                                SbcBucketEnt* run, int rl, int cap)
 {
+    constexpr int SCAN_UNROLL = FLAT_UNROLL;
     const int lane = lane_id();
     for (int ch = a; ch < b; ch += SCAN_UNROLL) {
         int32_t cc[SCAN_UNROLL];
