@@ -821,6 +821,66 @@ struct ProbeSources {
     __device__ int size() const { return below.size() + (dB - dA) + nr; }
 };
 
+// One lane's entry of a hub's term bucket (lanes >= n hold an empty entry
+// with the largest key).
+struct BucketLane {
+    int key;
+    int flag;
+    double add, sub;
+};
+
+__device__ __forceinline__ BucketLane bucket_lane(const SbcBucketEnt* b, int n)
+{
+    const int lane = lane_id();
+    BucketLane e{INF32, 0, 0.0, 0.0};
+    if (lane < n) {
+        e.key = b[lane].key;
+        e.flag = b[lane].flag;
+        e.add = b[lane].add;
+        e.sub = b[lane].sub;
+    }
+    return e;
+}
+
+__device__ __forceinline__ double shfl_d(double x, int src)
+{
+    int2 v = __builtin_bit_cast(int2, x);
+    v.x = __shfl(v.x, src);
+    v.y = __shfl(v.y, src);
+    return __builtin_bit_cast(double, v);
+}
+
+// base + the bucket's terms in position order: a wavefront bitonic sort of
+// (key, lane), then each lane takes the terms of the lane it sorted to.
+__device__ __forceinline__ double bucket_sum(const BucketLane& e, double base)
+{
+    const int lane = lane_id();
+    int key = e.key;
+    int idx = lane;
+    for (int k2 = 2; k2 <= 64; k2 <<= 1) {
+        for (int j = k2 >> 1; j > 0; j >>= 1) {
+            const int pk = __shfl_xor(key, j), pi = __shfl_xor(idx, j);
+            const bool up = (lane & k2) == 0;
+            const bool lower = (lane & j) == 0;
+            const bool take = lower ? ((pk < key) == up) : ((pk > key) == up);
+            if (take) { key = pk; idx = pi; }
+        }
+    }
+    const int fl = __shfl(e.flag, idx);
+    const double ad = shfl_d(e.add, idx);
+    const double sb2 = shfl_d(e.sub, idx);
+    const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
+    uint64_t m = cm | sm;
+    double acc = base;
+    while (m) {
+        const int l2 = __builtin_ctzll(m);
+        m &= m - 1;
+        if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
+        if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
+    }
+    return acc;
+}
+
 // New deltas of the ascent-only hubs hs.plist[0..np) of level lev (1 or 2)
 // without scanning their adjacency lists.  Their terms can only come from
 // processed vertices one level down (children) and moved descent vertices on
@@ -942,43 +1002,30 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     }
     if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
     __syncthreads();
-    // one wavefront per hub: sort its bucket by position, sum in that order
-    for (int hi = twave; hi < np; hi += TW) {
-        const int n = hs.bcnt[hi];
-        if (n > SBC_BUCKET) {
-            if (lane == 0) hs.kind[hi] = 2;
-            continue;
+    // one wavefront per hub, two hubs at a time (their loads overlap): sort
+    // the bucket by position, sum in that order
+    for (int h0 = twave; h0 < np; h0 += 2 * TW) {
+        BucketLane e[2];
+        int n[2];
+        double base[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int hi = h0 + t * TW;
+            n[t] = hi < np ? hs.bcnt[hi] : 0;
+            e[t] = bucket_lane(bk + (size_t)(hi < np ? hi : 0) * SBC_BUCKET, n[t] <= SBC_BUCKET ? n[t] : 0);
+            base[t] = hi < np ? X.delta[hs.plist[hi]] : 0.0;
         }
-        const int32_t k = hs.plist[hi];
-        const SbcBucketEnt* b = bk + (size_t)hi * SBC_BUCKET;
-        int key = lane < n ? b[lane].key : INF32;
-        int idx = lane;
-        for (int k2 = 2; k2 <= 64; k2 <<= 1) {
-            for (int j = k2 >> 1; j > 0; j >>= 1) {
-                const int pk = __shfl_xor(key, j), pi = __shfl_xor(idx, j);
-                const bool up = (lane & k2) == 0;
-                const bool lower = (lane & j) == 0;
-                const bool take = lower ? ((pk < key) == up) : ((pk > key) == up);
-                if (take) { key = pk; idx = pi; }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int hi = h0 + t * TW;
+            if (hi >= np) continue;
+            if (n[t] > SBC_BUCKET) {
+                if (lane == 0) hs.kind[hi] = 2;
+                continue;
             }
+            const double acc = bucket_sum(e[t], base[t]);
+            if (lane == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
         }
-        uint8_t fl = 0;
-        double ad = 0.0, sb2 = 0.0;
-        if (lane < n) {
-            fl = b[idx].flag;
-            ad = b[idx].add;
-            sb2 = b[idx].sub;
-        }
-        const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
-        uint64_t m = cm | sm;
-        double acc = X.delta[k];
-        while (m) {
-            const int l2 = __builtin_ctzll(m);
-            m &= m - 1;
-            if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
-            if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
-        }
-        if (lane == 0) st_reld(&X.ndel[k], acc);
     }
     __syncthreads();
 }
