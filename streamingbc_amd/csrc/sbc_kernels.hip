@@ -235,6 +235,7 @@ constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by o
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
 constexpr int PHASH = 2 * HUB_CAP;      // LDS hash: vertex -> number in the level's lookup list
+constexpr int TWOHOP_CAP = 1024;        // two-hop set of the root kept for level-3 lookups
 
 struct alignas(16) HubShared {
     int n;
@@ -249,6 +250,8 @@ struct alignas(16) HubShared {
     int phv[PHASH];       // hash values (number in plist)
     int pidx[HUB_CAP];    // queued hub -> number in plist, or -1
     uint8_t kind[HUB_CAP];
+    int ntwo;                 // entries of two (-1: the root's two-hop set is too large)
+    int32_t two[TWOHOP_CAP];  // the root's two-hop set: parent candidates of level-3 hubs
     // flat scan of the staged hubs: segment i = j + w holds the terms of hub
     // j found by wavefront w, entries [seg_b, seg_e) of w's run
     int seg_j[HUB_CAP + 16];
@@ -816,10 +819,38 @@ __device__ __forceinline__ int phash_find(const HubShared& hs, int32_t v)
 struct ProbeSources {
     LevelSets below;
     int dA, dB;          // descent vertices on the level
-    int32_t rstart;      // root adjacency
-    int nr;              // visible root neighbours (0 unless level 2)
+    const int32_t* pc;   // parent candidates: every vertex of level lev-1 (level 2: the root's
+                         // neighbours; level 3: the root's two-hop set), possibly more
+    int nr;              // number of parent candidates
     __device__ int size() const { return below.size() + (dB - dA) + nr; }
 };
+
+// The root's two-hop set (neighbours of its neighbours, duplicates kept)
+// into hs.two, or hs.ntwo = -1 if it has more than TWOHOP_CAP entries.
+// Every level-2 vertex is in it, and it holds no vertex below level 2, so a
+// member adjacent to a level-3 vertex is one of that vertex's parents.
+template <int TW>
+__device__ __forceinline__ void two_hop(const SbcParams& P, const Item& X, int32_t rstart, int len_r, HubShared& hs,
+                                        LightShared& ls)
+{
+    const int tid = threadIdx.x;
+    constexpr int NT = TW * 64;
+    int lx = 0;
+    int32_t xs = 0;
+    if (len_r <= NT && tid < len_r) {
+        const int32_t x = P.col[rstart + tid];
+        const SbcVInfo vi = P.vinfo[x];
+        lx = visible_len_thread(P, vi, X.q);
+        xs = vi.start;
+    }
+    int total;
+    const int off = block_exscan<TW>(lx, ls.scan, total);
+    const bool fits = len_r <= NT && total <= TWOHOP_CAP;
+    if (fits)
+        for (int e = 0; e < lx; ++e) hs.two[off + e] = P.col[xs + e];
+    if (tid == 0) hs.ntwo = fits ? total : -1;
+    __syncthreads();
+}
 
 // One lane's entry of a hub's term bucket (lanes >= n hold an empty entry
 // with the largest key).
@@ -923,7 +954,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         if (i < nsrc) {
             if (i < nb) w = src.below.at(X, i);
             else if (i < nb + ndl) w = X.list[src.dA + (i - nb)];
-            else w = P.col[src.rstart + (i - nb - ndl)];
+            else w = src.pc[i - nb - ndl];
             const SbcVInfo vi = P.vinfo[w];
             len = visible_len_thread(P, vi, X.q);
             start = vi.start;
@@ -964,7 +995,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const int hi2 = phash_find(hs, c);
             if (hi2 < 0) continue;
             if (ls.isD[own]) {
-                mark_parent(X, ts, ws, 1);
+                mark_parent(X, ts, ws, lev - 1);
             } else {
                 const int32_t k = hs.plist[hi2];
                 const int4 hb = P.hub[k];
@@ -980,7 +1011,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const int4 hb = P.hub[k];
             const int pos = hub_find(P.htab + hb.x, hb.z, ws);
             if (pos < 0) continue;
-            if (ls.eflag[si]) mark_parent(X, ts, ws, 1);
+            if (ls.eflag[si]) mark_parent(X, ts, ws, lev - 1);
             else probe_term(P, X, hs, bk, k, hi2, lev, ws, pos);
         }
         __syncthreads();
@@ -1341,6 +1372,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     LevelSets below{0, 0, 0, 0, 0, 0};
     SbcVInfo vr = P.vinfo[X.r];
     const int len_r = visible_len(P, vr, X.q);
+    bool two_done = false;  // two-hop parent candidates computed for this item
     for (int lev = base + nf - 1; lev >= 0; --lev) {
         const int jj = lev - base;
         int dA = 0, dB = 0, sA = 0, sB = 0;
@@ -1381,13 +1413,27 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 src.below = below;
                 src.dA = here.dA;
                 src.dB = here.dB;
-                src.rstart = vr.start;
-                src.nr = lev == 2 ? len_r : 0;
+                src.pc = nullptr;
+                src.nr = 0;
+                if (lev == 2) {
+                    src.pc = P.col + vr.start;
+                    src.nr = len_r;
+                } else if (lev == 3 && P.hub && P.bucket) {
+                    if (!two_done) {
+                        two_hop<TW>(P, X, vr.start, len_r, *hs, *ls);
+                        two_done = true;
+                    }
+                    if (hs->ntwo >= 0) {
+                        src.pc = hs->two;
+                        src.nr = hs->ntwo;
+                    }
+                }
                 const long cand = (long)src.size();
                 for (int h = threadIdx.x; h < nh; h += TW * 64) {
                     const int32_t k = hs->list[h];
                     bool pk = false;
-                    if (P.hub && P.bucket && lev >= 1 && lev <= 2 && P.hub[k].y > 0 && sb_type(ld_sb(&X.sb[k])) == 3)
+                    if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && P.hub[k].y > 0 &&
+                        sb_type(ld_sb(&X.sb[k])) == 3)
                         pk = cand < (long)P.hub[k].y;
                     int hi = -1;
                     if (pk) {

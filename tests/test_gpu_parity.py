@@ -345,3 +345,34 @@ def test_parked_items_match_oracle(recompute):
     assert ctx2.last_parked() == 0
     for a, b in zip(f.trees(), f2.trees()):
         assert np.array_equal(a, b)
+
+
+def test_hub_lookup_level3_leaf_roots():
+    """Roots that are leaves hanging off a low-degree vertex see the hubs on
+    level 3; their ascent-only hubs there are resolved by table lookups with
+    the root's two-hop set as parent candidates.  Exact against the oracle."""
+    hg = sbc.gen_rmat(14, 16 << 14, seed=3, keep_lcc=True)
+    deg = hg.degree()
+    leaves = []
+    for x in np.flatnonzero(deg == 1):
+        nb = hg.col[hg.rowptr[x]]
+        if 2 <= deg[nb] <= 24:
+            leaves.append(x)
+        if len(leaves) == 24:
+            break
+    assert len(leaves) >= 8
+    roots = np.array(leaves, dtype=np.int64)
+    u, v = _hub_stream(hg, 48, 8)
+    print(f"[level3] rmat14 k={len(roots)} n={len(u)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    lv, _, _ = f.tree(0)
+    assert (deg[lv == 3] > 512).any(), "expected hubs on level 3"
+    rec = f.insert_batched(g, ctx, u, v, sbc.BatchConfig(window=16, block_threads=1024), records=True)
+    for i in range(len(u)):
+        rec_o = o.insert(int(u[i]), int(v[i]), O_FINE, items=True)
+        for key in ("affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec[i][key], rec_o[key]), f"insertion {i} {key}"
+    _compare_trees(f, o, "level-3 lookups")
+    assert_bc_close(f.bc(), o.bc())
