@@ -713,24 +713,42 @@ __device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item&
     __syncthreads();
     for (int j = twave; j < nsh; j += TW) {
         if (hs.sovf[j]) continue;
+        // lane w < TW: wavefront w's segment of hub j (start, length); an
+        // inclusive prefix over them numbers the hub's terms in order
+        int sbeg = 0, slen = 0;
+        if (lane < TW && hs.seg_j[j + lane] == j) {
+            sbeg = hs.seg_b[j + lane];
+            slen = hs.seg_e[j + lane] - sbeg;
+        }
+        int pre = slen;
+        for (int d = 1; d < TW; d <<= 1) {
+            const int y = __shfl_up(pre, d);
+            if (lane >= d) pre += y;
+        }
+        const int T = __shfl(pre, TW - 1);
         double acc = ls.base[j];
-        for (int i = j; i < j + TW; ++i) {
-            if (hs.seg_j[i] != j) continue;
-            const SbcBucketEnt* r = runs + (size_t)(i - j) * share;
-            const int eb = hs.seg_b[i], ee = hs.seg_e[i];
-            for (int e0 = eb; e0 < ee; e0 += 64) {
-                const int e = e0 + lane;
-                const int fl = e < ee ? r[e].flag : 0;
-                const double ad = (fl & 1) ? r[e].add : 0.0;
-                const double sb2 = (fl & 2) ? r[e].sub : 0.0;
-                const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
-                uint64_t m = cm | sm;
-                while (m) {
-                    const int l2 = __builtin_ctzll(m);
-                    m &= m - 1;
-                    if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
-                    if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
-                }
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            const int t = t0 + lane;
+            int w = 0;  // segment of term t: segments ending at or before t
+            for (int q = 0; q < TW; ++q) w += t >= __shfl(pre, q) ? 1 : 0;
+            const int wq = w < TW ? w : TW - 1;
+            const int excl = __shfl(pre - slen, wq);
+            const int beg = __shfl(sbeg, wq);
+            int fl = 0;
+            double ad = 0.0, sb2 = 0.0;
+            if (t < T) {
+                const SbcBucketEnt& e = runs[(size_t)wq * share + beg + (t - excl)];
+                fl = e.flag;
+                ad = e.add;
+                sb2 = e.sub;
+            }
+            const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
+            uint64_t m = cm | sm;
+            while (m) {
+                const int l2 = __builtin_ctzll(m);
+                m &= m - 1;
+                if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
+                if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
             }
         }
         if (lane == 0) st_reld(&X.ndel[ls.k[j]], acc);
@@ -927,7 +945,7 @@ __device__ __forceinline__ double bucket_sum(const BucketLane& e, double base)
 template <int TW>
 __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item& X, int np, int lev,
                                                   TeamShared& ts, HubShared& hs, LightShared& ls,
-                                                  const ProbeSources& src)
+                                                  const ProbeSources& src, int64_t* tp)
 {
     const int tid = threadIdx.x, lane = lane_id(), twave = tid >> 6;
     constexpr int NT = TW * 64;
@@ -1033,32 +1051,73 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     }
     if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
     __syncthreads();
-    // one wavefront per hub, two hubs at a time (their loads overlap): sort
-    // the bucket by position, sum in that order
-    for (int h0 = twave; h0 < np; h0 += 2 * TW) {
-        BucketLane e[2];
-        int n[2];
-        double base[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int hi = h0 + t * TW;
-            n[t] = hi < np ? hs.bcnt[hi] : 0;
-            e[t] = bucket_lane(bk + (size_t)(hi < np ? hi : 0) * SBC_BUCKET, n[t] <= SBC_BUCKET ? n[t] : 0);
-            base[t] = hi < np ? X.delta[hs.plist[hi]] : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int hi = h0 + t * TW;
-            if (hi >= np) continue;
-            if (n[t] > SBC_BUCKET) {
-                if (lane == 0) hs.kind[hi] = 2;
-                continue;
+    if (tp && tid == 0) tp[12] -= (int64_t)wall_clock64();  // trace: sort-and-sum time
+    // buckets of at most 8 terms: eight hubs per wavefront, an 8-lane group
+    // each (bitonic sort of the group's keys, the group leader sums in key
+    // order); larger buckets are marked and done one wavefront per hub
+    {
+        const int g = lane >> 3, gl = lane & 7, gb = lane & ~7;
+        for (int h0 = twave * 8; h0 < np; h0 += TW * 8) {
+            const int hi = h0 + g;
+            const int n = hi < np ? hs.bcnt[hi] : 0;
+            const bool small = hi < np && n <= 8;
+            int key = INF32, fl = 0, idx = gl;
+            double ad = 0.0, sb2 = 0.0, base = 0.0;
+            if (small) {
+                if (gl < n) {
+                    const SbcBucketEnt& e = bk[(size_t)hi * SBC_BUCKET + gl];
+                    key = e.key;
+                    fl = e.flag;
+                    ad = e.add;
+                    sb2 = e.sub;
+                }
+                if (gl == 0) base = X.delta[hs.plist[hi]];
             }
-            const double acc = bucket_sum(e[t], base[t]);
+            for (int k2 = 2; k2 <= 8; k2 <<= 1) {
+                for (int jj = k2 >> 1; jj > 0; jj >>= 1) {
+                    const int pk = __shfl_xor(key, jj), pi = __shfl_xor(idx, jj);
+                    const bool up = k2 == 8 || (lane & k2) == 0;
+                    const bool lower = (lane & jj) == 0;
+                    const bool take = lower ? ((pk < key) == up) : ((pk > key) == up);
+                    if (take) { key = pk; idx = pi; }
+                }
+            }
+            const int src = gb | idx;
+            const int sfl = __shfl(fl, src);
+            const double sad = shfl_d(ad, src), ssb = shfl_d(sb2, src);
+            double acc = base;
+            for (int q = 0; q < 8; ++q) {
+                const int f2 = __shfl(sfl, gb + q);
+                const double a2 = shfl_d(sad, gb + q), b2 = shfl_d(ssb, gb + q);
+                if (gl == 0 && q < n) {
+                    if (f2 & 1) acc = acc + a2;
+                    if (f2 & 2) acc = acc - b2;
+                }
+            }
+            if (small && gl == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
+            if (hi < np && gl == 0) hs.kind[hi] = n > SBC_BUCKET ? 2 : (n > 8 ? 1 : 0);
+        }
+    }
+    __syncthreads();
+    // one wavefront per larger bucket (kind 1), two hubs at a time
+    for (int h0 = 0; h0 < np; h0 += 64) {
+        const int hl = h0 + lane;
+        uint64_t bm = ballot(hl < np && hs.kind[hl] == 1);
+        // the wavefronts share out the marked hubs of this group of 64
+        int r = 0;
+        while (bm) {
+            const int l0 = __builtin_ctzll(bm);
+            bm &= bm - 1;
+            if ((r++ % TW) != twave) continue;
+            const int hi = h0 + l0;
+            const int n = hs.bcnt[hi];
+            const BucketLane e = bucket_lane(bk + (size_t)hi * SBC_BUCKET, n);
+            const double acc = bucket_sum(e, X.delta[hs.plist[hi]]);
             if (lane == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
         }
     }
     __syncthreads();
+    if (tp && tid == 0) tp[12] += (int64_t)wall_clock64();
 }
 
 // All vertices of ascent level `here`: hubs are queued in hs.list for the
@@ -1382,7 +1441,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         const int tot = nd + ns + nu;
         if (tot == 0) break;
         int64_t tl0 = 0;
-        if (tp && leader) { tl0 = (int64_t)wall_clock64(); tp[11] += 1; tp[12] += tot; }
+        if (tp && leader) { tl0 = (int64_t)wall_clock64(); tp[11] += 1; }
         if constexpr (TW > 1) {
             ascent_light_level<TW>(P, X, here, tot, lev, ts, *hs, *ls, twave, my_proc, my_scan, tp);
         } else {
@@ -1445,7 +1504,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 }
                 __syncthreads();
                 const int np = hs->np;
-                if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src);
+                if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src, tp);
                 if (tp && leader) { const int64_t t = (int64_t)wall_clock64(); tp[9] += t - tl1; tl1 = t; }
                 // counters of every hub of the batch, one thread per hub; hubs
                 // not resolved by lookups (not in plist, or their bucket
