@@ -180,8 +180,9 @@ def main():
         plan, X = sched.plan(f, g, ctx, u[a:b], v[a:b])
         c = sbc.BatchConfig(window=W, block_threads=cfg.block_threads, grid_blocks=cfg.grid_blocks,
                             use_graph=cfg.use_graph, heavy_roots=cfg.heavy_roots if plan.heavy_roots < 0
-                            else plan.heavy_roots, root_order=plan.root_order, park_min=cfg.park_min)
-        rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or X is not None)
+                            else plan.heavy_roots, root_order=plan.root_order, park_min=cfg.park_min,
+                            use_context_order=plan.device_order)
+        rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or sched.needs_records)
         parked[0] += ctx.last_parked()
         sched.observe(plan, X, rec)
         return rec

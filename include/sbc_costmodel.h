@@ -65,6 +65,19 @@ sbcStatus streamingBCWindowFeatures(bcForest *forest, struct stinger *G, extraAr
                                     const int64_t *u, const int64_t *v, int64_t count, const float *root_hist,
                                     float *X);
 
+/* Schedules the next window on the device: features of every (insertion,
+ * root) item, BF16-MFMA predictions, per-root window cost
+ * sum(expm1(max(pred, 0)) + item_overhead), roots sorted by descending cost
+ * into the context's root order (used by the next batched call with
+ * use_context_order = 1; any batched call consumes it, so schedule before
+ * each call that uses it).  Ties keep root-array order.  heavy_out (optional): roots with cost >=
+ * light_max (light_max <= 0: -1, every root heavy).  pred_out (optional):
+ * the predictions, pred_out[i * K + r].  Needs K <= 4096. */
+sbcStatus streamingBCScheduleWindow(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,
+                                    sbcCostModel *model, const int64_t *u, const int64_t *v, int64_t count,
+                                    const float *root_hist, float light_max, float item_overhead,
+                                    int32_t *heavy_out, float *pred_out);
+
 #ifdef __cplusplus
 }
 #endif

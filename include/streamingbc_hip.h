@@ -195,6 +195,8 @@ typedef struct {
     const int32_t *root_order; /* NULL: root-array order */
     int32_t park_min;       /* an item whose descent frontier reaches this many vertices is finished by
                                kernels spanning the whole GPU (0: 4096, < 0: never) */
+    int32_t use_context_order; /* 1: root order left in the context by streamingBCScheduleWindow
+                                  (sbc_costmodel.h); root_order must be NULL */
 } sbcBatchConfig;
 
 sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,
@@ -204,6 +206,10 @@ sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, ext
 /* Number of items of the last streamingBCInsertEdgesBatched call that were
  * finished by the whole-GPU kernels (sbcBatchConfig.park_min). */
 int64_t streamingBCLastParked(extraArraysPerThread **eAPT);
+
+/* Compute units of the context's device.  With grid_blocks = 0 a batched call
+ * runs min(max(heavy roots, light teams), 2 x this) teams at once. */
+int32_t streamingBCComputeUnits(extraArraysPerThread **eAPT);
 
 /* Records of the last insertEdgeStreamingBC call (K entries). */
 sbcStatus streamingBCLastRecords(extraArraysPerThread **eAPT, sbcItemRecord *out, int64_t K);

@@ -67,7 +67,7 @@ class ExtraInfoC(C.Structure):
 class BatchConfigC(C.Structure):
     _fields_ = [("window", i32), ("block_threads", i32), ("grid_blocks", i32), ("use_graph", i32),
                 ("recompute_bc", i32), ("heavy_roots", i32), ("root_order", C.POINTER(C.c_int32)),
-                ("park_min", i32)]
+                ("park_min", i32), ("use_context_order", i32)]
 
 
 RECORD_DTYPE = np.dtype([("root_index", np.int32), ("insertion", np.int32), ("kase", np.int32),
@@ -124,6 +124,7 @@ _sig("streamingBCForestFootprint", i64, [i64, i64])
 _sig("streamingBCResidentFootprint", i64, [P64, i64, i64, i32])
 _sig("streamingBCSynchronize", C.c_int, [VPP])
 _sig("streamingBCLastParked", i64, [VPP])
+_sig("streamingBCComputeUnits", i32, [VPP])
 # cost model (include/sbc_costmodel.h)
 PF = C.POINTER(C.c_float)
 _sig("sbcCostModelNumParams", i32, [])
@@ -136,6 +137,8 @@ _sig("sbcCostModelGradients", C.c_int, [VP, PF, PF, i64, PF, PF])
 _sig("sbcCostModelPredict", C.c_int, [VP, PF, i64, PF])
 _sig("sbcCostModelPredictFP32", C.c_int, [VP, PF, i64, PF])
 _sig("streamingBCWindowFeatures", C.c_int, [FP, VP, VPP, P64, P64, i64, PF, PF])
+_sig("streamingBCScheduleWindow", C.c_int, [FP, VP, VPP, VP, P64, P64, i64, PF, C.c_float, C.c_float,
+                                            C.POINTER(C.c_int32), PF])
 
 
 def _check(st):
@@ -260,6 +263,11 @@ class Context:
         _check(lib.streamingBCKernelStats(self._p, kernel, C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
 
+    @property
+    def num_cu(self) -> int:
+        """Compute units of the context's device."""
+        return int(lib.streamingBCComputeUnits(self._p))
+
     def last_parked(self) -> int:
         """Items of the last batched call finished by the whole-GPU kernels."""
         return int(lib.streamingBCLastParked(self._p))
@@ -323,6 +331,7 @@ class BatchConfig:
     heavy_roots: int = -1
     root_order: np.ndarray | None = None
     park_min: int = 0  # 0: library default, < 0: never hand an item to the whole-GPU kernels
+    use_context_order: bool = False  # root order left by schedule_window (device)
 
 
 class Forest:
@@ -369,7 +378,8 @@ class Forest:
         n = u.shape[0]
         order = None
         c = BatchConfigC(cfg.window, cfg.block_threads, cfg.grid_blocks, 1 if cfg.use_graph else 0,
-                         1 if cfg.recompute_bc else 0, cfg.heavy_roots, None, cfg.park_min)
+                         1 if cfg.recompute_bc else 0, cfg.heavy_roots, None, cfg.park_min,
+                         1 if cfg.use_context_order else 0)
         if cfg.root_order is not None:
             order = np.ascontiguousarray(cfg.root_order, dtype=np.int32)
             c.root_order = order.ctypes.data_as(C.POINTER(C.c_int32))
@@ -494,3 +504,24 @@ def window_features(forest: Forest, g: Graph, ctx: Context, u, v, root_hist=None
         hp = _pf(h)
     _check(lib.streamingBCWindowFeatures(forest._p, g._p, ctx._p, _p64(u), _p64(v), n, hp, _pf(X)))
     return X
+
+
+def schedule_window(forest: Forest, g: Graph, ctx: Context, model: CostModel, u, v, root_hist=None,
+                    light_max: float = 0.0, item_overhead: float = 4.0, predictions: bool = False):
+    """Device scheduling of the next window (streamingBCScheduleWindow):
+    leaves the root order in the context for a batched call with
+    use_context_order=True.  Returns (heavy_roots, predictions (n, K) or None)."""
+    u = np.ascontiguousarray(u, dtype=np.int64)
+    v = np.ascontiguousarray(v, dtype=np.int64)
+    n = u.shape[0]
+    hp = None
+    if root_hist is not None:
+        h = _f32(root_hist)
+        if h.shape != (forest.k,):
+            raise ValueError("root_hist needs one value per root")
+        hp = _pf(h)
+    heavy = C.c_int32(-1)
+    pred = np.zeros((n, forest.k), dtype=np.float32) if predictions else None
+    _check(lib.streamingBCScheduleWindow(forest._p, g._p, ctx._p, model._p, _p64(u), _p64(v), n, hp, light_max,
+                                         item_overhead, C.byref(heavy), _pf(pred) if predictions else None))
+    return int(heavy.value), pred

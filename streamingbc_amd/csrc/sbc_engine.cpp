@@ -529,6 +529,16 @@ struct extraArraysPerThread {
     int64_t park_cap = 0;
     std::vector<int32_t> h_start;
     int64_t last_parked = 0;   // items finished by the wide kernels in the last batched call
+    // device scheduling (streamingBCScheduleWindow): buffers and a valid-order flag
+    int32_t* d_suv = nullptr;
+    float* d_shist = nullptr;
+    float* d_sX = nullptr;
+    float* d_spred = nullptr;
+    float* d_scost = nullptr;
+    int32_t* d_sheavy = nullptr;
+    int64_t sched_items = 0, sched_k = 0, sched_cnt = 0;
+    bool order_ready = false;
+    const void* order_forest = nullptr;  // forest the scheduled order belongs to
     int64_t order_cap = 0;
     int num_cu = 0;
     // optional per-item device timestamps, appended to the file named by
@@ -574,6 +584,9 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
+    for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
+                    (void*)C->d_sheavy})
+        if (q) (void)hipFree(q);
     if (C->win.d_desc) (void)hipFree(C->win.d_desc);
     if (C->win.d_ints) (void)hipFree(C->win.d_ints);
     if (C->win.d_vi) (void)hipFree(C->win.d_vi);
@@ -974,6 +987,7 @@ static sbcStatus ensure_order(extraArraysPerThread* C, int64_t K)
     if (st) return st;
     C->order_cap = K;
     C->gc.has_order = -1;
+    C->order_ready = false;
     return SBC_OK;
 }
 
@@ -1156,8 +1170,11 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     const int park_min = cfg->park_min == 0 ? 4096 : cfg->park_min;
     if (park_min > 0 && nwb > 1 && (st = ensure_park(C, F->K))) return st;
     int heavy = cfg->heavy_roots < 0 || cfg->heavy_roots > F->K ? (int)F->K : cfg->heavy_roots;
-    const int has_order = cfg->root_order != nullptr;
-    if (has_order) {
+    if (cfg->use_context_order && (cfg->root_order || !C->order_ready || C->order_forest != F))
+        return set_err(SBC_ERR_ARG, "use_context_order needs a preceding streamingBCScheduleWindow and no root_order");
+    const int has_order = cfg->root_order != nullptr || cfg->use_context_order;
+    C->order_ready = false;  // a scheduled order serves one batched call
+    if (cfg->root_order) {
         std::vector<char> chk(F->K, 0);
         for (int64_t i = 0; i < F->K; i++) {
             int32_t x = cfg->root_order[i];
@@ -1165,6 +1182,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             chk[x] = 1;
         }
         HIPCHK(hipMemcpyAsync(C->d_order, cfg->root_order, sizeof(int32_t) * F->K, hipMemcpyHostToDevice, C->stream));
+        C->order_ready = false;
         HIPCHK(hipStreamSynchronize(C->stream));
     }
     int grid = cfg->grid_blocks;
@@ -1453,4 +1471,79 @@ extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger*
 extern "C" int64_t streamingBCLastParked(extraArraysPerThread** eAPT)
 {
     return (eAPT && eAPT[0]) ? eAPT[0]->last_parked : -1;
+}
+
+extern "C" int32_t streamingBCComputeUnits(extraArraysPerThread** eAPT)
+{
+    return (eAPT && eAPT[0]) ? eAPT[0]->num_cu : -1;
+}
+
+// ------------------------------------------------------------ device scheduling
+
+extern "C" sbcStatus streamingBCScheduleWindow(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT,
+                                               sbcCostModel* model, const int64_t* u, const int64_t* v,
+                                               int64_t count, const float* root_hist, float light_max,
+                                               float item_overhead, int32_t* heavy_out, float* pred_out)
+{
+    if (!forest || !G || !eAPT || !eAPT[0] || !model || count <= 0 || count > (1 << 16) || !u || !v)
+        return set_err(SBC_ERR_ARG, "bad argument");
+    extraArraysPerThread* C = eAPT[0];
+    sbcForestDev* F = forest->dev;
+    if (!F->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (F->K > 4096) return set_err(SBC_ERR_ARG, "device scheduling supports at most 4096 roots");
+    sbcStatus st = graph_flush(G, C->stream);
+    if (st) return st;
+    if ((st = ensure_order(C, F->K))) return st;
+    const int64_t items = count * F->K;
+    if (C->sched_items < items || C->sched_k < F->K || C->sched_cnt < count) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
+                        (void*)C->d_sheavy})
+            if (q) HIPCHK(hipFree(q));
+        C->d_suv = nullptr; C->d_shist = nullptr; C->d_sX = nullptr; C->d_spred = nullptr; C->d_scost = nullptr;
+        C->d_sheavy = nullptr;
+        const int64_t ni = items > C->sched_items ? items : C->sched_items;
+        const int64_t nk = F->K > C->sched_k ? F->K : C->sched_k;
+        const int64_t nc = count > C->sched_cnt ? count : C->sched_cnt;
+        if ((st = dalloc(&C->d_suv, (size_t)(2 * nc))) || (st = dalloc(&C->d_shist, (size_t)nk)) ||
+            (st = dalloc(&C->d_sX, (size_t)ni * SBC_COST_FEATURES)) || (st = dalloc(&C->d_spred, (size_t)ni)) ||
+            (st = dalloc(&C->d_scost, (size_t)nk)) || (st = dalloc(&C->d_sheavy, 1)))
+            return st;
+        C->sched_items = ni;
+        C->sched_k = nk;
+        C->sched_cnt = nc;
+    }
+    std::vector<int32_t> uv(2 * count);
+    for (int64_t i = 0; i < count; i++) {
+        if (u[i] < 0 || v[i] < 0 || u[i] >= F->nv || v[i] >= F->nv) return set_err(SBC_ERR_ARG, "vertex out of range");
+        uv[i] = (int32_t)u[i];
+        uv[count + i] = (int32_t)v[i];
+    }
+    HIPCHK(hipMemcpyAsync(C->d_suv, uv.data(), sizeof(int32_t) * 2 * count, hipMemcpyHostToDevice, C->stream));
+    if (root_hist)
+        HIPCHK(hipMemcpyAsync(C->d_shist, root_hist, sizeof(float) * F->K, hipMemcpyHostToDevice, C->stream));
+    SbcFeatArgs A;
+    A.vinfo = G->d_vinfo;
+    A.dist = F->d_dist;
+    A.sigma = F->d_sigma;
+    A.delta = F->d_delta;
+    A.u = C->d_suv;
+    A.v = C->d_suv + count;
+    A.hist = root_hist ? C->d_shist : nullptr;
+    A.X = C->d_sX;
+    A.nv = (int32_t)F->nv;
+    A.K = (int32_t)F->K;
+    A.count = (int32_t)count;
+    HIPCHK(sbc_launch_features(&A, C->stream));
+    HIPCHK(sbc_cost_schedule(model, C->d_sX, (int)count, (int)F->K, light_max > 0 ? light_max : -1.0f, item_overhead,
+                             C->d_spred, C->d_scost, C->d_order, C->d_sheavy, C->stream));
+    int32_t heavy = 0;
+    HIPCHK(hipMemcpyAsync(&heavy, C->d_sheavy, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+    if (pred_out)
+        HIPCHK(hipMemcpyAsync(pred_out, C->d_spred, sizeof(float) * (size_t)items, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (heavy_out) *heavy_out = light_max > 0 ? heavy : -1;
+    C->order_ready = true;
+    C->order_forest = F;
+    return SBC_OK;
 }

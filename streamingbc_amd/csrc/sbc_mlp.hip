@@ -504,6 +504,91 @@ static sbcStatus predict(sbcCostModel* m, const float* X, int64_t n, float* out,
     return SBC_OK;
 }
 
+// ---------------------------------------------------------------- schedule
+
+namespace {
+
+// cost[r] = sum over the window's items of expm1(max(pred, 0)) + overhead
+__global__ void root_cost_kernel(const float* __restrict__ pred, int n, int K, float overhead, float* cost)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= K) return;
+    float c = 0.f;
+    for (int i = 0; i < n; ++i) c += expm1f(fmaxf(pred[(size_t)i * K + r], 0.f)) + overhead;
+    cost[r] = c;
+}
+
+__device__ inline uint32_t orderable(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// One workgroup: roots sorted by descending cost (ties: lower index first)
+// into order[K]; heavy = roots with cost >= light_max.
+constexpr int SORT_CAP = 4096;
+__global__ __launch_bounds__(1024) void root_order_kernel(const float* __restrict__ cost, int K, float light_max,
+                                                          int32_t* order, int32_t* heavy)
+{
+    __shared__ unsigned long long key[SORT_CAP];
+    __shared__ int nheavy;
+    int n2 = 1;
+    while (n2 < K) n2 <<= 1;
+    if (threadIdx.x == 0) nheavy = 0;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        if (i < K) {
+            // ascending key = descending cost
+            key[i] = ((unsigned long long)orderable(-cost[i]) << 32) | (uint32_t)i;
+        } else {
+            key[i] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int k2 = 2; k2 <= n2; k2 <<= 1) {
+        for (int j = k2 >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const bool up = (i & k2) == 0;
+                    const unsigned long long a = key[i], b = key[p];
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        const int r = (int)(uint32_t)key[i];
+        order[i] = r;
+        if (cost[r] >= light_max) atomicAdd(&nheavy, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *heavy = nheavy;
+}
+
+}  // namespace
+
+extern "C" hipError_t sbc_cost_schedule(sbcCostModel* m, const float* dX, int n, int K, float light_max,
+                                        float overhead, float* d_pred, float* d_cost, int32_t* d_order,
+                                        int32_t* d_heavy, hipStream_t st)
+{
+    if (!m || K <= 0 || K > SORT_CAP || n <= 0) return hipErrorInvalidValue;
+    if (m->bf16_stale) {
+        hipLaunchKernelGGL(mlp_tobf16_kernel, dim3((NH * NH + 255) / 256), dim3(256), 0, st, m->d_p, m->d_w1b,
+                           m->d_w2b);
+        m->bf16_stale = false;
+    }
+    const int64_t items = (int64_t)n * K;
+    hipLaunchKernelGGL(mlp_mfma_kernel, dim3((unsigned)((items + 63) / 64)), dim3(256), 0, st, dX, (int)items,
+                       m->d_w1b, m->d_w2b, m->d_p, d_pred);
+    hipLaunchKernelGGL(root_cost_kernel, dim3((K + 255) / 256), dim3(256), 0, st, d_pred, n, K, overhead, d_cost);
+    hipLaunchKernelGGL(root_order_kernel, dim3(1), dim3(1024), 0, st, d_cost, K, light_max, d_order, d_heavy);
+    return hipGetLastError();
+}
+
 extern "C" sbcStatus sbcCostModelPredict(sbcCostModel* m, const float* X, int64_t n, float* out)
 {
     return predict(m, X, n, out, true);

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import BatchConfig, CostModel, Context, Forest, Graph, window_features
+from . import BatchConfig, CostModel, Context, Forest, Graph, schedule_window, window_features
 
 # Fixed per-item work (in affected-vertex units) added to each prediction:
 # even an item that changes nothing reads its endpoints and syncs its team.
@@ -33,12 +33,14 @@ class WindowPlan:
     root_order: np.ndarray | None
     heavy_roots: int
     predicted: np.ndarray | None = None  # (n, K) predicted log1p(affected)
+    device_order: bool = False           # order left on the device by schedule_window
 
 
 class StaticSchedule:
     """Identity root order, every root in the heavy phase."""
 
     name = "static"
+    needs_records = False
 
     def plan(self, forest, g, ctx, u, v):
         return WindowPlan(None, -1), None
@@ -66,6 +68,7 @@ class LearnedSchedule:
     hist_decay: float = 0.5
     mfma: bool = True
     name: str = "mlp"
+    needs_records: bool = True  # observe() reads every window's item records (root history feature)
     model: CostModel | None = None
     _X: list = field(default_factory=list)
     _y: list = field(default_factory=list)
@@ -76,6 +79,8 @@ class LearnedSchedule:
     train_seconds: float = 0.0
     eval_pred: list = field(default_factory=list)
     eval_true: list = field(default_factory=list)
+    device: bool = True     # features, predictions and the root sort on the device
+    eval_every: int = 4     # trained windows whose predictions are copied back for the report
 
     def __post_init__(self):
         if self.model is None:
@@ -86,8 +91,26 @@ class LearnedSchedule:
             self._hist = np.zeros(forest.k, dtype=np.float32)
         return window_features(forest, g, ctx, u, v, self._hist)
 
+    def _predict_only(self, forest, g, ctx, u, v):
+        _, pred = schedule_window(forest, g, ctx, self.model, u, v, self._hist, 0.0, ITEM_OVERHEAD, predictions=True)
+        return pred
+
     def plan(self, forest: Forest, g: Graph, ctx: Context, u, v):
         """Returns (WindowPlan, features of the window)."""
+        if self._hist is None:
+            self._hist = np.zeros(forest.k, dtype=np.float32)
+        if self.trained and self.device and self.mfma and forest.k <= 4096:
+            self._sched_windows = getattr(self, "_sched_windows", 0) + 1
+            want = (self._sched_windows - 1) % self.eval_every == 0
+            if self.light_max <= 0 and forest.k <= 2 * ctx.num_cu:
+                # every root gets a team of its own at once: the order cannot
+                # change the window time, so only report windows are predicted
+                if not want:
+                    return WindowPlan(None, -1, None), None
+                return WindowPlan(None, -1, self._predict_only(forest, g, ctx, u, v)), None
+            heavy, pred = schedule_window(forest, g, ctx, self.model, u, v, self._hist, self.light_max,
+                                          ITEM_OVERHEAD, predictions=want)
+            return WindowPlan(None, heavy, pred, device_order=True), None
         X = self._features(forest, g, ctx, u, v)
         if not self.trained:
             return WindowPlan(None, -1, None), X
@@ -103,8 +126,9 @@ class LearnedSchedule:
         if plan.predicted is not None:
             self.eval_pred.append(plan.predicted.copy())
             self.eval_true.append(y.copy())
-        self._X.append(X.reshape(-1, X.shape[-1]))
-        self._y.append(y.reshape(-1))
+        if X is not None and not self.trained:
+            self._X.append(X.reshape(-1, X.shape[-1]))
+            self._y.append(y.reshape(-1))
         # per-root history feature: decayed mean label of the root's items
         self._hist = (self.hist_decay * self._hist + (1.0 - self.hist_decay) * y.mean(axis=0)).astype(np.float32)
         self._windows += 1
@@ -171,7 +195,7 @@ def run_stream(forest: Forest, g: Graph, ctx: Context, u, v, window: int, schedu
         plan, X = schedule.plan(forest, g, ctx, uu, vv)
         cfg = BatchConfig(window=window, block_threads=base.block_threads, grid_blocks=base.grid_blocks,
                           use_graph=base.use_graph, recompute_bc=base.recompute_bc, heavy_roots=plan.heavy_roots,
-                          root_order=plan.root_order)
+                          root_order=plan.root_order, park_min=base.park_min, use_context_order=plan.device_order)
         rec = forest.insert_batched(g, ctx, uu, vv, cfg, records=True)
         times.append(time.perf_counter() - t0)
         recs.append(rec)

@@ -210,11 +210,13 @@ def _run(hg, roots, u, v, sched, window=16):
 
 
 @pytest.mark.gpu
-def test_learned_schedule_is_result_invariant_and_predictive():
+@pytest.mark.parametrize("device,light_max", [(False, 64.0), (True, 64.0), (True, 0.0)])
+def test_learned_schedule_is_result_invariant_and_predictive(device, light_max):
     hg, roots, u, v = _stream_case()
-    print(f"[schedule] nv={hg.nv} k={len(roots)} n={len(u)}", flush=True)
+    print(f"[schedule] nv={hg.nv} k={len(roots)} n={len(u)} device={device} light_max={light_max}", flush=True)
     f0, r0 = _run(hg, roots, u, v, StaticSchedule())
-    sched = LearnedSchedule(train_after=8, epochs=20, batch=512, lr=3e-3, light_max=64.0)
+    sched = LearnedSchedule(train_after=8, epochs=20, batch=512, lr=3e-3, light_max=light_max, device=device,
+                            eval_every=1)
     f1, r1 = _run(hg, roots, u, v, sched)
     assert sched.trained
     assert len(sched.eval_pred) == 24 - 8
@@ -230,3 +232,46 @@ def test_learned_schedule_is_result_invariant_and_predictive():
     # held-out windows: per-root window work is ranked far better than chance
     assert rep["root_window_spearman"] > 0.5
     assert rep["rmse_log1p"] < 1.5
+
+
+@pytest.mark.gpu
+def test_device_schedule_matches_host_schedule():
+    """streamingBCScheduleWindow computes the features, the MFMA predictions,
+    the per-root costs and the heavy count on the device; they agree with the
+    host-side pipeline (window_features + predict + numpy ordering), and a
+    batched call that uses the device order gives the same trees and BC."""
+    hg, roots, u, v = _stream_case()
+    ctx = sbc.Context(hg.nv)
+    g = sbc.Graph(hg)
+    f = sbc.Forest(hg.nv, roots)
+    f.init_static(g, ctx)
+    sched = LearnedSchedule(train_after=4, epochs=8, batch=512, lr=3e-3, light_max=64.0, device=False)
+    run_stream(f, g, ctx, u[:64], v[:64], 16, sched, sbc.BatchConfig(window=16, block_threads=1024))
+    assert sched.trained
+    uu, vv = u[64:80], v[64:80]
+    X = sbc.window_features(f, g, ctx, uu, vv, sched._hist)
+    p_host = sched.model.predict(X, mfma=True)
+    heavy, p_dev = sbc.schedule_window(f, g, ctx, sched.model, uu, vv, sched._hist, 64.0, 4.0, predictions=True)
+    assert np.array_equal(p_host, p_dev)
+    work = np.expm1(np.maximum(p_host.astype(np.float64), 0.0)).sum(axis=0) + 4.0 * len(uu)
+    lo = int(np.count_nonzero(work >= 64.0 * (1 + 1e-5)))
+    hi = int(np.count_nonzero(work >= 64.0 * (1 - 1e-5)))
+    print(f"[schedule] heavy device={heavy} host=[{lo},{hi}]", flush=True)
+    assert lo <= heavy <= hi
+    # the order stays in the context for the next batched call
+    f.insert_batched(g, ctx, uu, vv, sbc.BatchConfig(window=16, block_threads=1024, heavy_roots=heavy,
+                                                     use_context_order=True))
+    f.recompute_bc(ctx)
+    # same stream without any schedule
+    g2 = sbc.Graph(hg)
+    f2 = sbc.Forest(hg.nv, roots)
+    f2.init_static(g2, ctx)
+    f2.insert_batched(g2, ctx, u[:80], v[:80], sbc.BatchConfig(window=16, block_threads=1024))
+    f2.recompute_bc(ctx)
+    for a, b in zip(f.trees(), f2.trees()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(f.bc(), f2.bc())
+    # a second batched call without a fresh schedule is refused
+    with pytest.raises(sbc.SbcError):
+        f.insert_batched(g, ctx, u[80:96], v[80:96], sbc.BatchConfig(window=16, block_threads=1024,
+                                                                     use_context_order=True))
