@@ -63,6 +63,9 @@ struct stinger {
     // host mirror
     std::vector<int32_t> start, deg, deg0, cap;
     std::vector<int32_t> col, seq;
+    // per-vertex host scratch of the batched call, all zero between calls
+    std::vector<int32_t> h_cnt;
+    std::vector<char> h_mark;
     int64_t total_cap = 0;
     int32_t seq_next = 1;
     // device image
@@ -1196,15 +1199,23 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     if ((st = ensure_bucket(C, G, grid))) return st;
     if ((st = ensure_hubq(C, grid))) return st;
     // capacity: relayout once up front if any endpoint lacks room for the whole batch
+    if ((int64_t)G->h_cnt.size() != G->nv) G->h_cnt.assign((size_t)G->nv, 0);
+    if ((int64_t)G->h_mark.size() != G->nv) G->h_mark.assign((size_t)G->nv, 0);
     {
-        std::vector<int32_t> extra(G->nv, 0);
+        std::vector<int32_t>& extra = G->h_cnt;
+        std::vector<int32_t> ends;
+        ends.reserve((size_t)count * 2);
         bool short_room = false;
-        for (int64_t i = 0; i < count; i++) { extra[u[i]]++; extra[v[i]]++; }
-        for (int64_t x = 0; x < G->nv; x++)
-            if (extra[x] && G->deg[x] + extra[x] > G->cap[x]) {
+        for (int64_t i = 0; i < count; i++)
+            for (int64_t x : {u[i], v[i]})
+                if (extra[x]++ == 0) ends.push_back((int32_t)x);
+        for (int32_t x : ends) {
+            if (G->deg[x] + extra[x] > G->cap[x]) {
                 G->cap[x] = G->deg[x] + extra[x] + slack_for(G->deg[x]);
                 short_room = true;
             }
+            extra[x] = 0;
+        }
         if (short_room) graph_relayout(G);
         if ((st = graph_flush(G, C->stream))) return st;
     }
@@ -1274,7 +1285,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     int64_t nparked = 0, nparked_before = 0;
     std::vector<int32_t> wu(W), wv(W), slot(2 * W), scol(2 * W), sseq(2 * W), vv, touched;
     std::vector<SbcVInfo> vi;
-    std::vector<char> mark(G->nv, 0);
+    std::vector<char>& mark = G->h_mark;  // cleared again at the end of each window's loop body
     for (int64_t w0 = 0; w0 < count; w0 += W) {
         const int n = (int)((count - w0) < W ? (count - w0) : W);
         const int32_t seq0 = G->seq_next;

@@ -208,6 +208,38 @@ __device__ __forceinline__ int block_exscan(int x, int* tmp, int& total)
     return res;
 }
 
+// Exclusive prefix sums of two values over a workgroup of NW wavefronts in
+// one pass (two barriers).  tmp: 2 * (NW + 1) ints of LDS, not read again
+// before the caller's next barrier.
+template <int NW>
+__device__ __forceinline__ int2 block_exscan2(int x, int y, int* tmp, int& total_x, int& total_y)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int vx = x, vy = y;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int ax = __shfl_up(vx, d), ay = __shfl_up(vy, d);
+        if (lane >= d) { vx += ax; vy += ay; }
+    }
+    if (lane == 63) { tmp[wave] = vx; tmp[NW + 1 + wave] = vy; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acx = 0, acy = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int tx = tmp[w], ty = tmp[NW + 1 + w];
+            tmp[w] = acx;
+            tmp[NW + 1 + w] = acy;
+            acx += tx;
+            acy += ty;
+        }
+        tmp[NW] = acx;
+        tmp[2 * NW + 1] = acy;
+    }
+    __syncthreads();
+    total_x = tmp[NW];
+    total_y = tmp[2 * NW + 1];
+    return make_int2(tmp[wave] + vx - x, tmp[NW + 1 + wave] + vy - y);
+}
+
 // (sa / sb) * (d + 1) in FP64 -- the dependency term of
 // src/streamingbc_init.c:159-161 and src/streamingbc_internal.c:185-198
 __device__ __forceinline__ double dep(int64_t sa, int64_t sb, double d)
@@ -276,12 +308,15 @@ __device__ __forceinline__ int hub_batch(const SbcParams& P)
 // every batched vertex is handled by one lane, its term kept at the edge's
 // index, then each vertex sums its edges' terms in list order.
 constexpr int LIGHT_CAP = 1024;  // vertices per batch (one per thread of a 1024-thread team)
-constexpr int EDGE_CAP = 2048;   // edges per batch
+#ifndef SBC_EDGE_CAP
+#define SBC_EDGE_CAP 2048
+#endif
+constexpr int EDGE_CAP = SBC_EDGE_CAP;  // edges per batch
 
 struct alignas(16) LightShared {
     int nb;                       // batched vertices
-    int novf;                     // vertices left to the one-wave-per-vertex path
-    int scan[17];
+    int novf;                     // hubs the queues could not take (one wavefront each)
+    int scan[36];
     int32_t k[LIGHT_CAP];
     int32_t start[LIGHT_CAP];
     int32_t off[LIGHT_CAP];       // first edge index of the vertex in the batch
@@ -290,7 +325,7 @@ struct alignas(16) LightShared {
     int64_t sk_old[LIGHT_CAP];
     int64_t sk_new[LIGHT_CAP];
     double base[LIGHT_CAP];
-    int32_t ovf[LIGHT_CAP];
+    int32_t ovf[LIGHT_CAP];       // light vertices of a round past the edge cap keep their slots
     uint8_t eflag[EDGE_CAP];
     double eadd[EDGE_CAP];
     double esub[EDGE_CAP];
@@ -490,11 +525,10 @@ __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
 // One wavefront computes the new delta of k, summing the terms in adjacency
 // order lane by lane.
 template <typename TS>
-__device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
-                                   int lev, TS& ts)
+__device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const Item& X, int32_t k, int32_t start,
+                                                      int len, const KState& ks, int lev, TS& ts)
 {
     const int lane = lane_id();
-    const KState ks = kstate(X, k);
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
     for (int ch = 0; ch < nch; ch += SCAN_UNROLL) {
@@ -505,7 +539,7 @@ __device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Ite
         for (int u = 0; u < SCAN_UNROLL; ++u) {
             const int e = (ch + u) * 64 + lane;
             ok[u] = (ch + u) < nch && e < len;
-            cc[u] = ok[u] ? P.col[vi.start + e] : 0;
+            cc[u] = ok[u] ? P.col[start + e] : 0;
         }
 #pragma unroll
         for (int u = 0; u < SCAN_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
@@ -526,6 +560,13 @@ __device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Ite
         }
     }
     if (lane == 0) st_reld(&X.ndel[k], acc);
+}
+
+template <typename TS>
+__device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
+                                   int lev, TS& ts)
+{
+    ascent_vertex_wave_ks(P, X, k, vi.start, len, kstate(X, k), lev, ts);
 }
 
 // The whole team computes the new delta of hub k: wave w scans a contiguous
@@ -1120,18 +1161,23 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     if (tp && tid == 0) tp[12] += (int64_t)wall_clock64();
 }
 
-// All vertices of ascent level `here`: hubs are queued in hs.list for the
-// hub passes; light vertices are batched (see LightShared); vertices that do
-// not fit a batch run one wavefront per vertex.
+// All vertices of ascent level `here`, in rounds of one vertex per thread:
+// hubs are queued in hs.list for the hub passes; light vertices get a slot
+// (their rank in the round) holding their adjacency range and ascent state.
+// The first light vertices whose edges fit EDGE_CAP form the batch (one lane
+// per edge, terms kept at the edge's index, then each vertex sums its terms
+// in list order); the rest of the round's light vertices run one wavefront
+// per vertex from their slots while the batch computes its terms.
 template <int TW>
 __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Item& X, const LevelSets& here, int tot,
                                                    int lev, TeamShared& ts, HubShared& hs, LightShared& ls,
                                                    int twave, unsigned long long& my_proc,
                                                    unsigned long long& my_scan, int64_t* tp)
 {
-    int64_t tq = 0;  // trace: batch setup (13), edge terms and sums (14), leftovers (15)
+    int64_t tq = 0;  // trace: round setup (13), edge terms and leftovers (14), batch sums (15)
     const int tid = threadIdx.x, lane = lane_id();
     constexpr int NT = TW * 64;
+    static_assert(NT <= LIGHT_CAP, "one slot per thread of a round");
     for (int r0 = 0; r0 < tot; r0 += NT) {
         if (tp && tid == 0) tq = (int64_t)wall_clock64();
         const int t = r0 + tid;
@@ -1154,12 +1200,10 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
                 light = true;
             }
         }
-        int nl_round, ne_round;
-        const int slot = block_exscan<TW>(light ? 1 : 0, ls.scan, nl_round);
-        const int eoff = block_exscan<TW>(light ? len : 0, ls.scan, ne_round);
-        const bool in_batch = light && eoff + len <= EDGE_CAP;
-        if (light && !in_batch) ls.ovf[atomicAdd(&ls.novf, 1)] = k;
-        if (in_batch) {
+        int nl, ne_round;
+        const int2 se = block_exscan2<TW>(light ? 1 : 0, light ? len : 0, ls.scan, nl, ne_round);
+        const int slot = se.x, eoff = se.y;
+        if (light) {
             ls.k[slot] = k;
             ls.start[slot] = vi.start;
             ls.off[slot] = eoff;
@@ -1169,13 +1213,14 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             ls.sk_old[slot] = kst.sk_old;
             ls.sk_new[slot] = kst.sk_new;
             ls.base[slot] = kst.base;
-            atomicMax(&ls.nb, slot + 1);
+            if (eoff + len <= EDGE_CAP) atomicMax(&ls.nb, slot + 1);
         }
         __syncthreads();
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[13] += x - tq; tq = x; }
         const int nb = ls.nb;
+        const int no = ls.novf;
         const int ne = nb > 0 ? ls.off[nb - 1] + ls.len[nb - 1] : 0;
-        // one lane per edge
+        // batch: one lane per edge
         for (int e = tid; e < ne; e += NT) {
             int lo = 0, hi = nb;  // last slot with off <= e
             while (hi - lo > 1) {
@@ -1194,8 +1239,36 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             ls.eadd[e] = tm.add;
             ls.esub[e] = tm.sub;
         }
+        // light vertices past the cap, one wavefront each, from their slots
+        for (int sl = nb + twave; sl < nl; sl += TW) {
+            KState ks;
+            ks.isD = ls.isD[sl] != 0;
+            ks.sk_old = ls.sk_old[sl];
+            ks.sk_new = ls.sk_new[sl];
+            ks.base = ls.base[sl];
+            const int l2 = ls.len[sl];
+            ascent_vertex_wave_ks(P, X, ls.k[sl], ls.start[sl], l2, ks, lev, ts);
+            my_proc += 1;
+            my_scan += (unsigned long long)l2;
+        }
+        // hubs no queue could take
+        for (int i = twave; i < no; i += TW) {
+            const int32_t kk = ls.ovf[i];
+            const SbcVInfo v2 = P.vinfo[kk];
+            const int l2 = visible_len(P, v2, X.q);
+            ascent_vertex_wave(P, X, kk, v2, l2, lev, ts);
+            my_proc += 1;
+            my_scan += (unsigned long long)l2;
+        }
+        if (tid == 0) {
+            my_proc += (unsigned long long)nb;
+            my_scan += (unsigned long long)ne;
+        }
         __syncthreads();
-        // one wavefront per vertex: sum its edges' terms in list order
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[14] += x - tq; tq = x; }
+        // every thread holds nb and no: the counters are free for the next round
+        if (tid == 0) { ls.nb = 0; ls.novf = 0; }
+        // one wavefront per batched vertex: sum its edges' terms in list order
         for (int sl = twave; sl < nb; sl += TW) {
             const int a = ls.off[sl], l = ls.len[sl];
             double acc = ls.base[sl];
@@ -1215,24 +1288,6 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             }
             if (lane == 0) st_reld(&X.ndel[ls.k[sl]], acc);
         }
-        if (tid == 0) {
-            my_proc += (unsigned long long)nb;
-            my_scan += (unsigned long long)ne;
-        }
-        __syncthreads();
-        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[14] += x - tq; tq = x; }
-        // leftovers, one wavefront each
-        const int no = ls.novf;
-        for (int i = twave; i < no; i += TW) {
-            const int32_t kk = ls.ovf[i];
-            const SbcVInfo v2 = P.vinfo[kk];
-            const int l2 = visible_len(P, v2, X.q);
-            ascent_vertex_wave(P, X, kk, v2, l2, lev, ts);
-            my_proc += 1;
-            my_scan += (unsigned long long)l2;
-        }
-        __syncthreads();
-        if (tid == 0) { ls.nb = 0; ls.novf = 0; }
         __syncthreads();
         if (tp && tid == 0) tp[15] += (int64_t)wall_clock64() - tq;
     }
