@@ -46,7 +46,7 @@ struct SbcBucketEnt {
     double sub;
 };
 
-#define SBC_TRACE_SLOTS 16
+#define SBC_TRACE_SLOTS 64
 #define SBC_HUBQ_CAP 8192   // hubs per workgroup and pass beyond the LDS queue (global overflow queue)
 
 struct SbcParams {
@@ -82,7 +82,9 @@ struct SbcParams {
     int64_t* tprof;       // [W][K][SBC_TRACE_SLOTS] or nullptr: timestamps (start, descent end, ascent end,
                           // commit end), hubs looked up, hubs scanned, neighbours of each; ascent time in the
                           // light-vertex passes, the hub lookups, the hub staging and scans; ascent levels,
-                          // vertices over the ascent levels
+                          // vertices over the ascent levels; light sub-phases; per ascent level 0..7 (deeper:
+                          // 7) its time; team-sorted buckets: staging, prefix, placing and summing time, terms,
+                          // buckets; per ascent level 0..7: lookup time, scan time, scanned hubs, candidates
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)

@@ -75,6 +75,45 @@ __device__ __forceinline__ int prefix_in_wave(uint64_t m)
     return __popcll(l == 0 ? 0ull : (m & (~0ull >> (64 - l))));
 }
 
+// acc + the wavefront's terms in lane order, where lane l's term adds `add`
+// if c, then subtracts `sub` if s: the terms are packed in order into the
+// wavefront's LDS row (a subtraction as the addition of the negated value,
+// which IEEE 754 defines it to be) and lane 0 adds them one after another.
+// acc must be the same in every lane; so is the result.  Wave-uniform
+// control flow only.
+__device__ __forceinline__ double wave_ordered_add(double acc, bool c, double add, bool s, double sub)
+{
+    __shared__ double wrow[16][128];
+    const uint64_t cm = ballot(c), sm = ballot(s);
+    if ((cm | sm) == 0) return acc;
+    const int lane = lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int pos = __popcll(cm & below) + __popcll(sm & below);
+    double* w = wrow[threadIdx.x >> 6];
+    if (c) w[pos++] = add;
+    if (s) w[pos] = -sub;
+    const int n = __popcll(cm) + __popcll(sm);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double r = acc;
+    if (lane == 0) {
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+            const double a0 = w[i], a1 = w[i + 1], a2 = w[i + 2], a3 = w[i + 3];
+            r = r + a0;
+            r = r + a1;
+            r = r + a2;
+            r = r + a3;
+        }
+        for (; i < n; ++i) r = r + w[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return readlane_d(r, 0);
+}
+
 // Appends the lanes with `flag` set to a list whose length lives in LDS.
 // Must be called by the whole wavefront.  Returns the slot or -1.
 __device__ __forceinline__ int wave_append(bool flag, int* ctr)
@@ -268,6 +307,11 @@ constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
 constexpr int PHASH = 2 * HUB_CAP;      // LDS hash: vertex -> number in the level's lookup list
 constexpr int TWOHOP_CAP = 1024;        // two-hop set of the root kept for level-3 lookups
+// Lookup buckets: a pass's HUB_CAP * SBC_BUCKET term slots are shared out
+// evenly between its lookup hubs, at most TEAM_SORT_CAP per hub; a bucket of
+// more than SBC_BUCKET terms is sorted by the whole team.
+constexpr int TEAM_SORT_CAP = 1024;
+static_assert(TEAM_SORT_CAP % 64 == 0 && TEAM_SORT_CAP <= HUB_CAP * SBC_BUCKET, "bucket shares");
 
 struct alignas(16) HubShared {
     int n;
@@ -275,6 +319,10 @@ struct alignas(16) HubShared {
     int pn;
     int np;
     int ns;               // hubs staged for the team scan
+    int bstride;          // term slots per lookup hub in this pass's buckets
+    int n3;               // lookup hubs whose buckets the whole team sorts
+    int dup;              // a team-sorted bucket holds a position twice
+    int plen[HUB_CAP];    // visible list length of each lookup hub
     int list[HUB_CAP];
     int plist[HUB_CAP];   // ascent-only hubs of the level handled by table lookups
     int bcnt[HUB_CAP];    // terms collected per lookup hub
@@ -330,6 +378,10 @@ struct alignas(16) LightShared {
     double eadd[EDGE_CAP];
     double esub[EDGE_CAP];
 };
+static_assert(TEAM_SORT_CAP <= LIGHT_CAP && TEAM_SORT_CAP <= EDGE_CAP, "team sort stages a bucket in LightShared");
+// positions of a team-sorted hub's list: a bitmap in LightShared::base, per-word prefixes in ::ovf
+constexpr int TEAM_SORT_BITS = 32 * LIGHT_CAP;
+static_assert(TEAM_SORT_BITS <= 64 * LIGHT_CAP, "bitmap fits LightShared::base");
 
 // Per-item constants.
 struct Item {
@@ -549,14 +601,7 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
             const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
             const int pu = wave_append(appU, &ts.cnt[2]);
             if (pu >= 0) X.list[pu] = cc[u];
-            const uint64_t cm = ballot(t.child), sm = ballot(t.hsub);
-            uint64_t m = cm | sm;
-            while (m) {
-                const int l = __builtin_ctzll(m);
-                m &= m - 1;
-                if ((cm >> l) & 1) acc = acc + readlane_d(t.add, l);
-                if ((sm >> l) & 1) acc = acc - readlane_d(t.sub, l);
-            }
+            acc = wave_ordered_add(acc, t.child, t.add, t.hsub, t.sub);
         }
     }
     if (lane == 0) st_reld(&X.ndel[k], acc);
@@ -783,14 +828,7 @@ __device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item&
                 ad = e.add;
                 sb2 = e.sub;
             }
-            const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
-            uint64_t m = cm | sm;
-            while (m) {
-                const int l2 = __builtin_ctzll(m);
-                m &= m - 1;
-                if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
-                if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
-            }
+            acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
         }
         if (lane == 0) st_reld(&X.ndel[ls.k[j]], acc);
     }
@@ -841,8 +879,8 @@ __device__ __forceinline__ void probe_term(const SbcParams& P, const Item& X, Hu
     const Term t = ascent_lane(P, X, k, false, sk, sk, lev, true, w, ld_sb(&X.sb[w]), appU);
     if (t.child || t.hsub) {
         const int slot = atomicAdd(&hs.bcnt[hi], 1);
-        if (slot < SBC_BUCKET) {
-            SbcBucketEnt& e = bk[(size_t)hi * SBC_BUCKET + slot];
+        if (slot < hs.bstride) {
+            SbcBucketEnt& e = bk[(size_t)hi * hs.bstride + slot];
             e.key = pos;
             e.flag = (t.child ? 1 : 0) | (t.hsub ? 2 : 0);
             e.add = t.add;
@@ -959,16 +997,7 @@ __device__ __forceinline__ double bucket_sum(const BucketLane& e, double base)
     const int fl = __shfl(e.flag, idx);
     const double ad = shfl_d(e.add, idx);
     const double sb2 = shfl_d(e.sub, idx);
-    const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
-    uint64_t m = cm | sm;
-    double acc = base;
-    while (m) {
-        const int l2 = __builtin_ctzll(m);
-        m &= m - 1;
-        if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
-        if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
-    }
-    return acc;
+    return wave_ordered_add(base, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
 }
 
 // New deltas of the ascent-only hubs hs.plist[0..np) of level lev (1 or 2)
@@ -981,8 +1010,8 @@ __device__ __forceinline__ double bucket_sum(const BucketLane& e, double base)
 // hub's list.  Inserted edges of the hubs are checked slot by slot.  Each
 // hub's terms are sorted by position and summed in that order, the order of
 // the full pull scan, so the result is bit-identical to it.  Hubs with more
-// than SBC_BUCKET terms are marked kind 2 and left to the caller, which scans
-// them (parent marks are idempotent).
+// terms than their bucket holds are marked kind 2 and left to the caller,
+// which scans them (parent marks are idempotent).
 template <int TW>
 __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item& X, int np, int lev,
                                                   TeamShared& ts, HubShared& hs, LightShared& ls,
@@ -993,6 +1022,11 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     SbcBucketEnt* bk = P.bucket + (size_t)blockIdx.x * HUB_CAP * SBC_BUCKET;
     for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
     for (int i = tid; i < np; i += NT) hs.bcnt[i] = 0;
+    if (tid == 0) {
+        const int share = ((HUB_CAP * SBC_BUCKET) / np) & ~63;
+        hs.bstride = share < TEAM_SORT_CAP ? share : TEAM_SORT_CAP;
+        hs.n3 = 0;
+    }
     __syncthreads();
     for (int hi = tid; hi < np; hi += NT) {
         const int32_t k = hs.plist[hi];
@@ -1080,6 +1114,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         const int32_t k = hs.plist[hi];
         const SbcVInfo vi = P.vinfo[k];
         const int len = visible_len_thread(P, vi, X.q);
+        hs.plen[hi] = len;
         for (int e = P.hub[k].y; e < len; ++e) {
             const int32_t c = P.col[vi.start + e];
             const uint8_t b = ld_sb(&X.sb[c]);
@@ -1106,7 +1141,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             double ad = 0.0, sb2 = 0.0, base = 0.0;
             if (small) {
                 if (gl < n) {
-                    const SbcBucketEnt& e = bk[(size_t)hi * SBC_BUCKET + gl];
+                    const SbcBucketEnt& e = bk[(size_t)hi * hs.bstride + gl];
                     key = e.key;
                     fl = e.flag;
                     ad = e.add;
@@ -1136,7 +1171,13 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                 }
             }
             if (small && gl == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
-            if (hi < np && gl == 0) hs.kind[hi] = n > SBC_BUCKET ? 2 : (n > 8 ? 1 : 0);
+            if (hi < np && gl == 0) {
+                const uint8_t kd = n > hs.bstride ? 2
+                                 : n > SBC_BUCKET ? (hs.plen[hi] <= TEAM_SORT_BITS ? 3 : 2)
+                                 : (n > 8 ? 1 : 0);
+                hs.kind[hi] = kd;
+                if (kd == 3) atomicAdd(&hs.n3, 1);
+            }
         }
     }
     __syncthreads();
@@ -1152,12 +1193,83 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             if ((r++ % TW) != twave) continue;
             const int hi = h0 + l0;
             const int n = hs.bcnt[hi];
-            const BucketLane e = bucket_lane(bk + (size_t)hi * SBC_BUCKET, n);
+            const BucketLane e = bucket_lane(bk + (size_t)hi * hs.bstride, n);
             const double acc = bucket_sum(e, X.delta[hs.plist[hi]]);
             if (lane == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
         }
     }
     __syncthreads();
+    // buckets of more than SBC_BUCKET terms (kind 3), one hub at a time by
+    // the whole team: the entries' positions are set in an LDS bitmap over
+    // the hub's list, each entry's rank is the number of bits below its own
+    // (a prefix of per-word popcounts plus a masked popcount), the entries
+    // are placed at their ranks and one wavefront sums them in that order.
+    // A position set twice would make two ranks equal: the hub is then
+    // handed to the scans (kind 2) instead.
+    const int n3 = hs.n3;
+    uint32_t* bits = reinterpret_cast<uint32_t*>(ls.base);  // TEAM_SORT_BITS bits
+    for (int hi = 0, f = 0; f < n3; ++hi) {
+        if (hs.kind[hi] != 3) continue;
+        ++f;
+        const int n = hs.bcnt[hi];
+        const SbcBucketEnt* b = bk + (size_t)hi * hs.bstride;
+        const int nw = (hs.plen[hi] + 31) >> 5;
+        int64_t tk = 0;
+        if (tp && tid == 0) { tk = (int64_t)wall_clock64(); tp[28] += n; tp[29] += 1; }
+        for (int i = tid; i < nw; i += NT) bits[i] = 0u;
+        if (tid == 0) hs.dup = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += NT) {
+            const SbcBucketEnt e = b[i];
+            ls.off[i] = e.key;
+            ls.isD[i] = (uint8_t)e.flag;
+            ls.sk_old[i] = __builtin_bit_cast(int64_t, e.add);
+            ls.sk_new[i] = __builtin_bit_cast(int64_t, e.sub);
+            const uint32_t m = 1u << (e.key & 31);
+            if (atomicOr(&bits[e.key >> 5], m) & m) hs.dup = 1;
+        }
+        __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[24] += x - tk; tk = x; }
+        int tot_bits;
+        int wpre = 0;
+        for (int w0 = 0; w0 < nw; w0 += NT) {
+            const int w = w0 + tid;
+            const int c = w < nw ? __popc(bits[w]) : 0;
+            const int off = block_exscan<TW>(c, ls.scan, tot_bits);
+            if (w < nw) ls.ovf[w] = wpre + off;
+            wpre += tot_bits;
+        }
+        __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[25] += x - tk; tk = x; }
+        const bool dup = hs.dup != 0;
+        if (!dup) {
+            for (int i = tid; i < n; i += NT) {
+                const int key = ls.off[i];
+                const int rank = ls.ovf[key >> 5] + __popc(bits[key >> 5] & ((1u << (key & 31)) - 1u));
+                ls.eflag[rank] = ls.isD[i];
+                ls.eadd[rank] = __builtin_bit_cast(double, ls.sk_old[i]);
+                ls.esub[rank] = __builtin_bit_cast(double, ls.sk_new[i]);
+            }
+        } else if (tid == 0) {
+            hs.kind[hi] = 2;
+        }
+        __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[26] += x - tk; tk = x; }
+        if (!dup && twave == 0) {
+            const int32_t k = hs.plist[hi];
+            double acc = X.delta[k];
+            for (int e0 = 0; e0 < n; e0 += 64) {
+                const int e = e0 + lane;
+                const uint8_t fl = e < n ? ls.eflag[e] : (uint8_t)0;
+                const double ad = (fl & 1) ? ls.eadd[e] : 0.0;
+                const double sb2 = (fl & 2) ? ls.esub[e] : 0.0;
+                acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
+            }
+            if (lane == 0) st_reld(&X.ndel[k], acc);
+        }
+        __syncthreads();
+        if (tp && tid == 0) tp[27] += (int64_t)wall_clock64() - tk;
+    }
     if (tp && tid == 0) tp[12] += (int64_t)wall_clock64();
 }
 
@@ -1277,14 +1389,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
                 const uint8_t fl = e < l ? ls.eflag[a + e] : (uint8_t)0;
                 const double ad = (fl & 1) ? ls.eadd[a + e] : 0.0;
                 const double sb2 = (fl & 2) ? ls.esub[a + e] : 0.0;
-                const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
-                uint64_t m = cm | sm;
-                while (m) {
-                    const int l2 = __builtin_ctzll(m);
-                    m &= m - 1;
-                    if ((cm >> l2) & 1) acc = acc + readlane_d(ad, l2);
-                    if ((sm >> l2) & 1) acc = acc - readlane_d(sb2, l2);
-                }
+                acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
             }
             if (lane == 0) st_reld(&X.ndel[ls.k[sl]], acc);
         }
@@ -1560,7 +1665,13 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 __syncthreads();
                 const int np = hs->np;
                 if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src, tp);
-                if (tp && leader) { const int64_t t = (int64_t)wall_clock64(); tp[9] += t - tl1; tl1 = t; }
+                if (tp && leader) {
+                    const int64_t t = (int64_t)wall_clock64();
+                    tp[9] += t - tl1;
+                    tp[32 + (lev < 7 ? lev : 7)] += t - tl1;
+                    if (np > 0) tp[56 + (lev < 7 ? lev : 7)] += cand;
+                    tl1 = t;
+                }
                 // counters of every hub of the batch, one thread per hub; hubs
                 // not resolved by lookups (not in plist, or their bucket
                 // overflowed: kind 2) are staged in LDS with their state for
@@ -1603,11 +1714,20 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     }
                 }
                 __syncthreads();
-                if (tp && leader) tp[10] += (int64_t)wall_clock64() - tl1;
+                if (tp && leader) {
+                    const int64_t t = (int64_t)wall_clock64();
+                    tp[10] += t - tl1;
+                    tp[40 + (lev < 7 ? lev : 7)] += t - tl1;
+                    tp[48 + (lev < 7 ? lev : 7)] += nsh;
+                }
             }
             if (leader) hs->n = 0;
         }
         team_sync<TW>();
+        if (tp && leader) {
+            const int ls7 = lev < 7 ? lev : 7;
+            tp[16 + ls7] += (int64_t)wall_clock64() - tl0;
+        }
         below = here;
         uA = uB;
         uB = ts.cnt[2];
