@@ -46,7 +46,7 @@ struct SbcBucketEnt {
     double sub;
 };
 
-#define SBC_TRACE_SLOTS 64
+#define SBC_TRACE_SLOTS 72
 #define SBC_HUBQ_CAP 8192   // hubs per workgroup and pass beyond the LDS queue (global overflow queue)
 
 struct SbcParams {
@@ -84,7 +84,9 @@ struct SbcParams {
                           // light-vertex passes, the hub lookups, the hub staging and scans; ascent levels,
                           // vertices over the ascent levels; light sub-phases; per ascent level 0..7 (deeper:
                           // 7) its time; team-sorted buckets: staging, prefix, placing and summing time, terms,
-                          // buckets; per ascent level 0..7: lookup time, scan time, scanned hubs, candidates
+                          // buckets; per ascent level 0..7: lookup time, scan time, scanned hubs, candidates;
+                          // level-2 light passes: vertices, batched vertices and edges, leftover vertices and
+                          // edges, rounds, light time, hub-pass time
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)

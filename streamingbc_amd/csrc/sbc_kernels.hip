@@ -279,6 +279,37 @@ __device__ __forceinline__ int2 block_exscan2(int x, int y, int* tmp, int& total
     return make_int2(tmp[wave] + vx - x, tmp[NW + 1 + wave] + vy - y);
 }
 
+// Three exclusive prefix sums in one pass (two barriers); tmp: 3 * (NW + 1)
+// ints of LDS, not read again before the caller's next barrier.
+template <int NW>
+__device__ __forceinline__ int3 block_exscan3(int x, int y, int z, int* tmp, int& tx, int& ty, int& tz)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int vx = x, vy = y, vz = z;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int ax = __shfl_up(vx, d), ay = __shfl_up(vy, d), az = __shfl_up(vz, d);
+        if (lane >= d) { vx += ax; vy += ay; vz += az; }
+    }
+    constexpr int S = NW + 1;
+    if (lane == 63) { tmp[wave] = vx; tmp[S + wave] = vy; tmp[2 * S + wave] = vz; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int* t = tmp + threadIdx.x * S;
+        int acc = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int v = t[w];
+            t[w] = acc;
+            acc += v;
+        }
+        t[NW] = acc;
+    }
+    __syncthreads();
+    tx = tmp[NW];
+    ty = tmp[S + NW];
+    tz = tmp[2 * S + NW];
+    return make_int3(tmp[wave] + vx - x, tmp[S + wave] + vy - y, tmp[2 * S + wave] + vz - z);
+}
+
 // (sa / sb) * (d + 1) in FP64 -- the dependency term of
 // src/streamingbc_init.c:159-161 and src/streamingbc_internal.c:185-198
 __device__ __forceinline__ double dep(int64_t sa, int64_t sb, double d)
@@ -296,7 +327,10 @@ struct TeamShared {
 
 // Vertices with more visible neighbours than this are scanned by every
 // wavefront of a multi-wave team (contiguous chunk ranges per wave).
-constexpr int HUB_MIN = 512;
+#ifndef SBC_HUB_MIN
+#define SBC_HUB_MIN 512
+#endif
+constexpr int HUB_MIN = SBC_HUB_MIN;
 // 64-neighbour chunks whose loads are issued together by one wavefront.
 #ifndef SBC_SCAN_UNROLL
 #define SBC_SCAN_UNROLL 2
@@ -364,7 +398,7 @@ constexpr int EDGE_CAP = SBC_EDGE_CAP;  // edges per batch
 struct alignas(16) LightShared {
     int nb;                       // batched vertices
     int novf;                     // hubs the queues could not take (one wavefront each)
-    int scan[36];
+    int scan[52];
     int32_t k[LIGHT_CAP];
     int32_t start[LIGHT_CAP];
     int32_t off[LIGHT_CAP];       // first edge index of the vertex in the batch
@@ -565,12 +599,17 @@ struct KState {
 
 __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
 {
+    // the four loads are issued together; the state byte selects which are used
     KState s;
-    const int kt = sb_type(ld_sb(&X.sb[k]));
+    const uint8_t b = ld_sb(&X.sb[k]);
+    const int64_t sg = X.sigma[k];
+    const int64_t dp = ld_acq64(&X.dpath[k]);
+    const double dl = X.delta[k];
+    const int kt = sb_type(b);
     s.isD = kt != 3;
-    s.sk_old = X.sigma[k];
-    s.sk_new = s.isD ? (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(kt == 1 ? s.sk_old : 0)) : s.sk_old;
-    s.base = s.isD ? 0.0 : X.delta[k];
+    s.sk_old = sg;
+    s.sk_new = s.isD ? (int64_t)((uint64_t)dp + (uint64_t)(kt == 1 ? sg : 0)) : sg;
+    s.base = s.isD ? 0.0 : dl;
     return s;
 }
 
@@ -585,7 +624,6 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     const int nch = (len + 63) >> 6;
     for (int ch = 0; ch < nch; ch += SCAN_UNROLL) {
         int32_t cc[SCAN_UNROLL];
-        uint8_t ww[SCAN_UNROLL];
         bool ok[SCAN_UNROLL];
 #pragma unroll
         for (int u = 0; u < SCAN_UNROLL; ++u) {
@@ -593,16 +631,22 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
             ok[u] = (ch + u) < nch && e < len;
             cc[u] = ok[u] ? P.col[start + e] : 0;
         }
+        uint8_t ww[SCAN_UNROLL];
 #pragma unroll
         for (int u = 0; u < SCAN_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+        // every chunk's term loads are issued before the first sum waits on them
+        Term t[SCAN_UNROLL];
+        bool appU[SCAN_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SCAN_UNROLL; ++u)
+            t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
 #pragma unroll
         for (int u = 0; u < SCAN_UNROLL; ++u) {
-            bool appU;
-            const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU);
-            const int pu = wave_append(appU, &ts.cnt[2]);
+            const int pu = wave_append(appU[u], &ts.cnt[2]);
             if (pu >= 0) X.list[pu] = cc[u];
-            acc = wave_ordered_add(acc, t.child, t.add, t.hsub, t.sub);
         }
+#pragma unroll
+        for (int u = 0; u < SCAN_UNROLL; ++u) acc = wave_ordered_add(acc, t[u].child, t[u].add, t[u].hsub, t[u].sub);
     }
     if (lane == 0) st_reld(&X.ndel[k], acc);
 }
@@ -1054,24 +1098,23 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             scan_mode = len <= 2 * np;
             table_mode = !scan_mode;
         }
-        int ne, nt;
-        const int eoff = block_exscan<TW>(scan_mode ? len : 0, ls.scan, ne);
-        const int toff = block_exscan<TW>(table_mode ? 1 : 0, ls.scan, nt);
+        // edge offsets of scan-mode sources, slots of table-mode sources and
+        // compact scan-mode slots (number of scan-mode threads before), in
+        // thread order
+        int ne, nt, nsc;
+        const int3 sc = block_exscan3<TW>(scan_mode ? len : 0, table_mode ? 1 : 0, scan_mode ? 1 : 0, ls.scan, ne,
+                                          nt, nsc);
+        const int eoff = sc.x, toff = sc.y, sslot = sc.z;
         if (scan_mode) {
             ls.k[tid] = w;
             ls.start[tid] = start;
             ls.off[tid] = eoff;
             ls.isD[tid] = (i >= nb + ndl) ? 1 : 0;  // 1: parent candidate
+            ls.len[sslot] = tid;                    // owner thread of slot
         }
         if (table_mode) {
             ls.ovf[toff] = w;
             ls.eflag[toff] = (i >= nb + ndl) ? 1 : 0;
-        }
-        // compact scan-mode slots in thread order: slot = number of scan-mode threads before
-        int nsc;
-        const int sslot = block_exscan<TW>(scan_mode ? 1 : 0, ls.scan, nsc);
-        if (scan_mode) {
-            ls.len[sslot] = tid;  // owner thread of slot
         }
         __syncthreads();
         // edges of scan-mode sources
@@ -1296,10 +1339,12 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
         int32_t k = 0;
         int len = 0;
         SbcVInfo vi{0, 0, 0, 0};
+        KState kst{false, 0, 0, 0.0};
         bool light = false;
         if (t < tot) {
             k = here.at(X, t);
             vi = P.vinfo[k];
+            kst = kstate(X, k);  // loaded with vinfo; unused for hubs
             len = visible_len_thread(P, vi, X.q);
             if (len > HUB_MIN) {
                 const int pos = atomicAdd(&hs.n, 1);
@@ -1320,7 +1365,6 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             ls.start[slot] = vi.start;
             ls.off[slot] = eoff;
             ls.len[slot] = len;
-            const KState kst = kstate(X, k);
             ls.isD[slot] = kst.isD ? 1 : 0;
             ls.sk_old[slot] = kst.sk_old;
             ls.sk_new[slot] = kst.sk_new;
@@ -1332,24 +1376,52 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
         const int nb = ls.nb;
         const int no = ls.novf;
         const int ne = nb > 0 ? ls.off[nb - 1] + ls.len[nb - 1] : 0;
-        // batch: one lane per edge
-        for (int e = tid; e < ne; e += NT) {
-            int lo = 0, hi = nb;  // last slot with off <= e
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (ls.off[mid] <= e) lo = mid;
-                else hi = mid;
+        if (tp && tid == 0 && lev == 2) {
+            tp[64] += nl;
+            tp[65] += nb;
+            tp[66] += ne;
+            tp[67] += nl - nb;
+            tp[68] += ne_round - ne;
+            tp[69] += 1;
+        }
+        // batch: one lane per edge, two edges per thread in flight
+        for (int e0 = tid; e0 < ne; e0 += 2 * NT) {
+            int slot2[2], ee[2];
+            int32_t c2[2];
+            bool ok2[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = e0 + u * NT;
+                ok2[u] = e < ne;
+                ee[u] = e;
+                int lo = 0, hi = nb;  // last slot with off <= e
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (ls.off[mid] <= e) lo = mid;
+                    else hi = mid;
+                }
+                slot2[u] = lo;
+                c2[u] = ok2[u] ? P.col[ls.start[lo] + (e - ls.off[lo])] : 0;
             }
-            const int p = e - ls.off[lo];
-            const int32_t kk = ls.k[lo];
-            const int32_t c = P.col[ls.start[lo] + p];
-            bool appU;
-            const Term tm = ascent_lane(P, X, kk, ls.isD[lo] != 0, ls.sk_old[lo], ls.sk_new[lo], lev, true, c,
-                                        ld_sb(&X.sb[c]), appU);
-            if (appU) X.list[atomicAdd(&ts.cnt[2], 1)] = c;
-            ls.eflag[e] = (uint8_t)((tm.child ? 1 : 0) | (tm.hsub ? 2 : 0));
-            ls.eadd[e] = tm.add;
-            ls.esub[e] = tm.sub;
+            uint8_t b2[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) b2[u] = ok2[u] ? ld_sb(&X.sb[c2[u]]) : (uint8_t)0;
+            Term tm[2];
+            bool appU[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int lo = slot2[u];
+                tm[u] = ascent_lane(P, X, ls.k[lo], ls.isD[lo] != 0, ls.sk_old[lo], ls.sk_new[lo], lev, ok2[u], c2[u],
+                                    b2[u], appU[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (!ok2[u]) continue;
+                if (appU[u]) X.list[atomicAdd(&ts.cnt[2], 1)] = c2[u];
+                ls.eflag[ee[u]] = (uint8_t)((tm[u].child ? 1 : 0) | (tm[u].hsub ? 2 : 0));
+                ls.eadd[ee[u]] = tm[u].add;
+                ls.esub[ee[u]] = tm[u].sub;
+            }
         }
         // light vertices past the cap, one wavefront each, from their slots
         for (int sl = nb + twave; sl < nl; sl += TW) {
@@ -1616,7 +1688,10 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         }
         if constexpr (TW > 1) {
             __syncthreads();
-            if (tp && leader) tp[8] += (int64_t)wall_clock64() - tl0;
+            if (tp && leader) {
+                tp[8] += (int64_t)wall_clock64() - tl0;
+                if (lev == 2) tp[70] += (int64_t)wall_clock64() - tl0;
+            }
             const int nall = hubs_queued(P, *hs);
             for (int hb = 0; hb < nall; hb += hub_batch(P)) {
                 const int nh = min(nall - hb, hub_batch(P));
@@ -1727,6 +1802,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         if (tp && leader) {
             const int ls7 = lev < 7 ? lev : 7;
             tp[16 + ls7] += (int64_t)wall_clock64() - tl0;
+            if (lev == 2) tp[71] += (int64_t)wall_clock64() - tl0;
         }
         below = here;
         uA = uB;
