@@ -344,11 +344,8 @@ struct sbcForestDev {
     int32_t* d_dist = nullptr;
     uint8_t* d_sb = nullptr;     // state bytes, sb_stride per root
     int64_t sb_stride = 0;
-    int64_t* d_sigma = nullptr;
-    double* d_delta = nullptr;
+    SbcVRec* d_vr = nullptr;     // [K][nv] path counts, dependencies, update scratch
     double* d_bc = nullptr;
-    int64_t* d_dpath = nullptr;
-    double* d_ndel = nullptr;
     int32_t* d_list = nullptr;
     int32_t* d_lvl = nullptr;
     int32_t* d_err = nullptr;
@@ -358,7 +355,7 @@ static int32_t lcap_for(int64_t nv) { return (int32_t)(nv + 2 < 4096 ? nv + 2 : 
 
 extern "C" int64_t streamingBCForestFootprint(int64_t NV, int64_t NK)
 {
-    int64_t per = 4 + 1 + 8 + 8 + 8 + 8 + 4; // dist, state byte, sigma, delta, dpath, ndel, list
+    int64_t per = 4 + 1 + (int64_t)sizeof(SbcVRec) + 4; // dist, state byte, path record, list
     return NK * NV * per + NK * 2 * (int64_t)lcap_for(NV) * 4 + NV * 8 + NK * 4 + 64;
 }
 
@@ -385,8 +382,7 @@ extern "C" int64_t streamingBCResidentFootprint(const int64_t* rowptr, int64_t N
 static void forest_free_dev(sbcForestDev* F)
 {
     if (!F) return;
-    void* ptrs[] = {F->d_roots, F->d_dist, F->d_sb, F->d_sigma, F->d_delta, F->d_bc,
-                    F->d_dpath, F->d_ndel, F->d_list, F->d_lvl, F->d_err};
+    void* ptrs[] = {F->d_roots, F->d_dist, F->d_sb, F->d_vr, F->d_bc, F->d_list, F->d_lvl, F->d_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete F;
@@ -426,9 +422,8 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
     sbcStatus st = SBC_OK;
     F->sb_stride = (NV + 63) / 64 * 64;
     if ((st = dalloc(&F->d_roots, NK)) || (st = dalloc(&F->d_dist, kn)) ||
-        (st = dalloc(&F->d_sb, (size_t)NK * (size_t)F->sb_stride)) || (st = dalloc(&F->d_sigma, kn)) ||
-        (st = dalloc(&F->d_delta, kn)) || (st = dalloc(&F->d_bc, NV)) ||
-        (st = dalloc(&F->d_dpath, kn)) || (st = dalloc(&F->d_ndel, kn)) || (st = dalloc(&F->d_list, kn)) ||
+        (st = dalloc(&F->d_sb, (size_t)NK * (size_t)F->sb_stride)) || (st = dalloc(&F->d_vr, kn)) ||
+        (st = dalloc(&F->d_bc, NV)) || (st = dalloc(&F->d_list, kn)) ||
         (st = dalloc(&F->d_lvl, (size_t)NK * 2 * F->lcap)) || (st = dalloc(&F->d_err, 4))) {
         forest_free_dev(F);
         return st;
@@ -436,8 +431,7 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMemcpy(F->d_roots, r32.data(), sizeof(int32_t) * NK, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(F->d_sb, 0, (size_t)NK * (size_t)F->sb_stride);
-    if (e == hipSuccess) e = hipMemset(F->d_dpath, 0, sizeof(int64_t) * kn);
-    if (e == hipSuccess) e = hipMemset(F->d_ndel, 0, sizeof(double) * kn);
+    if (e == hipSuccess) e = hipMemset(F->d_vr, 0, sizeof(SbcVRec) * kn);
     if (e == hipSuccess) e = hipMemset(F->d_bc, 0, sizeof(double) * NV);
     if (e == hipSuccess) e = hipMemset(F->d_err, 0, sizeof(int32_t) * 4);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -745,11 +739,8 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.dist = F->d_dist;
     P.sb = F->d_sb;
     P.sb_stride = F->sb_stride;
-    P.sigma = F->d_sigma;
-    P.delta = F->d_delta;
+    P.vr = F->d_vr;
     P.bc = F->d_bc;
-    P.dpath = F->d_dpath;
-    P.ndel = F->d_ndel;
     P.list = F->d_list;
     P.lvl = F->d_lvl;
     P.roots = F->d_roots;
@@ -921,8 +912,14 @@ extern "C" sbcStatus streamingBCGetTree(bcForest* forest, int64_t ri, int64_t* l
         HIPCHK(hipMemcpy(d.data(), F->d_dist + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
         for (int64_t x = 0; x < F->nv; x++) level[x] = d[x] == INT32_MAX ? LONG_MAX : d[x];
     }
-    if (sigma) HIPCHK(hipMemcpy(sigma, F->d_sigma + off, sizeof(int64_t) * F->nv, hipMemcpyDeviceToHost));
-    if (delta) HIPCHK(hipMemcpy(delta, F->d_delta + off, sizeof(double) * F->nv, hipMemcpyDeviceToHost));
+    if (sigma || delta) {
+        std::vector<SbcVRec> rs((size_t)F->nv);
+        HIPCHK(hipMemcpy(rs.data(), F->d_vr + off, sizeof(SbcVRec) * F->nv, hipMemcpyDeviceToHost));
+        for (int64_t x = 0; x < F->nv; x++) {
+            if (sigma) sigma[x] = rs[x].sigma;
+            if (delta) delta[x] = rs[x].delta;
+        }
+    }
     return SBC_OK;
 }
 
@@ -935,7 +932,7 @@ extern "C" sbcStatus streamingBCRecomputeBC(bcForest* forest, extraArraysPerThre
     memset(&P, 0, sizeof(P));
     P.nv = (int32_t)F->nv;
     P.K = (int32_t)F->K;
-    P.delta = F->d_delta;
+    P.vr = F->d_vr;
     P.bc = F->d_bc;
     P.roots = F->d_roots;
     hipEvent_t e0;
@@ -1461,8 +1458,7 @@ extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger*
     SbcFeatArgs A;
     A.vinfo = G->d_vinfo;
     A.dist = F->d_dist;
-    A.sigma = F->d_sigma;
-    A.delta = F->d_delta;
+    A.vr = F->d_vr;
     A.u = d_uv;
     A.v = d_uv + count;
     A.hist = root_hist ? d_hist : nullptr;
@@ -1536,8 +1532,7 @@ extern "C" sbcStatus streamingBCScheduleWindow(bcForest* forest, struct stinger*
     SbcFeatArgs A;
     A.vinfo = G->d_vinfo;
     A.dist = F->d_dist;
-    A.sigma = F->d_sigma;
-    A.delta = F->d_delta;
+    A.vr = F->d_vr;
     A.u = C->d_suv;
     A.v = C->d_suv + count;
     A.hist = root_hist ? C->d_shist : nullptr;

@@ -49,6 +49,19 @@ struct SbcBucketEnt {
 #define SBC_TRACE_SLOTS 72
 #define SBC_HUBQ_CAP 8192   // hubs per workgroup and pass beyond the LDS queue (global overflow queue)
 
+// Path count and dependency of one vertex in one root's tree, with the
+// update scratch, in one 32-byte record: an ascent term reads all four, so
+// they arrive in one cache line.  Levels and state bytes stay in dense
+// arrays of their own (neighbour scans read only those, and the popular
+// low-id vertices of a power-law graph keep them cache-resident).
+struct alignas(32) SbcVRec {
+    int64_t sigma;    // shortest-path count (LONG_MAX: unreachable)
+    double delta;     // dependency
+    int64_t dpath;    // update scratch: added paths (the reference's diffPath)
+    double ndel;      // update scratch: new dependency
+};
+static_assert(sizeof(SbcVRec) == 32, "four 8-byte fields");
+
 struct SbcParams {
     const SbcVInfo* vinfo;
     const int32_t* col;
@@ -63,11 +76,8 @@ struct SbcParams {
     int32_t* dist;        // levels, INT32_MAX = unreachable
     uint8_t* sb;          // state bytes, sb_stride bytes per root
     int64_t sb_stride;
-    int64_t* sigma;
-    double* delta;
+    SbcVRec* vr;          // [K][nv] path counts, dependencies, update scratch
     double* bc;
-    int64_t* dpath;
-    double* ndel;
     int32_t* list;
     int32_t* lvl;
     const int32_t* roots;
@@ -141,8 +151,7 @@ struct SbcAppend {
 struct SbcFeatArgs {
     const SbcVInfo* vinfo;
     const int32_t* dist;
-    const int64_t* sigma;
-    const double* delta;
+    const SbcVRec* vr;
     const int32_t* u;
     const int32_t* v;
     const float* hist;

@@ -421,10 +421,7 @@ static_assert(TEAM_SORT_BITS <= 64 * LIGHT_CAP, "bitmap fits LightShared::base")
 struct Item {
     int32_t* dist;    // levels (unchanged until commit)
     uint8_t* sb;      // state bytes
-    int64_t* sigma;
-    double* delta;
-    int64_t* dpath;
-    double* ndel;
+    SbcVRec* vr;      // the root's path counts, dependencies and scratch
     int32_t* list;
     int nv;
     int32_t q;        // visibility sequence
@@ -475,12 +472,12 @@ __device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X
                         const uint8_t seen = cas_sb(&X.sb[k], b, want);
                         appD = (seen == b);
                         if (seen == b || seen == want)
-                            add64_wg(&X.dpath[k], dc);
+                            add64_wg(&X.vr[k].dpath, dc);
                     } else if (cd == -1 && cmoved) {
                         appS = (cas_sb(&X.sb[k], b, sb_make(lc, 3)) == b);
                     }
                 } else if (b == wantStay || b == wantMove) {
-                    add64_wg(&X.dpath[k], dc);
+                    add64_wg(&X.vr[k].dpath, dc);
                 }
             }
             const int pd = wave_append(appD, &ts.cnt[0]);
@@ -515,7 +512,7 @@ __device__ __forceinline__ void descent_hubs_flat(const SbcParams& P, const Item
         ls.start[h] = vi.start;
         ls.len[h] = len;
         ls.isD[h] = cm ? 1 : 0;
-        ls.sk_old[h] = ld_acq64(&X.dpath[c]);  // added paths of c
+        ls.sk_old[h] = ld_acq64(&X.vr[c].dpath);  // added paths of c
         ls.sk_new[h] = X.dist[c];              // old level of c
         mine += (len + 63) >> 6;
         atomicAdd(&ts.stat[0], 1ull);
@@ -573,17 +570,17 @@ __device__ __forceinline__ Term ascent_lane(const SbcParams& P, const Item& X, i
     if (ctype != 0) {
         if (rel == 1) {
             t.child = true;
-            const int64_t sc_old = X.sigma[c];
+            const int64_t sc_old = X.vr[c].sigma;
             const int64_t sc_new = ctype == 3 ? sc_old
-                : (int64_t)((uint64_t)ld_acq64(&X.dpath[c]) + (uint64_t)(ctype == 1 ? sc_old : 0));
-            t.add = dep(sk_new, sc_new, ld_accd(&X.ndel[c]));
+                : (int64_t)((uint64_t)ld_acq64(&X.vr[c].dpath) + (uint64_t)(ctype == 1 ? sc_old : 0));
+            t.add = dep(sk_new, sc_new, ld_accd(&X.vr[c].ndel));
             if (!isD && !(k == X.p && c == X.s)) {
                 t.hsub = true;
-                t.sub = dep(sk_old, sc_old, X.delta[c]);
+                t.sub = dep(sk_old, sc_old, X.vr[c].delta);
             }
         } else if (!isD && rel == 0 && ctype == 2) {
             t.hsub = true;
-            t.sub = dep(sk_old, X.sigma[c], X.delta[c]);
+            t.sub = dep(sk_old, X.vr[c].sigma, X.vr[c].delta);
         }
     } else if (lev >= 1 && rel == -1) {
         appU = (cas_sb(&X.sb[c], bc, sb_make(lev - 1, 3)) == bc);
@@ -602,9 +599,9 @@ __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
     // the four loads are issued together; the state byte selects which are used
     KState s;
     const uint8_t b = ld_sb(&X.sb[k]);
-    const int64_t sg = X.sigma[k];
-    const int64_t dp = ld_acq64(&X.dpath[k]);
-    const double dl = X.delta[k];
+    const int64_t sg = X.vr[k].sigma;
+    const int64_t dp = ld_acq64(&X.vr[k].dpath);
+    const double dl = X.vr[k].delta;
     const int kt = sb_type(b);
     s.isD = kt != 3;
     s.sk_old = sg;
@@ -648,7 +645,7 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
 #pragma unroll
         for (int u = 0; u < SCAN_UNROLL; ++u) acc = wave_ordered_add(acc, t[u].child, t[u].add, t[u].hsub, t[u].sub);
     }
-    if (lane == 0) st_reld(&X.ndel[k], acc);
+    if (lane == 0) st_reld(&X.vr[k].ndel, acc);
 }
 
 template <typename TS>
@@ -715,7 +712,7 @@ __device__ __forceinline__ void ascent_vertex_team(const SbcParams& P, const Ite
                     if (fl & 1) acc = acc + hs.radd[w][i];
                     if (fl & 2) acc = acc - hs.rsub[w][i];
                 }
-            st_reld(&X.ndel[k], acc);
+            st_reld(&X.vr[k].ndel, acc);
         }
     }
     __syncthreads();
@@ -874,7 +871,7 @@ __device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item&
             }
             acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
         }
-        if (lane == 0) st_reld(&X.ndel[ls.k[j]], acc);
+        if (lane == 0) st_reld(&X.vr[ls.k[j]].ndel, acc);
     }
     __syncthreads();
     for (int j = 0; j < nsh; ++j) {
@@ -919,7 +916,7 @@ __device__ __forceinline__ void probe_term(const SbcParams& P, const Item& X, Hu
                                            int32_t k, int hi, int lev, int32_t w, int pos)
 {
     bool appU;
-    const int64_t sk = X.sigma[k];
+    const int64_t sk = X.vr[k].sigma;
     const Term t = ascent_lane(P, X, k, false, sk, sk, lev, true, w, ld_sb(&X.sb[w]), appU);
     if (t.child || t.hsub) {
         const int slot = atomicAdd(&hs.bcnt[hi], 1);
@@ -1190,7 +1187,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                     ad = e.add;
                     sb2 = e.sub;
                 }
-                if (gl == 0) base = X.delta[hs.plist[hi]];
+                if (gl == 0) base = X.vr[hs.plist[hi]].delta;
             }
             for (int k2 = 2; k2 <= 8; k2 <<= 1) {
                 for (int jj = k2 >> 1; jj > 0; jj >>= 1) {
@@ -1213,7 +1210,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                     if (f2 & 2) acc = acc - b2;
                 }
             }
-            if (small && gl == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
+            if (small && gl == 0) st_reld(&X.vr[hs.plist[hi]].ndel, acc);
             if (hi < np && gl == 0) {
                 const uint8_t kd = n > hs.bstride ? 2
                                  : n > SBC_BUCKET ? (hs.plen[hi] <= TEAM_SORT_BITS ? 3 : 2)
@@ -1237,8 +1234,8 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const int hi = h0 + l0;
             const int n = hs.bcnt[hi];
             const BucketLane e = bucket_lane(bk + (size_t)hi * hs.bstride, n);
-            const double acc = bucket_sum(e, X.delta[hs.plist[hi]]);
-            if (lane == 0) st_reld(&X.ndel[hs.plist[hi]], acc);
+            const double acc = bucket_sum(e, X.vr[hs.plist[hi]].delta);
+            if (lane == 0) st_reld(&X.vr[hs.plist[hi]].ndel, acc);
         }
     }
     __syncthreads();
@@ -1300,7 +1297,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[26] += x - tk; tk = x; }
         if (!dup && twave == 0) {
             const int32_t k = hs.plist[hi];
-            double acc = X.delta[k];
+            double acc = X.vr[k].delta;
             for (int e0 = 0; e0 < n; e0 += 64) {
                 const int e = e0 + lane;
                 const uint8_t fl = e < n ? ls.eflag[e] : (uint8_t)0;
@@ -1308,7 +1305,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                 const double sb2 = (fl & 2) ? ls.esub[e] : 0.0;
                 acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
             }
-            if (lane == 0) st_reld(&X.ndel[k], acc);
+            if (lane == 0) st_reld(&X.vr[k].ndel, acc);
         }
         __syncthreads();
         if (tp && tid == 0) tp[27] += (int64_t)wall_clock64() - tk;
@@ -1463,7 +1460,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
                 const double sb2 = (fl & 2) ? ls.esub[a + e] : 0.0;
                 acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
             }
-            if (lane == 0) st_reld(&X.ndel[ls.k[sl]], acc);
+            if (lane == 0) st_reld(&X.vr[ls.k[sl]].ndel, acc);
         }
         __syncthreads();
         if (tp && tid == 0) tp[15] += (int64_t)wall_clock64() - tq;
@@ -1521,10 +1518,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     Item X;
     X.dist = P.dist + off;
     X.sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
-    X.sigma = P.sigma + off;
-    X.delta = P.delta + off;
-    X.dpath = P.dpath + off;
-    X.ndel = P.ndel + off;
+    X.vr = P.vr + off;
     X.list = P.list + off;
     X.nv = nv;
     X.r = P.roots[ri];
@@ -1560,7 +1554,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
 
     if (leader) {
         st_sb(&X.sb[X.s], sb_make(base, smoved ? 2 : 1));
-        st_rel64(&X.dpath[X.s], X.sigma[X.p]);
+        st_rel64(&X.vr[X.s].dpath, X.vr[X.p].sigma);
         X.list[0] = X.s;
         lvlD[0] = 0;
         lvlS[0] = 0;
@@ -1586,7 +1580,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             const int len = visible_len(P, vi, X.q);
             if (queue_hub<TW>(P, hs, c, len)) continue;
             const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
-            const int64_t dc = ld_acq64(&X.dpath[c]);
+            const int64_t dc = ld_acq64(&X.vr[c].dpath);
             descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, ts);
             my_proc += 1;
             my_scan += (unsigned long long)len;
@@ -1821,28 +1815,28 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         for (int w = lvlD[jf] + twave * 64 + lane; w < lvlD[jf + 1]; w += TW * 64) {
             const int32_t k = X.list[w];
             const int ktype = sb_type(ld_sb(&X.sb[k]));
-            const double nd = ld_accd(&X.ndel[k]);
-            const double od = X.delta[k];
-            const int64_t so = X.sigma[k];
-            X.sigma[k] = (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
-            X.delta[k] = nd;
+            const double nd = ld_accd(&X.vr[k].ndel);
+            const double od = X.vr[k].delta;
+            const int64_t so = X.vr[k].sigma;
+            X.vr[k].sigma = (int64_t)((uint64_t)ld_acq64(&X.vr[k].dpath) + (uint64_t)(ktype == 1 ? so : 0));
+            X.vr[k].delta = nd;
             X.dist[k] = nl;
             st_sb(&X.sb[k], sb_make(nl, 0));
             if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
-            st_rel64(&X.dpath[k], 0);
-            st_reld(&X.ndel[k], 0.0);
+            st_rel64(&X.vr[k].dpath, 0);
+            st_reld(&X.vr[k].ndel, 0.0);
         }
     }
     // ascent-only vertices keep their level
     for (int w = twave * 64 + lane; w < total - nD; w += TW * 64) {
         const int32_t k = w < nUend - nD ? X.list[nD + w] : X.list[nv - 1 - (w - (nUend - nD))];
         const uint8_t b = ld_sb(&X.sb[k]);
-        const double nd = ld_accd(&X.ndel[k]);
-        const double od = X.delta[k];
-        X.delta[k] = nd;
+        const double nd = ld_accd(&X.vr[k].ndel);
+        const double od = X.vr[k].delta;
+        X.vr[k].delta = nd;
         st_sb(&X.sb[k], (uint8_t)(b & ~3));
         if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
-        st_reld(&X.ndel[k], 0.0);
+        st_reld(&X.vr[k].ndel, 0.0);
     }
     if (lane == 0) {
         atomicAdd(&ts.stat[0], my_proc);
@@ -1917,10 +1911,7 @@ __device__ __forceinline__ Item parked_item(const SbcParams& P, const SbcWideIte
     Item X;
     X.dist = P.dist + off;
     X.sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
-    X.sigma = P.sigma + off;
-    X.delta = P.delta + off;
-    X.dpath = P.dpath + off;
-    X.ndel = P.ndel + off;
+    X.vr = P.vr + off;
     X.list = P.list + off;
     X.nv = P.nv;
     X.q = w.q;
@@ -1944,7 +1935,7 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_kernel(SbcParams P, 
         const SbcVInfo vi = P.vinfo[c];
         const int len = visible_len(P, vi, X.q);
         const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
-        const int64_t dc = ld_acq64(&X.dpath[c]);
+        const int64_t dc = ld_acq64(&X.vr[c].dpath);
         descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, w);
         my_proc += 1;
         my_scan += (unsigned long long)len;
@@ -2001,28 +1992,28 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_commit_kernel(SbcParams P, S
         for (int i = a + gt; i < b; i += nthreads) {
             const int32_t k = X.list[i];
             const int ktype = sb_type(ld_sb(&X.sb[k]));
-            const double nd = ld_accd(&X.ndel[k]);
-            const double od = X.delta[k];
-            const int64_t so = X.sigma[k];
-            X.sigma[k] = (int64_t)((uint64_t)ld_acq64(&X.dpath[k]) + (uint64_t)(ktype == 1 ? so : 0));
-            X.delta[k] = nd;
+            const double nd = ld_accd(&X.vr[k].ndel);
+            const double od = X.vr[k].delta;
+            const int64_t so = X.vr[k].sigma;
+            X.vr[k].sigma = (int64_t)((uint64_t)ld_acq64(&X.vr[k].dpath) + (uint64_t)(ktype == 1 ? so : 0));
+            X.vr[k].delta = nd;
             X.dist[k] = nl;
             st_sb(&X.sb[k], sb_make(nl, 0));
             if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
-            st_rel64(&X.dpath[k], 0);
-            st_reld(&X.ndel[k], 0.0);
+            st_rel64(&X.vr[k].dpath, 0);
+            st_reld(&X.vr[k].ndel, 0.0);
         }
     }
     const int rest = nUend + nS - nD;
     for (int i = gt; i < rest; i += nthreads) {
         const int32_t k = i < nUend - nD ? X.list[nD + i] : X.list[X.nv - 1 - (i - (nUend - nD))];
         const uint8_t b = ld_sb(&X.sb[k]);
-        const double nd = ld_accd(&X.ndel[k]);
-        const double od = X.delta[k];
-        X.delta[k] = nd;
+        const double nd = ld_accd(&X.vr[k].ndel);
+        const double od = X.vr[k].delta;
+        X.vr[k].delta = nd;
         st_sb(&X.sb[k], (uint8_t)(b & ~3));
         if (P.bc_atomic && k != X.r) atomicAdd(&P.bc[k], nd - od);
-        st_reld(&X.ndel[k], 0.0);
+        st_reld(&X.vr[k].ndel, 0.0);
     }
     if (gt == 0 && P.rec) {
         SbcItemRec* rec = &P.rec[(size_t)w.item * P.K + ri];
@@ -2059,23 +2050,21 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
         const size_t off = (size_t)ri * (size_t)nv;
         int32_t* dist = P.dist + off;
         uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
-        int64_t* sigma = P.sigma + off;
-        double* delta = P.delta + off;
-        int64_t* acc_s = P.dpath + off;
+        SbcVRec* vr = P.vr + off;
         int32_t* list = P.list + off;
         int32_t* lvl = P.lvl + (size_t)ri * 2 * P.lcap;
         const int32_t r = P.roots[ri];
         for (int x = threadIdx.x; x < nv; x += NWB * 64) {
             st_rel32(&dist[x], INF32);
             sb[x] = SB_INF;
-            sigma[x] = SIGMA_INF;
-            delta[x] = 0.0;
+            vr[x].sigma = SIGMA_INF;
+            vr[x].delta = 0.0;
         }
         __syncthreads();
         if (leader) {
             st_rel32(&dist[r], 0);
             sb[r] = sb_make(0, 0);
-            sigma[r] = 1;
+            vr[r].sigma = 1;
             list[0] = r;
             lvl[0] = 0;
             lvl[1] = 1;
@@ -2089,7 +2078,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
             const int32_t next = j + 1;
             for (int idx = fs + wave; idx < fe; idx += NWB) {
                 const int32_t c = list[idx];
-                const int64_t sc = sigma[c];
+                const int64_t sc = vr[c].sigma;
                 const SbcVInfo vi = P.vinfo[c];
                 const int len = vi.deg; // the static pass sees the whole graph
                 for (int e0 = 0; e0 < len; e0 += 64) {
@@ -2104,7 +2093,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
                             app = (old == unseen);
                             wk = app ? next : old;
                         }
-                        if (wk == next) add64_wg(&acc_s[k], sc);
+                        if (wk == next) add64_wg(&vr[k].dpath, sc);
                     }
                     const int pos = wave_append(app, &ts.cnt[0]);
                     if (pos >= 0) list[pos] = k;
@@ -2115,9 +2104,9 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
             __syncthreads();
             for (int x = fe + threadIdx.x; x < nfe; x += NWB * 64) {
                 const int32_t k = list[x];
-                sigma[k] = ld_acq64(&acc_s[k]);
+                vr[k].sigma = ld_acq64(&vr[k].dpath);
                 sb[k] = sb_make(j + 1, 0);
-                st_rel64(&acc_s[k], 0);
+                st_rel64(&vr[k].dpath, 0);
             }
             j++;
             if (j + 1 >= P.lcap) { overflow = true; break; }
@@ -2137,7 +2126,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
             const int32_t childw = L + 1;
             for (int idx = a + wave; idx < b; idx += NWB) {
                 const int32_t k = list[idx];
-                const int64_t sk = sigma[k];
+                const int64_t sk = vr[k].sigma;
                 const SbcVInfo vi = P.vinfo[k];
                 double acc = 0.0;
                 for (int e0 = 0; e0 < vi.deg; e0 += 64) {
@@ -2148,7 +2137,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
                         const int32_t c = P.col[vi.start + e];
                         if (ld_acq32(&dist[c]) == childw) {
                             child = true;
-                            add = dep(sk, sigma[c], delta[c]);
+                            add = dep(sk, vr[c].sigma, vr[c].delta);
                         }
                     }
                     uint64_t m = ballot(child);
@@ -2158,7 +2147,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
                         acc = acc + readlane_d(add, l);
                     }
                 }
-                if (lane == 0) delta[k] = acc;
+                if (lane == 0) vr[k].delta = acc;
             }
             __syncthreads();
         }
@@ -2173,7 +2162,7 @@ __global__ void sbc_bc_reduce_kernel(SbcParams P)
     double acc = 0.0;
     for (int ri = 0; ri < P.K; ++ri) {
         if (P.roots[ri] == v) continue;
-        acc += P.delta[(size_t)ri * (size_t)P.nv + v];
+        acc += P.vr[(size_t)ri * (size_t)P.nv + v].delta;
     }
     P.bc[v] = 0.0 + acc;
 }

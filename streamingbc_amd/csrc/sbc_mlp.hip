@@ -251,10 +251,10 @@ __global__ void sbc_feature_kernel(SbcFeatArgs A)
         f[4] = (float)(dp < 16 ? dp : 16) / 16.f;
         f[5] = log1pf((float)A.vinfo[s].deg) / 10.f;
         f[6] = log1pf((float)A.vinfo[pv].deg) / 10.f;
-        f[7] = dp == INF ? 0.f : (float)log1p((double)A.sigma[off + pv]) / 20.f;
-        f[8] = ds == INF ? 0.f : (float)log1p((double)A.sigma[off + s]) / 20.f;
-        f[9] = (float)log1p(A.delta[off + s]) / 10.f;
-        f[10] = (float)log1p(A.delta[off + pv]) / 10.f;
+        f[7] = dp == INF ? 0.f : (float)log1p((double)A.vr[off + pv].sigma) / 20.f;
+        f[8] = ds == INF ? 0.f : (float)log1p((double)A.vr[off + s].sigma) / 20.f;
+        f[9] = (float)log1p(A.vr[off + s].delta) / 10.f;
+        f[10] = (float)log1p(A.vr[off + pv].delta) / 10.f;
     }
     f[11] = A.hist ? A.hist[r] / 10.f : 0.f;
     for (int q = 0; q < NF; ++q) A.X[t * NF + q] = f[q];
