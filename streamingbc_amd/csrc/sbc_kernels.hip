@@ -1,13 +1,15 @@
 // sbc_kernels.hip -- gfx950 kernels of the streaming-BC engine.
 //
-// Per root r the forest holds, in HBM, structure-of-arrays slices of NV
-// entries: dist (int32 level, INT32_MAX = unreachable), sb (one state byte,
-// see below; the only per-root array gathered while scanning adjacency
-// lists), sigma (int64, LONG_MAX = unreachable), delta (FP64), and per-root
-// scratch used only while an update of that root runs: dpath (int64, the
-// added-path count the reference calls diffPath), ndel (FP64, the new delta)
-// and list (int32, the processed vertices).  Scratch is cleared on the
-// touched entries only, at commit, as src/streamingbc_internal.c:213-242 does.
+// Per root r the forest holds, in HBM, slices of NV entries: dist (int32
+// level, INT32_MAX = unreachable), sb (one state byte, see below; gathered
+// for every neighbour while scanning adjacency lists, dense so that popular
+// vertices stay cache-resident), vr (one 32-byte record per vertex: sigma
+// (int64, LONG_MAX = unreachable), delta (FP64) and the scratch used only
+// while an update of that root runs, dpath (int64, the added-path count the
+// reference calls diffPath) and ndel (FP64, the new delta); an ascent term
+// reads all four from one cache line) and list (int32, the processed
+// vertices).  Scratch is cleared on the touched entries only, at commit, as
+// src/streamingbc_internal.c:213-242 does.
 //
 // One update item = (root, inserted edge).  A team (one wavefront, or the
 // whole workgroup) runs it:
