@@ -338,6 +338,12 @@ constexpr int HUB_MIN = SBC_HUB_MIN;
 #define SBC_SCAN_UNROLL 2
 #endif
 constexpr int SCAN_UNROLL = SBC_SCAN_UNROLL;
+// chunks whose loads one wavefront issues together when it ascends a single
+// vertex on its own
+#ifndef SBC_WAVE_UNROLL
+#define SBC_WAVE_UNROLL 2
+#endif
+constexpr int WAVE_UNROLL = SBC_WAVE_UNROLL;
 constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
@@ -359,6 +365,9 @@ struct alignas(16) HubShared {
     int n3;               // lookup hubs whose buckets the whole team sorts
     int dup;              // a team-sorted bucket holds a position twice
     int plen[HUB_CAP];    // visible list length of each lookup hub
+    int pstart[HUB_CAP];  // first slot of each lookup hub
+    int hlen[HUB_CAP];    // visible list length of each queued hub of the pass
+    int hstart[HUB_CAP];  // first slot of each queued hub of the pass
     int list[HUB_CAP];
     int plist[HUB_CAP];   // ascent-only hubs of the level handled by table lookups
     int bcnt[HUB_CAP];    // terms collected per lookup hub
@@ -621,31 +630,31 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     const int lane = lane_id();
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
-    for (int ch = 0; ch < nch; ch += SCAN_UNROLL) {
-        int32_t cc[SCAN_UNROLL];
-        bool ok[SCAN_UNROLL];
+    for (int ch = 0; ch < nch; ch += WAVE_UNROLL) {
+        int32_t cc[WAVE_UNROLL];
+        bool ok[WAVE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < SCAN_UNROLL; ++u) {
+        for (int u = 0; u < WAVE_UNROLL; ++u) {
             const int e = (ch + u) * 64 + lane;
             ok[u] = (ch + u) < nch && e < len;
             cc[u] = ok[u] ? P.col[start + e] : 0;
         }
-        uint8_t ww[SCAN_UNROLL];
+        uint8_t ww[WAVE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < SCAN_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+        for (int u = 0; u < WAVE_UNROLL; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
         // every chunk's term loads are issued before the first sum waits on them
-        Term t[SCAN_UNROLL];
-        bool appU[SCAN_UNROLL];
+        Term t[WAVE_UNROLL];
+        bool appU[WAVE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < SCAN_UNROLL; ++u)
+        for (int u = 0; u < WAVE_UNROLL; ++u)
             t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
 #pragma unroll
-        for (int u = 0; u < SCAN_UNROLL; ++u) {
+        for (int u = 0; u < WAVE_UNROLL; ++u) {
             const int pu = wave_append(appU[u], &ts.cnt[2]);
             if (pu >= 0) X.list[pu] = cc[u];
         }
 #pragma unroll
-        for (int u = 0; u < SCAN_UNROLL; ++u) acc = wave_ordered_add(acc, t[u].child, t[u].add, t[u].hsub, t[u].sub);
+        for (int u = 0; u < WAVE_UNROLL; ++u) acc = wave_ordered_add(acc, t[u].child, t[u].add, t[u].hsub, t[u].sub);
     }
     if (lane == 0) st_reld(&X.vr[k].ndel, acc);
 }
@@ -1154,11 +1163,10 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     // inserted edges of the hubs (a few each), one thread per hub
     for (int hi = tid; hi < np; hi += NT) {
         const int32_t k = hs.plist[hi];
-        const SbcVInfo vi = P.vinfo[k];
-        const int len = visible_len_thread(P, vi, X.q);
-        hs.plen[hi] = len;
+        const int len = hs.plen[hi];
+        const int start = hs.pstart[hi];
         for (int e = P.hub[k].y; e < len; ++e) {
-            const int32_t c = P.col[vi.start + e];
+            const int32_t c = P.col[start + e];
             const uint8_t b = ld_sb(&X.sb[c]);
             if (sb_type(b) == 0) {
                 if (!(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
@@ -1719,16 +1727,29 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     }
                 }
                 const long cand = (long)src.size();
+                // one thread per hub: its descriptor, table and state byte in
+                // one round trip; its visible length and first slot kept for
+                // the lookups, the counters and the scans
                 for (int h = threadIdx.x; h < nh; h += TW * 64) {
                     const int32_t k = hs->list[h];
+                    const SbcVInfo vi = P.vinfo[k];
+                    const int4 hb = P.hub ? P.hub[k] : make_int4(0, 0, 0, 0);
+                    const uint8_t kb = ld_sb(&X.sb[k]);
+                    const int len = visible_len_thread(P, vi, X.q);
+                    hs->hlen[h] = len;
+                    hs->hstart[h] = vi.start;
+                    atomicAdd(&ts.stat[0], 1ull);
+                    atomicAdd(&ts.stat[1], (unsigned long long)len);
                     bool pk = false;
-                    if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && P.hub[k].y > 0 &&
-                        sb_type(ld_sb(&X.sb[k])) == 3)
-                        pk = cand < (long)P.hub[k].y;
+                    if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && hb.y > 0 &&
+                        sb_type(kb) == 3)
+                        pk = cand < (long)hb.y;
                     int hi = -1;
                     if (pk) {
                         hi = atomicAdd(&hs->np, 1);
                         hs->plist[hi] = k;
+                        hs->plen[hi] = len;
+                        hs->pstart[hi] = vi.start;
                         hs->kind[hi] = 0;
                     }
                     hs->pidx[h] = hi;
@@ -1753,10 +1774,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     const int32_t k = hs->list[h];
                     const int hi = hs->pidx[h];
                     const bool looked_up = hi >= 0 && hs->kind[hi] != 2;
-                    const SbcVInfo vi = P.vinfo[k];
-                    const int len = visible_len_thread(P, vi, X.q);
-                    atomicAdd(&ts.stat[0], 1ull);
-                    atomicAdd(&ts.stat[1], (unsigned long long)len);
+                    const int len = hs->hlen[h];
                     if (tp) {
                         atomicAdd((unsigned long long*)&tp[looked_up ? 4 : 5], 1ull);
                         atomicAdd((unsigned long long*)&tp[looked_up ? 6 : 7], (unsigned long long)len);
@@ -1765,7 +1783,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                         const int j = atomicAdd(&hs->ns, 1);
                         const KState kst = kstate(X, k);
                         ls->k[j] = k;
-                        ls->start[j] = vi.start;
+                        ls->start[j] = hs->hstart[h];
                         ls->len[j] = len;
                         ls->isD[j] = kst.isD ? 1 : 0;
                         ls->sk_old[j] = kst.sk_old;
