@@ -390,6 +390,36 @@ struct alignas(16) HubShared {
     uint8_t rflag[16][RUN_CAP];
 };
 
+__device__ __forceinline__ int phash_slot(int32_t v) { return (int)(((uint32_t)v * 0x9E3779B1u) >> 21) & (PHASH - 1); }
+
+// Membership set of an ascent level's possible sources, in hs.phk: every
+// vertex that can give a light vertex of the level a term (the processed
+// vertices one level down and the descent vertices on the level) or that the
+// level may mark as a parent (every vertex one level up).  A neighbour
+// outside it contributes nothing, so its state byte is not gathered.
+constexpr int SHASH_MAX = PHASH / 2;
+
+__device__ __forceinline__ void shash_insert(HubShared& hs, int32_t v)
+{
+    int h = phash_slot(v);
+    for (;;) {
+        const int old = atomicCAS(&hs.phk[h], 0, v + 1);
+        if (old == 0 || old == v + 1) return;
+        h = (h + 1) & (PHASH - 1);
+    }
+}
+
+__device__ __forceinline__ bool shash_has(const HubShared& hs, int32_t v)
+{
+    int h = phash_slot(v);
+    for (;;) {
+        const int key = hs.phk[h];
+        if (key == v + 1) return true;
+        if (key == 0) return false;
+        h = (h + 1) & (PHASH - 1);
+    }
+}
+
 // Hubs per hub pass: the LDS queue's capacity (smaller only to test the
 // batched passes).
 __device__ __forceinline__ int hub_batch(const SbcParams& P)
@@ -625,7 +655,8 @@ __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
 // order lane by lane.
 template <typename TS>
 __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const Item& X, int32_t k, int32_t start,
-                                                      int len, const KState& ks, int lev, TS& ts)
+                                                      int len, const KState& ks, int lev, TS& ts,
+                                                      const HubShared* srcset = nullptr)
 {
     const int lane = lane_id();
     double acc = ks.base;
@@ -638,6 +669,10 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
             const int e = (ch + u) * 64 + lane;
             ok[u] = (ch + u) < nch && e < len;
             cc[u] = ok[u] ? P.col[start + e] : 0;
+        }
+        if (srcset) {
+#pragma unroll
+            for (int u = 0; u < WAVE_UNROLL; ++u) ok[u] = ok[u] && shash_has(*srcset, cc[u]);
         }
         uint8_t ww[WAVE_UNROLL];
 #pragma unroll
@@ -948,7 +983,6 @@ __device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32
         X.list[atomicAdd(&ts.cnt[2], 1)] = x;
 }
 
-__device__ __forceinline__ int phash_slot(int32_t v) { return (int)(((uint32_t)v * 0x9E3779B1u) >> 21) & (PHASH - 1); }
 
 __device__ __forceinline__ int phash_find(const HubShared& hs, int32_t v)
 {
@@ -1323,6 +1357,22 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     if (tp && tid == 0) tp[12] += (int64_t)wall_clock64();
 }
 
+// Inputs of an ascent level's source set (see shash_insert): the processed
+// vertices one level down, the descent vertices list[dA..dB) on the level,
+// and every vertex one level up (npc of them at pcs; npc == 1 with pcs null:
+// the root; npc < 0: unknown, no set).
+struct SrcSet {
+    LevelSets below;
+    int dA, dB;
+    const int32_t* pcs;
+    int npc;
+};
+// Light edges of a round from which the source set pays for itself.
+#ifndef SBC_SET_MIN_EDGES
+#define SBC_SET_MIN_EDGES 4096
+#endif
+constexpr int SET_MIN_EDGES = SBC_SET_MIN_EDGES;
+
 // All vertices of ascent level `here`, in rounds of one vertex per thread:
 // hubs are queued in hs.list for the hub passes; light vertices get a slot
 // (their rank in the round) holding their adjacency range and ascent state.
@@ -1334,12 +1384,13 @@ template <int TW>
 __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Item& X, const LevelSets& here, int tot,
                                                    int lev, TeamShared& ts, HubShared& hs, LightShared& ls,
                                                    int twave, unsigned long long& my_proc,
-                                                   unsigned long long& my_scan, int64_t* tp)
+                                                   unsigned long long& my_scan, int64_t* tp, const SrcSet& ss)
 {
     int64_t tq = 0;  // trace: round setup (13), edge terms and leftovers (14), batch sums (15)
     const int tid = threadIdx.x, lane = lane_id();
     constexpr int NT = TW * 64;
     static_assert(NT <= LIGHT_CAP, "one slot per thread of a round");
+    bool set_built = false;  // the level's source set, kept for the later rounds
     for (int r0 = 0; r0 < tot; r0 += NT) {
         if (tp && tid == 0) tq = (int64_t)wall_clock64();
         const int t = r0 + tid;
@@ -1379,6 +1430,22 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             if (eoff + len <= EDGE_CAP) atomicMax(&ls.nb, slot + 1);
         }
         __syncthreads();
+        const int nsrc = ss.npc < 0 ? -1 : ss.below.size() + (ss.dB - ss.dA) + ss.npc;
+        const bool use_set = set_built || (nsrc >= 0 && nsrc <= SHASH_MAX && ne_round >= SET_MIN_EDGES);
+        if (use_set && !set_built) {
+            set_built = true;
+            for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
+            __syncthreads();
+            const int nbl = ss.below.size(), ndl = ss.dB - ss.dA;
+            for (int i = tid; i < nsrc; i += NT) {
+                int32_t w;
+                if (i < nbl) w = ss.below.at(X, i);
+                else if (i < nbl + ndl) w = X.list[ss.dA + (i - nbl)];
+                else w = ss.pcs ? ss.pcs[i - nbl - ndl] : X.r;
+                shash_insert(hs, w);
+            }
+            __syncthreads();
+        }
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[13] += x - tq; tq = x; }
         const int nb = ls.nb;
         const int no = ls.novf;
@@ -1410,15 +1477,18 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
                 slot2[u] = lo;
                 c2[u] = ok2[u] ? P.col[ls.start[lo] + (e - ls.off[lo])] : 0;
             }
+            bool g2[2];  // neighbour in the level's source set (or no set)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) g2[u] = ok2[u] && (!use_set || shash_has(hs, c2[u]));
             uint8_t b2[2];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) b2[u] = ok2[u] ? ld_sb(&X.sb[c2[u]]) : (uint8_t)0;
+            for (int u = 0; u < 2; ++u) b2[u] = g2[u] ? ld_sb(&X.sb[c2[u]]) : (uint8_t)0;
             Term tm[2];
             bool appU[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int lo = slot2[u];
-                tm[u] = ascent_lane(P, X, ls.k[lo], ls.isD[lo] != 0, ls.sk_old[lo], ls.sk_new[lo], lev, ok2[u], c2[u],
+                tm[u] = ascent_lane(P, X, ls.k[lo], ls.isD[lo] != 0, ls.sk_old[lo], ls.sk_new[lo], lev, g2[u], c2[u],
                                     b2[u], appU[u]);
             }
 #pragma unroll
@@ -1438,7 +1508,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             ks.sk_new = ls.sk_new[sl];
             ks.base = ls.base[sl];
             const int l2 = ls.len[sl];
-            ascent_vertex_wave_ks(P, X, ls.k[sl], ls.start[sl], l2, ks, lev, ts);
+            ascent_vertex_wave_ks(P, X, ls.k[sl], ls.start[sl], l2, ks, lev, ts, use_set ? &hs : nullptr);
             my_proc += 1;
             my_scan += (unsigned long long)l2;
         }
@@ -1679,7 +1749,13 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         int64_t tl0 = 0;
         if (tp && leader) { tl0 = (int64_t)wall_clock64(); tp[11] += 1; }
         if constexpr (TW > 1) {
-            ascent_light_level<TW>(P, X, here, tot, lev, ts, *hs, *ls, twave, my_proc, my_scan, tp);
+            // levels 1 and 2: every vertex one level up is known (the root, its
+            // neighbours), so the level's source set can be built when the
+            // light vertices have many edges to scan
+            SrcSet ss{below, here.dA, here.dB, nullptr, -1};
+            if (lev == 1) ss.npc = 1;
+            else if (lev == 2) { ss.pcs = P.col + vr.start; ss.npc = len_r; }
+            ascent_light_level<TW>(P, X, here, tot, lev, ts, *hs, *ls, twave, my_proc, my_scan, tp, ss);
         } else {
             for (int w = 0; w < tot; ++w) {
                 const int32_t k = here.at(X, w);
