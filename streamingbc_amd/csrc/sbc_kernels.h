@@ -89,7 +89,7 @@ struct SbcParams {
     int32_t u0;
     int32_t v0;
     SbcItemRec* rec;      // [W][K] or nullptr
-    int64_t* tprof;       // [W][K][SBC_TRACE_SLOTS] or nullptr: timestamps (start, descent end, ascent end,
+    int64_t* tprof;       // [W][K][SBC_TRACE_SLOTS] or nullptr (recorded by 1024-thread teams only): timestamps (start, descent end, ascent end,
                           // commit end), hubs looked up, hubs scanned, neighbours of each; ascent time in the
                           // light-vertex passes, the hub lookups, the hub staging and scans; ascent levels,
                           // vertices over the ascent levels; light sub-phases; per ascent level 0..7 (deeper:

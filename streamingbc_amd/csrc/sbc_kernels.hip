@@ -1586,8 +1586,10 @@ __device__ __forceinline__ void load_hub_batch(const SbcParams& P, HubShared& hs
 }
 
 // Applies window item `item` to root ri.  Returns true if the item was
-// parked (its remaining levels are left to the wide kernels).
-template <int TW>
+// parked (its remaining levels are left to the wide kernels).  TRACE: record
+// per-item timestamps and counters in P.tprof (a separate instantiation, so
+// that the untraced kernel carries none of the trace state).
+template <int TW, bool TRACE = false>
 __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs,
                                              LightShared* ls, int twave)
 {
@@ -1613,7 +1615,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     if (du == INF32 && dv == INF32) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
     SbcItemRec* rec = P.rec ? &P.rec[(size_t)item * P.K + ri] : nullptr;
-    int64_t* tp = P.tprof ? &P.tprof[((size_t)item * P.K + ri) * SBC_TRACE_SLOTS] : nullptr;
+    int64_t* tp = (TRACE && P.tprof) ? &P.tprof[((size_t)item * P.K + ri) * SBC_TRACE_SLOTS] : nullptr;
     if (tp && leader) {
         const int64_t t0 = (int64_t)wall_clock64();
         tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0;
@@ -1959,7 +1961,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
 // [0, heavy) of the queue order; phase 2: each wavefront pops slots
 // [heavy, K).  A team applies every insertion of the window to its root in
 // stream order before popping the next root.
-template <int NWB>
+template <int NWB, bool TRACE = false>
 __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
 {
     __shared__ TeamShared ts[NWB];
@@ -1976,7 +1978,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         if (t >= P.heavy || t >= P.qlen) break;
         const int ri = P.order ? P.order[t] : t;
         for (int i = P.start_item ? P.start_item[ri] : 0; i < W; ++i)
-            if (process_item<NWB>(P, ri, i, ts[0], &hs, &ls, wave)) break;
+            if (process_item<NWB, TRACE>(P, ri, i, ts[0], &hs, &ls, wave)) break;
     }
     // phase 2: light, one wavefront per root
     for (;;) {
@@ -1987,7 +1989,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         if (t >= P.qlen) break;
         const int ri = P.order ? P.order[t] : t;
         for (int i = P.start_item ? P.start_item[ri] : 0; i < W; ++i)
-            process_item<1>(P, ri, i, ts[wave], nullptr, nullptr, 0);
+            process_item<1, TRACE>(P, ri, i, ts[wave], nullptr, nullptr, 0);
     }
 }
 
@@ -2293,6 +2295,11 @@ __global__ void sbc_append_kernel(SbcAppend A)
 
 extern "C" hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st)
 {
+    // per-item tracing is compiled into the 1024-thread kernel only
+    if (P->tprof && nwb == 16) {
+        hipLaunchKernelGGL((sbc_window_kernel<16, true>), dim3(grid), dim3(1024), 0, st, *P);
+        return hipGetLastError();
+    }
     switch (nwb) {
     case 1: hipLaunchKernelGGL(sbc_window_kernel<1>, dim3(grid), dim3(64), 0, st, *P); break;
     case 2: hipLaunchKernelGGL(sbc_window_kernel<2>, dim3(grid), dim3(128), 0, st, *P); break;
