@@ -1383,8 +1383,8 @@ constexpr int SET_MIN_EDGES = SBC_SET_MIN_EDGES;
 template <int TW>
 __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Item& X, const LevelSets& here, int tot,
                                                    int lev, TeamShared& ts, HubShared& hs, LightShared& ls,
-                                                   int twave, unsigned long long& my_proc,
-                                                   unsigned long long& my_scan, int64_t* tp, const SrcSet& ss)
+                                                   int twave, uint32_t& my_proc, uint32_t& my_scan, int64_t* tp,
+                                                   const SrcSet& ss)
 {
     int64_t tq = 0;  // trace: round setup (13), edge terms and leftovers (14), batch sums (15)
     const int tid = threadIdx.x, lane = lane_id();
@@ -1510,7 +1510,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             const int l2 = ls.len[sl];
             ascent_vertex_wave_ks(P, X, ls.k[sl], ls.start[sl], l2, ks, lev, ts, use_set ? &hs : nullptr);
             my_proc += 1;
-            my_scan += (unsigned long long)l2;
+            my_scan += (uint32_t)l2;
         }
         // hubs no queue could take
         for (int i = twave; i < no; i += TW) {
@@ -1519,11 +1519,11 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             const int l2 = visible_len(P, v2, X.q);
             ascent_vertex_wave(P, X, kk, v2, l2, lev, ts);
             my_proc += 1;
-            my_scan += (unsigned long long)l2;
+            my_scan += (uint32_t)l2;
         }
         if (tid == 0) {
-            my_proc += (unsigned long long)nb;
-            my_scan += (unsigned long long)ne;
+            my_proc += (uint32_t)nb;
+            my_scan += (uint32_t)ne;
         }
         __syncthreads();
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[14] += x - tq; tq = x; }
@@ -1649,7 +1649,8 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     }
     team_sync<TW>();
 
-    unsigned long long my_proc = 0, my_scan = 0, my_moved = 0;
+    // per-thread work counts of this item (an item scans fewer than 2^32 entries)
+    uint32_t my_proc = 0, my_scan = 0, my_moved = 0;
 
     // ---------------- descent
     int fs = 0, fe = 1, j = 0;
@@ -1665,7 +1666,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             const int64_t dc = ld_acq64(&X.vr[c].dpath);
             descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, ts);
             my_proc += 1;
-            my_scan += (unsigned long long)len;
+            my_scan += (uint32_t)len;
             my_moved += cmoved ? 1 : 0;
         }
         if constexpr (TW > 1) {
@@ -1765,7 +1766,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 const int len = visible_len(P, vi, X.q);
                 ascent_vertex_wave(P, X, k, vi, len, lev, ts);
                 my_proc += 1;
-                my_scan += (unsigned long long)len;
+                my_scan += (uint32_t)len;
             }
         }
         if constexpr (TW > 1) {
