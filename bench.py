@@ -250,6 +250,11 @@ def main():
             t = json.load(fh)
         roofline["traffic"] = t["hbm_bytes_per_launch"]
         roofline["traffic_source"] = f"profiles/traffic_{args.workload}.json ({t['windows']})"
+        # the bytes the kernel actually moves, per second of launch: far below
+        # `achieved`, whose formula counts every adjacency entry of every
+        # processed vertex (most of which lookups and source sets never read)
+        if launch_ms > 0:
+            roofline["measured_GBs"] = round(t["hbm_bytes_per_launch"] / (launch_ms * 1e-3) / 1e9, 2)
 
     out = {
         "metric": METRIC,
