@@ -375,7 +375,7 @@ extern "C" int64_t streamingBCResidentFootprint(const int64_t* rowptr, int64_t N
     }
     int64_t graph = slots * 8 + NV * (int64_t)sizeof(SbcVInfo) + NV * 16 + htab * 8;
     // windowed-update scratch: hub term buckets for up to 4 workgroups per CU
-    int64_t scratch = htab ? 4LL * num_cu * kBucketHubs * SBC_BUCKET * (int64_t)sizeof(SbcBucketEnt) : 0;
+    int64_t scratch = htab ? 4LL * num_cu * SBC_BUCKET_SLOTS * (int64_t)sizeof(SbcBucketEnt) : 0;
     return streamingBCForestFootprint(NV, NK) + graph + scratch;
 }
 
@@ -703,7 +703,7 @@ static sbcStatus ensure_bucket(extraArraysPerThread* C, const stinger* G, int64_
     HIPCHK(hipStreamSynchronize(C->stream));
     if (C->d_bucket) HIPCHK(hipFree(C->d_bucket));
     C->d_bucket = nullptr;
-    sbcStatus st = dalloc(&C->d_bucket, (size_t)grid * kBucketHubs * SBC_BUCKET);
+    sbcStatus st = dalloc(&C->d_bucket, (size_t)grid * SBC_BUCKET_SLOTS);
     if (st) return st;
     C->bucket_blocks = grid;
     C->gc.with_rec = -1; // cached graph holds the old pointer

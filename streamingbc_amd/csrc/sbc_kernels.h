@@ -39,6 +39,9 @@ enum { SBC_KERR_LEVELS = 1 };
 
 // Per-hub term bucket entry of the hub lookup pass (per-workgroup scratch).
 #define SBC_BUCKET 64
+// term slots of one workgroup's lookup buckets (shared out between the hubs
+// of a pass; also the flat hub scan's run space): 256 per hub at 512 hubs
+#define SBC_BUCKET_SLOTS (512 * 256)
 struct SbcBucketEnt {
     int32_t key;   // position in the hub's adjacency list
     int32_t flag;  // 1: add term, 2: sub term
@@ -68,7 +71,7 @@ struct SbcParams {
     const int32_t* seq;
     const int4* hub;      // per vertex {offset into htab, indexed count, log2 table size, 0} (count 0: none)
     const int2* htab;     // open-addressing tables {neighbour + 1, position} of indexed vertices
-    SbcBucketEnt* bucket; // per workgroup: HUB_CAP x SBC_BUCKET term slots (nullptr: no lookups)
+    SbcBucketEnt* bucket; // per workgroup: SBC_BUCKET_SLOTS term slots (nullptr: no lookups)
     int32_t nv;
     int32_t K;
     int32_t lcap;

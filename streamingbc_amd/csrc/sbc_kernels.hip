@@ -349,11 +349,14 @@ constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
 constexpr int PHASH = 2 * HUB_CAP;      // LDS hash: vertex -> number in the level's lookup list
 constexpr int TWOHOP_CAP = 1024;        // two-hop set of the root kept for level-3 lookups
-// Lookup buckets: a pass's HUB_CAP * SBC_BUCKET term slots are shared out
-// evenly between its lookup hubs, at most TEAM_SORT_CAP per hub; a bucket of
-// more than SBC_BUCKET terms is sorted by the whole team.
+// Lookup buckets: a pass's SBC_BUCKET_SLOTS term slots are shared out evenly
+// between its lookup hubs, at most TEAM_SORT_CAP per hub; a bucket of up to
+// WAVE_SORT_CAP terms is sorted by one wavefront, a larger one by the whole
+// team.
 constexpr int TEAM_SORT_CAP = 1024;
-static_assert(TEAM_SORT_CAP % 64 == 0 && TEAM_SORT_CAP <= HUB_CAP * SBC_BUCKET, "bucket shares");
+constexpr int WAVE_SORT_CAP = 256;
+static_assert(TEAM_SORT_CAP % 64 == 0 && TEAM_SORT_CAP <= SBC_BUCKET_SLOTS, "bucket shares");
+static_assert(SBC_BUCKET_SLOTS / HUB_CAP >= WAVE_SORT_CAP, "every pass can hold wave-sorted buckets");
 
 struct alignas(16) HubShared {
     int n;
@@ -832,10 +835,10 @@ __device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item&
                                                  HubShared& hs, LightShared& ls, int twave)
 {
     constexpr int NT = TW * 64;
-    constexpr int share = (HUB_CAP * SBC_BUCKET) / TW;  // run stride
+    constexpr int share = SBC_BUCKET_SLOTS / TW;  // run stride
     const int cap = (P.run_cap > 0 && P.run_cap < share) ? P.run_cap : share;
     const int tid = threadIdx.x, lane = lane_id();
-    SbcBucketEnt* runs = P.bucket + (size_t)blockIdx.x * HUB_CAP * SBC_BUCKET;
+    SbcBucketEnt* runs = P.bucket + (size_t)blockIdx.x * SBC_BUCKET_SLOTS;
     const int per = (nsh + NT - 1) / NT;
     const int h0 = min(tid * per, nsh), h1 = min(h0 + per, nsh);
     int mine = 0;
@@ -1064,6 +1067,45 @@ __device__ __forceinline__ double shfl_d(double x, int src)
     return __builtin_bit_cast(double, v);
 }
 
+// Ascending sort of up to 256 (key, idx) pairs held by a wavefront as
+// element e = j * 64 + lane in key[j], idx[j]: a bitonic network whose
+// strides below 64 exchange between lanes and whose strides of 64 and 128
+// exchange between a lane's registers.  Unused elements carry INF32 keys.
+__device__ __forceinline__ void wave_sort256(int (&key)[4], int (&idx)[4])
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int k2 = 2; k2 <= 256; k2 <<= 1) {
+#pragma unroll
+        for (int d = k2 >> 1; d > 0; d >>= 1) {
+            if (d >= 64) {
+                const int dj = d >> 6;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int pj = j ^ dj;
+                    if (pj > j) {
+                        const bool up = ((j * 64 + lane) & k2) == 0;
+                        if (up ? key[j] > key[pj] : key[j] < key[pj]) {
+                            const int tk = key[j], ti = idx[j];
+                            key[j] = key[pj]; idx[j] = idx[pj];
+                            key[pj] = tk; idx[pj] = ti;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int pk = __shfl_xor(key[j], d), pi = __shfl_xor(idx[j], d);
+                    const bool up = ((j * 64 + lane) & k2) == 0;
+                    const bool lower = (lane & d) == 0;
+                    const bool take = lower ? ((pk < key[j]) == up) : ((pk > key[j]) == up);
+                    if (take) { key[j] = pk; idx[j] = pi; }
+                }
+            }
+        }
+    }
+}
+
 // base + the bucket's terms in position order: a wavefront bitonic sort of
 // (key, lane), then each lane takes the terms of the lane it sorted to.
 __device__ __forceinline__ double bucket_sum(const BucketLane& e, double base)
@@ -1105,11 +1147,11 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
 {
     const int tid = threadIdx.x, lane = lane_id(), twave = tid >> 6;
     constexpr int NT = TW * 64;
-    SbcBucketEnt* bk = P.bucket + (size_t)blockIdx.x * HUB_CAP * SBC_BUCKET;
+    SbcBucketEnt* bk = P.bucket + (size_t)blockIdx.x * SBC_BUCKET_SLOTS;
     for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
     for (int i = tid; i < np; i += NT) hs.bcnt[i] = 0;
     if (tid == 0) {
-        const int share = ((HUB_CAP * SBC_BUCKET) / np) & ~63;
+        const int share = (SBC_BUCKET_SLOTS / np) & ~63;
         hs.bstride = share < TEAM_SORT_CAP ? share : TEAM_SORT_CAP;
         hs.n3 = 0;
     }
@@ -1256,8 +1298,11 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             }
             if (small && gl == 0) st_reld(&X.vr[hs.plist[hi]].ndel, acc);
             if (hi < np && gl == 0) {
+                // kinds: 0 (<= 8 terms, done here), 1 (<= 64, a wavefront), 4 (<= 256,
+                // a wavefront's 256-key sort), 3 (more, the team), 2 (overflowed: scanned)
                 const uint8_t kd = n > hs.bstride ? 2
-                                 : n > SBC_BUCKET ? (hs.plen[hi] <= TEAM_SORT_BITS ? 3 : 2)
+                                 : n > WAVE_SORT_CAP ? (hs.plen[hi] <= TEAM_SORT_BITS ? 3 : 2)
+                                 : n > SBC_BUCKET ? 4
                                  : (n > 8 ? 1 : 0);
                 hs.kind[hi] = kd;
                 if (kd == 3) atomicAdd(&hs.n3, 1);
@@ -1265,10 +1310,10 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         }
     }
     __syncthreads();
-    // one wavefront per larger bucket (kind 1), two hubs at a time
+    // one wavefront per larger bucket (kinds 1 and 4)
     for (int h0 = 0; h0 < np; h0 += 64) {
         const int hl = h0 + lane;
-        uint64_t bm = ballot(hl < np && hs.kind[hl] == 1);
+        uint64_t bm = ballot(hl < np && (hs.kind[hl] == 1 || hs.kind[hl] == 4));
         // the wavefronts share out the marked hubs of this group of 64
         int r = 0;
         while (bm) {
@@ -1277,8 +1322,33 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             if ((r++ % TW) != twave) continue;
             const int hi = h0 + l0;
             const int n = hs.bcnt[hi];
-            const BucketLane e = bucket_lane(bk + (size_t)hi * hs.bstride, n);
-            const double acc = bucket_sum(e, X.vr[hs.plist[hi]].delta);
+            const SbcBucketEnt* b = bk + (size_t)hi * hs.bstride;
+            double acc;
+            if (hs.kind[hi] == 1) {
+                acc = bucket_sum(bucket_lane(b, n), X.vr[hs.plist[hi]].delta);
+            } else {
+                int key[4], idx[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int e = j * 64 + lane;
+                    key[j] = e < n ? b[e].key : INF32;
+                    idx[j] = e;
+                }
+                wave_sort256(key, idx);
+                acc = X.vr[hs.plist[hi]].delta;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    int fl = 0;
+                    double ad = 0.0, sb2 = 0.0;
+                    if (key[j] != INF32) {
+                        const SbcBucketEnt& e = b[idx[j]];
+                        fl = e.flag;
+                        ad = e.add;
+                        sb2 = e.sub;
+                    }
+                    acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
+                }
+            }
             if (lane == 0) st_reld(&X.vr[hs.plist[hi]].ndel, acc);
         }
     }

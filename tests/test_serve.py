@@ -22,7 +22,8 @@ def test_resident_footprint_counts_forest_graph_and_scratch():
     hub = sbc.gen_chung_lu(1 << 14, 2.0, 16.0, seed=3)
     assert hub.degree().max() > 512
     fh = sbc.resident_footprint(hub, k, 256)
-    assert fh - sbc.resident_footprint(hub, k, 128) == 128 * 4 * 512 * 64 * 24
+    # lookup-bucket scratch: 4 workgroups per CU, 512 * 256 term slots of 24 bytes each
+    assert fh - sbc.resident_footprint(hub, k, 128) == 128 * 4 * 512 * 256 * 24
     assert sbc.resident_footprint(hg, 0, 256) == -1
 
 
