@@ -505,6 +505,7 @@ struct GraphCache {
     uint64_t generation = 0;
     int nwb = 0, grid = 0, heavy = 0, recompute = 0, W = 0, has_order = 0, with_rec = 0, park_min = 0;
     int run_cap = 0, hub_batch = 0;  // test hooks baked into the captured arguments
+    int64_t park_scan = 0;
 };
 
 struct extraArraysPerThread {
@@ -751,6 +752,7 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     P.park = nullptr;
     P.parked = nullptr;
     P.park_min = 0;
+    P.park_scan = 0;
     P.start_item = nullptr;
     P.qlen = (int32_t)F->K;
     // test hook: a small run capacity forces the flat hub scan's overflow path
@@ -1023,6 +1025,10 @@ static int wide_grid(const extraArraysPerThread* C, int64_t n)
 // ascent levels and the commit, one whole-GPU launch per level; the level
 // offsets and list counters are read back between launches to drive the
 // same loop as a team's.
+static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri,
+                                    const SbcWideItem& w, const std::vector<int32_t>& lv, int nf, int nD, int nS,
+                                    int lev0, int uA, int uB, bool light_counted);
+
 static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri)
 {
     SbcWideItem w;
@@ -1035,8 +1041,12 @@ static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, co
     HIPCHK(hipMemcpyAsync(&w, dw, sizeof(w), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipMemcpyAsync(lv.data(), dlvl, sizeof(int32_t) * lv.size(), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    if (w.j < 1 || w.j >= lcap || w.fs < 0 || w.fs > w.fe || w.fe > F->nv || w.item < 0)
+    if (w.j < 1 || w.j >= lcap || w.fs < 0 || w.fs > w.fe || w.fe > F->nv || w.item < 0 || w.phase < 0 ||
+        w.phase > 1 || (w.phase == 1 && (w.uA < 0 || w.uA > w.uB || w.uB > F->nv || w.lev < 0 || w.lev >= lcap)))
         return set_err(SBC_ERR_STATE, "parked item of root %d is inconsistent", (int)ri);
+    // parked in an ascent level: the descent is complete, the team counted
+    // the level's light vertices
+    if (w.phase == 1) return wide_ascent_commit(F, C, P, ri, w, lv, w.j, w.cnt[0], w.cnt[1], w.lev, w.uA, w.uB, true);
     int j = w.j, fs = w.fs, fe = w.fe;
     int32_t c01[2] = {w.cnt[0], w.cnt[1]};
     while (fs < fe) {
@@ -1057,14 +1067,30 @@ static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, co
     HIPCHK(hipMemcpyAsync(dlvl, lv.data(), sizeof(int32_t) * lv.size(), hipMemcpyHostToDevice, C->stream));
     HIPCHK(hipMemcpyAsync(&dw->cnt[2], &u, sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));  // lv and u are host stack data
-    int uA = nD, uB = nD;
-    for (int lev = w.base + nf - 1; lev >= 0; --lev) {
+    return wide_ascent_commit(F, C, P, ri, w, lv, nf, nD, nS, w.base + nf - 1, nD, nD, false);
+}
+
+// Ascent levels lev0 .. 0 of a parked item (level lev0's ascent set is
+// list[uA, uB); the item's counter cnt[2] holds the end of the ascent list),
+// then its commit, with the wide kernels.  light_counted: the team counted
+// level lev0's light vertices.
+static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri,
+                                    const SbcWideItem& w, const std::vector<int32_t>& lv, int nf, int nD, int nS,
+                                    int lev0, int uA, int uB, bool light_counted)
+{
+    SbcWideItem* dw = C->d_park + ri;
+    const int lcap = F->lcap;
+    const int32_t* lvlD = lv.data();
+    const int32_t* lvlS = lvlD + lcap;
+    int32_t u = 0;
+    for (int lev = lev0; lev >= 0; --lev) {
         const int jj = lev - w.base;
         int32_t h[6] = {0, 0, 0, 0, uA, uB};
         if (jj >= 0) { h[0] = lvlD[jj]; h[1] = lvlD[jj + 1]; h[2] = lvlS[jj]; h[3] = lvlS[jj + 1]; }
         const int64_t tot = (int64_t)(h[1] - h[0]) + (h[3] - h[2]) + (h[5] - h[4]);
         if (tot == 0) break;
-        HIPCHK(sbc_launch_wide_ascent(&P, dw, ri, h, lev, wide_grid(C, tot), C->stream));
+        HIPCHK(sbc_launch_wide_ascent(&P, dw, ri, h, lev, light_counted && lev == lev0 ? 1 : 0, wide_grid(C, tot),
+                                      C->stream));
         HIPCHK(hipMemcpyAsync(&u, &dw->cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
         if (u < uB || (int64_t)u + nS > F->nv)
@@ -1239,6 +1265,14 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         P.park = C->d_park;
         P.parked = C->d_parked;
         P.park_min = park_min;
+        // an ascent level whose hubs leave this many adjacency entries to
+        // scan is handed over too: one workgroup scans 50-150 million entries
+        // a second, the wide kernels put every hub on a wavefront of its own
+        // across the GPU; the rest of the root's window then runs in a launch
+        // of its own, so only levels worth tens of milliseconds qualify
+        P.park_scan = (int64_t)park_min * 512;
+        const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
+        if (ps) P.park_scan = atoll(ps);
     }
     SbcAppend A;
     memset(&A, 0, sizeof(A));
@@ -1258,7 +1292,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         (!gc.exec || gc.forest != F || gc.g != G || gc.generation != G->generation || gc.nwb != nwb ||
          gc.grid != grid || gc.heavy != heavy || gc.recompute != cfg->recompute_bc || gc.W != W ||
          gc.has_order != has_order || gc.with_rec != 1 || gc.park_min != P.park_min ||
-         gc.run_cap != P.run_cap || gc.hub_batch != P.hub_batch);
+         gc.run_cap != P.run_cap || gc.hub_batch != P.hub_batch || gc.park_scan != P.park_scan);
     if (need_capture) {
         if (gc.exec) { (void)hipGraphExecDestroy(gc.exec); gc.exec = nullptr; }
         if (gc.graph) { (void)hipGraphDestroy(gc.graph); gc.graph = nullptr; }
@@ -1276,6 +1310,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         gc.park_min = P.park_min;
         gc.run_cap = P.run_cap;
         gc.hub_batch = P.hub_batch;
+        gc.park_scan = P.park_scan;
     }
 
     std::vector<int32_t> prev_touched;

@@ -106,12 +106,15 @@ struct SbcParams {
     int32_t* hubq;        // per workgroup: SBC_HUBQ_CAP queued hubs beyond the LDS queue, or nullptr
     int32_t hub_batch;    // hubs per pass (<= the LDS queue; smaller only to test the batching)
     // parked items: a whole-workgroup team whose next descent frontier has at
-    // least park_min vertices stops, saves the item in park[ri] and appends
-    // ri to parked (count in qctr[2]); the host finishes it with the wide
-    // kernels and resumes the root at start_item[ri]
+    // least park_min vertices, or whose ascent level has hubs with at least
+    // park_scan adjacency entries that lookups cannot serve, stops, saves the
+    // item in park[ri] and appends ri to parked (count in qctr[2]); the host
+    // finishes it with the wide kernels and resumes the root at
+    // start_item[ri]
     struct SbcWideItem* park;   // [K] or nullptr (never park)
     int32_t* parked;            // [K]
     int32_t park_min;
+    int64_t park_scan;
     const int32_t* start_item;  // [K] first item per root, or nullptr (0)
     int32_t qlen;               // queue slots in use (K, or the roots resumed)
 };
@@ -124,7 +127,12 @@ struct SbcWideItem {
     unsigned long long stat[4];  // processed, scanned, moved, unused
     int32_t item, s, p, q;       // window item, deeper and shallower endpoint, visibility sequence
     int32_t base, j, fs, fe;     // new level of s, frontiers done, next frontier list[fs, fe)
-    int32_t gap, smoved, pad[2];
+    int32_t gap, smoved;
+    // phase 0: parked during the descent (j, fs, fe as above).  Phase 1:
+    // parked in ascent level lev (its ascent set list[uA, uB)) after the
+    // team had processed the level's light vertices; the descent is
+    // complete (j frontiers) and the counts hold those light vertices.
+    int32_t phase, lev, uA, uB;
 };
 
 #define SBC_APPEND_INLINE 8
@@ -172,7 +180,7 @@ hipError_t sbc_cost_schedule(sbcCostModel* m, const float* dX, int n, int K, flo
                              float* d_pred, float* d_cost, int32_t* d_order, int32_t* d_heavy, hipStream_t st);
 hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc, int grid,
                                    hipStream_t st);
-hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* here6, int lev, int grid,
+hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* here6, int lev, int hubs_only, int grid,
                                   hipStream_t st);
 hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, int nf, int nD, int nUend, int nS,
                                   int grid, hipStream_t st);

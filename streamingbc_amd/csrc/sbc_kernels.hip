@@ -364,6 +364,8 @@ struct alignas(16) HubShared {
     int pn;
     int np;
     int ns;               // hubs staged for the team scan
+    long long svol;       // adjacency entries of the pass's hubs that lookups cannot serve
+    int handover;         // the item goes to the wide kernels at the end of this level's hub passes
     int bstride;          // term slots per lookup hub in this pass's buckets
     int n3;               // lookup hubs whose buckets the whole team sorts
     int dup;              // a team-sorted bucket holds a position twice
@@ -1655,6 +1657,49 @@ __device__ __forceinline__ void load_hub_batch(const SbcParams& P, HubShared& hs
     __syncthreads();
 }
 
+// Hands the item over to the wide kernels in ascent level lev, after the
+// team processed the level's light vertices and looked up its first pass of
+// hubs (the light vertices' counts stay in the team's; the wide kernels redo
+// the level, counting only its hubs).
+template <int TW>
+__device__ __forceinline__ void park_in_ascent(const SbcParams& P, int ri, int item, int32_t xs, int32_t xp, int32_t xq,
+                                            TeamShared& ts,
+                                            int32_t base, int nf, int64_t gap, bool smoved, int lev, int uA, int uB,
+                                            uint32_t my_proc, uint32_t my_scan, uint32_t my_moved, bool leader,
+                                            int lane)
+{
+    if (lane == 0) {
+        atomicAdd(&ts.stat[0], (unsigned long long)my_proc);
+        atomicAdd(&ts.stat[1], (unsigned long long)my_scan);
+        atomicAdd(&ts.stat[2], (unsigned long long)my_moved);
+    }
+    team_sync<TW>();
+    if (leader) {
+        SbcWideItem& w = P.park[ri];
+        w.cnt[0] = ts.cnt[0];
+        w.cnt[1] = ts.cnt[1];
+        w.cnt[2] = ts.cnt[2];
+        w.cnt[3] = 0;
+        for (int i = 0; i < 4; ++i) w.stat[i] = ts.stat[i];
+        w.item = item;
+        w.s = xs;
+        w.p = xp;
+        w.q = xq;
+        w.base = base;
+        w.j = nf;
+        w.fs = 0;
+        w.fe = 0;
+        w.gap = (int32_t)(gap > INF32 ? INF32 : (gap < -INF32 ? -INF32 : gap));
+        w.smoved = smoved ? 1 : 0;
+        w.phase = 1;
+        w.lev = lev;
+        w.uA = uA;
+        w.uB = uB;
+        P.parked[atomicAdd(&P.qctr[2], 1)] = ri;
+    }
+    team_sync<TW>();
+}
+
 // Applies window item `item` to root ri.  Returns true if the item was
 // parked (its remaining levels are left to the wide kernels).  TRACE: record
 // per-item timestamps and counters in P.tprof (a separate instantiation, so
@@ -1714,7 +1759,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         ts.cnt[1] = 0;
         ts.cnt[2] = 0;
         ts.stat[0] = 0; ts.stat[1] = 0; ts.stat[2] = 0;
-        if (hs) { hs->n = 0; hs->overflow = 0; }
+        if (hs) { hs->n = 0; hs->overflow = 0; hs->svol = 0; hs->handover = 0; }
         if (ls) { ls->nb = 0; ls->novf = 0; }
     }
     team_sync<TW>();
@@ -1787,6 +1832,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     w.fe = fe;
                     w.gap = (int32_t)(gap > INF32 ? INF32 : (gap < -INF32 ? -INF32 : gap));
                     w.smoved = smoved ? 1 : 0;
+                    w.phase = 0;
                     P.parked[atomicAdd(&P.qctr[2], 1)] = ri;
                 }
                 team_sync<TW>();
@@ -1887,12 +1933,12 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     const int len = visible_len_thread(P, vi, X.q);
                     hs->hlen[h] = len;
                     hs->hstart[h] = vi.start;
-                    atomicAdd(&ts.stat[0], 1ull);
-                    atomicAdd(&ts.stat[1], (unsigned long long)len);
                     bool pk = false;
                     if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && hb.y > 0 &&
                         sb_type(kb) == 3)
                         pk = cand < (long)hb.y;
+                    atomicAdd(&ts.stat[0], 1ull);
+                    atomicAdd(&ts.stat[1], (unsigned long long)len);
                     int hi = -1;
                     if (pk) {
                         hi = atomicAdd(&hs->np, 1);
@@ -1930,6 +1976,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     }
                     if (!looked_up) {
                         const int j = atomicAdd(&hs->ns, 1);
+                        if (P.park) atomicAdd((unsigned long long*)&hs->svol, (unsigned long long)len);
                         const KState kst = kstate(X, k);
                         ls->k[j] = k;
                         ls->start[j] = hs->hstart[h];
@@ -1941,6 +1988,17 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     }
                 }
                 __syncthreads();
+                if (hb == 0 && P.park && hs->svol >= P.park_scan && hs->n <= hub_batch(P) + (P.hubq ? SBC_HUBQ_CAP : 0)) {
+                    // the pass's hubs left to scan are too much for one workgroup:
+                    // take back the pass's hub counts (the wide kernels count the
+                    // level's hubs) and hand the item over
+                    for (int h = threadIdx.x; h < nh; h += TW * 64) {
+                        atomicAdd(&ts.stat[0], ~0ull);
+                        atomicAdd(&ts.stat[1], (unsigned long long)(-(long long)hs->hlen[h]));
+                    }
+                    if (leader) hs->handover = 1;
+                    break;
+                }
                 const int nsh = hs->ns;
                 if (P.bucket) {
                     if (nsh > 0) ascent_hubs_flat<TW>(P, X, nsh, lev, ts, *hs, *ls, twave);
@@ -1959,7 +2017,13 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     tp[48 + (lev < 7 ? lev : 7)] += nsh;
                 }
             }
-            if (leader) hs->n = 0;
+            if (leader) { hs->n = 0; hs->svol = 0; }
+            __syncthreads();
+            if (hs->handover) {
+                park_in_ascent<TW>(P, ri, item, X.s, X.p, X.q, ts, base, nf, gap, smoved, lev, uA, uB, my_proc,
+                                   my_scan, my_moved, leader, lane);
+                return true;
+            }
         }
         team_sync<TW>();
         if (tp && leader) {
@@ -2121,7 +2185,8 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_kernel(SbcParams P, 
 // same-level set, ascent set), one wavefront per vertex.
 template <int NW>
 __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_kernel(SbcParams P, SbcWideItem* wi, int ri, int dA, int dB,
-                                                                  int sA, int sB, int uA, int uB, int lev)
+                                                                  int sA, int sB, int uA, int uB, int lev,
+                                                                  int hubs_only)
 {
     SbcWideItem& w = *wi;
     const Item X = parked_item(P, w, ri);
@@ -2134,8 +2199,11 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_kernel(SbcParams P, S
         const SbcVInfo vi = P.vinfo[k];
         const int len = visible_len(P, vi, X.q);
         ascent_vertex_wave(P, X, k, vi, len, lev, w);
-        my_proc += 1;
-        my_scan += (unsigned long long)len;
+        // hubs_only: the level's light vertices were counted by the team
+        if (!hubs_only || len > HUB_MIN) {
+            my_proc += 1;
+            my_scan += (unsigned long long)len;
+        }
     }
     if (lane_id() == 0 && my_proc > 0) {
         atomicAdd(&w.stat[0], my_proc);
@@ -2392,10 +2460,10 @@ extern "C" hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* w
 }
 
 extern "C" hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* h, int lev,
-                                             int grid, hipStream_t st)
+                                             int hubs_only, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(sbc_wide_ascent_kernel<WIDE_NW>, dim3(grid), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, h[0], h[1],
-                       h[2], h[3], h[4], h[5], lev);
+                       h[2], h[3], h[4], h[5], lev, hubs_only);
     return hipGetLastError();
 }
 

@@ -347,6 +347,36 @@ def test_parked_items_match_oracle(recompute):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("window", [1, 16])
+def test_items_handed_over_in_the_ascent_match_oracle(window, monkeypatch):
+    """An ascent level whose hubs leave more entries to scan than the
+    threshold than lookups can serve is handed over to the whole-GPU level
+    kernels after the team's light-vertex pass (they redo the level, counting
+    only its hubs).  With the frontier threshold out of reach and a tiny scan
+    threshold every such item takes that path; trees, records and BC stay
+    exactly the oracle's."""
+    monkeypatch.setenv("SBC_PARK_SCAN", "600")
+    hg = sbc.gen_rmat(12, 16 << 12, seed=9, keep_lcc=True)
+    roots = hg.pick_roots(48, 4)
+    u, v = hg.insertions(64, 21)
+    print(f"[ascent handover] rmat12 nv={hg.nv} k={len(roots)} n={len(u)} window={window}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    cfg = sbc.BatchConfig(window=window, block_threads=1024, park_min=1 << 20, recompute_bc=True)
+    rec = f.insert_batched(g, ctx, u, v, cfg, records=True)
+    parked = ctx.last_parked()
+    print(f"[ascent handover] items handed over in the last window: {parked}", flush=True)
+    for i in range(len(u)):
+        rec_o = o.insert(int(u[i]), int(v[i]), O_FINE, items=True)
+        for key in ("kase", "affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec[i][key], rec_o[key]), f"insertion {i} {key}"
+    _compare_trees(f, o, "ascent handover")
+    f.recompute_bc(ctx)
+    assert_bc_close(f.bc(), o.bc())
+
+
 def test_hub_lookup_level3_leaf_roots():
     """Roots that are leaves hanging off a low-degree vertex see the hubs on
     level 3; their ascent-only hubs there are resolved by table lookups with
