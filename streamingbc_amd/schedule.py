@@ -102,9 +102,10 @@ class LearnedSchedule:
         if self.trained and self.device and self.mfma and forest.k <= 4096:
             self._sched_windows = getattr(self, "_sched_windows", 0) + 1
             want = (self._sched_windows - 1) % self.eval_every == 0
-            if self.light_max <= 0 and forest.k <= 2 * ctx.num_cu:
-                # every root gets a team of its own at once: the order cannot
-                # change the window time, so only report windows are predicted
+            if self.light_max <= 0 and forest.k <= ctx.num_cu:
+                # every root gets a team of its own at once (a 1024-thread team
+                # takes a whole CU's LDS): the order cannot change the window
+                # time, so only report windows are predicted
                 if not want:
                     return WindowPlan(None, -1, None), None
                 return WindowPlan(None, -1, self._predict_only(forest, g, ctx, u, v)), None
