@@ -89,6 +89,7 @@ struct stinger {
 // update kernel looks neighbours up in it instead of scanning hub lists).
 static const int32_t kHubIndexMin = 512;
 static const int64_t kBucketHubs = 512; // HUB_CAP of the kernels
+static const int64_t kParkScan = 512 * 1024; // adjacency entries left to scan that hand an ascent level over
 
 static int32_t slack_for(int32_t d) { return 4 + d / 16; }
 
@@ -1271,7 +1272,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         // across the GPU; the rest of the root's window then runs in a launch
         // of its own (headline workload: 2M entries 10.65K insertions/s, 1M
         // 10.89K, 512K 11.49K, 256K 11.26K, 128K 10.59K)
-        P.park_scan = (int64_t)park_min * 128;
+        P.park_scan = kParkScan;
         const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
         if (ps) P.park_scan = atoll(ps);
     }
