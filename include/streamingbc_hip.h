@@ -193,9 +193,10 @@ typedef struct {
     int32_t recompute_bc;   /* 1: BC = sum over roots after each window (deterministic) */
     int32_t heavy_roots;    /* roots run in whole-workgroup mode (-1: all) */
     const int32_t *root_order; /* NULL: root-array order */
-    int32_t park_min;       /* an item whose descent frontier reaches this many vertices, or one of whose
-                               ascent levels leaves more than 512K adjacency entries of hubs to scan, is
-                               finished by kernels spanning the whole GPU (0: 4096, < 0: never) */
+    int32_t park_min;       /* an item whose descent frontier reaches this many vertices, whose descent
+                               scans 512K adjacency entries, or one of whose ascent levels leaves more
+                               than 512K entries of hubs to scan, is finished by kernels spanning the
+                               whole GPU (0: 4096, < 0: never) */
     int32_t use_context_order; /* 1: root order left in the context by streamingBCScheduleWindow
                                   (sbc_costmodel.h); root_order must be NULL */
 } sbcBatchConfig;

@@ -106,8 +106,9 @@ struct SbcParams {
     int32_t* hubq;        // per workgroup: SBC_HUBQ_CAP queued hubs beyond the LDS queue, or nullptr
     int32_t hub_batch;    // hubs per pass (<= the LDS queue; smaller only to test the batching)
     // parked items: a whole-workgroup team whose next descent frontier has at
-    // least park_min vertices, or whose ascent level has hubs with at least
-    // park_scan adjacency entries that lookups cannot serve, stops, saves the
+    // least park_min vertices, whose descent has scanned park_scan adjacency
+    // entries, or whose ascent level has hubs with at least park_scan entries
+    // that lookups cannot serve, stops, saves the
     // item in park[ri] and appends ri to parked (count in qctr[2]); the host
     // finishes it with the wide kernels and resumes the root at
     // start_item[ri]

@@ -322,7 +322,7 @@ __device__ __forceinline__ double dep(int64_t sa, int64_t sb, double d)
 // Per-team LDS block.
 struct TeamShared {
     int cnt[4];                 // 0: |D|, 1: |S|, 2: end of U list, 3: pop result
-    unsigned long long stat[4]; // processed, scanned, moved, (unused)
+    unsigned long long stat[4]; // processed, scanned, moved, adjacency entries scanned by the descent
 };
 
 // ------------------------------------------------------------ update item
@@ -563,6 +563,7 @@ __device__ __forceinline__ void descent_hubs_flat(const SbcParams& P, const Item
         mine += (len + 63) >> 6;
         atomicAdd(&ts.stat[0], 1ull);
         atomicAdd(&ts.stat[1], (unsigned long long)len);
+        atomicAdd(&ts.stat[3], (unsigned long long)len);
         if (cm) atomicAdd(&ts.stat[2], 1ull);
     }
     int total;
@@ -1758,7 +1759,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         ts.cnt[0] = 1;
         ts.cnt[1] = 0;
         ts.cnt[2] = 0;
-        ts.stat[0] = 0; ts.stat[1] = 0; ts.stat[2] = 0;
+        ts.stat[0] = 0; ts.stat[1] = 0; ts.stat[2] = 0; ts.stat[3] = 0;
         if (hs) { hs->n = 0; hs->overflow = 0; hs->svol = 0; hs->handover = 0; }
         if (ls) { ls->nb = 0; ls->novf = 0; }
     }
@@ -1783,6 +1784,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             my_proc += 1;
             my_scan += (uint32_t)len;
             my_moved += cmoved ? 1 : 0;
+            if (TW > 1 && P.park && lane == 0) atomicAdd(&ts.stat[3], (unsigned long long)len);
         }
         if constexpr (TW > 1) {
             __syncthreads();
@@ -1807,7 +1809,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         fs = fe;
         fe = nfe;
         if constexpr (TW > 1) {
-            if (P.park && fe - fs >= P.park_min) {
+            if (P.park && (fe - fs >= P.park_min || (long long)ts.stat[3] >= P.park_scan)) {
                 // too wide for one workgroup: save the item, stop this root
                 if (lane == 0) {
                     atomicAdd(&ts.stat[0], my_proc);
