@@ -198,20 +198,23 @@ def main():
     parked[0] = 0
     ctx.reset_stats(timing=True)
     lat = []
-    recs = []
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        recs.append(window(pos, pos + W, records=True))
+        window(pos, pos + W)
         lat.append(time.perf_counter() - ts)
         pos += W
     ctx.synchronize()
     t_batched = time.perf_counter() - t0
     n_launch, k_ms = ctx.kernel_stats(sbc.KSTAT_UPDATE)
-    rec = np.concatenate([r.reshape(-1) for r in recs])
-    alg_bytes = item_bytes(rec)
-    per_root = np.bincount(rec["root_index"], weights=rec["scanned"], minlength=K)
+    # work of the timed windows, summed per root on the device (the item
+    # records' fields without copying the records in the timed region)
+    tot = ctx.work_totals(K)
+    n_items = args.steps * W * K
+    alg_bytes = float(8 * n_items + 8 * tot["processed"].sum() + 8 * tot["scanned"].sum()
+                      + 48 * tot["affected"].sum() + 4 * tot["moved"].sum())
+    per_root = tot["scanned"].astype(np.float64)
     value = args.steps * W / t_batched
     print(f"[bench] batched: {value:.1f} ins/s, window {1e3 * t_batched / args.steps:.2f} ms, "
           f"kernel avg {k_ms / max(n_launch, 1):.2f} ms", file=sys.stderr, flush=True)
@@ -277,10 +280,10 @@ def main():
         "plain_gpu": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "roofline": roofline,
-        "items": {"count": int(rec.size), "nonzero_gap": int((rec["kase"] > 0).sum()),
+        "items": {"count": n_items, "nonzero_gap": int(tot["nonzero"].sum()),
                   "root_scan_max_over_mean": float(per_root.max() / max(per_root.mean(), 1)),
-                  "mean_affected": float(rec["affected"].mean()), "max_affected": int(rec["affected"].max()),
-                  "mean_scanned": float(rec["scanned"].mean()),
+                  "mean_affected": float(tot["affected"].sum() / n_items), "max_affected": int(tot["max_affected"].max()),
+                  "mean_scanned": float(tot["scanned"].sum() / n_items),
                   "parked": int(parked[0])},
         "static_pass_s": round(t_static, 3),
         "schedule": {"name": sched.name, **(sched.report() if args.schedule == "mlp" else {})},

@@ -209,6 +209,13 @@ sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, ext
  * finished by the whole-GPU kernels (sbcBatchConfig.park_min). */
 int64_t streamingBCLastParked(extraArraysPerThread **eAPT);
 
+/* Per-root work totals of the batched calls since the last
+ * streamingBCResetStats: out[r * 8 + f] for root index r < K, f = nonzero
+ * items, processed, scanned, affected, moved, largest affected of one item,
+ * then two zeros (the sums of the item records' fields, without copying the
+ * records). */
+sbcStatus streamingBCWorkTotals(extraArraysPerThread **eAPT, int64_t K, int64_t *out);
+
 /* Compute units of the context's device.  With grid_blocks = 0 a batched call
  * runs min(max(heavy roots, light teams), 2 x this) teams at once. */
 int32_t streamingBCComputeUnits(extraArraysPerThread **eAPT);

@@ -125,6 +125,7 @@ _sig("streamingBCResidentFootprint", i64, [P64, i64, i64, i32])
 _sig("streamingBCSynchronize", C.c_int, [VPP])
 _sig("streamingBCLastParked", i64, [VPP])
 _sig("streamingBCComputeUnits", i32, [VPP])
+_sig("streamingBCWorkTotals", C.c_int, [VPP, i64, P64])
 # cost model (include/sbc_costmodel.h)
 PF = C.POINTER(C.c_float)
 _sig("sbcCostModelNumParams", i32, [])
@@ -267,6 +268,16 @@ class Context:
     def num_cu(self) -> int:
         """Compute units of the context's device."""
         return int(lib.streamingBCComputeUnits(self._p))
+
+    def work_totals(self, k: int) -> dict:
+        """Per-root work totals of the batched calls since reset_stats: arrays
+        of K for nonzero items, processed, scanned, affected, moved and the
+        largest affected count of one item."""
+        out = np.zeros(8 * k, dtype=np.int64)
+        _check(lib.streamingBCWorkTotals(self._p, k, _p64(out)))
+        t = out.reshape(k, 8)
+        return {"nonzero": t[:, 0], "processed": t[:, 1], "scanned": t[:, 2], "affected": t[:, 3],
+                "moved": t[:, 4], "max_affected": t[:, 5]}
 
     def last_parked(self) -> int:
         """Items of the last batched call finished by the whole-GPU kernels."""

@@ -544,6 +544,8 @@ struct extraArraysPerThread {
     // SBC_ITEM_TRACE after each batched call (diagnostics)
     int64_t* d_tprof = nullptr;
     int64_t tprof_cap = 0;
+    int64_t* d_wtot = nullptr;   // [K][8] per-root work totals since the last stats reset
+    int64_t wtot_k = 0;
     // per-workgroup term buckets of the hub lookup pass
     SbcBucketEnt* d_bucket = nullptr;
     int64_t bucket_blocks = 0;
@@ -581,6 +583,7 @@ static void ctx_free(extraArraysPerThread* C)
     for (void* q : {(void*)C->d_park, (void*)C->d_parked, (void*)C->d_order2, (void*)C->d_start})
         if (q) (void)hipFree(q);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
+    if (C->d_wtot) (void)hipFree(C->d_wtot);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
     for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
@@ -675,6 +678,10 @@ extern "C" sbcStatus streamingBCResetStats(extraArraysPerThread** eAPT, int32_t 
     if (st) return st;
     for (int k = 0; k < KSTAT_N; k++) { C->launches[k] = 0; C->total_ms[k] = 0; }
     C->timing = enable_timing != 0;
+    if (C->d_wtot) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        HIPCHK(hipMemset(C->d_wtot, 0, sizeof(int64_t) * 8 * (size_t)C->wtot_k));
+    }
     return SBC_OK;
 }
 
@@ -1252,8 +1259,18 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         C->tprof_cap = (int64_t)W * F->K * SBC_TRACE_SLOTS;
         C->gc.with_rec = -1;
     }
+    if (C->wtot_k < F->K) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (C->d_wtot) HIPCHK(hipFree(C->d_wtot));
+        C->d_wtot = nullptr;
+        if ((st = dalloc(&C->d_wtot, (size_t)8 * F->K))) return st;
+        HIPCHK(hipMemset(C->d_wtot, 0, sizeof(int64_t) * 8 * (size_t)F->K));
+        C->wtot_k = F->K;
+        C->gc.with_rec = -1;  // the captured launch holds the old pointer
+    }
     SbcParams P = base_params(F, G, C);
     P.tprof = trace_path ? C->d_tprof : nullptr;
+    P.wtot = C->d_wtot;
     P.heavy = heavy;
     P.order = has_order ? C->d_order : nullptr;
     P.win = C->win.d_desc;
@@ -1515,6 +1532,16 @@ extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger*
 extern "C" int64_t streamingBCLastParked(extraArraysPerThread** eAPT)
 {
     return (eAPT && eAPT[0]) ? eAPT[0]->last_parked : -1;
+}
+
+extern "C" sbcStatus streamingBCWorkTotals(extraArraysPerThread** eAPT, int64_t K, int64_t* out)
+{
+    if (!eAPT || !eAPT[0] || !out || K < 1) return set_err(SBC_ERR_ARG, "bad argument");
+    extraArraysPerThread* C = eAPT[0];
+    if (K > C->wtot_k) return set_err(SBC_ERR_ARG, "no work totals for %ld roots", (long)K);
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(out, C->d_wtot, sizeof(int64_t) * 8 * (size_t)K, hipMemcpyDeviceToHost));
+    return SBC_OK;
 }
 
 extern "C" int32_t streamingBCComputeUnits(extraArraysPerThread** eAPT)

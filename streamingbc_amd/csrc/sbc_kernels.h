@@ -92,6 +92,9 @@ struct SbcParams {
     int32_t u0;
     int32_t v0;
     SbcItemRec* rec;      // [W][K] or nullptr
+    int64_t* wtot;        // [K][8] per-root work totals of the batched form (nonzero items, processed,
+                          // scanned, affected, moved, largest affected, -, -), added by the root's own
+                          // team or by the wide commit, or nullptr
     int64_t* tprof;       // [W][K][SBC_TRACE_SLOTS] or nullptr (recorded by 1024-thread teams only): timestamps (start, descent end, ascent end,
                           // commit end), hubs looked up, hubs scanned, neighbours of each; ascent time in the
                           // light-vertex passes, the hub lookups, the hub staging and scans; ascent levels,

@@ -1658,6 +1658,18 @@ __device__ __forceinline__ void load_hub_batch(const SbcParams& P, HubShared& hs
     __syncthreads();
 }
 
+// Adds a finished nonzero item to its root's work totals (only the root's
+// team, or the wide commit after it, ever writes them).
+__device__ __forceinline__ void add_work_totals(int64_t* t, const unsigned long long* stat, int64_t affected)
+{
+    t[0] += 1;
+    t[1] += (int64_t)stat[0];
+    t[2] += (int64_t)stat[1];
+    t[3] += affected;
+    t[4] += (int64_t)stat[2];
+    if (affected > t[5]) t[5] = affected;
+}
+
 // Hands the item over to the wide kernels in ascent level lev, after the
 // team processed the level's light vertices and looked up its first pass of
 // hubs (the light vertices' counts stay in the team's; the wide kernels redo
@@ -2090,6 +2102,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         rec->affected = total;
         rec->moved = (int64_t)ts.stat[2];
     }
+    if (P.wtot && leader) add_work_totals(P.wtot + (size_t)ri * 8, ts.stat, total);
     team_sync<TW>();
     return false;
 }
@@ -2265,6 +2278,7 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_commit_kernel(SbcParams P, S
         rec->affected = nUend + nS;
         rec->moved = (int64_t)w.stat[2];
     }
+    if (gt == 0 && P.wtot) add_work_totals(P.wtot + (size_t)ri * 8, w.stat, nUend + nS);
 }
 
 // ------------------------------------------------------------ static pass

@@ -377,6 +377,32 @@ def test_items_handed_over_in_the_ascent_match_oracle(window, monkeypatch):
     assert_bc_close(f.bc(), o.bc())
 
 
+@pytest.mark.gpu
+def test_work_totals_equal_record_sums(monkeypatch):
+    """The per-root work totals kept on the device equal the sums of the item
+    records (including items finished by the wide kernels)."""
+    hg = sbc.gen_rmat(12, 16 << 12, seed=9, keep_lcc=True)
+    roots = hg.pick_roots(48, 4)
+    u, v = hg.insertions(64, 21)
+    ctx, g, f = _engine(hg, roots)
+    ctx.reset_stats(False)
+    rec = f.insert_batched(g, ctx, u, v, sbc.BatchConfig(window=16, block_threads=1024, park_min=16), records=True)
+    tot = ctx.work_totals(len(roots))
+    r = rec.reshape(-1)
+    print(f"[totals] parked {ctx.last_parked()} nonzero {int(tot['nonzero'].sum())}", flush=True)
+    for key in ("processed", "scanned", "affected", "moved"):
+        want = np.bincount(r["root_index"], weights=r[key].astype(np.float64), minlength=len(roots))
+        assert np.array_equal(tot[key], want.astype(np.int64)), key
+    nz = np.bincount(r["root_index"], weights=(r["kase"] > 0).astype(np.float64), minlength=len(roots))
+    assert np.array_equal(tot["nonzero"], nz.astype(np.int64))
+    mx = np.zeros(len(roots), dtype=np.int64)
+    np.maximum.at(mx, r["root_index"], r["affected"])
+    assert np.array_equal(tot["max_affected"], mx)
+    # a reset starts them again
+    ctx.reset_stats(False)
+    assert ctx.work_totals(len(roots))["scanned"].sum() == 0
+
+
 def test_hub_lookup_level3_leaf_roots():
     """Roots that are leaves hanging off a low-degree vertex see the hubs on
     level 3; their ascent-only hubs there are resolved by table lookups with
