@@ -1423,20 +1423,38 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         }
     }
     C->last_parked = nparked;
-    // make the last window's appends visible to everyone
+    // make the last window's appends visible to everyone: one launch with the
+    // vertex descriptors staged like a window's (no slots)
     for (int32_t x : prev_touched) G->deg0[x] = G->deg[x];
-    for (size_t i = 0; i < prev_touched.size(); i += SBC_APPEND_INLINE) {
+    if (!prev_touched.empty()) {
+        WinStage& S = C->win;
+        const int b = S.h_cur;
+        S.h_cur ^= 1;
+        HIPCHK(hipEventSynchronize(S.h_done[b]));
+        char* hb = (char*)S.h_buf[b];
+        SbcWinDesc* hd = (SbcWinDesc*)hb;
+        memset(hd, 0, sizeof(SbcWinDesc));
+        hd->nvupd = (int32_t)prev_touched.size();
+        int32_t* hvv = (int32_t*)(hb + 64) + 5 * 4 * W;
+        SbcVInfo* hv = (SbcVInfo*)(hb + 64 + sizeof(int32_t) * 6 * 4 * (size_t)W);
+        for (size_t i = 0; i < prev_touched.size(); i++) {
+            hvv[i] = prev_touched[i];
+            hv[i] = vinfo_of(G, prev_touched[i]);
+        }
+        HIPCHK(hipMemcpyAsync(S.d_desc, hd, sizeof(SbcWinDesc), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(S.d_ints + 5 * 4 * W, hvv, sizeof(int32_t) * prev_touched.size(), hipMemcpyHostToDevice,
+                              C->stream));
+        HIPCHK(hipMemcpyAsync(S.d_vi, hv, sizeof(SbcVInfo) * prev_touched.size(), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipEventRecord(S.h_done[b], C->stream));
         SbcAppend B;
         memset(&B, 0, sizeof(B));
         B.vinfo = G->d_vinfo;
         B.col = G->d_col;
         B.seq = G->d_seq;
-        for (size_t j = i; j < prev_touched.size() && B.nvupd < SBC_APPEND_INLINE; j++) {
-            B.vv[B.nvupd] = prev_touched[j];
-            B.vi[B.nvupd] = vinfo_of(G, prev_touched[j]);
-            B.nvupd++;
-        }
-        HIPCHK(sbc_launch_append(&B, SBC_APPEND_INLINE, C->stream));
+        B.desc = S.d_desc;
+        B.d_vv = S.d_ints + 5 * 4 * W;
+        B.d_vi = S.d_vi;
+        HIPCHK(sbc_launch_append(&B, (int)prev_touched.size(), C->stream));
     }
     return check_kernel_err(F, C);
 }
