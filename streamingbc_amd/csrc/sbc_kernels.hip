@@ -83,9 +83,30 @@ __device__ __forceinline__ int prefix_in_wave(uint64_t m)
 // which IEEE 754 defines it to be) and lane 0 adds them one after another.
 // acc must be the same in every lane; so is the result.  Wave-uniform
 // control flow only.
+// Packing rows of the ordered sums: one row of 128 values per wavefront of
+// the workgroup (a single wavefront may use all of them while the others
+// wait at a barrier).
+__shared__ double sbc_wrow[16][128];
+
+// lane 0's sequential sum acc + v[0] + ... + v[n - 1] over values in LDS
+// (loads run ahead of the add chain); the result is that of the plain loop.
+__device__ __forceinline__ double lds_chain_sum(double acc, const double* v, int n)
+{
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = v[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + x[u];
+    }
+    for (; i < n; ++i) acc = acc + v[i];
+    return acc;
+}
+
 __device__ __forceinline__ double wave_ordered_add(double acc, bool c, double add, bool s, double sub)
 {
-    __shared__ double wrow[16][128];
+    double (&wrow)[16][128] = sbc_wrow;
     const uint64_t cm = ballot(c), sm = ballot(s);
     if ((cm | sm) == 0) return acc;
     const int lane = lane_id();
@@ -99,17 +120,7 @@ __device__ __forceinline__ double wave_ordered_add(double acc, bool c, double ad
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double r = acc;
-    if (lane == 0) {
-        int i = 0;
-        for (; i + 4 <= n; i += 4) {
-            const double a0 = w[i], a1 = w[i + 1], a2 = w[i + 2], a3 = w[i + 3];
-            r = r + a0;
-            r = r + a1;
-            r = r + a2;
-            r = r + a3;
-        }
-        for (; i < n; ++i) r = r + w[i];
-    }
+    if (lane == 0) r = lds_chain_sum(acc, w, n);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -459,6 +470,7 @@ struct alignas(16) LightShared {
     double esub[EDGE_CAP];
 };
 static_assert(TEAM_SORT_CAP <= LIGHT_CAP && TEAM_SORT_CAP <= EDGE_CAP, "team sort stages a bucket in LightShared");
+static_assert(2 * TEAM_SORT_CAP <= 16 * 128, "a team-sorted bucket's values fit the packing rows");
 // positions of a team-sorted hub's list: a bitmap in LightShared::base, per-word prefixes in ::ovf
 constexpr int TEAM_SORT_BITS = 32 * LIGHT_CAP;
 static_assert(TEAM_SORT_BITS <= 64 * LIGHT_CAP, "bitmap fits LightShared::base");
@@ -1413,16 +1425,26 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         __syncthreads();
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[26] += x - tk; tk = x; }
         if (!dup && twave == 0) {
+            // every value in order into the packing rows (a subtraction as the
+            // addition of the negated value, which IEEE 754 defines it to be),
+            // then one add chain by lane 0
             const int32_t k = hs.plist[hi];
-            double acc = X.vr[k].delta;
+            double* row = &sbc_wrow[0][0];  // 2048 values >= 2 * TEAM_SORT_CAP
+            const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            int nval = 0;
             for (int e0 = 0; e0 < n; e0 += 64) {
                 const int e = e0 + lane;
                 const uint8_t fl = e < n ? ls.eflag[e] : (uint8_t)0;
-                const double ad = (fl & 1) ? ls.eadd[e] : 0.0;
-                const double sb2 = (fl & 2) ? ls.esub[e] : 0.0;
-                acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
+                const uint64_t cm = ballot(fl & 1), sm = ballot(fl & 2);
+                int pos = nval + __popcll(cm & below) + __popcll(sm & below);
+                if (fl & 1) row[pos++] = ls.eadd[e];
+                if (fl & 2) row[pos] = -ls.esub[e];
+                nval += __popcll(cm) + __popcll(sm);
             }
-            if (lane == 0) st_reld(&X.vr[k].ndel, acc);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) st_reld(&X.vr[k].ndel, lds_chain_sum(X.vr[k].delta, row, nval));
         }
         __syncthreads();
         if (tp && tid == 0) tp[27] += (int64_t)wall_clock64() - tk;
