@@ -384,6 +384,8 @@ struct alignas(16) HubShared {
     int pstart[HUB_CAP];  // first slot of each lookup hub
     int hlen[HUB_CAP];    // visible list length of each queued hub of the pass
     int hstart[HUB_CAP];  // first slot of each queued hub of the pass
+    int hidx[HUB_CAP];    // indexed neighbours of each hub queued in LDS by an ascent level (hub table count)
+    uint8_t hasc[HUB_CAP];  // that hub is ascent-only
     int list[HUB_CAP];
     int plist[HUB_CAP];   // ascent-only hubs of the level handled by table lookups
     int bcnt[HUB_CAP];    // terms collected per lookup hub
@@ -1502,7 +1504,14 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             if (len > HUB_MIN) {
                 const int pos = atomicAdd(&hs.n, 1);
                 const int hcap = hub_batch(P);
-                if (pos < hcap) hs.list[pos] = k;
+                if (pos < hcap) {
+                    // what the first hub pass needs, from this round's loads
+                    hs.list[pos] = k;
+                    hs.hlen[pos] = len;
+                    hs.hstart[pos] = vi.start;
+                    hs.hidx[pos] = P.hub ? P.hub[k].y : 0;
+                    hs.hasc[pos] = kst.isD ? 0 : 1;
+                }
                 else if (P.hubq && pos < hcap + SBC_HUBQ_CAP)
                     P.hubq[(size_t)blockIdx.x * SBC_HUBQ_CAP + (pos - hcap)] = k;
                 else ls.ovf[atomicAdd(&ls.novf, 1)] = k;
@@ -1958,21 +1967,29 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     }
                 }
                 const long cand = (long)src.size();
-                // one thread per hub: its descriptor, table and state byte in
-                // one round trip; its visible length and first slot kept for
-                // the lookups, the counters and the scans
+                // one thread per hub: its descriptor, table count and state (the
+                // first pass's were kept in LDS by the light pass; later passes
+                // load them in one round trip); its visible length and first
+                // slot kept for the lookups, the counters and the scans
                 for (int h = threadIdx.x; h < nh; h += TW * 64) {
                     const int32_t k = hs->list[h];
-                    const SbcVInfo vi = P.vinfo[k];
-                    const int4 hb = P.hub ? P.hub[k] : make_int4(0, 0, 0, 0);
-                    const uint8_t kb = ld_sb(&X.sb[k]);
-                    const int len = visible_len_thread(P, vi, X.q);
-                    hs->hlen[h] = len;
-                    hs->hstart[h] = vi.start;
+                    int len, hy;
+                    bool asc;
+                    if (hb == 0) {
+                        len = hs->hlen[h];
+                        hy = hs->hidx[h];
+                        asc = hs->hasc[h] != 0;
+                    } else {
+                        const SbcVInfo vi = P.vinfo[k];
+                        hy = P.hub ? P.hub[k].y : 0;
+                        asc = sb_type(ld_sb(&X.sb[k])) == 3;
+                        len = visible_len_thread(P, vi, X.q);
+                        hs->hlen[h] = len;
+                        hs->hstart[h] = vi.start;
+                    }
                     bool pk = false;
-                    if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && hb.y > 0 &&
-                        sb_type(kb) == 3)
-                        pk = cand < (long)hb.y;
+                    if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && hy > 0 && asc)
+                        pk = cand < (long)hy;
                     atomicAdd(&ts.stat[0], 1ull);
                     atomicAdd(&ts.stat[1], (unsigned long long)len);
                     int hi = -1;
@@ -1980,7 +1997,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                         hi = atomicAdd(&hs->np, 1);
                         hs->plist[hi] = k;
                         hs->plen[hi] = len;
-                        hs->pstart[hi] = vi.start;
+                        hs->pstart[hi] = hs->hstart[h];
                         hs->kind[hi] = 0;
                     }
                     hs->pidx[h] = hi;
