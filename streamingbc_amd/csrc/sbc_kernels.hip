@@ -681,6 +681,33 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     const int lane = lane_id();
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
+    if (srcset) {
+        // neighbour ids of up to 8 chunks in flight, then chunk by chunk:
+        // only members of the level's source set can give a term or need a
+        // mark, and a chunk without one is skipped
+        constexpr int SU = 8;
+        for (int ch0 = 0; ch0 < nch; ch0 += SU) {
+            int32_t cc[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int e = (ch0 + u) * 64 + lane;
+                cc[u] = (ch0 + u) < nch && e < len ? P.col[start + e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const bool ok = cc[u] >= 0 && shash_has(*srcset, cc[u]);
+                if (ballot(ok) == 0) continue;
+                const uint8_t ww = ok ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+                bool appU;
+                const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok, cc[u], ww, appU);
+                const int pu = wave_append(appU, &ts.cnt[2]);
+                if (pu >= 0) X.list[pu] = cc[u];
+                acc = wave_ordered_add(acc, t.child, t.add, t.hsub, t.sub);
+            }
+        }
+        if (lane == 0) st_reld(&X.vr[k].ndel, acc);
+        return;
+    }
     for (int ch = 0; ch < nch; ch += WAVE_UNROLL) {
         int32_t cc[WAVE_UNROLL];
         bool ok[WAVE_UNROLL];
