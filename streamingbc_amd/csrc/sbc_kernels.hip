@@ -1221,10 +1221,13 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             else if (i < nb + ndl) w = X.list[src.dA + (i - nb)];
             else w = src.pc[i - nb - ndl];
             const SbcVInfo vi = P.vinfo[w];
+            // a parent candidate that is already marked (or processed) can
+            // only be left as it is, so it is not looked up at all
+            const bool done = i >= nb + ndl && sb_type(ld_sb(&X.sb[w])) != 0;
             len = visible_len_thread(P, vi, X.q);
             start = vi.start;
-            scan_mode = len <= 2 * np;
-            table_mode = !scan_mode;
+            scan_mode = !done && len <= 2 * np;
+            table_mode = !done && !scan_mode;
         }
         // edge offsets of scan-mode sources, slots of table-mode sources and
         // compact scan-mode slots (number of scan-mode threads before), in
