@@ -1517,6 +1517,40 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
     const int tid = threadIdx.x, lane = lane_id();
     constexpr int NT = TW * 64;
     static_assert(NT <= LIGHT_CAP, "one slot per thread of a round");
+    if (tot <= TW) {
+        // a small level: one wavefront per vertex, no batch and no block-wide
+        // scans (the caller's barrier follows); hubs are queued as in a round
+        if (twave < tot) {
+            const int32_t k = here.at(X, twave);
+            const SbcVInfo vi = P.vinfo[k];
+            const int len = visible_len(P, vi, X.q);
+            int pos = -1;  // place in the hub queues, or -1 (the wavefront ascends it)
+            if (len > HUB_MIN) {
+                if (lane == 0) pos = atomicAdd(&hs.n, 1);
+                pos = __shfl(pos, 0);
+                const int hcap = hub_batch(P);
+                if (pos < hcap) {
+                    if (lane == 0) {
+                        hs.list[pos] = k;
+                        hs.hlen[pos] = len;
+                        hs.hstart[pos] = vi.start;
+                        hs.hidx[pos] = P.hub ? P.hub[k].y : 0;
+                        hs.hasc[pos] = sb_type(ld_sb(&X.sb[k])) == 3 ? 1 : 0;
+                    }
+                } else if (P.hubq && pos < hcap + SBC_HUBQ_CAP) {
+                    if (lane == 0) P.hubq[(size_t)blockIdx.x * SBC_HUBQ_CAP + (pos - hcap)] = k;
+                } else {
+                    pos = -1;  // no queue takes it
+                }
+            }
+            if (pos < 0) {
+                ascent_vertex_wave(P, X, k, vi, len, lev, ts);
+                my_proc += 1;
+                my_scan += (uint32_t)len;
+            }
+        }
+        return;
+    }
     bool set_built = false;  // the level's source set, kept for the later rounds
     for (int r0 = 0; r0 < tot; r0 += NT) {
         if (tp && tid == 0) tq = (int64_t)wall_clock64();
