@@ -50,8 +50,10 @@ typedef struct {
     int64_t *vertexPointerArray;
 } csrGraph;
 
-/* src/streamingbc_aux.h:42-48 (returned zeroed, as the reference does:
- * src/streamingbc.c:220-225; per-root case data is in sbcItemRecord) */
+/* src/streamingbc_aux.h:42-48 (insertEdgeStreamingBC returns it zeroed, as the
+ * reference does: src/streamingbc.c:220-225, per-root case data is in
+ * sbcItemRecord; deleteEdgeStreamingBC fills movement, adjacent and sameLevel
+ * with root counts, as src/streamingbc.c:408-414 does) */
 typedef struct {
     int64_t movement;
     int64_t adjacent;
@@ -81,7 +83,8 @@ typedef enum {
     SBC_ERR_STATE = 4,     /* call order violated (e.g. update before static pass) */
     SBC_ERR_EXISTS = 5,    /* edge already present or self loop */
     SBC_ERR_LEVELS = 6,    /* BFS depth beyond the forest's level capacity */
-    SBC_ERR_BUDGET = 7     /* admission refused: footprint above the HBM budget */
+    SBC_ERR_BUDGET = 7,    /* admission refused: footprint above the HBM budget */
+    SBC_ERR_ABSENT = 8     /* edge to delete is not in the graph */
 } sbcStatus;
 
 /* ------------------------------------------- 1. reference entry points */
@@ -116,6 +119,19 @@ StreamingExtraInfo insertEdgeStreamingBC(bcForest *forest, struct stinger *sStin
                                          int64_t NV, int64_t NT, extraArraysPerThread **eAPT,
                                          uint32_t loadBalancing, uint32_t granularity);
 
+/* src/streamingbc.h:57-58, src/streamingbc.c:258-416 -- the edge (oldU,oldV)
+ * must already have been removed with stinger_remove_edge in both directions,
+ * and every inserted edge applied.  Per root: endpoints on the same level
+ * leave the tree as it is; otherwise (the reference's adjacent and movement
+ * cases) the root's tree is rebuilt on the device by the static pass over the
+ * post-deletion graph.  BC is then folded again from every tree.  Returns the
+ * root counts of the three cases.  loadBalancing, granularity and NT are
+ * ignored. */
+StreamingExtraInfo deleteEdgeStreamingBC(bcForest *forest, struct stinger *sStinger, int64_t oldU,
+                                         int64_t oldV, int64_t *rootArrayForApproximation, int64_t NK,
+                                         int64_t NV, int64_t NT, extraArraysPerThread **eAPT,
+                                         uint32_t loadBalancing, uint32_t granularity);
+
 /* src/streamingbc.h:68 -- builds the device graph from a CSR whose
  * vertexPointerArray has NV+1 entries and edgeArray NE entries */
 void CreateStingerFromCSR(csrGraph *csr, struct stinger **stingerGraph);
@@ -127,6 +143,10 @@ void ParseDimacsGraph(char initial_graph_name[1024], struct stinger **stingerGra
  * type, weight and timestamp are ignored. */
 int stinger_insert_edge(struct stinger *G, int64_t type, int64_t from, int64_t to, int64_t weight,
                         int64_t timestamp);
+/* src/stinger/stinger.c:1055-1090 -- removes `to` from `from`'s list, keeping
+ * the order of the other neighbours.  Returns 1 if removed, -1 if absent or on
+ * error (self loop, bad id).  type is ignored. */
+int stinger_remove_edge(struct stinger *G, int64_t type, int64_t from, int64_t to);
 /* src/stinger/stinger.c:72-95 */
 int64_t stinger_outdegree(struct stinger *G, int64_t v);
 /* src/stinger/stinger.c:558-590 -- frees the graph, returns NULL */
@@ -146,6 +166,12 @@ sbcStatus sbcInitStatic(bcForest *forest, struct stinger *G, extraArraysPerThrea
  * (src/sbcMain_utils.c:458-471). */
 sbcStatus sbcInsertEdge(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t u,
                         int64_t v, uint32_t granularity);
+/* Removes (u,v) from the graph in both directions, then runs the deletion
+ * update: stinger_remove_edge x2 + deleteEdgeStreamingBC as the reference's
+ * driver does (src/sbcMain_utils.c:473-481, 577-591).  SBC_ERR_ABSENT (nothing
+ * modified) if the edge is not there.  info (may be NULL) gets the case counts. */
+sbcStatus sbcDeleteEdge(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t u,
+                        int64_t v, StreamingExtraInfo *info);
 
 /* Copies BC (NV doubles) to out (may be NULL) and refreshes forest->totalBC. */
 sbcStatus streamingBCGetBC(bcForest *forest, double *out);

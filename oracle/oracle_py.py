@@ -49,6 +49,8 @@ def oracle_lib():
             ("orc_graph_new", VP, [i64, P64, P64]),
             ("orc_graph_free", None, [VP]),
             ("orc_graph_insert", C.c_int, [VP, i64, i64]),
+            ("orc_graph_remove", C.c_int, [VP, i64, i64]),
+            ("orc_delete", None, [VP, VP, i64, i64, P64]),
             ("orc_graph_degree", i64, [VP, i64]),
             ("orc_forest_new", VP, [i64, i64, P64]),
             ("orc_forest_free", None, [VP]),
@@ -81,6 +83,7 @@ def ref_lib():
             ("ref_new", VP, [i64, i64, P64, P64, i64, P64, i64]),
             ("ref_static", C.c_double, [VP]),
             ("ref_insert", C.c_double, [VP, i64, i64, i64, i64]),
+            ("ref_delete", C.c_double, [VP, i64, i64, i64, i64, P64]),
             ("ref_get_tree", None, [VP, i64, P64, P64, PD]),
             ("ref_get_bc", None, [VP, PD]),
             ("ref_counters", None, [VP, P64]),
@@ -137,6 +140,16 @@ class Oracle:
         if r != 1:
             raise ValueError(f"edge ({u},{v}) not insertable: {r}")
         return self.update(u, v, mode, items)
+
+    def delete(self, u, v):
+        """Removes (u,v) and runs the deletion update; returns the root counts
+        (sameLevel, adjacent, movement)."""
+        L = oracle_lib()
+        if L.orc_graph_remove(self.g, u, v) != 1:
+            raise ValueError(f"edge ({u},{v}) not present")
+        out = np.zeros(3, dtype=np.int64)
+        L.orc_delete(self.f, self.g, u, v, _p64(out))
+        return out
 
     def tree(self, ri):
         lv = np.zeros(self.nv, dtype=np.int64)
@@ -207,6 +220,14 @@ class Reference:
         if t < 0:
             raise ValueError(f"edge ({u},{v}) already present")
         return t
+
+    def delete(self, u, v, lb=0, gran=COARSE):
+        """Returns the reference's (sameLevel, adjacent, movement) counts."""
+        out = np.zeros(3, dtype=np.int64)
+        t = ref_lib().ref_delete(self.h, u, v, lb, gran, _p64(out))
+        if t < 0:
+            raise ValueError(f"edge ({u},{v}) not present")
+        return out
 
     def tree(self, ri):
         lv = np.zeros(self.nv, dtype=np.int64)

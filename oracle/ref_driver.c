@@ -90,6 +90,27 @@ double ref_insert(void *h, int64_t u, int64_t v, int64_t lb, int64_t gran)
     return now_s() - t0;
 }
 
+/* Removes (u,v) from STINGER (both directions) and runs deleteEdgeStreamingBC,
+ * as DeleteEdge does (src/sbcMain_utils.c:473-591).  counts3 gets the
+ * returned sameLevel, adjacent and movement counts.  Returns the seconds spent
+ * in deleteEdgeStreamingBC, or -1 if the edge was not present. */
+double ref_delete(void *h, int64_t u, int64_t v, int64_t lb, int64_t gran, int64_t *counts3)
+{
+    ref_ctx *c = (ref_ctx *)h;
+    int r1 = stinger_remove_edge(c->G, 0, u, v);
+    int r2 = stinger_remove_edge(c->G, 0, v, u);
+    if (r1 != 1 || r2 != 1) return -1.0;
+    omp_set_num_threads((int)c->NT);
+    double t0 = now_s();
+    c->last = deleteEdgeStreamingBC(c->F, c->G, u, v, c->roots, c->NK, c->NV, c->NT, c->eAPT,
+                                    (uint32_t)lb, (uint32_t)gran);
+    double t = now_s() - t0;
+    counts3[0] = c->last.sameLevel;
+    counts3[1] = c->last.adjacent;
+    counts3[2] = c->last.movement;
+    return t;
+}
+
 void ref_get_tree(void *h, int64_t ri, int64_t *level, int64_t *sigma, double *delta)
 {
     ref_ctx *c = (ref_ctx *)h;

@@ -84,6 +84,28 @@ int orc_graph_insert(orc_graph *g, int64_t u, int64_t v)
     return 1;
 }
 
+/* Drops v from u's list keeping the order of the rest: STINGER marks the
+ * slot deleted and every traversal skips it (src/stinger/stinger.c:1055-1090,
+ * src/stinger/stinger-traversal.h:61-76). */
+static int drop_adj(orc_graph *g, int64_t u, int64_t v)
+{
+    for (int64_t i = 0; i < g->deg[u]; i++)
+        if (g->adj[u][i] == v) {
+            memmove(&g->adj[u][i], &g->adj[u][i + 1], (size_t)(g->deg[u] - i - 1) * sizeof(int64_t));
+            g->deg[u]--;
+            return 1;
+        }
+    return 0;
+}
+
+int orc_graph_remove(orc_graph *g, int64_t u, int64_t v)
+{
+    if (u == v || u < 0 || v < 0 || u >= g->nv || v >= g->nv) return -1;
+    if (!drop_adj(g, u, v)) return -1;
+    drop_adj(g, v, u);
+    return 1;
+}
+
 /* ------------------------------------------------------------------ forest */
 
 orc_forest *orc_forest_new(int64_t nv, int64_t K, const int64_t *roots)
@@ -200,42 +222,88 @@ static int64_t static_bfs(orc_forest *f, const orc_graph *g, int64_t ri, int64_t
     return qt;
 }
 
+/* One root's tree from scratch; the visit order is left in f->qd and its
+ * length returned. */
+static int64_t static_root(orc_forest *f, const orc_graph *g, int64_t ri, int order)
+{
+    int64_t nv = f->nv;
+    int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+    double *D = f->delta + (size_t)ri * nv;
+    int64_t n = static_bfs(f, g, ri, f->qd);
+    if (order == ORC_PUSH) {
+        /* stack ascent, src/streamingbc_init.c:155-180 */
+        for (int64_t i = n - 1; i >= 0; i--) {
+            int64_t c = f->qd[i];
+            for (int64_t e = 0; e < g->deg[c]; e++) {
+                int64_t k = g->adj[c][e];
+                if (L[k] == L[c] - 1) D[k] += dep(S[k], S[c], D[c]);
+            }
+        }
+    } else {
+        /* pull: each vertex sums its children in adjacency order */
+        for (int64_t i = n - 1; i >= 0; i--) {
+            int64_t k = f->qd[i];
+            double acc = 0.0;
+            for (int64_t e = 0; e < g->deg[k]; e++) {
+                int64_t c = g->adj[k][e];
+                if (L[c] == L[k] + 1) acc += dep(S[k], S[c], D[c]);
+            }
+            D[k] = acc;
+        }
+    }
+    return n;
+}
+
 void orc_static(orc_forest *f, const orc_graph *g, int order)
 {
     int64_t nv = f->nv;
     for (int64_t v = 0; v < nv; v++) f->bc[v] = 0.0;
     for (int64_t ri = 0; ri < f->K; ri++) {
         int64_t r = f->roots[ri];
-        int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
         double *D = f->delta + (size_t)ri * nv;
-        int64_t n = static_bfs(f, g, ri, f->qd);
-        if (order == ORC_PUSH) {
-            /* stack ascent, src/streamingbc_init.c:155-180 */
-            for (int64_t i = n - 1; i >= 0; i--) {
-                int64_t c = f->qd[i];
-                for (int64_t e = 0; e < g->deg[c]; e++) {
-                    int64_t k = g->adj[c][e];
-                    if (L[k] == L[c] - 1) D[k] += dep(S[k], S[c], D[c]);
-                }
-                if (c != r) f->bcpart[c] += D[c];
-            }
-        } else {
-            /* pull: each vertex sums its children in adjacency order */
-            for (int64_t i = n - 1; i >= 0; i--) {
-                int64_t k = f->qd[i];
-                double acc = 0.0;
-                for (int64_t e = 0; e < g->deg[k]; e++) {
-                    int64_t c = g->adj[k][e];
-                    if (L[c] == L[k] + 1) acc += dep(S[k], S[c], D[c]);
-                }
-                D[k] = acc;
-            }
-            for (int64_t i = n - 1; i >= 0; i--) {
-                int64_t c = f->qd[i];
-                if (c != r) f->bcpart[c] += D[c];
-            }
+        int64_t n = static_root(f, g, ri, order);
+        /* BC partial in stack order, src/streamingbc_init.c:175-177 */
+        for (int64_t i = n - 1; i >= 0; i--) {
+            int64_t c = f->qd[i];
+            if (c != r) f->bcpart[c] += D[c];
         }
     }
+    fold_bc(f);
+}
+
+/* deleteEdgeStreamingBC, src/streamingbc.c:258-416 (NT = 1, NO_BALANCE), for
+ * (u,v) already removed from g.  Per root the reference's dispatch (310-345):
+ * endpoints on the same level, nothing changes; otherwise the deeper endpoint
+ * either keeps a neighbour one level up (adjacent case,
+ * removeEdgeWithoutMovementBrandes, src/streamingbc_internal.c:607-868) or not
+ * (movement case, moveDownTreeBrandes, 880-1401).  Both cases end in the tree
+ * of the post-deletion graph, so that tree is recomputed here (pull order) and
+ * the change of each vertex's dependency added to BC, as the reference adds
+ * newDelta - delta per touched vertex. */
+void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3])
+{
+    int64_t nv = f->nv;
+    counts[0] = counts[1] = counts[2] = 0;
+    double *old = (double *)malloc((size_t)nv * sizeof(double));
+    for (int64_t ri = 0; ri < f->K; ri++) {
+        int64_t r = f->roots[ri];
+        int64_t *L = f->level + (size_t)ri * nv;
+        double *D = f->delta + (size_t)ri * nv;
+        if (L[u] == L[v]) {
+            counts[0]++;
+            continue;
+        }
+        int64_t c = L[u] > L[v] ? u : v;
+        int64_t extra = 0;
+        for (int64_t e = 0; e < g->deg[c]; e++)
+            if (L[g->adj[c][e]] == L[c] - 1) extra++;
+        counts[extra >= 1 ? 1 : 2]++;
+        memcpy(old, D, (size_t)nv * sizeof(double));
+        static_root(f, g, ri, ORC_PULL);
+        for (int64_t x = 0; x < nv; x++)
+            if (x != r && D[x] != old[x]) f->bcpart[x] += D[x] - old[x];
+    }
+    free(old);
     fold_bc(f);
 }
 

@@ -20,6 +20,13 @@
  *                               addEdgeWithoutMovementBrandesFG / moveUpTreeBrandesFG
  *                               (src/streamingbc_internal_fine.c:11-340, 343-1016), i.e.
  *                               level-synchronous descent and pull-form ascent
+ *   orc_delete                  deleteEdgeStreamingBC, NT=1, NO_BALANCE (src/streamingbc.c:258-416):
+ *                               the reference's per-root dispatch, and each affected
+ *                               root's tree recomputed by the static pull pass over the
+ *                               post-deletion graph (the tree both deletion cases,
+ *                               src/streamingbc_internal.c:607-1401, arrive at)
+ *   orc_graph_remove            stinger_remove_edge x2 as called by DeleteEdge
+ *                               (src/sbcMain_utils.c:473-481; src/stinger/stinger.c:1055-1090)
  *   orc_graph_insert            stinger_insert_edge x2 as called by InsertEdge
  *                               (src/sbcMain_utils.c:458-471; src/stinger/stinger.c:778-869):
  *                               neighbour appended at the end of each list
@@ -32,7 +39,11 @@
  *   - insert coarse / fine: bit-exact (level, sigma, delta, BC) against the
  *     reference's COARSE / FINE update at NT=1, NO_BALANCE (golden fixtures in
  *     tests/golden written by the compiled reference), and equal to a static
- *     recompute of the post-insertion graph (dist/sigma exact, delta/BC 1e-9).
+ *     recompute of the post-insertion graph (dist/sigma exact, delta/BC 1e-9);
+ *   - delete: case counts equal and level/sigma bit-exact, delta/BC within
+ *     1e-9, against the reference's deleteEdgeStreamingBC (golden fixtures
+ *     written by the compiled reference), and equal to a static recompute of
+ *     the post-deletion graph.
  *
  * Conventions: level and sigma are int64 with LONG_MAX for unreachable
  * vertices, delta and BC are double, BC excludes each tree's own root.
@@ -68,6 +79,10 @@ void orc_graph_free(orc_graph *g);
 /* Appends v to u's list and u to v's list.  Returns 1 if inserted, 0 if the
  * edge already exists, -1 for a self loop or out-of-range id. */
 int orc_graph_insert(orc_graph *g, int64_t u, int64_t v);
+/* Removes v from u's list and u from v's list, keeping the order of the other
+ * neighbours (STINGER leaves a hole that traversals skip).  Returns 1 if
+ * removed, -1 if absent, a self loop or an out-of-range id. */
+int orc_graph_remove(orc_graph *g, int64_t u, int64_t v);
 int64_t orc_graph_degree(const orc_graph *g, int64_t v);
 
 orc_forest *orc_forest_new(int64_t nv, int64_t K, const int64_t *roots);
@@ -76,6 +91,9 @@ void orc_forest_free(orc_forest *f);
 void orc_static(orc_forest *f, const orc_graph *g, int order);
 /* The edge (u,v) must already be in g.  items: K records or NULL. */
 void orc_insert(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int mode, orc_item *items);
+/* The edge (u,v) must already be removed from g.  counts: roots of the
+ * same-level, adjacent and movement cases. */
+void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3]);
 
 void orc_get_tree(const orc_forest *f, int64_t ri, int64_t *level, int64_t *sigma, double *delta);
 void orc_set_tree(orc_forest *f, int64_t ri, const int64_t *level, const int64_t *sigma, const double *delta);

@@ -42,7 +42,8 @@ class SbcError(RuntimeError):
         self.status = status
 
 
-SBC_OK, SBC_ERR_ARG, SBC_ERR_HIP, SBC_ERR_NOMEM, SBC_ERR_STATE, SBC_ERR_EXISTS, SBC_ERR_LEVELS, SBC_ERR_BUDGET = range(8)
+(SBC_OK, SBC_ERR_ARG, SBC_ERR_HIP, SBC_ERR_NOMEM, SBC_ERR_STATE, SBC_ERR_EXISTS, SBC_ERR_LEVELS, SBC_ERR_BUDGET,
+ SBC_ERR_ABSENT) = range(9)
 COARSE, FINE = 0, 1
 NO_BALANCE, BALANCE = 0, 1
 
@@ -105,11 +106,13 @@ _sig("sbcGraphCreate", C.c_int, [i64, P64, P64, VPP])
 _sig("sbcForestCreate", C.c_int, [i64, i64, P64, C.POINTER(FP)])
 _sig("sbcInitStatic", C.c_int, [FP, VP, VPP])
 _sig("sbcInsertEdge", C.c_int, [FP, VP, VPP, i64, i64, u32])
+_sig("sbcDeleteEdge", C.c_int, [FP, VP, VPP, i64, i64, C.POINTER(ExtraInfoC)])
 _sig("streamingBCCreateAuxilary", VPP, [i64, i64])
 _sig("streamingBCDeleteAuxilary", None, [VPP, i64, i64])
 _sig("streamingBCDeleteForestApproximate", None, [C.POINTER(FP), i64, P64])
 _sig("stinger_free_all", VP, [VP])
 _sig("stinger_insert_edge", C.c_int, [VP, i64, i64, i64, i64, i64])
+_sig("stinger_remove_edge", C.c_int, [VP, i64, i64, i64])
 _sig("stinger_outdegree", i64, [VP, i64])
 _sig("streamingBCGetBC", C.c_int, [FP, PD])
 _sig("streamingBCGetTree", C.c_int, [FP, i64, P64, P64, PD])
@@ -117,6 +120,7 @@ _sig("streamingBCRecomputeBC", C.c_int, [FP, VPP])
 _sig("streamingBCInsertEdgesBatched", C.c_int, [FP, VP, VPP, P64, P64, i64, C.POINTER(BatchConfigC),
                                                  C.POINTER(ItemRecord)])
 _sig("insertEdgeStreamingBC", ExtraInfoC, [FP, VP, i64, i64, P64, i64, i64, i64, VPP, u32, u32])
+_sig("deleteEdgeStreamingBC", ExtraInfoC, [FP, VP, i64, i64, P64, i64, i64, i64, VPP, u32, u32])
 _sig("streamingBCLastRecords", C.c_int, [VPP, C.POINTER(ItemRecord), i64])
 _sig("streamingBCKernelStats", C.c_int, [VPP, i32, P64, PD])
 _sig("streamingBCResetStats", C.c_int, [VPP, i32])
@@ -319,6 +323,10 @@ class Graph:
         """stinger_insert_edge(u -> v); 1 inserted, 0 exists, -1 invalid."""
         return lib.stinger_insert_edge(self._p, 0, u, v, 0, 0)
 
+    def remove_edge(self, u: int, v: int) -> int:
+        """stinger_remove_edge(u -> v); 1 removed, -1 absent or invalid."""
+        return lib.stinger_remove_edge(self._p, 0, u, v)
+
     def degree(self, v: int) -> int:
         return lib.stinger_outdegree(self._p, v)
 
@@ -382,6 +390,18 @@ class Forest:
         """insertEdgeStreamingBC for an edge already added with Graph.insert_edge."""
         lib.insertEdgeStreamingBC(self._p, g._p, u, v, _p64(self.roots), self.k, self.nv, 1, ctx._p, 0,
                                   granularity)
+
+    def delete_edge(self, g: Graph, ctx: Context, u: int, v: int) -> dict:
+        """Removes (u,v) from the graph and updates every root; returns the
+        root counts of the reference's three deletion cases."""
+        info = ExtraInfoC()
+        _check(lib.sbcDeleteEdge(self._p, g._p, ctx._p, u, v, C.byref(info)))
+        return {"sameLevel": info.sameLevel, "adjacent": info.adjacent, "movement": info.movement}
+
+    def update_deleted(self, g: Graph, ctx: Context, u: int, v: int) -> dict:
+        """deleteEdgeStreamingBC for an edge already removed with Graph.remove_edge."""
+        info = lib.deleteEdgeStreamingBC(self._p, g._p, u, v, _p64(self.roots), self.k, self.nv, 1, ctx._p, 0, FINE)
+        return {"sameLevel": info.sameLevel, "adjacent": info.adjacent, "movement": info.movement}
 
     def insert_batched(self, g: Graph, ctx: Context, u, v, cfg: BatchConfig, records: bool = False):
         u = np.ascontiguousarray(u, dtype=np.int64)

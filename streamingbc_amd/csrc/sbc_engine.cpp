@@ -83,6 +83,7 @@ struct stinger {
     std::vector<int2> htab;
     int4* d_hub = nullptr;
     int2* d_htab = nullptr;
+    std::vector<int32_t> p_hub;  // vertices whose table changed since the last upload (edge removals)
 };
 
 // Vertices with more initial neighbours than this get a sorted index (the
@@ -203,6 +204,27 @@ static SbcVInfo vinfo_of(const stinger* G, int32_t v)
     return x;
 }
 
+// Sends the tables that edge removals rebuilt (entry and table slots of each
+// vertex in p_hub) and waits for the copies, so the host tables may change
+// again right after.
+static sbcStatus hub_flush(stinger* G, hipStream_t st)
+{
+    if (G->p_hub.empty() || !G->d_hub) {
+        G->p_hub.clear();
+        return SBC_OK;
+    }
+    for (int32_t v : G->p_hub) {
+        const int4 hb = G->hub[v];
+        HIPCHK(hipMemcpyAsync(G->d_hub + v, &G->hub[v], sizeof(int4), hipMemcpyHostToDevice, st));
+        if (hb.z > 0)
+            HIPCHK(hipMemcpyAsync(G->d_htab + hb.x, &G->htab[hb.x], sizeof(int2) * ((size_t)1 << hb.z),
+                                  hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    G->p_hub.clear();
+    return SBC_OK;
+}
+
 static sbcStatus graph_upload_full(stinger* G, hipStream_t st)
 {
     HIPCHK(hipStreamSynchronize(st));
@@ -230,7 +252,7 @@ static sbcStatus graph_upload_full(stinger* G, hipStream_t st)
     HIPCHK(hipMemcpy(G->d_seq, G->seq.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice));
     G->full_dirty = false;
     G->p_slot.clear(); G->p_col.clear(); G->p_seq.clear(); G->p_v.clear();
-    return SBC_OK;
+    return hub_flush(G, st);
 }
 
 // Sends pending appends (legacy stinger_insert_edge path) to the device.
@@ -261,7 +283,7 @@ static sbcStatus graph_flush(stinger* G, hipStream_t st)
         HIPCHK(sbc_launch_append(&A, SBC_APPEND_INLINE, st));
     }
     G->p_slot.clear(); G->p_col.clear(); G->p_seq.clear(); G->p_v.clear();
-    return SBC_OK;
+    return hub_flush(G, st);
 }
 
 static void graph_free(stinger* G)
@@ -319,6 +341,59 @@ extern "C" int stinger_insert_edge(struct stinger* G, int64_t type, int64_t from
         G->p_v.push_back((int32_t)from);
     }
     return 1;
+}
+
+// Drops position p of v's list, keeping the order of the rest (the
+// reference's STINGER leaves a hole there and skips it in traversals, so the
+// remaining neighbours are visited in the same order).  A lookup table over
+// v's list is rebuilt when p lies in its indexed part.
+static void graph_remove_at(stinger* G, int32_t v, int32_t p)
+{
+    const int32_t s0 = G->start[v], d = G->deg[v];
+    for (int32_t i = p; i + 1 < d; i++) {
+        G->col[s0 + i] = G->col[s0 + i + 1];
+        G->seq[s0 + i] = G->seq[s0 + i + 1];
+    }
+    G->col[s0 + d - 1] = 0;
+    G->seq[s0 + d - 1] = kNoSeq;
+    G->deg[v] = d - 1;
+    G->deg0[v] = d - 1;  // every remaining neighbour is visible
+    if (!G->full_dirty) {
+        for (int32_t i = p; i < d; i++) {
+            G->p_slot.push_back(s0 + i);
+            G->p_col.push_back(G->col[s0 + i]);
+            G->p_seq.push_back(G->seq[s0 + i]);
+        }
+        G->p_v.push_back(v);
+    }
+    int4& hb = G->hub[v];
+    if (hb.y > 0 && p < hb.y) {
+        const int32_t n = hb.y - 1;  // indexed entries left, at positions 0..n-1
+        const uint32_t mask = (1u << hb.z) - 1u;
+        int2* tab = &G->htab[hb.x];
+        for (uint32_t i = 0; i <= mask; i++) tab[i] = int2{0, 0};
+        for (int32_t i = 0; i < n; i++) {
+            const int32_t w = G->col[s0 + i];
+            uint32_t h = ((uint32_t)w * 0x9E3779B1u) >> (32 - hb.z);
+            while (tab[h].x != 0) h = (h + 1) & mask;
+            tab[h] = int2{w + 1, i};
+        }
+        hb.y = n;
+        G->p_hub.push_back(v);
+    }
+}
+
+extern "C" int stinger_remove_edge(struct stinger* G, int64_t type, int64_t from, int64_t to)
+{
+    (void)type;
+    if (!G || from == to || from < 0 || to < 0 || from >= G->nv || to >= G->nv) return -1;
+    const int32_t* a = &G->col[G->start[from]];
+    for (int32_t i = 0; i < G->deg[from]; i++)
+        if (a[i] == to) {
+            graph_remove_at(G, (int32_t)from, i);
+            return 1;
+        }
+    return -1;
 }
 
 extern "C" int64_t stinger_outdegree(struct stinger* G, int64_t v)
@@ -551,6 +626,8 @@ struct extraArraysPerThread {
     int64_t bucket_blocks = 0;
     int32_t* d_hubq = nullptr;  // per workgroup: hubs queued beyond the LDS queue
     int64_t hubq_blocks = 0;
+    int32_t* d_del = nullptr;   // edge deletion: 4 counts, then the affected root indices
+    int64_t del_k = 0;
     WinStage win;
     GraphCache gc;
     int refs = 1;
@@ -586,6 +663,7 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_wtot) (void)hipFree(C->d_wtot);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
+    if (C->d_del) (void)hipFree(C->d_del);
     for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
                     (void*)C->d_sheavy})
         if (q) (void)hipFree(q);
@@ -885,6 +963,96 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
     if (!eAPT || run_insert_plain(forest, sStinger, eAPT[0], newU, newV, granularity) != SBC_OK)
         legacy_abort("insertEdgeStreamingBC");
     return r;
+}
+
+// ------------------------------------------------------------ edge deletion
+
+// The update after (u, v) left the graph in both directions.  Per root the
+// reference dispatches on the endpoints' levels (src/streamingbc.c:258-416):
+// same level, nothing to do; otherwise the adjacent case
+// (removeEdgeWithoutMovementBrandes, src/streamingbc_internal.c:607-868) or
+// the movement case (moveDownTreeBrandes, 880-1401).  Both end in the tree
+// of the post-deletion graph, so the affected roots are rebuilt by the static
+// pass over that graph (dist and sigma exact, delta in the static pass's
+// order), and BC is folded again from every tree.  counts: same-level,
+// adjacent and movement roots, as the reference's StreamingExtraInfo.
+static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* C, int64_t u, int64_t v,
+                            int64_t counts[3])
+{
+    if (!forest || !G || !C) return set_err(SBC_ERR_ARG, "null argument");
+    sbcForestDev* F = forest->dev;
+    if (!F->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (F->nv != G->nv) return set_err(SBC_ERR_ARG, "forest NV %ld != graph NV %ld", (long)F->nv, (long)G->nv);
+    if (u < 0 || v < 0 || u >= F->nv || v >= F->nv || u == v) return set_err(SBC_ERR_ARG, "bad edge (%ld,%ld)", (long)u, (long)v);
+    sbcStatus st = graph_flush(G, C->stream);
+    if (st) return st;
+    if (C->del_k < F->K) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (C->d_del) HIPCHK(hipFree(C->d_del));
+        C->d_del = nullptr;
+        if ((st = dalloc(&C->d_del, (size_t)(4 + F->K)))) return st;
+        C->del_k = F->K;
+    }
+    SbcParams P = base_params(F, G, C);
+    HIPCHK(sbc_launch_reset(C->d_del, 4, C->stream));
+    HIPCHK(sbc_launch_delete_classify(&P, (int32_t)u, (int32_t)v, C->d_del, C->stream));
+    P.sroots = C->d_del + 4;
+    P.scnt = C->d_del;
+    HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+    hipEvent_t e0;
+    if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
+    int grid = (int)(F->K < 4LL * C->num_cu ? F->K : 4LL * C->num_cu);
+    HIPCHK(sbc_launch_static(&P, 16, grid, C->stream));
+    if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+    if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
+    HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
+    if ((st = ev_end(C, KSTAT_BC, e0))) return st;
+    int32_t h[4];
+    HIPCHK(hipMemcpyAsync(h, C->d_del, sizeof(h), hipMemcpyDeviceToHost, C->stream));
+    if ((st = check_kernel_err(F, C))) return st;
+    counts[0] = h[1];
+    counts[1] = h[2];
+    counts[2] = h[3];
+    return SBC_OK;
+}
+
+extern "C" StreamingExtraInfo deleteEdgeStreamingBC(bcForest* forest, struct stinger* sStinger, int64_t oldU,
+                                                    int64_t oldV, int64_t* rootArrayForApproximation, int64_t NK,
+                                                    int64_t NV, int64_t NT, extraArraysPerThread** eAPT,
+                                                    uint32_t loadBalancing, uint32_t granularity)
+{
+    (void)rootArrayForApproximation; (void)NT; (void)loadBalancing; (void)granularity;
+    StreamingExtraInfo r = {0, 0, 0, 0};
+    if (!forest || NK != forest->dev->K || NV != forest->dev->nv) {
+        set_err(SBC_ERR_ARG, "NK/NV do not match the forest");
+        legacy_abort("deleteEdgeStreamingBC");
+    }
+    int64_t c[3];
+    if (!eAPT || run_delete(forest, sStinger, eAPT[0], oldU, oldV, c) != SBC_OK) legacy_abort("deleteEdgeStreamingBC");
+    r.sameLevel = c[0];
+    r.adjacent = c[1];
+    r.movement = c[2];
+    return r;
+}
+
+extern "C" sbcStatus sbcDeleteEdge(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT, int64_t u,
+                                   int64_t v, StreamingExtraInfo* info)
+{
+    if (!forest || !G || !eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null argument");
+    if (!forest->dev->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (u < 0 || v < 0 || u >= G->nv || v >= G->nv || u == v) return set_err(SBC_ERR_ARG, "bad edge (%ld,%ld)", (long)u, (long)v);
+    if (!graph_has_edge(G, (int32_t)u, (int32_t)v)) return set_err(SBC_ERR_ABSENT, "edge (%ld,%ld) absent", (long)u, (long)v);
+    stinger_remove_edge(G, 0, u, v);
+    stinger_remove_edge(G, 0, v, u);
+    int64_t c[3];
+    sbcStatus st = run_delete(forest, G, eAPT[0], u, v, c);
+    if (st == SBC_OK && info) {
+        info->sameLevel = c[0];
+        info->adjacent = c[1];
+        info->movement = c[2];
+        info->connectedComponents = 0;
+    }
+    return st;
 }
 
 extern "C" sbcStatus streamingBCLastRecords(extraArraysPerThread** eAPT, sbcItemRecord* out, int64_t K)

@@ -121,6 +121,10 @@ struct SbcParams {
     int64_t park_scan;
     const int32_t* start_item;  // [K] first item per root, or nullptr (0)
     int32_t qlen;               // queue slots in use (K, or the roots resumed)
+    // static pass over some roots only: root indices sroots[0, *scnt), or
+    // every root when sroots is nullptr
+    const int32_t* sroots;
+    const int32_t* scnt;
 };
 
 // A parked item (one per root slot), in global memory.  cnt/stat have the
@@ -191,6 +195,10 @@ hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, i
 hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
+// Per root, the dispatch of deleting (u, v) (already removed from the graph):
+// out[0] affected roots, listed at out[4..]; out[1..3] same-level, adjacent
+// and movement cases.  out[0..3] must be zero.
+hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out, hipStream_t st);
 hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
 hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);
 }

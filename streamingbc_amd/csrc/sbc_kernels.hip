@@ -355,6 +355,14 @@ constexpr int SCAN_UNROLL = SBC_SCAN_UNROLL;
 #define SBC_WAVE_UNROLL 2
 #endif
 constexpr int WAVE_UNROLL = SBC_WAVE_UNROLL;
+// (source, hub) lookups whose first table slots one thread loads together
+#ifndef SBC_PROBE_UNROLL
+#define SBC_PROBE_UNROLL 2
+#endif
+constexpr int PROBE_UNROLL = SBC_PROBE_UNROLL;
+#ifndef SBC_SET_COMPACT
+#define SBC_SET_COMPACT 1
+#endif
 constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
@@ -382,6 +390,8 @@ struct alignas(16) HubShared {
     int dup;              // a team-sorted bucket holds a position twice
     int plen[HUB_CAP];    // visible list length of each lookup hub
     int pstart[HUB_CAP];  // first slot of each lookup hub
+    int poff[HUB_CAP];    // each lookup hub's table: first slot in htab
+    uint8_t plog[HUB_CAP];  // and log2 of its size
     int hlen[HUB_CAP];    // visible list length of each queued hub of the pass
     int hstart[HUB_CAP];  // first slot of each queued hub of the pass
     int hidx[HUB_CAP];    // indexed neighbours of each hub queued in LDS by an ascent level (hub table count)
@@ -682,10 +692,14 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
     if (srcset) {
-        // neighbour ids of up to 8 chunks in flight, then chunk by chunk:
-        // only members of the level's source set can give a term or need a
-        // mark, and a chunk without one is skipped
+        // neighbour ids of up to 8 chunks in flight; only members of the
+        // level's source set can give a term or need a mark.  When the
+        // chunks hold at most 64 members, they are packed in list order into
+        // the wavefront's LDS row and handled one per lane, so their state
+        // bytes and terms are loaded in one round trip each; otherwise chunk
+        // by chunk, skipping chunks without members
         constexpr int SU = 8;
+        int32_t* mrow = reinterpret_cast<int32_t*>(sbc_wrow[threadIdx.x >> 6]);
         for (int ch0 = 0; ch0 < nch; ch0 += SU) {
             int32_t cc[SU];
 #pragma unroll
@@ -693,9 +707,38 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
                 const int e = (ch0 + u) * 64 + lane;
                 cc[u] = (ch0 + u) < nch && e < len ? P.col[start + e] : -1;
             }
+            bool okk[SU];
+            int tot = 0;
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
-                const bool ok = cc[u] >= 0 && shash_has(*srcset, cc[u]);
+                okk[u] = cc[u] >= 0 && shash_has(*srcset, cc[u]);
+                tot += __popcll(ballot(okk[u]));
+            }
+            if (tot == 0) continue;
+            if (SBC_SET_COMPACT && tot <= 64) {
+                int nm = 0;
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    const uint64_t m = ballot(okk[u]);
+                    if (okk[u]) mrow[nm + prefix_in_wave(m)] = cc[u];
+                    nm += __popcll(m);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const bool ok = lane < nm;
+                const int32_t c = ok ? mrow[lane] : 0;
+                const uint8_t ww = ok ? ld_sb(&X.sb[c]) : (uint8_t)0;
+                bool appU;
+                const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok, c, ww, appU);
+                const int pu = wave_append(appU, &ts.cnt[2]);
+                if (pu >= 0) X.list[pu] = c;
+                acc = wave_ordered_add(acc, t.child, t.add, t.hsub, t.sub);
+                continue;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const bool ok = okk[u];
                 if (ballot(ok) == 0) continue;
                 const uint8_t ww = ok ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
                 bool appU;
@@ -977,16 +1020,23 @@ __device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item&
 
 // Position of w in the indexed (initial) part of a vertex's adjacency list,
 // looked up in the vertex's open-addressing table, or -1.
-__device__ __forceinline__ int hub_find(const int2* __restrict__ tab, int logT, int32_t w)
+__device__ __forceinline__ uint32_t hub_slot(int32_t w, int logT) { return ((uint32_t)w * 0x9E3779B1u) >> (32 - logT); }
+
+// The same search from slot h on (linear probing).
+__device__ __forceinline__ int hub_find_from(const int2* __restrict__ tab, int logT, int32_t w, uint32_t h)
 {
     const uint32_t mask = (1u << logT) - 1u;
-    uint32_t h = ((uint32_t)w * 0x9E3779B1u) >> (32 - logT);
     for (;;) {
         const int2 e = tab[h];
         if (e.x == w + 1) return e.y;
         if (e.x == 0) return -1;
         h = (h + 1) & mask;
     }
+}
+
+__device__ __forceinline__ int hub_find(const int2* __restrict__ tab, int logT, int32_t w)
+{
+    return hub_find_from(tab, logT, w, hub_slot(w, logT));
 }
 
 // Processed-vertex ranges of one ascent level: descent frontier list[dA..dB),
@@ -1202,6 +1252,9 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     __syncthreads();
     for (int hi = tid; hi < np; hi += NT) {
         const int32_t k = hs.plist[hi];
+        const int4 hb = P.hub[k];
+        hs.poff[hi] = hb.x;
+        hs.plog[hi] = (uint8_t)hb.z;
         int h = phash_slot(k);
         while (atomicCAS(&hs.phk[h], 0, k + 1) != 0) h = (h + 1) & (PHASH - 1);
         hs.phv[h] = hi;
@@ -1264,22 +1317,46 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             if (ls.isD[own]) {
                 mark_parent(X, ts, ws, lev - 1);
             } else {
-                const int32_t k = hs.plist[hi2];
-                const int4 hb = P.hub[k];
-                const int pos = hub_find(P.htab + hb.x, hb.z, ws);
-                if (pos >= 0) probe_term(P, X, hs, bk, k, hi2, lev, ws, pos);
+                const int pos = hub_find(P.htab + hs.poff[hi2], hs.plog[hi2], ws);
+                if (pos >= 0) probe_term(P, X, hs, bk, hs.plist[hi2], hi2, lev, ws, pos);
             }
         }
-        // table-mode sources against every hub
-        for (int t = tid; t < nt * np; t += NT) {
-            const int si = t / np, hi2 = t - si * np;
-            const int32_t ws = ls.ovf[si];
-            const int32_t k = hs.plist[hi2];
-            const int4 hb = P.hub[k];
-            const int pos = hub_find(P.htab + hb.x, hb.z, ws);
-            if (pos < 0) continue;
-            if (ls.eflag[si]) mark_parent(X, ts, ws, lev - 1);
-            else probe_term(P, X, hs, bk, k, hi2, lev, ws, pos);
+        // table-mode sources against every hub: PROBE_UNROLL (source, hub)
+        // pairs per thread, whose first table slots are loaded together
+        const int npairs = nt * np;
+        for (int t0 = tid; t0 < npairs; t0 += PROBE_UNROLL * NT) {
+            int si[PROBE_UNROLL], hi2[PROBE_UNROLL];
+            int2 e[PROBE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < PROBE_UNROLL; ++u) {
+                const int t = t0 + u * NT;
+                si[u] = -1;
+                hi2[u] = 0;
+                e[u] = int2{0, 0};
+                if (t < npairs) {
+                    si[u] = t / np;
+                    hi2[u] = t - si[u] * np;
+                    e[u] = P.htab[hs.poff[hi2[u]] + hub_slot(ls.ovf[si[u]], hs.plog[hi2[u]])];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PROBE_UNROLL; ++u) {
+                if (si[u] < 0) continue;
+                const int32_t ws = ls.ovf[si[u]];
+                int pos;
+                if (e[u].x == ws + 1) {
+                    pos = e[u].y;
+                } else if (e[u].x == 0) {
+                    pos = -1;
+                } else {
+                    const int lg = hs.plog[hi2[u]];
+                    pos = hub_find_from(P.htab + hs.poff[hi2[u]], lg, ws,
+                                        (hub_slot(ws, lg) + 1u) & ((1u << lg) - 1u));
+                }
+                if (pos < 0) continue;
+                if (ls.eflag[si[u]]) mark_parent(X, ts, ws, lev - 1);
+                else probe_term(P, X, hs, bk, hs.plist[hi2[u]], hi2[u], lev, ws, pos);
+            }
         }
         __syncthreads();
     }
@@ -2397,12 +2474,14 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
     const int wave = threadIdx.x >> 6, lane = lane_id();
     const bool leader = threadIdx.x == 0;
     const int nv = P.nv;
+    const int nq = P.sroots ? *P.scnt : P.K;
     for (;;) {
         if (leader) ts.cnt[3] = atomicAdd(&P.qctr[0], 1);
         __syncthreads();
-        const int ri = ts.cnt[3];
+        const int qi = ts.cnt[3];
         __syncthreads();
-        if (ri >= P.K) break;
+        if (qi >= nq) break;
+        const int ri = P.sroots ? P.sroots[qi] : qi;
         const size_t off = (size_t)ri * (size_t)nv;
         int32_t* dist = P.dist + off;
         uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
@@ -2523,6 +2602,32 @@ __global__ void sbc_bc_reduce_kernel(SbcParams P)
     P.bc[v] = 0.0 + acc;
 }
 
+// Edge deletion, one thread per root, after (u, v) left the graph: the
+// reference's per-root dispatch (src/streamingbc.c:310-345).  Endpoints on
+// the same level (or both unreachable): the edge is on no shortest path and
+// the tree stands.  Otherwise the deeper endpoint c keeps its level if
+// another neighbour lies one level up (adjacent case) or moves down
+// (movement case); either way the root's tree is rebuilt by the static pass.
+__global__ void sbc_delete_classify_kernel(SbcParams P, int32_t u, int32_t v, int32_t* out)
+{
+    const int ri = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ri >= P.K) return;
+    const int32_t* dist = P.dist + (size_t)ri * (size_t)P.nv;
+    const int32_t du = dist[u], dv = dist[v];
+    if (du == dv) {
+        atomicAdd(&out[1], 1);
+        return;
+    }
+    const int32_t c = du > dv ? u : v;
+    const int32_t up = (du > dv ? du : dv) - 1;
+    const SbcVInfo vi = P.vinfo[c];
+    int extra = 0;
+    for (int e = 0; e < vi.deg && !extra; ++e)
+        if (dist[P.col[vi.start + e]] == up) extra = 1;
+    atomicAdd(&out[extra ? 2 : 3], 1);
+    out[4 + atomicAdd(&out[0], 1)] = ri;
+}
+
 // Clears the work-queue counters (a kernel, not a memset, so it can sit in
 // a captured graph).
 __global__ void sbc_reset_kernel(int32_t* q, int n)
@@ -2609,6 +2714,14 @@ extern "C" hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st)
 {
     const int bs = 256;
     hipLaunchKernelGGL(sbc_bc_reduce_kernel, dim3((P->nv + bs - 1) / bs), dim3(bs), 0, st, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out,
+                                                 hipStream_t st)
+{
+    const int bs = 256;
+    hipLaunchKernelGGL(sbc_delete_classify_kernel, dim3((P->K + bs - 1) / bs), dim3(bs), 0, st, *P, u, v, out);
     return hipGetLastError();
 }
 

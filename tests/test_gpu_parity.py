@@ -432,3 +432,142 @@ def test_hub_lookup_level3_leaf_roots():
             assert np.array_equal(rec[i][key], rec_o[key]), f"insertion {i} {key}"
     _compare_trees(f, o, "level-3 lookups")
     assert_bc_close(f.bc(), o.bc())
+
+
+# ------------------------------------------------------------ edge deletion
+# The device deletion rebuilds each affected root with the static pass, so
+# after any mix of insertions and deletions the per-root trees equal the
+# oracle's bit for bit (same adjacency order on both sides: removal keeps the
+# order, insertion appends); BC is folded again from the trees, so it agrees
+# within 1e-9 with the oracle's incremental BC and the reference's fixtures.
+
+import hashlib
+import os
+
+from sbc_testutil import GOLDEN
+
+
+def _apply_ops(f, g, ctx, o, ops):
+    counts = []
+    for op, a, b in ops:
+        a, b = int(a), int(b)
+        if op == 1:
+            f.insert_edge(g, ctx, a, b, sbc.FINE)
+            o.insert(a, b, O_FINE)
+            counts.append((-1, -1, -1))
+        else:
+            c = f.delete_edge(g, ctx, a, b)
+            co = o.delete(a, b)
+            got = (c["sameLevel"], c["adjacent"], c["movement"])
+            assert got == tuple(int(x) for x in co), f"case counts of deleting ({a},{b}): {got} vs {tuple(co)}"
+            counts.append(got)
+    return np.array(counts, dtype=np.int64)
+
+
+@pytest.mark.parametrize("name", ["karate", "dolphins"])
+def test_delete_matches_oracle_and_reference(name):
+    fx = np.load(os.path.join(GOLDEN, f"refdel_{name}.npz"))
+    hg = sbc.read_dimacs(os.path.join(GOLDEN, "graphs", f"{name}.graph"))
+    roots = np.arange(hg.nv)
+    print(f"[delete] {name} ops={len(fx['ops'])}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    for i, op in enumerate(fx["ops"]):
+        c = _apply_ops(f, g, ctx, o, [op])[0]
+        if op[0] == 0:
+            assert np.array_equal(c, fx["fine_counts"][i]), f"counts of operation {i} vs the reference"
+        assert_bc_close(f.bc(), fx["fine_bc"][i + 1], what=f"BC after operation {i} vs the reference")
+        assert_bc_close(f.bc(), o.bc(), what=f"BC after operation {i} vs the oracle")
+    _compare_trees(f, o, f"{name} after deletions")
+    lv, sg, _ = f.trees()
+    assert np.array_equal(lv, fx["fine_level"]) and np.array_equal(sg, fx["fine_sigma"])
+
+
+def test_delete_rmat10_matches_reference_hashes():
+    fx = np.load(os.path.join(GOLDEN, "refdel_rmat10.npz"))
+    hg = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
+    assert hg.nv == int(fx["nv"])
+    roots = np.arange(hg.nv)
+    print(f"[delete] rmat10 deletions={len(fx['ops'])}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    for i, (op, a, b) in enumerate(fx["ops"]):
+        c = f.delete_edge(g, ctx, int(a), int(b))
+        assert (c["sameLevel"], c["adjacent"], c["movement"]) == tuple(int(x) for x in fx["coarse_counts"][i])
+        if i % 8 == 0 or i == len(fx["ops"]) - 1:
+            lv, sg, _ = f.trees()
+            h = hashlib.sha256(lv.tobytes() + sg.tobytes()).digest()
+            assert h == fx["coarse_tree_sha256"][i].tobytes(), f"trees differ after deletion {i}"
+        if (i + 1) % 8 == 0:
+            assert_bc_close(f.bc(), fx["coarse_bc"][(i + 1) // 8], what=f"BC after deletion {i}")
+
+
+def _hub_graph():
+    hg = sbc.gen_rmat(13, 16 * 8192, seed=2, keep_lcc=True)
+    deg = np.diff(hg.rowptr)
+    assert deg.max() > 700, "the graph must hold vertices with lookup tables (more than 512 neighbours)"
+    return hg, deg
+
+
+def test_delete_hub_neighbours_then_insert_matches_oracle():
+    """Deleting entries of a hub's list rebuilds its lookup table; later
+    insertions, plain and batched, look neighbours up in it."""
+    hg, deg = _hub_graph()
+    roots = hg.pick_roots(24, 4)
+    hub = int(np.argmax(deg))
+    nb = hg.col[hg.rowptr[hub]:hg.rowptr[hub + 1]]
+    print(f"[delete] hub {hub} degree {deg[hub]} nv={hg.nv} k={len(roots)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    picks = [0, 1, 7, len(nb) // 2, len(nb) - 2]
+    _apply_ops(f, g, ctx, o, [(0, hub, int(nb[p])) for p in picks])
+    _compare_trees(f, o, "after hub deletions")
+    u, v = hg.insertions(40, 6)
+    far = next(x for x in range(1, hg.nv) if x != hub and x not in set(nb.tolist()) and deg[x] > 0)
+    _apply_ops(f, g, ctx, o, [(1, hub, far)] + [(1, int(a), int(b)) for a, b in zip(u[:10], v[:10])])
+    _compare_trees(f, o, "plain insertions after hub deletions")
+    f.insert_batched(g, ctx, u[10:30], v[10:30], sbc.BatchConfig(window=8, block_threads=1024))
+    for a, b in zip(u[10:30], v[10:30]):
+        o.insert(int(a), int(b), O_FINE)
+    _compare_trees(f, o, "batched insertions after hub deletions")
+    # an edge appended to the hub after its table was built, one in the
+    # table's part of the list, and three inserted edges elsewhere
+    later = [(0, int(a), int(b)) for a, b in zip(u[:3], v[:3])]
+    _apply_ops(f, g, ctx, o, [(0, hub, far), (0, hub, int(nb[3]))] + later)
+    f.insert_batched(g, ctx, u[30:], v[30:], sbc.BatchConfig(window=5, block_threads=1024))
+    for a, b in zip(u[30:], v[30:]):
+        o.insert(int(a), int(b), O_FINE)
+    _compare_trees(f, o, "after the second round")
+    assert_bc_close(f.bc(), o.bc())
+
+
+def test_delete_disconnects_and_legacy_form():
+    """A bridge deletion leaves vertices unreachable (LONG_MAX sentinels); the
+    legacy two-step form gives the same trees; deleting an absent edge is
+    refused before anything changes."""
+    a = [(i, i + 1) for i in range(1, 8)]
+    hg = sbc.graph_from_edges(9, np.array([x for x, y in a]), np.array([y for x, y in a]))
+    roots = np.arange(hg.nv)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    _apply_ops(f, g, ctx, o, [(0, 4, 5)])
+    _compare_trees(f, o, "path cut")
+    lv, sg, _ = f.tree(1)
+    assert np.all(lv[5:9] == LONG_MAX) and np.all(sg[5:9] == LONG_MAX)
+    bc = f.bc()
+    for k in range(1, 5):
+        assert bc[k] == 2 * (k - 1) * (4 - k)
+    assert g.remove_edge(2, 3) == 1 and g.remove_edge(3, 2) == 1
+    c = f.update_deleted(g, ctx, 2, 3)
+    co = o.delete(2, 3)
+    assert (c["sameLevel"], c["adjacent"], c["movement"]) == tuple(int(x) for x in co)
+    _compare_trees(f, o, "legacy deletion")
+    before = f.trees()
+    with pytest.raises(sbc.SbcError) as ei:
+        f.delete_edge(g, ctx, 2, 3)
+    assert ei.value.status == sbc.SBC_ERR_ABSENT
+    after = f.trees()
+    assert all(np.array_equal(x, y) for x, y in zip(before, after))
+    assert g.degree(2) == 1 and g.degree(3) == 1

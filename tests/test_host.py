@@ -113,3 +113,18 @@ def test_footprint_formula():
     # 41 B per (root, vertex) + level offsets + BC
     fp = sbc.forest_footprint(1 << 20, 256)
     assert 256 * (1 << 20) * 41 <= fp < 256 * (1 << 20) * 41 + 64 << 20
+
+
+def test_remove_edge_return_codes():
+    """stinger_remove_edge: 1 removed, -1 absent / self loop / bad id; an
+    insertion after a removal is accepted again."""
+    hg = sbc.read_dimacs(os.path.join(ROOT, "tests", "golden", "graphs", "karate.graph"))
+    g = sbc.Graph(hg)
+    d1, d2 = g.degree(1), g.degree(2)
+    assert g.remove_edge(1, 2) == 1 and g.remove_edge(2, 1) == 1
+    assert g.degree(1) == d1 - 1 and g.degree(2) == d2 - 1
+    assert g.remove_edge(1, 2) == -1
+    assert g.remove_edge(3, 3) == -1
+    assert g.remove_edge(1, hg.nv) == -1
+    assert g.insert_edge(1, 2) == 1 and g.degree(1) == d1
+    g.close()

@@ -272,3 +272,142 @@ def test_sigma_sentinels():
     assert lv[0] == 0 and sg[0] == 1
     assert np.all(lv[1:] == LONG_MAX) and np.all(sg[1:] == LONG_MAX) and np.all(dl == 0)
     assert not o.sigma_overflowed()
+
+
+# ------------------------------------------------------------ edge deletion
+# orc_delete is pinned by the reference's deleteEdgeStreamingBC (fixtures
+# refdel_*.npz written by the compiled reference: case counts exact, level and
+# sigma exact, delta and BC within 1e-9), by a static recompute of the
+# post-deletion graph, and by closed forms on paths.
+
+def _replay(o, ops, mode):
+    counts = []
+    for op, a, b in ops:
+        if op == 1:
+            o.insert(int(a), int(b), mode)
+            counts.append(np.full(3, -1, dtype=np.int64))
+        else:
+            counts.append(o.delete(int(a), int(b)))
+        yield np.array(counts[-1])
+
+
+@pytest.mark.parametrize("name", ["karate", "dolphins"])
+@pytest.mark.parametrize("mode,key", [(COARSE, "coarse"), (FINE, "fine")])
+def test_oracle_delete_matches_reference_golden(name, mode, key):
+    g = np.load(os.path.join(GOLDEN, f"refdel_{name}.npz"))
+    hg = load_graph(name)
+    o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+    o.static(PUSH)
+    assert np.array_equal(o.bc(), g[f"{key}_bc"][0])
+    assert (g["ops"][:, 0] == 0).sum() >= 10 and (g["ops"][:, 0] == 1).sum() >= 3
+    for i, c in enumerate(_replay(o, g["ops"], mode)):
+        assert np.array_equal(c, g[f"{key}_counts"][i]), f"case counts of operation {i}"
+        assert_bc_close(o.bc(), g[f"{key}_bc"][i + 1], what=f"BC after operation {i}")
+    lv, sg, dl = o.trees()
+    assert np.array_equal(lv, g[f"{key}_level"])
+    assert np.array_equal(sg, g[f"{key}_sigma"])
+    assert_bc_close(dl, g[f"{key}_delta"], what="delta")
+
+
+def test_oracle_delete_matches_reference_golden_rmat10():
+    g = np.load(os.path.join(GOLDEN, "refdel_rmat10.npz"))
+    hg = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
+    assert hg.nv == int(g["nv"]) and hg.ne == int(g["ne"]), "generator changed: fixtures are stale"
+    o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+    o.static(PUSH)
+    assert np.array_equal(o.bc(), g["coarse_bc"][0])
+    kinds = np.zeros(3, dtype=np.int64)
+    for i, c in enumerate(_replay(o, g["ops"], COARSE)):
+        assert np.array_equal(c, g["coarse_counts"][i]), f"case counts of deletion {i}"
+        kinds += c > 0
+        if i % 8 == 0 or i == len(g["ops"]) - 1:
+            lv, sg, _ = o.trees()
+            h = hashlib.sha256(lv.tobytes() + sg.tobytes()).digest()
+            assert h == g["coarse_tree_sha256"][i].tobytes(), f"trees differ after deletion {i}"
+        if (i + 1) % 8 == 0:
+            assert_bc_close(o.bc(), g["coarse_bc"][(i + 1) // 8], what=f"BC after deletion {i}")
+    assert np.all(kinds > 0), "the stream must hold same-level, adjacent and movement roots"
+    lv, sg, dl = o.trees()
+    assert np.array_equal(lv, g["coarse_level"])
+    assert np.array_equal(sg, g["coarse_sigma"])
+    assert_bc_close(dl[:64], g["coarse_delta"], what="delta")
+
+
+def test_delete_equals_static_recompute():
+    """After deletions alone, every tree equals a fresh pull-order static pass
+    over the final graph bit for bit (same-level deletions remove no term of
+    any pull sum), and BC equals it within rounding."""
+    hg = sbc.gen_chung_lu(1500, 2.3, 8.0, seed=2, keep_lcc=True)
+    roots = hg.pick_roots(40, 3)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    rng = np.random.default_rng(5)
+    src = np.repeat(np.arange(hg.nv), np.diff(hg.rowptr))
+    edges = [(int(a), int(b)) for a, b in zip(src, hg.col) if a < b]
+    gone = [edges[i] for i in rng.choice(len(edges), 80, replace=False)]
+    tot = np.zeros(3, dtype=np.int64)
+    for a, b in gone:
+        tot += o.delete(a, b)
+    assert tot.sum() == 80 * len(roots) and tot[1] > 0 and tot[2] > 0
+    keep = sorted(set(edges) - set(gone))
+    # same adjacency order as the oracle's graph: the original lists minus the deleted entries
+    dead = set(gone) | {(b, a) for a, b in gone}
+    col = [int(x) for a, x in zip(src, hg.col) if (int(a), int(x)) not in dead]
+    rowptr = np.zeros(hg.nv + 1, dtype=np.int64)
+    np.add.at(rowptr, np.array([a for a, x in zip(src, hg.col) if (int(a), int(x)) not in dead]) + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    assert rowptr[-1] == 2 * len(keep)
+    s = Oracle(rowptr, np.array(col, dtype=np.int64), roots)
+    s.static(PULL)
+    lv, sg, dl = o.trees()
+    ls, ss, ds = s.trees()
+    assert np.array_equal(lv, ls) and np.array_equal(sg, ss)
+    assert np.array_equal(dl, ds)
+    assert_bc_close(o.bc(), s.bc())
+
+
+def test_delete_closed_forms():
+    """Cutting path 1..9 at (4,5) leaves paths of 4 and 5 vertices, whose BC is
+    2(k-1)(m-k) for the k-th vertex of an m-vertex path; cutting a leaf of a
+    star lowers the centre's BC from (n-1)(n-2) to (n-2)(n-3)."""
+    n = 9
+    path = from_edges(n, [(i, i + 1) for i in range(1, n)])
+    for mode in (COARSE, FINE):
+        o = Oracle(path.rowptr, path.col, np.arange(path.nv))
+        o.static(PUSH)
+        c = o.delete(4, 5)
+        bc = o.bc()
+        for k in range(1, 5):
+            assert bc[k] == 2 * (k - 1) * (4 - k)
+        for k in range(5, 10):
+            assert bc[k] == 2 * (k - 5) * (9 - k)
+        # roots 1..9: all see (4,5) on different levels; no other parent: movement
+        assert c[0] == 1 and c[1] == 0 and c[2] == 9
+        star = from_edges(n, [(1, i) for i in range(2, n + 1)])
+        o = Oracle(star.rowptr, star.col, np.arange(star.nv))
+        o.static(PUSH)
+        o.delete(1, 9)
+        bc = o.bc()
+        assert bc[1] == (n - 2) * (n - 3) and np.all(bc[2:] == 0)
+        lv, sg, _ = o.tree(9)
+        assert lv[9] == 0 and np.all(lv[1:9] == LONG_MAX) and np.all(sg[1:9] == LONG_MAX)
+
+
+def test_planted_delete_error_is_caught():
+    """Skipping the recompute of one affected root (a plausible dispatch
+    mistake) makes the trees differ from the static recompute."""
+    hg = load_graph("karate")
+    roots = np.arange(hg.nv)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    saved = o.trees()
+    c = o.delete(1, 2)
+    assert c[1] + c[2] > 0
+    lv, sg, dl = o.trees()
+    changed = [ri for ri in range(hg.nv) if not np.array_equal(dl[ri], saved[2][ri])]
+    assert changed
+    bad = [x.copy() for x in (lv, sg, dl)]
+    ri = changed[0]
+    for x, y in zip(bad, saved):
+        x[ri] = y[ri]
+    assert not (np.array_equal(bad[0], lv) and np.array_equal(bad[1], sg) and np.array_equal(bad[2], dl))
