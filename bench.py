@@ -30,6 +30,7 @@ Algorithmic bytes per (root, insertion) item:
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -73,22 +74,58 @@ def make_workload(name, n_ins):
     return hg, roots, u, v, K, W, desc, time.time() - t0
 
 
-def cpu_baseline(hg, roots, u, v, budget_s, threads, gran):
+def cpu_baseline(hg, roots, u, v, budget_s, threads, gran, dels=()):
+    """The reference's update timed on the host cores; then, if dels is given,
+    its deleteEdgeStreamingBC on those edges (at most budget_s / 2 of it)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_py import Reference, ref_available
     if not ref_available():
-        return None
+        return None, None
     ref = Reference(hg.rowptr, hg.col, roots, nt=threads)
     t_static = ref.static()
     spent, n = 0.0, 0
     while n < len(u) and (spent < budget_s or n == 0):
         spent += ref.insert(int(u[n]), int(v[n]), 1, gran)  # BALANCE, as every reference run script
         n += 1
+    ins = {"value": n / spent, "unit": "insertions/s", "cores": threads, "kind": "reference",
+           "granularity": "COARSE" if gran == 0 else "FINE",
+           "sample": f"first {n} insertions of the same stream, {spent:.1f} s inside insertEdgeStreamingBC "
+                     f"(static pass {t_static:.1f} s untimed)"}
+    dl = None
+    if len(dels):
+        from oracle_py import ref_lib
+        L = ref_lib()
+        cnt = np.zeros(3, dtype=np.int64)
+        dspent, nd = 0.0, 0
+        for a, b in dels:
+            if nd and dspent >= budget_s / 2:
+                break
+            t = L.ref_delete(ref.h, int(a), int(b), 1, gran, cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+            if t < 0:
+                continue
+            dspent += t
+            nd += 1
+        if nd:
+            dl = {"value": nd / dspent, "unit": "deletions/s", "cores": threads, "kind": "reference",
+                  "sample": f"first {nd} of the same deletions after the {n} insertions above, {dspent:.1f} s "
+                            f"inside deleteEdgeStreamingBC"}
     ref.close()
-    return {"value": n / spent, "unit": "insertions/s", "cores": threads, "kind": "reference",
-            "granularity": "COARSE" if gran == 0 else "FINE",
-            "sample": f"first {n} insertions of the same stream, {spent:.1f} s inside insertEdgeStreamingBC "
-                      f"(static pass {t_static:.1f} s untimed)"}
+    return ins, dl
+
+
+def pick_deletions(hg, n, seed=13):
+    """n distinct edges of the initial graph, as (u, v) with u < v."""
+    rng = np.random.default_rng(seed)
+    src = np.repeat(np.arange(hg.nv, dtype=np.int64), np.diff(hg.rowptr))
+    out, seen = [], set()
+    while len(out) < n:
+        e = int(rng.integers(hg.ne))
+        a, b = int(src[e]), int(hg.col[e])
+        key = (min(a, b), max(a, b))
+        if a != b and key not in seen:
+            seen.add(key)
+            out.append(key)
+    return out
 
 
 def main():
@@ -114,6 +151,8 @@ def main():
     ap.add_argument("--park-min", type=int, default=0,
                     help="descent frontier size at which an item is finished by whole-GPU kernels "
                          "(0: library default, <0: never)")
+    ap.add_argument("--deletions", type=int, default=4,
+                    help="edges of the initial graph deleted (and timed) after the insertion legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (PMC traffic calibration)")
     args = ap.parse_args()
@@ -238,6 +277,40 @@ def main():
                  "achieved_GBs": item_bytes(prec) / (pms * 1e-3) / 1e9 if pms > 0 else None}
         print(f"[bench] plain: {plain['value']:.1f} ins/s", file=sys.stderr, flush=True)
 
+    # deletions of edges of the initial graph, after every insertion leg
+    deletion = None
+    dels = pick_deletions(hg, args.deletions) if args.deletions > 0 else []
+    if dels:
+        ctx.reset_stats(timing=True)
+        cases = np.zeros(3, dtype=np.int64)
+        dlat = []
+        ctx.synchronize()
+        td = time.perf_counter()
+        for a, b in dels:
+            ts = time.perf_counter()
+            c = f.delete_edge(g, ctx, a, b)
+            dlat.append(time.perf_counter() - ts)
+            cases += [c["sameLevel"], c["adjacent"], c["movement"]]
+        ctx.synchronize()
+        t_del = time.perf_counter() - td
+        sn, sms = ctx.kernel_stats(sbc.KSTAT_STATIC)
+        un, ums = ctx.kernel_stats(sbc.KSTAT_UPDATE)
+        rebuilt = int(cases[2])
+        # the static pass of a rebuilt (movement-case) root reads every
+        # adjacency entry with the level of its far end (8 B) and writes
+        # level, state and record of every vertex (4 + 1 + 16 B)
+        sbytes = rebuilt * (8.0 * hg.ne + 21.0 * hg.nv)
+        deletion = {"value": round(len(dels) / t_del, 3), "unit": "deletions/s", "deletions": len(dels),
+                    "ms_p50": round(1e3 * float(np.percentile(dlat, 50)), 3),
+                    "cases": {"sameLevel": int(cases[0]), "adjacent": int(cases[1]), "movement": int(cases[2])},
+                    "update_kernel_ms_avg": round(ums / max(un, 1), 3),
+                    "static_kernel_ms_avg": round(sms / max(sn, 1), 3),
+                    "static_GBs": round(sbytes / (sms * 1e-3) / 1e9, 1) if sms > 0 else None,
+                    "method": "adjacent-case roots: update kernel with negated added paths; movement-case "
+                              "roots: rebuilt by the static pass; BC folded again"}
+        print(f"[bench] deletions: {deletion['value']:.2f} del/s, cases {deletion['cases']}",
+              file=sys.stderr, flush=True)
+
     launch_ms = k_ms / max(n_launch, 1)
     achieved = alg_bytes / max(n_launch, 1) / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -286,6 +359,7 @@ def main():
                   "mean_scanned": float(tot["scanned"].sum() / n_items),
                   "parked": int(parked[0])},
         "static_pass_s": round(t_static, 3),
+        "deletion": deletion,
         "schedule": {"name": sched.name, **(sched.report() if args.schedule == "mlp" else {})},
         "autotune": tune,
     }
@@ -294,10 +368,13 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         f.close()
         g.close()
-        cb = cpu_baseline(hg, roots, u, v, args.cpu_budget, threads, 0)
+        cb, cdl = cpu_baseline(hg, roots, u, v, args.cpu_budget, threads, 0, dels)
         if cb:
             out["cpu_baseline"] = cb
             out["speedup_vs_cpu"] = round(value / cb["value"], 2)
+        if cdl and deletion:
+            deletion["cpu_baseline"] = cdl
+            deletion["speedup_vs_cpu"] = round(deletion["value"] / cdl["value"], 2)
     print(json.dumps(out), flush=True)
 
 

@@ -271,15 +271,23 @@ void orc_static(orc_forest *f, const orc_graph *g, int order)
     fold_bc(f);
 }
 
+static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s, int64_t p,
+                        int moved, int del, orc_item *it);
+
 /* deleteEdgeStreamingBC, src/streamingbc.c:258-416 (NT = 1, NO_BALANCE), for
  * (u,v) already removed from g.  Per root the reference's dispatch (310-345):
- * endpoints on the same level, nothing changes; otherwise the deeper endpoint
- * either keeps a neighbour one level up (adjacent case,
- * removeEdgeWithoutMovementBrandes, src/streamingbc_internal.c:607-868) or not
- * (movement case, moveDownTreeBrandes, 880-1401).  Both cases end in the tree
- * of the post-deletion graph, so that tree is recomputed here (pull order) and
- * the change of each vertex's dependency added to BC, as the reference adds
- * newDelta - delta per touched vertex. */
+ * endpoints on the same level, nothing changes; otherwise s (the deeper
+ * endpoint) either keeps a neighbour one level up or not.
+ *  - Adjacent case (removeEdgeWithoutMovementBrandes, 607-868 of
+ *    src/streamingbc_internal.c; the FG form, src/streamingbc_internal_fine.c:
+ *    1019-1400): no level changes; s and its descendants lose sigma[p] paths
+ *    (the level-synchronous descent with negative added paths), and
+ *    dependencies are pulled again level by level.  p, which no scan reaches
+ *    through the removed edge, is marked once s's level is done, and its pull
+ *    starts from its old dependency less s's old term.
+ *  - Movement case (moveDownTreeBrandes, 880-1401): s moves down or out of
+ *    reach; the root's tree is recomputed by the static pull pass over g, the
+ *    tree that case ends in, and the change of each dependency added to BC. */
 void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3])
 {
     int64_t nv = f->nv;
@@ -293,15 +301,22 @@ void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t
             counts[0]++;
             continue;
         }
-        int64_t c = L[u] > L[v] ? u : v;
+        int64_t s = L[u] > L[v] ? u : v, p = L[u] > L[v] ? v : u;
         int64_t extra = 0;
-        for (int64_t e = 0; e < g->deg[c]; e++)
-            if (L[g->adj[c][e]] == L[c] - 1) extra++;
-        counts[extra >= 1 ? 1 : 2]++;
-        memcpy(old, D, (size_t)nv * sizeof(double));
-        static_root(f, g, ri, ORC_PULL);
-        for (int64_t x = 0; x < nv; x++)
-            if (x != r && D[x] != old[x]) f->bcpart[x] += D[x] - old[x];
+        for (int64_t e = 0; e < g->deg[s]; e++)
+            if (L[g->adj[s][e]] == L[p]) extra++;
+        if (extra >= 1) {
+            counts[1]++;
+            orc_item it;
+            memset(&it, 0, sizeof(it));
+            fine_update(f, g, ri, s, p, 0, 1, &it);
+        } else {
+            counts[2]++;
+            memcpy(old, D, (size_t)nv * sizeof(double));
+            static_root(f, g, ri, ORC_PULL);
+            for (int64_t x = 0; x < nv; x++)
+                if (x != r && D[x] != old[x]) f->bcpart[x] += D[x] - old[x];
+        }
     }
     free(old);
     fold_bc(f);
@@ -520,7 +535,7 @@ static void coarse_move(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s
  * 560-999).  s = deeper endpoint, p = shallower, moved = |gap| >= 2.
  * touched[k]: >0 descent (frontier index + 1), -1 ascent-only (U/S). */
 static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s, int64_t p,
-                        int moved, orc_item *it)
+                        int moved, int del, orc_item *it)
 {
     int64_t nv = f->nv, r = f->roots[ri];
     int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
@@ -536,8 +551,9 @@ static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s
     touched[s] = 1;
     im[s] = moved ? 1 : 0;
     nlev[s] = base;
-    diff[s] = S[p];
-    nsig[s] = moved ? S[p] : sig_add(f, S[s], S[p]);
+    /* a deletion (adjacent case) takes sigma[p] paths away from s */
+    diff[s] = del ? (int64_t)(0ull - (uint64_t)S[p]) : S[p];
+    nsig[s] = moved ? S[p] : sig_add(f, S[s], diff[s]);
     int64_t nf = 0; /* frontiers: qd[ld[j] .. ld[j+1]) */
     ld[0] = 0;
     ls[0] = 0;
@@ -597,6 +613,9 @@ static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s
             int isD = touched[k] > 0;
             int64_t sk = isD ? nsig[k] : S[k];
             double acc = isD ? 0.0 : D[k];
+            /* a deletion's p no longer meets s in its list: its pull starts
+             * without s's old term */
+            if (del && k == p) acc = acc - dep(S[p], S[s], D[s]);
             for (int64_t e = 0; e < g->deg[k]; e++) {
                 int64_t c = g->adj[k][e];
                 int64_t tcv = touched[c];
@@ -616,6 +635,12 @@ static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s
             it->processed++;
             it->scanned += g->deg[k];
             if (k != r) f->bcpart[k] += ndel[k] - D[k];
+        }
+        /* a deletion's p lost its child s (no scan reaches p through the
+         * removed edge): it is updated on the next level */
+        if (del && lev == base && touched[p] == 0) {
+            touched[p] = -1;
+            qu[un++] = p;
         }
     }
     for (int64_t i = 0; i < nd; i++) {
@@ -656,14 +681,14 @@ void orc_insert(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int mod
         if (ad == 1) {
             it->kase = 1;
             if (mode == ORC_COARSE) coarse_adjacent(f, g, ri, s, p, S[p], it);
-            else fine_update(f, g, ri, s, p, 0, it);
+            else fine_update(f, g, ri, s, p, 0, 0, it);
         } else {
             it->kase = 2;
             if (mode == ORC_COARSE) {
                 int64_t prev = (L[s] == INF_LEVEL) ? (INF_LEVEL - L[p] - 1) : ad - 1;
                 coarse_move(f, g, ri, s, p, prev, it);
             } else {
-                fine_update(f, g, ri, s, p, 1, it);
+                fine_update(f, g, ri, s, p, 1, 0, it);
             }
         }
     }

@@ -21,10 +21,11 @@
  *                               (src/streamingbc_internal_fine.c:11-340, 343-1016), i.e.
  *                               level-synchronous descent and pull-form ascent
  *   orc_delete                  deleteEdgeStreamingBC, NT=1, NO_BALANCE (src/streamingbc.c:258-416):
- *                               the reference's per-root dispatch, and each affected
- *                               root's tree recomputed by the static pull pass over the
- *                               post-deletion graph (the tree both deletion cases,
- *                               src/streamingbc_internal.c:607-1401, arrive at)
+ *                               the reference's per-root dispatch; the adjacent case
+ *                               (src/streamingbc_internal.c:607-868) as the FINE update with
+ *                               negated added paths, the shallower endpoint dropping the
+ *                               deeper one's old term; the movement case (880-1401) by the
+ *                               static pull pass over the post-deletion graph
  *   orc_graph_remove            stinger_remove_edge x2 as called by DeleteEdge
  *                               (src/sbcMain_utils.c:473-481; src/stinger/stinger.c:1055-1090)
  *   orc_graph_insert            stinger_insert_edge x2 as called by InsertEdge

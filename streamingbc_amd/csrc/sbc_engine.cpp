@@ -969,13 +969,15 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
 
 // The update after (u, v) left the graph in both directions.  Per root the
 // reference dispatches on the endpoints' levels (src/streamingbc.c:258-416):
-// same level, nothing to do; otherwise the adjacent case
-// (removeEdgeWithoutMovementBrandes, src/streamingbc_internal.c:607-868) or
-// the movement case (moveDownTreeBrandes, 880-1401).  Both end in the tree
-// of the post-deletion graph, so the affected roots are rebuilt by the static
-// pass over that graph (dist and sigma exact, delta in the static pass's
-// order), and BC is folded again from every tree.  counts: same-level,
-// adjacent and movement roots, as the reference's StreamingExtraInfo.
+// same level, nothing to do; the deeper endpoint s keeps another neighbour one
+// level up: the adjacent case (removeEdgeWithoutMovementBrandes,
+// src/streamingbc_internal.c:607-868), in which no level changes, s and its
+// descendants lose sigma[p] paths and dependencies are pulled again up to the
+// root -- the update kernel's adjacent case with the added paths negated;
+// otherwise the movement case (moveDownTreeBrandes, 880-1401), whose roots
+// are rebuilt by the static pass over the post-deletion graph.  BC is then
+// folded again from every tree.  counts: same-level, adjacent and movement
+// roots, as the reference's StreamingExtraInfo.
 static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* C, int64_t u, int64_t v,
                             int64_t counts[3])
 {
@@ -990,25 +992,50 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(hipStreamSynchronize(C->stream));
         if (C->d_del) HIPCHK(hipFree(C->d_del));
         C->d_del = nullptr;
-        if ((st = dalloc(&C->d_del, (size_t)(4 + F->K)))) return st;
+        if ((st = dalloc(&C->d_del, (size_t)(4 + 2 * F->K)))) return st;
         C->del_k = F->K;
     }
+    if ((st = ensure_rec(C, F->K))) return st;
+    if ((st = ensure_bucket(C, G, 4LL * C->num_cu))) return st;
+    if ((st = ensure_hubq(C, 4LL * C->num_cu))) return st;
     SbcParams P = base_params(F, G, C);
     HIPCHK(sbc_launch_reset(C->d_del, 4, C->stream));
     HIPCHK(sbc_launch_delete_classify(&P, (int32_t)u, (int32_t)v, C->d_del, C->stream));
-    P.sroots = C->d_del + 4;
-    P.scnt = C->d_del;
-    HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+    int32_t h[4];
+    HIPCHK(hipMemcpyAsync(h, C->d_del, sizeof(h), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
     hipEvent_t e0;
-    if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
-    int grid = (int)(F->K < 4LL * C->num_cu ? F->K : 4LL * C->num_cu);
-    HIPCHK(sbc_launch_static(&P, 16, grid, C->stream));
-    if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+    if (h[2] > 0) {
+        // adjacent roots: the single-item update in deletion mode
+        SbcParams Q = P;
+        Q.u0 = (int32_t)u;
+        Q.v0 = (int32_t)v;
+        Q.rec = C->d_rec;
+        Q.del = 1;
+        Q.order = C->d_del + 4;
+        Q.qlen = h[2];
+        Q.heavy = h[2];
+        int grid = (int)(h[2] < 2LL * C->num_cu ? h[2] : 2LL * C->num_cu);
+        HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+        if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
+        HIPCHK(sbc_launch_window(&Q, 16, grid, C->stream));
+        if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+    }
+    if (h[3] > 0) {
+        // movement roots: rebuilt by the static pass
+        SbcParams Q = P;
+        Q.sroots = C->d_del + 4 + F->K;
+        Q.scnt = C->d_del + 3;
+        HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+        if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
+        int grid = (int)(h[3] < 4LL * C->num_cu ? h[3] : 4LL * C->num_cu);
+        HIPCHK(sbc_launch_static(&Q, 16, grid, C->stream));
+        if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+    }
     if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
     HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
     if ((st = ev_end(C, KSTAT_BC, e0))) return st;
-    int32_t h[4];
-    HIPCHK(hipMemcpyAsync(h, C->d_del, sizeof(h), hipMemcpyDeviceToHost, C->stream));
+    C->last_rec_n = 0;
     if ((st = check_kernel_err(F, C))) return st;
     counts[0] = h[1];
     counts[1] = h[2];

@@ -125,6 +125,10 @@ struct SbcParams {
     // every root when sroots is nullptr
     const int32_t* sroots;
     const int32_t* scnt;
+    // edge deletion (single-item form): the item's edge has left the graph;
+    // s loses p's paths (added paths -sigma[p]), and p, marked as a parent
+    // once s's level is done, pulls from its old dependency less s's old term
+    int32_t del;
 };
 
 // A parked item (one per root slot), in global memory.  cnt/stat have the
@@ -196,8 +200,9 @@ hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t 
 hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
 // Per root, the dispatch of deleting (u, v) (already removed from the graph):
-// out[0] affected roots, listed at out[4..]; out[1..3] same-level, adjacent
-// and movement cases.  out[0..3] must be zero.
+// out[1..3] count the same-level, adjacent and movement roots; adjacent roots
+// are listed at out[4..4+K), movement roots at out[4+K..4+2K).  out[0..3]
+// must be zero.
 hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out, hipStream_t st);
 hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
 hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);

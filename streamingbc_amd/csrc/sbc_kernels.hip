@@ -360,9 +360,6 @@ constexpr int WAVE_UNROLL = SBC_WAVE_UNROLL;
 #define SBC_PROBE_UNROLL 2
 #endif
 constexpr int PROBE_UNROLL = SBC_PROBE_UNROLL;
-#ifndef SBC_SET_COMPACT
-#define SBC_SET_COMPACT 1
-#endif
 constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
@@ -665,7 +662,18 @@ struct KState {
     double base;
 };
 
-__device__ __forceinline__ KState kstate(const Item& X, int32_t k)
+// Old dependency of ascent-only vertex k, where its pull starts: in an edge
+// deletion the shallower endpoint p no longer meets s in its list, so it
+// starts without s's old term.  The stored value stays as it is: p's own
+// parents subtract p's old term from it.
+__device__ __forceinline__ double asc_base(const SbcParams& P, const Item& X, int32_t k)
+{
+    const double d = X.vr[k].delta;
+    if (P.del && k == X.p) return d - dep(X.vr[k].sigma, X.vr[X.s].sigma, X.vr[X.s].delta);
+    return d;
+}
+
+__device__ __forceinline__ KState kstate(const SbcParams& P, const Item& X, int32_t k)
 {
     // the four loads are issued together; the state byte selects which are used
     KState s;
@@ -677,7 +685,7 @@ __device__ __forceinline__ KState kstate(const Item& X, int32_t k)
     s.isD = kt != 3;
     s.sk_old = sg;
     s.sk_new = s.isD ? (int64_t)((uint64_t)dp + (uint64_t)(kt == 1 ? sg : 0)) : sg;
-    s.base = s.isD ? 0.0 : dl;
+    s.base = s.isD ? 0.0 : ((P.del && k == X.p) ? asc_base(P, X, k) : dl);
     return s;
 }
 
@@ -692,14 +700,10 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
     if (srcset) {
-        // neighbour ids of up to 8 chunks in flight; only members of the
-        // level's source set can give a term or need a mark.  When the
-        // chunks hold at most 64 members, they are packed in list order into
-        // the wavefront's LDS row and handled one per lane, so their state
-        // bytes and terms are loaded in one round trip each; otherwise chunk
-        // by chunk, skipping chunks without members
+        // neighbour ids of up to 8 chunks in flight, then chunk by chunk:
+        // only members of the level's source set can give a term or need a
+        // mark, and a chunk without one is skipped
         constexpr int SU = 8;
-        int32_t* mrow = reinterpret_cast<int32_t*>(sbc_wrow[threadIdx.x >> 6]);
         for (int ch0 = 0; ch0 < nch; ch0 += SU) {
             int32_t cc[SU];
 #pragma unroll
@@ -707,38 +711,9 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
                 const int e = (ch0 + u) * 64 + lane;
                 cc[u] = (ch0 + u) < nch && e < len ? P.col[start + e] : -1;
             }
-            bool okk[SU];
-            int tot = 0;
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
-                okk[u] = cc[u] >= 0 && shash_has(*srcset, cc[u]);
-                tot += __popcll(ballot(okk[u]));
-            }
-            if (tot == 0) continue;
-            if (SBC_SET_COMPACT && tot <= 64) {
-                int nm = 0;
-#pragma unroll
-                for (int u = 0; u < SU; ++u) {
-                    const uint64_t m = ballot(okk[u]);
-                    if (okk[u]) mrow[nm + prefix_in_wave(m)] = cc[u];
-                    nm += __popcll(m);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const bool ok = lane < nm;
-                const int32_t c = ok ? mrow[lane] : 0;
-                const uint8_t ww = ok ? ld_sb(&X.sb[c]) : (uint8_t)0;
-                bool appU;
-                const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok, c, ww, appU);
-                const int pu = wave_append(appU, &ts.cnt[2]);
-                if (pu >= 0) X.list[pu] = c;
-                acc = wave_ordered_add(acc, t.child, t.add, t.hsub, t.sub);
-                continue;
-            }
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const bool ok = okk[u];
+                const bool ok = cc[u] >= 0 && shash_has(*srcset, cc[u]);
                 if (ballot(ok) == 0) continue;
                 const uint8_t ww = ok ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
                 bool appU;
@@ -788,7 +763,7 @@ template <typename TS>
 __device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
                                    int lev, TS& ts)
 {
-    ascent_vertex_wave_ks(P, X, k, vi.start, len, kstate(X, k), lev, ts);
+    ascent_vertex_wave_ks(P, X, k, vi.start, len, kstate(P, X, k), lev, ts);
 }
 
 // The whole team computes the new delta of hub k: wave w scans a contiguous
@@ -1397,7 +1372,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                     ad = e.add;
                     sb2 = e.sub;
                 }
-                if (gl == 0) base = X.vr[hs.plist[hi]].delta;
+                if (gl == 0) base = asc_base(P, X, hs.plist[hi]);
             }
             for (int k2 = 2; k2 <= 8; k2 <<= 1) {
                 for (int jj = k2 >> 1; jj > 0; jj >>= 1) {
@@ -1449,7 +1424,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const SbcBucketEnt* b = bk + (size_t)hi * hs.bstride;
             double acc;
             if (hs.kind[hi] == 1) {
-                acc = bucket_sum(bucket_lane(b, n), X.vr[hs.plist[hi]].delta);
+                acc = bucket_sum(bucket_lane(b, n), asc_base(P, X, hs.plist[hi]));
             } else {
                 int key[4], idx[4];
 #pragma unroll
@@ -1459,7 +1434,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                     idx[j] = e;
                 }
                 wave_sort256(key, idx);
-                acc = X.vr[hs.plist[hi]].delta;
+                acc = asc_base(P, X, hs.plist[hi]);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     int fl = 0;
@@ -1553,7 +1528,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane == 0) st_reld(&X.vr[k].ndel, lds_chain_sum(X.vr[k].delta, row, nval));
+            if (lane == 0) st_reld(&X.vr[k].ndel, lds_chain_sum(asc_base(P, X, k), row, nval));
         }
         __syncthreads();
         if (tp && tid == 0) tp[27] += (int64_t)wall_clock64() - tk;
@@ -1640,7 +1615,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
         if (t < tot) {
             k = here.at(X, t);
             vi = P.vinfo[k];
-            kst = kstate(X, k);  // loaded with vinfo; unused for hubs
+            kst = kstate(P, X, k);  // loaded with vinfo; unused for hubs
             len = visible_len_thread(P, vi, X.q);
             if (len > HUB_MIN) {
                 const int pos = atomicAdd(&hs.n, 1);
@@ -1936,7 +1911,8 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
 
     if (leader) {
         st_sb(&X.sb[X.s], sb_make(base, smoved ? 2 : 1));
-        st_rel64(&X.vr[X.s].dpath, X.vr[X.p].sigma);
+        const int64_t sp = X.vr[X.p].sigma;
+        st_rel64(&X.vr[X.s].dpath, P.del ? (int64_t)(0ull - (uint64_t)sp) : sp);
         X.list[0] = X.s;
         lvlD[0] = 0;
         lvlS[0] = 0;
@@ -2171,7 +2147,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     if (!looked_up) {
                         const int j = atomicAdd(&hs->ns, 1);
                         if (P.park) atomicAdd((unsigned long long*)&hs->svol, (unsigned long long)len);
-                        const KState kst = kstate(X, k);
+                        const KState kst = kstate(P, X, k);
                         ls->k[j] = k;
                         ls->start[j] = hs->hstart[h];
                         ls->len[j] = len;
@@ -2219,6 +2195,9 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 return true;
             }
         }
+        // a deletion's p lost its child s: it is a parent to update even if
+        // no scan of this level reaches it (its pull starts from asc_base)
+        if (P.del && lev == base && leader) mark_parent(X, ts, X.p, lev - 1);
         team_sync<TW>();
         if (tp && leader) {
             const int ls7 = lev < 7 ? lev : 7;
@@ -2605,9 +2584,10 @@ __global__ void sbc_bc_reduce_kernel(SbcParams P)
 // Edge deletion, one thread per root, after (u, v) left the graph: the
 // reference's per-root dispatch (src/streamingbc.c:310-345).  Endpoints on
 // the same level (or both unreachable): the edge is on no shortest path and
-// the tree stands.  Otherwise the deeper endpoint c keeps its level if
-// another neighbour lies one level up (adjacent case) or moves down
-// (movement case); either way the root's tree is rebuilt by the static pass.
+// the tree stands.  Otherwise the deeper endpoint s keeps its level if
+// another neighbour lies one level up (adjacent case: dist unchanged, the
+// update kernel runs with del set) or moves down (movement case: the static
+// pass rebuilds the tree).
 __global__ void sbc_delete_classify_kernel(SbcParams P, int32_t u, int32_t v, int32_t* out)
 {
     const int ri = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2618,14 +2598,17 @@ __global__ void sbc_delete_classify_kernel(SbcParams P, int32_t u, int32_t v, in
         atomicAdd(&out[1], 1);
         return;
     }
-    const int32_t c = du > dv ? u : v;
-    const int32_t up = (du > dv ? du : dv) - 1;
-    const SbcVInfo vi = P.vinfo[c];
+    const int32_t s = du > dv ? u : v, p = du > dv ? v : u;
+    const int32_t up = dist[p];
+    const SbcVInfo vi = P.vinfo[s];
     int extra = 0;
     for (int e = 0; e < vi.deg && !extra; ++e)
         if (dist[P.col[vi.start + e]] == up) extra = 1;
-    atomicAdd(&out[extra ? 2 : 3], 1);
-    out[4 + atomicAdd(&out[0], 1)] = ri;
+    if (extra) {
+        out[4 + atomicAdd(&out[2], 1)] = ri;
+    } else {
+        out[4 + P.K + atomicAdd(&out[3], 1)] = ri;
+    }
 }
 
 // Clears the work-queue counters (a kernel, not a memset, so it can sit in
