@@ -978,6 +978,42 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
 // are rebuilt by the static pass over the post-deletion graph.  BC is then
 // folded again from every tree.  counts: same-level, adjacent and movement
 // roots, as the reference's StreamingExtraInfo.
+// Movement-case roots up to this many per deletion are rebuilt one at a
+// time by the whole GPU (one launch per level); more, one workgroup each.
+static const int32_t kWideRebuildMax = 8;
+
+static int32_t wide_rebuild_max()
+{
+    const char* e = getenv("SBC_WIDE_REBUILD_MAX");
+    return e ? (int32_t)atoi(e) : kWideRebuildMax;
+}
+
+// The static pass of root ri spread over the whole GPU: host-driven, one
+// launch per BFS level (the frontier's end read back after each) and per
+// ascent level.  fr: one device counter.
+static sbcStatus rebuild_root_wide(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri,
+                                   int32_t* fr)
+{
+    HIPCHK(sbc_launch_root_reset(&P, ri, fr, C->stream));
+    std::vector<int32_t> lv = {0, 1};
+    int32_t j = 0;
+    for (;;) {
+        const int32_t fs = lv[j], fe = lv[j + 1];
+        HIPCHK(sbc_launch_root_bfs(&P, ri, fs, fe, j, fr, C->stream));
+        int32_t ne = 0;
+        HIPCHK(hipMemcpyAsync(&ne, fr, sizeof(ne), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (ne == fe) break;
+        HIPCHK(sbc_launch_root_settle(&P, ri, fe, ne, j, C->stream));
+        j++;
+        if (j + 1 >= F->lcap) return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", F->lcap);
+        lv.push_back(ne);
+    }
+    // levels 0..j, level L = list[lv[L], lv[L+1]); the deepest has no children
+    for (int32_t L = j - 1; L >= 0; --L) HIPCHK(sbc_launch_root_ascent(&P, ri, lv[L], lv[L + 1], L, C->stream));
+    return SBC_OK;
+}
+
 static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* C, int64_t u, int64_t v,
                             int64_t counts[3])
 {
@@ -992,7 +1028,7 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(hipStreamSynchronize(C->stream));
         if (C->d_del) HIPCHK(hipFree(C->d_del));
         C->d_del = nullptr;
-        if ((st = dalloc(&C->d_del, (size_t)(4 + 2 * F->K)))) return st;
+        if ((st = dalloc(&C->d_del, (size_t)(8 + 2 * F->K)))) return st;
         C->del_k = F->K;
     }
     if ((st = ensure_rec(C, F->K))) return st;
@@ -1021,8 +1057,18 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(sbc_launch_window(&Q, 16, grid, C->stream));
         if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
     }
-    if (h[3] > 0) {
-        // movement roots: rebuilt by the static pass
+    if (h[3] > 0 && h[3] <= wide_rebuild_max()) {
+        // few movement roots: each rebuilt by the whole GPU
+        std::vector<int32_t> mr((size_t)h[3]);
+        HIPCHK(hipMemcpyAsync(mr.data(), C->d_del + 4 + F->K, sizeof(int32_t) * mr.size(), hipMemcpyDeviceToHost,
+                              C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
+        for (int32_t ri : mr)
+            if ((st = rebuild_root_wide(F, C, P, ri, C->d_del + 4 + 2 * F->K))) return st;
+        if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+    } else if (h[3] > 0) {
+        // movement roots: rebuilt by the static pass, one workgroup each
         SbcParams Q = P;
         Q.sroots = C->d_del + 4 + F->K;
         Q.scnt = C->d_del + 3;

@@ -203,6 +203,14 @@ hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
 // out[1..3] count the same-level, adjacent and movement roots; adjacent roots
 // are listed at out[4..4+K), movement roots at out[4+K..4+2K).  out[0..3]
 // must be zero.
+// The static pass of root ri over the whole GPU: reset (fr[0] = list end =
+// 1), then per level j the frontier list[fs, fe) reaches level j + 1
+// (appending at fr[0]) and the new level list[a, b) is settled; then per
+// level L, deepest first, the pull of list[a, b).
+hipError_t sbc_launch_root_reset(const SbcParams* P, int ri, int32_t* fr, hipStream_t st);
+hipError_t sbc_launch_root_bfs(const SbcParams* P, int ri, int fs, int fe, int j, int32_t* fr, hipStream_t st);
+hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, int b, int j, hipStream_t st);
+hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, hipStream_t st);
 hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out, hipStream_t st);
 hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
 hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);

@@ -2568,6 +2568,133 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
     }
 }
 
+// ------------------------------------------------------------ one root, whole GPU
+//
+// The static pass of a single root spread over the whole GPU (an edge
+// deletion's movement-case roots, when there are few of them): one launch
+// per BFS level and per ascent level, wavefronts taking the level's
+// vertices in a grid stride, each running the per-vertex code of
+// sbc_static_kernel -- the CAS on dist, the added paths, then the pull in
+// lane order -- so the tree is bit-identical to that kernel's.  Every
+// workgroup writes the root's arrays here, so the atomics use device scope.
+
+__device__ __forceinline__ int32_t cas32_dev(int32_t* p, int32_t expect, int32_t want)
+{
+    __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expect;
+}
+
+// Resets root ri's tree to "only the root reached"; fr[0] = 1 (list end).
+__global__ void sbc_root_reset_kernel(SbcParams P, int ri, int32_t* fr)
+{
+    const size_t off = (size_t)ri * (size_t)P.nv;
+    const int32_t r = P.roots[ri];
+    uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < P.nv; x += gridDim.x * blockDim.x) {
+        const bool root = x == r;
+        P.dist[off + x] = root ? 0 : INF32;
+        sb[x] = root ? sb_make(0, 0) : SB_INF;
+        P.vr[off + x].sigma = root ? 1 : SIGMA_INF;
+        P.vr[off + x].delta = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.list[off] = r;
+        fr[0] = 1;
+    }
+}
+
+// Frontier list[fs, fe) (level j): reaches level j + 1, appending at fr[0].
+__global__ void sbc_root_bfs_kernel(SbcParams P, int ri, int fs, int fe, int j, int32_t* fr)
+{
+    const size_t off = (size_t)ri * (size_t)P.nv;
+    int32_t* dist = P.dist + off;
+    SbcVRec* vr = P.vr + off;
+    int32_t* list = P.list + off;
+    const int lane = lane_id();
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int32_t next = j + 1;
+    for (int idx = fs + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6); idx < fe; idx += nw) {
+        const int32_t c = list[idx];
+        const int64_t sc = vr[c].sigma;
+        const SbcVInfo vi = P.vinfo[c];
+        for (int e0 = 0; e0 < vi.deg; e0 += 64) {
+            const int e = e0 + lane;
+            bool app = false;
+            int32_t k = 0;
+            if (e < vi.deg) {
+                k = P.col[vi.start + e];
+                int32_t wk = __hip_atomic_load(&dist[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (wk == INF32) {
+                    const int32_t old = cas32_dev(&dist[k], INF32, next);
+                    app = old == INF32;
+                    wk = app ? next : old;
+                }
+                if (wk == next)
+                    __hip_atomic_fetch_add((unsigned long long*)&vr[k].dpath, (unsigned long long)sc,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const uint64_t m = ballot(app);
+            if (m) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&fr[0], __popcll(m));
+                base = __shfl(base, 0);
+                if (app) list[base + prefix_in_wave(m)] = k;
+            }
+        }
+    }
+}
+
+// Level j + 1 = list[a, b): path counts from the added paths, state bytes.
+__global__ void sbc_root_settle_kernel(SbcParams P, int ri, int a, int b, int j)
+{
+    const size_t off = (size_t)ri * (size_t)P.nv;
+    uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
+    for (int x = a + blockIdx.x * blockDim.x + threadIdx.x; x < b; x += gridDim.x * blockDim.x) {
+        const int32_t k = P.list[off + x];
+        SbcVRec& r = P.vr[off + k];
+        r.sigma = r.dpath;
+        r.dpath = 0;
+        sb[k] = sb_make(j + 1, 0);
+    }
+}
+
+// Level L = list[a, b): delta pulled from the children in adjacency order.
+__global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L)
+{
+    const size_t off = (size_t)ri * (size_t)P.nv;
+    const int32_t* dist = P.dist + off;
+    SbcVRec* vr = P.vr + off;
+    const int lane = lane_id();
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const int32_t childw = L + 1;
+    for (int idx = a + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6); idx < b; idx += nw) {
+        const int32_t k = P.list[off + idx];
+        const int64_t sk = vr[k].sigma;
+        const SbcVInfo vi = P.vinfo[k];
+        double acc = 0.0;
+        for (int e0 = 0; e0 < vi.deg; e0 += 64) {
+            const int e = e0 + lane;
+            bool child = false;
+            double add = 0.0;
+            if (e < vi.deg) {
+                const int32_t c = P.col[vi.start + e];
+                if (dist[c] == childw) {
+                    child = true;
+                    add = dep(sk, vr[c].sigma, vr[c].delta);
+                }
+            }
+            uint64_t m = ballot(child);
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                acc = acc + readlane_d(add, l);
+            }
+        }
+        if (lane == 0) vr[k].delta = acc;
+    }
+}
+
 // BC[v] = sum over roots, in root-array order, of delta_r[v] (r != v).
 __global__ void sbc_bc_reduce_kernel(SbcParams P)
 {
@@ -2705,6 +2832,37 @@ extern "C" hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, 
 {
     const int bs = 256;
     hipLaunchKernelGGL(sbc_delete_classify_kernel, dim3((P->K + bs - 1) / bs), dim3(bs), 0, st, *P, u, v, out);
+    return hipGetLastError();
+}
+
+static int root_grid(long items, int per_block)
+{
+    const long g = (items + per_block - 1) / per_block;
+    return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+extern "C" hipError_t sbc_launch_root_reset(const SbcParams* P, int ri, int32_t* fr, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_root_reset_kernel, dim3(root_grid(P->nv, 256)), dim3(256), 0, st, *P, ri, fr);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_root_bfs(const SbcParams* P, int ri, int fs, int fe, int j, int32_t* fr,
+                                          hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_root_bfs_kernel, dim3(root_grid(fe - fs, 4)), dim3(256), 0, st, *P, ri, fs, fe, j, fr);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, int b, int j, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_root_settle_kernel, dim3(root_grid(b - a, 256)), dim3(256), 0, st, *P, ri, a, b, j);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_root_ascent_kernel, dim3(root_grid(b - a, 4)), dim3(256), 0, st, *P, ri, a, b, L);
     return hipGetLastError();
 }
 

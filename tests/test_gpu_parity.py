@@ -464,12 +464,16 @@ def _apply_ops(f, g, ctx, o, ops):
     return np.array(counts, dtype=np.int64)
 
 
+# movement-case roots: SBC_WIDE_REBUILD_MAX = 0 rebuilds them one workgroup
+# per root, a large value one root at a time over the whole GPU
+@pytest.mark.parametrize("wide", ["0", "1000"])
 @pytest.mark.parametrize("name", ["karate", "dolphins"])
-def test_delete_matches_oracle_and_reference(name):
+def test_delete_matches_oracle_and_reference(name, wide, monkeypatch):
+    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
     fx = np.load(os.path.join(GOLDEN, f"refdel_{name}.npz"))
     hg = sbc.read_dimacs(os.path.join(GOLDEN, "graphs", f"{name}.graph"))
     roots = np.arange(hg.nv)
-    print(f"[delete] {name} ops={len(fx['ops'])}", flush=True)
+    print(f"[delete] {name} ops={len(fx['ops'])} wide={wide}", flush=True)
     ctx, g, f = _engine(hg, roots)
     o = Oracle(hg.rowptr, hg.col, roots)
     o.static(PULL)
@@ -484,12 +488,14 @@ def test_delete_matches_oracle_and_reference(name):
     assert np.array_equal(lv, fx["fine_level"]) and np.array_equal(sg, fx["fine_sigma"])
 
 
-def test_delete_rmat10_matches_reference_hashes():
+@pytest.mark.parametrize("wide", ["0", "1000"])
+def test_delete_rmat10_matches_reference_hashes(wide, monkeypatch):
+    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
     fx = np.load(os.path.join(GOLDEN, "refdel_rmat10.npz"))
     hg = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
     assert hg.nv == int(fx["nv"])
     roots = np.arange(hg.nv)
-    print(f"[delete] rmat10 deletions={len(fx['ops'])}", flush=True)
+    print(f"[delete] rmat10 deletions={len(fx['ops'])} wide={wide}", flush=True)
     ctx, g, f = _engine(hg, roots)
     for i, (op, a, b) in enumerate(fx["ops"]):
         c = f.delete_edge(g, ctx, int(a), int(b))
