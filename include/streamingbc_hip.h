@@ -231,8 +231,9 @@ sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, ext
                                         const int64_t *u, const int64_t *v, int64_t count,
                                         const sbcBatchConfig *cfg, sbcItemRecord *records);
 
-/* Number of items of the last streamingBCInsertEdgesBatched call that were
- * finished by the whole-GPU kernels (sbcBatchConfig.park_min). */
+/* Number of items of the last streamingBCInsertEdgesBatched call (or of the
+ * last deletion) that were finished by the whole-GPU kernels
+ * (sbcBatchConfig.park_min; deletions always hand large items over). */
 int64_t streamingBCLastParked(extraArraysPerThread **eAPT);
 
 /* Per-root work totals of the batched calls since the last

@@ -978,6 +978,10 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
 // are rebuilt by the static pass over the post-deletion graph.  BC is then
 // folded again from every tree.  counts: same-level, adjacent and movement
 // roots, as the reference's StreamingExtraInfo.
+static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K);
+static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
+                              int64_t* total);
+
 // Movement-case roots up to this many per deletion are rebuilt one at a
 // time by the whole GPU (one launch per level); more, one workgroup each.
 static const int32_t kWideRebuildMax = 8;
@@ -1034,7 +1038,9 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
     if ((st = ensure_rec(C, F->K))) return st;
     if ((st = ensure_bucket(C, G, 4LL * C->num_cu))) return st;
     if ((st = ensure_hubq(C, 4LL * C->num_cu))) return st;
+    if ((st = ensure_park(C, F->K))) return st;
     SbcParams P = base_params(F, G, C);
+    C->last_parked = 0;
     HIPCHK(sbc_launch_reset(C->d_del, 4, C->stream));
     HIPCHK(sbc_launch_delete_classify(&P, (int32_t)u, (int32_t)v, C->d_del, C->stream));
     int32_t h[4];
@@ -1051,11 +1057,22 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         Q.order = C->d_del + 4;
         Q.qlen = h[2];
         Q.heavy = h[2];
+        // items whose descent or ascent grows large go to the wide kernels,
+        // as in the windowed form
+        Q.park = C->d_park;
+        Q.parked = C->d_parked;
+        Q.park_min = 4096;
+        Q.park_scan = kParkScan;
+        const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
+        if (ps) Q.park_scan = atoll(ps);
         int grid = (int)(h[2] < 2LL * C->num_cu ? h[2] : 2LL * C->num_cu);
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
         if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
         HIPCHK(sbc_launch_window(&Q, 16, grid, C->stream));
+        int64_t parked = 0;
+        if ((st = finish_parked(F, C, Q, 16, grid, &parked))) return st;
         if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+        C->last_parked = parked;
     }
     if (h[3] > 0 && h[3] <= wide_rebuild_max()) {
         // few movement roots: each rebuilt by the whole GPU
@@ -1340,6 +1357,8 @@ static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, co
         if (tot == 0) break;
         HIPCHK(sbc_launch_wide_ascent(&P, dw, ri, h, lev, light_counted && lev == lev0 ? 1 : 0, wide_grid(C, tot),
                                       C->stream));
+        // a deletion's p lost its child s: a parent to update once s's level is done
+        if (P.del && lev == w.base) HIPCHK(sbc_launch_wide_mark(&P, dw, ri, w.p, lev - 1, C->stream));
         HIPCHK(hipMemcpyAsync(&u, &dw->cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
         if (u < uB || (int64_t)u + nS > F->nv)

@@ -211,6 +211,10 @@ hipError_t sbc_launch_root_reset(const SbcParams* P, int ri, int32_t* fr, hipStr
 hipError_t sbc_launch_root_bfs(const SbcParams* P, int ri, int fs, int fe, int j, int32_t* fr, hipStream_t st);
 hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, int b, int j, hipStream_t st);
 hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, hipStream_t st);
+// Marks x as a parent on level lev_parent for parked item wi of root ri (a
+// deletion's p once the level of s is done).
+hipError_t sbc_launch_wide_mark(const SbcParams* P, struct SbcWideItem* wi, int ri, int32_t x, int lev_parent,
+                                hipStream_t st);
 hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out, hipStream_t st);
 hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
 hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);

@@ -2328,6 +2328,17 @@ __device__ __forceinline__ Item parked_item(const SbcParams& P, const SbcWideIte
     return X;
 }
 
+// A deletion's p, marked as a parent on level lev_parent once the parked
+// item's level of s is done, and appended to the item's ascent list.
+__global__ void sbc_wide_mark_kernel(SbcParams P, SbcWideItem* wi, int ri, int32_t x, int lev_parent)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const Item X = parked_item(P, *wi, ri);
+    const uint8_t b = ld_sb(&X.sb[x]);
+    if (sb_type(b) == 0 && cas_sb(&X.sb[x], b, sb_make(lev_parent, 3)) == b)
+        X.list[atomicAdd(&wi->cnt[2], 1)] = x;
+}
+
 // Descent of frontier list[fs, fe) (new level lc of the children).
 template <int NW>
 __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_kernel(SbcParams P, SbcWideItem* wi, int ri, int fs, int fe,
@@ -2832,6 +2843,13 @@ extern "C" hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, 
 {
     const int bs = 256;
     hipLaunchKernelGGL(sbc_delete_classify_kernel, dim3((P->K + bs - 1) / bs), dim3(bs), 0, st, *P, u, v, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_wide_mark(const SbcParams* P, SbcWideItem* wi, int ri, int32_t x, int lev_parent,
+                                           hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_wide_mark_kernel, dim3(1), dim3(64), 0, st, *P, wi, ri, x, lev_parent);
     return hipGetLastError();
 }
 

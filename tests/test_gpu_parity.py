@@ -577,3 +577,34 @@ def test_delete_disconnects_and_legacy_form():
     after = f.trees()
     assert all(np.array_equal(x, y) for x, y in zip(before, after))
     assert g.degree(2) == 1 and g.degree(3) == 1
+
+
+@pytest.mark.parametrize("park_scan", ["64", "2000"])
+def test_delete_items_handed_to_wide_kernels_match_oracle(park_scan, monkeypatch):
+    """Deletion items whose descent or ascent scans more than SBC_PARK_SCAN
+    entries are finished by the wide kernels, which mark the shallower
+    endpoint as a parent once the deeper one's level is done."""
+    monkeypatch.setenv("SBC_PARK_SCAN", park_scan)
+    hg, deg = _hub_graph()
+    roots = hg.pick_roots(24, 7)
+    print(f"[delete] parked items, park_scan={park_scan} nv={hg.nv}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    rng = np.random.default_rng(11)
+    src = np.repeat(np.arange(hg.nv), np.diff(hg.rowptr))
+    picks = rng.choice(hg.ne, 12, replace=False)
+    ops = []
+    for e in picks:
+        a, b = int(src[e]), int(hg.col[e])
+        if (0, min(a, b), max(a, b)) not in ops:
+            ops.append((0, min(a, b), max(a, b)))
+    parked = 0
+    adjacent = 0
+    for op in ops:
+        c = _apply_ops(f, g, ctx, o, [op])[0]
+        parked += ctx.last_parked()
+        adjacent += int(c[1])
+    assert adjacent > 0 and parked > 0, f"no deletion item was handed over (adjacent roots {adjacent})"
+    _compare_trees(f, o, f"deletions with park_scan {park_scan}")
+    assert_bc_close(f.bc(), o.bc())
