@@ -2589,6 +2589,9 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
 // lane order -- so the tree is bit-identical to that kernel's.  Every
 // workgroup writes the root's arrays here, so the atomics use device scope.
 
+// chunks of 64 neighbours whose loads one wavefront issues together
+constexpr int ROOT_UNROLL = 8;
+
 __device__ __forceinline__ int32_t cas32_dev(int32_t* p, int32_t expect, int32_t want)
 {
     __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -2629,29 +2632,51 @@ __global__ void sbc_root_bfs_kernel(SbcParams P, int ri, int fs, int fe, int j, 
         const int32_t c = list[idx];
         const int64_t sc = vr[c].sigma;
         const SbcVInfo vi = P.vinfo[c];
-        for (int e0 = 0; e0 < vi.deg; e0 += 64) {
-            const int e = e0 + lane;
-            bool app = false;
-            int32_t k = 0;
-            if (e < vi.deg) {
-                k = P.col[vi.start + e];
-                int32_t wk = __hip_atomic_load(&dist[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (wk == INF32) {
-                    const int32_t old = cas32_dev(&dist[k], INF32, next);
-                    app = old == INF32;
-                    wk = app ? next : old;
+        // ROOT_UNROLL chunks per round: ids, then levels, then the CASes in
+        // flight together; one list reservation for all their appends, then
+        // the added paths (no value returned)
+        for (int e0 = 0; e0 < vi.deg; e0 += 64 * ROOT_UNROLL) {
+            int32_t k[ROOT_UNROLL], wk[ROOT_UNROLL];
+            bool app[ROOT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                const int e = e0 + u * 64 + lane;
+                k[u] = e < vi.deg ? P.col[vi.start + e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u)
+                wk[u] = k[u] >= 0 ? __hip_atomic_load(&dist[k[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                app[u] = false;
+                if (k[u] >= 0 && wk[u] == INF32) {
+                    const int32_t old = cas32_dev(&dist[k[u]], INF32, next);
+                    app[u] = old == INF32;
+                    wk[u] = app[u] ? next : old;
                 }
-                if (wk == next)
-                    __hip_atomic_fetch_add((unsigned long long*)&vr[k].dpath, (unsigned long long)sc,
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            const uint64_t m = ballot(app);
-            if (m) {
+            uint64_t m[ROOT_UNROLL];
+            int tot = 0;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                m[u] = ballot(app[u]);
+                tot += __popcll(m[u]);
+            }
+            if (tot) {
                 int base = 0;
-                if (lane == 0) base = atomicAdd(&fr[0], __popcll(m));
+                if (lane == 0) base = atomicAdd(&fr[0], tot);
                 base = __shfl(base, 0);
-                if (app) list[base + prefix_in_wave(m)] = k;
+#pragma unroll
+                for (int u = 0; u < ROOT_UNROLL; ++u) {
+                    if (app[u]) list[base + prefix_in_wave(m[u])] = k[u];
+                    base += __popcll(m[u]);
+                }
             }
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u)
+                if (k[u] >= 0 && wk[u] == next)
+                    __hip_atomic_fetch_add((unsigned long long*)&vr[k[u]].dpath, (unsigned long long)sc,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -2684,22 +2709,29 @@ __global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L)
         const int64_t sk = vr[k].sigma;
         const SbcVInfo vi = P.vinfo[k];
         double acc = 0.0;
-        for (int e0 = 0; e0 < vi.deg; e0 += 64) {
-            const int e = e0 + lane;
-            bool child = false;
-            double add = 0.0;
-            if (e < vi.deg) {
-                const int32_t c = P.col[vi.start + e];
-                if (dist[c] == childw) {
-                    child = true;
-                    add = dep(sk, vr[c].sigma, vr[c].delta);
-                }
+        // ROOT_UNROLL chunks per round: ids, levels, then the children's
+        // records in flight together; the adds stay in adjacency order
+        for (int e0 = 0; e0 < vi.deg; e0 += 64 * ROOT_UNROLL) {
+            int32_t c[ROOT_UNROLL];
+            bool child[ROOT_UNROLL];
+            double add[ROOT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                const int e = e0 + u * 64 + lane;
+                c[u] = e < vi.deg ? P.col[vi.start + e] : -1;
             }
-            uint64_t m = ballot(child);
-            while (m) {
-                const int l = __builtin_ctzll(m);
-                m &= m - 1;
-                acc = acc + readlane_d(add, l);
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) child[u] = c[u] >= 0 && dist[c[u]] == childw;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) add[u] = child[u] ? dep(sk, vr[c[u]].sigma, vr[c[u]].delta) : 0.0;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                uint64_t m = ballot(child[u]);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    acc = acc + readlane_d(add[u], l);
+                }
             }
         }
         if (lane == 0) vr[k].delta = acc;
