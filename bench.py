@@ -151,7 +151,7 @@ def main():
     ap.add_argument("--park-min", type=int, default=0,
                     help="descent frontier size at which an item is finished by whole-GPU kernels "
                          "(0: library default, <0: never)")
-    ap.add_argument("--deletions", type=int, default=4,
+    ap.add_argument("--deletions", type=int, default=16,
                     help="edges of the initial graph deleted (and timed) after the insertion legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (PMC traffic calibration)")
@@ -307,7 +307,7 @@ def main():
                     "static_kernel_ms_avg": round(sms / max(sn, 1), 3),
                     "static_GBs": round(sbytes / (sms * 1e-3) / 1e9, 1) if sms > 0 else None,
                     "method": "adjacent-case roots: update kernel with negated added paths; movement-case "
-                              "roots: rebuilt by the static pass; BC folded again"}
+                              "roots: rebuilt by the static pass; BC updated by the same changes"}
         print(f"[bench] deletions: {deletion['value']:.2f} del/s, cases {deletion['cases']}",
               file=sys.stderr, flush=True)
 

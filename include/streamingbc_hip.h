@@ -122,10 +122,12 @@ StreamingExtraInfo insertEdgeStreamingBC(bcForest *forest, struct stinger *sStin
 /* src/streamingbc.h:57-58, src/streamingbc.c:258-416 -- the edge (oldU,oldV)
  * must already have been removed with stinger_remove_edge in both directions,
  * and every inserted edge applied.  Per root: endpoints on the same level
- * leave the tree as it is; otherwise (the reference's adjacent and movement
- * cases) the root's tree is rebuilt on the device by the static pass over the
- * post-deletion graph.  BC is then folded again from every tree.  Returns the
- * root counts of the three cases.  loadBalancing, granularity and NT are
+ * leave the tree as it is; the adjacent case (the deeper endpoint keeps a
+ * parent) runs the device update with the lost paths subtracted; the movement
+ * case rebuilds the root's tree by the static pass over the post-deletion
+ * graph.  BC changes by the same amounts (device FP64 atomics, as after an
+ * insertion; streamingBCRecomputeBC folds it deterministically).  Returns
+ * the root counts of the three cases.  loadBalancing, granularity and NT are
  * ignored. */
 StreamingExtraInfo deleteEdgeStreamingBC(bcForest *forest, struct stinger *sStinger, int64_t oldU,
                                          int64_t oldV, int64_t *rootArrayForApproximation, int64_t NK,

@@ -975,9 +975,10 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
 // descendants lose sigma[p] paths and dependencies are pulled again up to the
 // root -- the update kernel's adjacent case with the added paths negated;
 // otherwise the movement case (moveDownTreeBrandes, 880-1401), whose roots
-// are rebuilt by the static pass over the post-deletion graph.  BC is then
-// folded again from every tree.  counts: same-level, adjacent and movement
-// roots, as the reference's StreamingExtraInfo.
+// are rebuilt by the static pass over the post-deletion graph.  BC changes by
+// the adjacent roots' atomic updates and by taking the movement roots out
+// before their rebuild and back after.  counts: same-level, adjacent and
+// movement roots, as the reference's StreamingExtraInfo.
 static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K);
 static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
                               int64_t* total);
@@ -1074,6 +1075,10 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
         C->last_parked = parked;
     }
+    // movement roots leave BC before their rebuild and return after it; the
+    // update kernel adds the adjacent roots' changes itself (as insertions do)
+    int32_t* mlist = C->d_del + 4 + F->K;
+    if (h[3] > 0) HIPCHK(sbc_launch_bc_roots(&P, mlist, C->d_del + 3, -1.0, C->stream));
     if (h[3] > 0 && h[3] <= wide_rebuild_max()) {
         // few movement roots: each rebuilt by the whole GPU
         std::vector<int32_t> mr((size_t)h[3]);
@@ -1095,9 +1100,7 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(sbc_launch_static(&Q, 16, grid, C->stream));
         if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
     }
-    if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
-    HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
-    if ((st = ev_end(C, KSTAT_BC, e0))) return st;
+    if (h[3] > 0) HIPCHK(sbc_launch_bc_roots(&P, mlist, C->d_del + 3, 1.0, C->stream));
     C->last_rec_n = 0;
     if ((st = check_kernel_err(F, C))) return st;
     counts[0] = h[1];

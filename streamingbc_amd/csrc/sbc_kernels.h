@@ -199,6 +199,9 @@ hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, i
 hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
+// BC[v] += sign * sum over root indices list[0, *cnt) of delta_r[v] (r != v).
+hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* list, const int32_t* cnt, double sign,
+                               hipStream_t st);
 // Per root, the dispatch of deleting (u, v) (already removed from the graph):
 // out[1..3] count the same-level, adjacent and movement roots; adjacent roots
 // are listed at out[4..4+K), movement roots at out[4+K..4+2K).  out[0..3]

@@ -2738,6 +2738,23 @@ __global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L)
     }
 }
 
+// BC[v] += sign * (sum over the listed roots list[0, *cnt) of delta_r[v],
+// r != v): an edge deletion takes the roots it rebuilds out of BC before the
+// rebuild (sign -1) and puts them back after (+1).
+__global__ void sbc_bc_roots_kernel(SbcParams P, const int32_t* list, const int32_t* cnt, double sign)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.nv) return;
+    const int n = *cnt;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int ri = list[i];
+        if (P.roots[ri] == v) continue;
+        acc += P.vr[(size_t)ri * (size_t)P.nv + v].delta;
+    }
+    if (acc != 0.0) P.bc[v] = P.bc[v] + sign * acc;
+}
+
 // BC[v] = sum over roots, in root-array order, of delta_r[v] (r != v).
 __global__ void sbc_bc_reduce_kernel(SbcParams P)
 {
@@ -2913,6 +2930,14 @@ extern "C" hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, 
 extern "C" hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, hipStream_t st)
 {
     hipLaunchKernelGGL(sbc_root_ascent_kernel, dim3(root_grid(b - a, 4)), dim3(256), 0, st, *P, ri, a, b, L);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* list, const int32_t* cnt, double sign,
+                                          hipStream_t st)
+{
+    const int bs = 256;
+    hipLaunchKernelGGL(sbc_bc_roots_kernel, dim3((P->nv + bs - 1) / bs), dim3(bs), 0, st, *P, list, cnt, sign);
     return hipGetLastError();
 }
 
