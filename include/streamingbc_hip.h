@@ -134,6 +134,28 @@ StreamingExtraInfo deleteEdgeStreamingBC(bcForest *forest, struct stinger *sStin
                                          int64_t NV, int64_t NT, extraArraysPerThread **eAPT,
                                          uint32_t loadBalancing, uint32_t granularity);
 
+/* src/streamingbc.h:61-62, src/streamingbc.c:56-77 -- connects src to each
+ * vertex of adjacencyArray (inserting both directions itself) and updates
+ * every root, as one insertion per edge in list order (run as one batched
+ * call: the same trees).  Edges already present, self loops and bad ids are
+ * skipped (the reference would update a present edge again).  Returns zero
+ * counts, as the reference's insertions do. */
+StreamingExtraInfo insertVertexStreamingBC(bcForest *forest, struct stinger *sStinger, int64_t src,
+                                           int64_t *adjacencyArray, int64_t adjacencySize,
+                                           int64_t *rootArrayForApproximation, int64_t NK, int64_t NV,
+                                           int64_t NT, extraArraysPerThread **eAPT);
+/* src/streamingbc.h:63-65, src/streamingbc.c:229-256 -- removes every edge of
+ * src in list order, one deletion update each (deleteEdgeStreamingBC).  The
+ * removed neighbours are written to adjacencyArray and their number to
+ * *adjacencySize; a positive *adjacencySize on entry is the array's capacity
+ * (otherwise it must hold src's degree).  (The reference never advances its
+ * output index: it reports 0 and keeps only the last neighbour.)  Returns the
+ * summed case counts of the deletions. */
+StreamingExtraInfo deleteVertexStreamingBC(bcForest *forest, struct stinger *sStinger, int64_t src,
+                                           int64_t *adjacencyArray, int64_t *adjacencySize,
+                                           int64_t *rootArrayForApproximation, int64_t NK, int64_t NV,
+                                           int64_t NT, extraArraysPerThread **eAPT);
+
 /* src/streamingbc.h:68 -- builds the device graph from a CSR whose
  * vertexPointerArray has NV+1 entries and edgeArray NE entries */
 void CreateStingerFromCSR(csrGraph *csr, struct stinger **stingerGraph);
@@ -174,6 +196,13 @@ sbcStatus sbcInsertEdge(bcForest *forest, struct stinger *G, extraArraysPerThrea
  * modified) if the edge is not there.  info (may be NULL) gets the case counts. */
 sbcStatus sbcDeleteEdge(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t u,
                         int64_t v, StreamingExtraInfo *info);
+/* Status forms of insertVertexStreamingBC / deleteVertexStreamingBC: adj may
+ * be NULL for a deletion; more neighbours than cap is SBC_ERR_ARG before
+ * anything changes; *n (may be NULL) gets the number removed. */
+sbcStatus sbcInsertVertex(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t src,
+                          const int64_t *adj, int64_t n);
+sbcStatus sbcDeleteVertex(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t src,
+                          int64_t *adj, int64_t cap, int64_t *n, StreamingExtraInfo *info);
 
 /* Copies BC (NV doubles) to out (may be NULL) and refreshes forest->totalBC. */
 sbcStatus streamingBCGetBC(bcForest *forest, double *out);

@@ -107,6 +107,8 @@ _sig("sbcForestCreate", C.c_int, [i64, i64, P64, C.POINTER(FP)])
 _sig("sbcInitStatic", C.c_int, [FP, VP, VPP])
 _sig("sbcInsertEdge", C.c_int, [FP, VP, VPP, i64, i64, u32])
 _sig("sbcDeleteEdge", C.c_int, [FP, VP, VPP, i64, i64, C.POINTER(ExtraInfoC)])
+_sig("sbcInsertVertex", C.c_int, [FP, VP, VPP, i64, P64, i64])
+_sig("sbcDeleteVertex", C.c_int, [FP, VP, VPP, i64, P64, i64, P64, C.POINTER(ExtraInfoC)])
 _sig("streamingBCCreateAuxilary", VPP, [i64, i64])
 _sig("streamingBCDeleteAuxilary", None, [VPP, i64, i64])
 _sig("streamingBCDeleteForestApproximate", None, [C.POINTER(FP), i64, P64])
@@ -397,6 +399,20 @@ class Forest:
         info = ExtraInfoC()
         _check(lib.sbcDeleteEdge(self._p, g._p, ctx._p, u, v, C.byref(info)))
         return {"sameLevel": info.sameLevel, "adjacent": info.adjacent, "movement": info.movement}
+
+    def insert_vertex(self, g: Graph, ctx: Context, src: int, adj) -> None:
+        """Connects src to every listed vertex (edges already there skipped)."""
+        adj = np.ascontiguousarray(adj, dtype=np.int64)
+        _check(lib.sbcInsertVertex(self._p, g._p, ctx._p, src, _p64(adj), adj.shape[0]))
+
+    def delete_vertex(self, g: Graph, ctx: Context, src: int):
+        """Removes every edge of src; returns (removed neighbours in list
+        order, summed case counts)."""
+        out = np.zeros(max(g.degree(src), 1), dtype=np.int64)
+        n = np.zeros(1, dtype=np.int64)
+        info = ExtraInfoC()
+        _check(lib.sbcDeleteVertex(self._p, g._p, ctx._p, src, _p64(out), out.shape[0], _p64(n), C.byref(info)))
+        return out[:int(n[0])], {"sameLevel": info.sameLevel, "adjacent": info.adjacent, "movement": info.movement}
 
     def update_deleted(self, g: Graph, ctx: Context, u: int, v: int) -> dict:
         """deleteEdgeStreamingBC for an edge already removed with Graph.remove_edge."""

@@ -1898,3 +1898,124 @@ extern "C" sbcStatus streamingBCScheduleWindow(bcForest* forest, struct stinger*
     C->order_forest = F;
     return SBC_OK;
 }
+
+// ------------------------------------------------------------ vertex updates
+
+// src/streamingbc.c:56-77: connects src to every listed vertex.  The edges not
+// yet in the graph (self loops, bad ids and duplicates skipped: the reference
+// would run the update again on an edge already there) are inserted in list
+// order by one batched call, whose trees equal those of one update per edge.
+static sbcStatus insert_vertex(bcForest* forest, stinger* G, extraArraysPerThread** eAPT, int64_t src,
+                               const int64_t* adj, int64_t n)
+{
+    if (!forest || !G || !eAPT || !eAPT[0] || (n > 0 && !adj)) return set_err(SBC_ERR_ARG, "null argument");
+    if (src < 0 || src >= G->nv || n < 0) return set_err(SBC_ERR_ARG, "vertex id or list size out of range");
+    std::vector<int64_t> u, v;
+    std::vector<char> seen;
+    for (int64_t d = 0; d < n; d++) {
+        const int64_t x = adj[d];
+        if (x < 0 || x >= G->nv || x == src || graph_has_edge(G, (int32_t)src, (int32_t)x)) continue;
+        if (seen.empty()) seen.assign((size_t)G->nv, 0);
+        if (seen[(size_t)x]) continue;
+        seen[(size_t)x] = 1;
+        u.push_back(src);
+        v.push_back(x);
+    }
+    if (u.empty()) return SBC_OK;
+    sbcBatchConfig cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.window = u.size() < 128 ? (int32_t)u.size() : 128;
+    cfg.block_threads = 1024;
+    cfg.heavy_roots = -1;
+    return streamingBCInsertEdgesBatched(forest, G, eAPT, u.data(), v.data(), (int64_t)u.size(), &cfg, nullptr);
+}
+
+// src/streamingbc.c:229-256: removes every edge of src in list order, one
+// deletion update each.  The removed neighbours go to adj (at most cap of
+// them; more is an error before anything changes) and their number to *n;
+// counts sum the deletions' case counts.  (The reference never advances its
+// output index, so it reports 0 neighbours and keeps only the last one.)
+static sbcStatus delete_vertex(bcForest* forest, stinger* G, extraArraysPerThread** eAPT, int64_t src,
+                               int64_t* adj, int64_t cap, int64_t* n, int64_t counts[3])
+{
+    if (!forest || !G || !eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null argument");
+    if (src < 0 || src >= G->nv) return set_err(SBC_ERR_ARG, "vertex id out of range");
+    const std::vector<int32_t> nb(G->col.begin() + G->start[src], G->col.begin() + G->start[src] + G->deg[src]);
+    if (adj && (int64_t)nb.size() > cap)
+        return set_err(SBC_ERR_ARG, "vertex %ld has %ld neighbours, the array holds %ld", (long)src, (long)nb.size(),
+                       (long)cap);
+    counts[0] = counts[1] = counts[2] = 0;
+    int64_t d = 0;
+    for (int32_t x : nb) {
+        stinger_remove_edge(G, 0, src, x);
+        stinger_remove_edge(G, 0, x, src);
+        int64_t c[3];
+        sbcStatus st = run_delete(forest, G, eAPT[0], src, x, c);
+        if (st) return st;
+        for (int i = 0; i < 3; i++) counts[i] += c[i];
+        if (adj) adj[d] = x;
+        d++;
+    }
+    if (n) *n = d;
+    return SBC_OK;
+}
+
+extern "C" StreamingExtraInfo insertVertexStreamingBC(bcForest* forest, struct stinger* sStinger, int64_t src,
+                                                      int64_t* adjacencyArray, int64_t adjacencySize,
+                                                      int64_t* rootArrayForApproximation, int64_t NK, int64_t NV,
+                                                      int64_t NT, extraArraysPerThread** eAPT)
+{
+    (void)rootArrayForApproximation; (void)NT;
+    StreamingExtraInfo r = {0, 0, 0, 0};
+    if (!forest || NK != forest->dev->K || NV != forest->dev->nv) {
+        set_err(SBC_ERR_ARG, "NK/NV do not match the forest");
+        legacy_abort("insertVertexStreamingBC");
+    }
+    if (insert_vertex(forest, sStinger, eAPT, src, adjacencyArray, adjacencySize) != SBC_OK)
+        legacy_abort("insertVertexStreamingBC");
+    return r;
+}
+
+extern "C" StreamingExtraInfo deleteVertexStreamingBC(bcForest* forest, struct stinger* sStinger, int64_t src,
+                                                      int64_t* adjacencyArray, int64_t* adjacencySize,
+                                                      int64_t* rootArrayForApproximation, int64_t NK, int64_t NV,
+                                                      int64_t NT, extraArraysPerThread** eAPT)
+{
+    (void)rootArrayForApproximation; (void)NT;
+    StreamingExtraInfo r = {0, 0, 0, 0};
+    if (!forest || !sStinger || NK != forest->dev->K || NV != forest->dev->nv || src < 0 || src >= sStinger->nv) {
+        set_err(SBC_ERR_ARG, "NK/NV do not match the forest, or bad vertex");
+        legacy_abort("deleteVertexStreamingBC");
+    }
+    // capacity: *adjacencySize when positive on entry, else the degree
+    const int64_t cap = adjacencySize && *adjacencySize > 0 ? *adjacencySize : sStinger->deg[src];
+    int64_t c[3];
+    if (delete_vertex(forest, sStinger, eAPT, src, adjacencyArray, cap, adjacencySize, c) != SBC_OK)
+        legacy_abort("deleteVertexStreamingBC");
+    r.sameLevel = c[0];
+    r.adjacent = c[1];
+    r.movement = c[2];
+    return r;
+}
+
+extern "C" sbcStatus sbcInsertVertex(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT, int64_t src,
+                                     const int64_t* adj, int64_t n)
+{
+    if (forest && !forest->dev->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    return insert_vertex(forest, G, eAPT, src, adj, n);
+}
+
+extern "C" sbcStatus sbcDeleteVertex(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT, int64_t src,
+                                     int64_t* adj, int64_t cap, int64_t* n, StreamingExtraInfo* info)
+{
+    if (forest && !forest->dev->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    int64_t c[3] = {0, 0, 0};
+    sbcStatus st = delete_vertex(forest, G, eAPT, src, adj, cap, n, c);
+    if (st == SBC_OK && info) {
+        info->sameLevel = c[0];
+        info->adjacent = c[1];
+        info->movement = c[2];
+        info->connectedComponents = 0;
+    }
+    return st;
+}

@@ -608,3 +608,54 @@ def test_delete_items_handed_to_wide_kernels_match_oracle(park_scan, monkeypatch
     assert adjacent > 0 and parked > 0, f"no deletion item was handed over (adjacent roots {adjacent})"
     _compare_trees(f, o, f"deletions with park_scan {park_scan}")
     assert_bc_close(f.bc(), o.bc())
+
+
+def _assert_fold_exact(f, ctx, o, roots):
+    """streamingBCRecomputeBC folds the trees in root order: equal bit for bit
+    to the same fold of the oracle's trees."""
+    f.recompute_bc(ctx)
+    _, _, dl = o.trees()
+    acc = np.zeros(dl.shape[1])
+    for ri, r in enumerate(roots):
+        d = dl[ri].copy()
+        d[int(r)] = 0.0
+        acc = acc + d
+    assert np.array_equal(f.bc(), 0.0 + acc)
+
+
+def test_vertex_delete_and_insert_match_oracle():
+    """deleteVertexStreamingBC removes every edge of a vertex in list order,
+    one deletion update each; insertVertexStreamingBC connects it again (one
+    batched call, equal to one insertion per edge); duplicates and self loops
+    in the list are skipped."""
+    hg = sbc.read_dimacs(karate_path())
+    roots = np.arange(hg.nv)
+    print("[vertex] karate delete/insert vertex 1", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    nb = [int(x) for x in hg.col[hg.rowptr[1]:hg.rowptr[2]]]
+    removed, c = f.delete_vertex(g, ctx, 1)
+    assert list(removed) == nb and g.degree(1) == 0
+    tot = np.zeros(3, dtype=np.int64)
+    for x in nb:
+        tot += o.delete(1, x)
+    assert (c["sameLevel"], c["adjacent"], c["movement"]) == tuple(int(t) for t in tot)
+    _compare_trees(f, o, "after deleting vertex 1")
+    # 16 consecutive deletions take the most central vertex's BC (and its
+    # neighbours' shares) to 0: device and oracle each keep rounding residues
+    # of ~1e-12 there, above 1e-9 x the 1e-6 x max|BC| floor
+    assert_bc_close(f.bc(), o.bc(), rel=1e-8)
+    _assert_fold_exact(f, ctx, o, roots)
+    again = [34, 2, 3, 3, 1, 12, 33, 2]
+    f.insert_vertex(g, ctx, 1, again)
+    seen = set()
+    for x in again:
+        if x != 1 and x not in seen:
+            seen.add(x)
+            o.insert(1, x, O_FINE)
+    assert g.degree(1) == len(seen)
+    _compare_trees(f, o, "after inserting vertex 1 again")
+    _assert_fold_exact(f, ctx, o, roots)
+    removed, _ = f.delete_vertex(g, ctx, 1)
+    assert list(removed) == [34, 2, 3, 12, 33]
