@@ -204,6 +204,17 @@ sbcStatus sbcInsertVertex(bcForest *forest, struct stinger *G, extraArraysPerThr
 sbcStatus sbcDeleteVertex(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t src,
                           int64_t *adj, int64_t cap, int64_t *n, StreamingExtraInfo *info);
 
+/* Checkpoint / resume (the reference's only precedent is STINGER's own
+ * save/open, src/stinger/stinger.c:1808-2038): sbcCheckpointSave writes the
+ * graph (slot layout with slack, insertion sequence numbers, lookup tables)
+ * and every root's tree, state bytes and BC to one file after the device is
+ * idle; sbcCheckpointLoad rebuilds a graph and forest from it that continue
+ * bit-identically.  A file that is not a checkpoint of this format, or is
+ * truncated, is SBC_ERR_ARG before any device allocation.  Host memory: one
+ * root's arrays at a time. */
+sbcStatus sbcCheckpointSave(const bcForest *forest, const struct stinger *G, const char *path);
+sbcStatus sbcCheckpointLoad(const char *path, bcForest **forest, struct stinger **G);
+
 /* Copies BC (NV doubles) to out (may be NULL) and refreshes forest->totalBC. */
 sbcStatus streamingBCGetBC(bcForest *forest, double *out);
 

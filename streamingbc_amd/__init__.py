@@ -108,6 +108,8 @@ _sig("sbcInitStatic", C.c_int, [FP, VP, VPP])
 _sig("sbcInsertEdge", C.c_int, [FP, VP, VPP, i64, i64, u32])
 _sig("sbcDeleteEdge", C.c_int, [FP, VP, VPP, i64, i64, C.POINTER(ExtraInfoC)])
 _sig("sbcInsertVertex", C.c_int, [FP, VP, VPP, i64, P64, i64])
+_sig("sbcCheckpointSave", C.c_int, [FP, VP, C.c_char_p])
+_sig("sbcCheckpointLoad", C.c_int, [C.c_char_p, C.POINTER(FP), C.POINTER(C.c_void_p)])
 _sig("sbcDeleteVertex", C.c_int, [FP, VP, VPP, i64, P64, i64, P64, C.POINTER(ExtraInfoC)])
 _sig("streamingBCCreateAuxilary", VPP, [i64, i64])
 _sig("streamingBCDeleteAuxilary", None, [VPP, i64, i64])
@@ -331,6 +333,29 @@ class Graph:
 
     def degree(self, v: int) -> int:
         return lib.stinger_outdegree(self._p, v)
+
+
+def checkpoint_save(forest: "Forest", g: Graph, path: str) -> None:
+    """Writes the graph and every root's tree and BC to path (sbcCheckpointSave)."""
+    _check(lib.sbcCheckpointSave(forest._p, g._p, os.fsencode(path)))
+
+
+def checkpoint_load(path: str):
+    """Rebuilds (Forest, Graph) from a checkpoint (sbcCheckpointLoad)."""
+    fp = FP()
+    gp = C.c_void_p()
+    _check(lib.sbcCheckpointLoad(os.fsencode(path), C.byref(fp), C.byref(gp)))
+    hdr = np.fromfile(path, dtype=np.int64, count=8, offset=8)
+    nv, k = int(hdr[1]), int(hdr[2])
+    g = Graph.__new__(Graph)
+    g.nv = nv
+    g._p = gp
+    f = Forest.__new__(Forest)
+    f.nv = nv
+    f.roots = np.fromfile(path, dtype=np.int64, count=k, offset=8 + 64)
+    f.k = k
+    f._p = fp
+    return f, g
 
 
 def forest_footprint(nv: int, k: int) -> int:

@@ -2019,3 +2019,157 @@ extern "C" sbcStatus sbcDeleteVertex(bcForest* forest, struct stinger* G, extraA
     }
     return st;
 }
+
+// ------------------------------------------------------------ checkpoint / resume
+//
+// One file per resident graph and its forest (the reference's only precedent
+// is STINGER's own save/open, src/stinger/stinger.c:1808-2038).  Layout,
+// little-endian: "SBCCKPT1", int64 {version 1, nv, K, exact, initialized,
+// slot count, table slot count, next sequence number}, roots[K] (int64); the
+// graph's host mirror: start/deg/deg0/cap[nv] (int32), col/seq[slots] (int32),
+// hub[nv] (int4), htab[table slots] (int2); per root: dist[nv] (int32), state
+// bytes[nv], records[nv] (32 B); BC[nv] (double); "SBCEND01".  Loading checks
+// the header and the exact file size before any device allocation.
+
+static const char kCkptMagic[8] = {'S', 'B', 'C', 'C', 'K', 'P', 'T', '1'};
+static const char kCkptEnd[8] = {'S', 'B', 'C', 'E', 'N', 'D', '0', '1'};
+
+static int64_t ckpt_size(int64_t nv, int64_t K, int64_t slots, int64_t tslots)
+{
+    return 8 + 8 * 8 + 8 * K + 16 * nv + 8 * slots + 16 * nv + 8 * tslots +
+           K * nv * (4 + 1 + (int64_t)sizeof(SbcVRec)) + 8 * nv + 8;
+}
+
+extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stinger* G, const char* path)
+{
+    if (!forest || !G || !path) return set_err(SBC_ERR_ARG, "null argument");
+    const sbcForestDev* F = forest->dev;
+    if (F->nv != G->nv) return set_err(SBC_ERR_ARG, "forest NV %ld != graph NV %ld", (long)F->nv, (long)G->nv);
+    HIPCHK(hipDeviceSynchronize());
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return set_err(SBC_ERR_ARG, "cannot open %s for writing", path);
+    const int64_t nv = F->nv, K = F->K;
+    const int64_t hdr[8] = {1, nv, K, F->exact ? 1 : 0, F->initialized ? 1 : 0, G->total_cap,
+                            (int64_t)G->htab.size(), G->seq_next};
+    bool ok = fwrite(kCkptMagic, 1, 8, fp) == 8 && fwrite(hdr, 8, 8, fp) == 8 &&
+              fwrite(F->roots.data(), 8, (size_t)K, fp) == (size_t)K;
+    for (const std::vector<int32_t>* a : {&G->start, &G->deg, &G->deg0, &G->cap})
+        ok = ok && fwrite(a->data(), 4, (size_t)nv, fp) == (size_t)nv;
+    ok = ok && fwrite(G->col.data(), 4, (size_t)G->total_cap, fp) == (size_t)G->total_cap &&
+         fwrite(G->seq.data(), 4, (size_t)G->total_cap, fp) == (size_t)G->total_cap &&
+         fwrite(G->hub.data(), sizeof(int4), (size_t)nv, fp) == (size_t)nv &&
+         fwrite(G->htab.data(), sizeof(int2), G->htab.size(), fp) == G->htab.size();
+    // per root, through one host buffer of nv records
+    std::vector<int32_t> d((size_t)nv);
+    std::vector<uint8_t> b((size_t)nv);
+    std::vector<SbcVRec> r((size_t)nv);
+    for (int64_t ri = 0; ok && ri < K; ri++) {
+        const size_t off = (size_t)ri * (size_t)nv;
+        if (hipMemcpy(d.data(), F->d_dist + off, 4 * (size_t)nv, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(b.data(), F->d_sb + (size_t)ri * (size_t)F->sb_stride, (size_t)nv, hipMemcpyDeviceToHost) !=
+                hipSuccess ||
+            hipMemcpy(r.data(), F->d_vr + off, sizeof(SbcVRec) * (size_t)nv, hipMemcpyDeviceToHost) != hipSuccess) {
+            fclose(fp);
+            return set_err(SBC_ERR_HIP, "checkpoint: device copy failed");
+        }
+        ok = fwrite(d.data(), 4, (size_t)nv, fp) == (size_t)nv && fwrite(b.data(), 1, (size_t)nv, fp) == (size_t)nv &&
+             fwrite(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv;
+    }
+    std::vector<double> bc((size_t)nv);
+    if (ok && hipMemcpy(bc.data(), F->d_bc, 8 * (size_t)nv, hipMemcpyDeviceToHost) != hipSuccess) {
+        fclose(fp);
+        return set_err(SBC_ERR_HIP, "checkpoint: device copy failed");
+    }
+    ok = ok && fwrite(bc.data(), 8, (size_t)nv, fp) == (size_t)nv && fwrite(kCkptEnd, 1, 8, fp) == 8;
+    if (fclose(fp) != 0) ok = false;
+    if (!ok) return set_err(SBC_ERR_ARG, "checkpoint: writing %s failed", path);
+    return SBC_OK;
+}
+
+extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, struct stinger** graph)
+{
+    if (!path || !forest || !graph) return set_err(SBC_ERR_ARG, "null argument");
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return set_err(SBC_ERR_ARG, "cannot open %s", path);
+    char magic[8];
+    int64_t hdr[8];
+    bool ok = fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(hdr, 8, 8, fp) == 8;
+    const int64_t nv = ok ? hdr[1] : 0, K = ok ? hdr[2] : 0, slots = ok ? hdr[5] : 0, tslots = ok ? hdr[6] : 0;
+    ok = ok && hdr[0] == 1 && nv >= 1 && nv < INT32_MAX / 4 && K >= 1 && K <= nv && slots >= 0 &&
+         slots < INT32_MAX && tslots >= 0 && tslots < INT32_MAX && hdr[7] >= 1 && hdr[7] < INT32_MAX;
+    if (ok) {
+        // the exact size, before anything is allocated
+        if (fseek(fp, 0, SEEK_END) != 0) ok = false;
+        const long long sz = ok ? (long long)ftello(fp) : -1;
+        ok = ok && sz == ckpt_size(nv, K, slots, tslots) && fseek(fp, 8 + 64, SEEK_SET) == 0;
+    }
+    if (!ok) {
+        fclose(fp);
+        return set_err(SBC_ERR_ARG, "%s is not a checkpoint of this format, or is truncated", path);
+    }
+    std::vector<int64_t> roots((size_t)K);
+    stinger* G = new stinger();
+    G->nv = nv;
+    G->total_cap = slots;
+    G->seq_next = (int32_t)hdr[7];
+    for (std::vector<int32_t>* a : {&G->start, &G->deg, &G->deg0, &G->cap}) a->resize((size_t)nv);
+    G->col.resize((size_t)slots);
+    G->seq.resize((size_t)slots);
+    G->hub.resize((size_t)nv);
+    G->htab.resize((size_t)tslots);
+    ok = fread(roots.data(), 8, (size_t)K, fp) == (size_t)K;
+    for (std::vector<int32_t>* a : {&G->start, &G->deg, &G->deg0, &G->cap})
+        ok = ok && fread(a->data(), 4, (size_t)nv, fp) == (size_t)nv;
+    ok = ok && fread(G->col.data(), 4, (size_t)slots, fp) == (size_t)slots &&
+         fread(G->seq.data(), 4, (size_t)slots, fp) == (size_t)slots &&
+         fread(G->hub.data(), sizeof(int4), (size_t)nv, fp) == (size_t)nv &&
+         fread(G->htab.data(), sizeof(int2), (size_t)tslots, fp) == (size_t)tslots;
+    // the graph arrays must describe lists inside the slots and tables inside htab
+    for (int64_t v = 0; ok && v < nv; v++) {
+        const int4 h = G->hub[v];
+        ok = G->start[v] >= 0 && G->cap[v] >= 0 && G->deg[v] >= 0 && G->deg0[v] >= 0 && G->deg0[v] <= G->deg[v] &&
+             G->deg[v] <= G->cap[v] && (int64_t)G->start[v] + G->cap[v] <= slots &&
+             (h.y == 0 || (h.z > 0 && h.z < 31 && h.x >= 0 && (int64_t)h.x + (1LL << h.z) <= tslots && h.y <= G->deg[v]));
+    }
+    for (int64_t i = 0; ok && i < K; i++) ok = roots[(size_t)i] >= 0 && roots[(size_t)i] < nv;
+    for (int64_t e = 0; ok && e < slots; e++) ok = G->col[(size_t)e] >= 0 && G->col[(size_t)e] < nv;
+    if (!ok) {
+        fclose(fp);
+        delete G;
+        return set_err(SBC_ERR_ARG, "%s: inconsistent graph in checkpoint", path);
+    }
+    G->full_dirty = true;
+    bcForest* f = nullptr;
+    sbcStatus st = forest_create(nv, K, roots.data(), hdr[3] != 0, &f);
+    if (st) {
+        fclose(fp);
+        delete G;
+        return st;
+    }
+    sbcForestDev* F = f->dev;
+    std::vector<int32_t> d((size_t)nv);
+    std::vector<uint8_t> b((size_t)nv);
+    std::vector<SbcVRec> r((size_t)nv);
+    for (int64_t ri = 0; ok && ri < K; ri++) {
+        const size_t off = (size_t)ri * (size_t)nv;
+        ok = fread(d.data(), 4, (size_t)nv, fp) == (size_t)nv && fread(b.data(), 1, (size_t)nv, fp) == (size_t)nv &&
+             fread(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv &&
+             hipMemcpy(F->d_dist + off, d.data(), 4 * (size_t)nv, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(F->d_sb + (size_t)ri * (size_t)F->sb_stride, b.data(), (size_t)nv, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMemcpy(F->d_vr + off, r.data(), sizeof(SbcVRec) * (size_t)nv, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    ok = ok && fread(f->totalBC, 8, (size_t)nv, fp) == (size_t)nv &&
+         hipMemcpy(F->d_bc, f->totalBC, 8 * (size_t)nv, hipMemcpyHostToDevice) == hipSuccess &&
+         fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkptEnd, 8) == 0;
+    fclose(fp);
+    if (!ok) {
+        forest_destroy(&f);
+        delete G;
+        return set_err(SBC_ERR_ARG, "%s: reading the forest failed", path);
+    }
+    F->initialized = hdr[4] != 0;
+    *forest = f;
+    *graph = G;
+    return SBC_OK;
+}

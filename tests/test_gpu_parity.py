@@ -659,3 +659,39 @@ def test_vertex_delete_and_insert_match_oracle():
     _assert_fold_exact(f, ctx, o, roots)
     removed, _ = f.delete_vertex(g, ctx, 1)
     assert list(removed) == [34, 2, 3, 12, 33]
+
+
+def test_checkpoint_resume_is_bit_identical(tmp_path):
+    """A checkpoint taken after insertions, deletions (one rebuilding a hub's
+    lookup table) and a batched window restores trees and BC exactly, and the
+    resumed forest stays bit-identical to the original under further updates."""
+    hg, deg = _hub_graph()
+    roots = hg.pick_roots(24, 9)
+    hub = int(np.argmax(deg))
+    nb = hg.col[hg.rowptr[hub]:hg.rowptr[hub + 1]]
+    print(f"[checkpoint] nv={hg.nv} k={len(roots)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    u, v = hg.insertions(48, 12)
+    for a, b in zip(u[:4], v[:4]):
+        f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+    f.delete_edge(g, ctx, hub, int(nb[2]))
+    f.delete_edge(g, ctx, int(u[0]), int(v[0]))
+    f.insert_batched(g, ctx, u[4:20], v[4:20], sbc.BatchConfig(window=8, block_threads=1024))
+    path = str(tmp_path / "state.ckpt")
+    sbc.checkpoint_save(f, g, path)
+    f2, g2 = sbc.checkpoint_load(path)
+    ctx2 = sbc.Context(hg.nv)
+    for x, y in zip(f.trees(), f2.trees()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(f.bc(), f2.bc())
+    assert all(g.degree(x) == g2.degree(x) for x in range(hg.nv))
+    for ff, gg, cc in ((f, g, ctx), (f2, g2, ctx2)):
+        ff.insert_batched(gg, cc, u[20:40], v[20:40], sbc.BatchConfig(window=8, block_threads=1024))
+        ff.delete_edge(gg, cc, hub, int(nb[5]))
+        ff.insert_edge(gg, cc, int(u[40]), int(v[40]), sbc.FINE)
+        ff.recompute_bc(cc)
+    for x, y in zip(f.trees(), f2.trees()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(f.bc(), f2.bc())
+    f2.close()
+    g2.close()

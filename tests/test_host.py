@@ -128,3 +128,22 @@ def test_remove_edge_return_codes():
     assert g.remove_edge(1, hg.nv) == -1
     assert g.insert_edge(1, 2) == 1 and g.degree(1) == d1
     g.close()
+
+
+def test_checkpoint_load_rejects_bad_files(tmp_path):
+    """sbcCheckpointLoad checks the header and the exact file size before any
+    device allocation: a missing file, a wrong magic or a truncated body is
+    SBC_ERR_ARG."""
+    p = tmp_path / "state.ckpt"
+    with pytest.raises(sbc.SbcError) as e:
+        sbc.checkpoint_load(str(p))
+    assert e.value.status == sbc.SBC_ERR_ARG
+    p.write_bytes(b"NOTACKPT" + bytes(64))
+    with pytest.raises(sbc.SbcError) as e:
+        sbc.checkpoint_load(str(p))
+    assert e.value.status == sbc.SBC_ERR_ARG
+    hdr = np.array([1, 35, 35, 1, 1, 200, 0, 1], dtype=np.int64)
+    p.write_bytes(b"SBCCKPT1" + hdr.tobytes() + bytes(1000))
+    with pytest.raises(sbc.SbcError) as e:
+        sbc.checkpoint_load(str(p))
+    assert e.value.status == sbc.SBC_ERR_ARG
