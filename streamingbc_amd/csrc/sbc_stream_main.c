@@ -3,14 +3,19 @@
  * caller of the reference's streamingbc.h drives it: the InsertEdge /
  * updateEdgeNEW loop of src/sbcMain_utils.c:458-574 (stinger_insert_edge in
  * both directions, then insertEdgeStreamingBC timed with CLOCK_MONOTONIC),
- * against include/streamingbc_hip.h.
+ * and the DeleteEdge step of src/sbcMain_utils.c:473-591 (stinger_remove_edge
+ * in both directions, then deleteEdgeStreamingBC), against
+ * include/streamingbc_hip.h.
  *
  *   sbc_stream -g file.graph | -r SCALE  [-k ROOTS] [-e INSERTIONS] [-s SEED]
- *              [-G COARSE|FINE] [-w WINDOW]
+ *              [-G COARSE|FINE] [-w WINDOW] [-d DELETIONS]
  *
  * -r generates an undirected R-MAT graph (16 draws per vertex, largest
- * component); -w > 0 applies the insertions with the batched windowed form.
- * Prints one JSON line: insertions/s and a BC checksum.
+ * component); -w > 0 applies the insertions with the batched windowed form;
+ * -d then deletes the first DELETIONS inserted edges again, newest first.
+ * Prints one JSON line: insertions/s, deletions/s and BC checksums (after the
+ * static pass and at the end: equal within rounding when every insertion is
+ * deleted again).
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdint.h>
@@ -32,7 +37,7 @@ static double now_s(void)
 static void usage(void)
 {
     fprintf(stderr, "usage: sbc_stream -g file.graph | -r SCALE [-k ROOTS] [-e INSERTIONS] [-s SEED] "
-                    "[-G COARSE|FINE] [-w WINDOW]\n");
+                    "[-G COARSE|FINE] [-w WINDOW] [-d DELETIONS]\n");
     exit(2);
 }
 
@@ -40,7 +45,7 @@ int main(int argc, char **argv)
 {
     const char *path = NULL;
     int scale = 0, window = 0;
-    int64_t NK = 0, E = 64;
+    int64_t NK = 0, E = 64, D = 0;
     uint64_t seed = 1;
     uint32_t gran = FINE;
     for (int i = 1; i < argc; i++) {
@@ -50,6 +55,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-e") && i + 1 < argc) E = atoll(argv[++i]);
         else if (!strcmp(argv[i], "-s") && i + 1 < argc) seed = strtoull(argv[++i], NULL, 10);
         else if (!strcmp(argv[i], "-w") && i + 1 < argc) window = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-d") && i + 1 < argc) D = atoll(argv[++i]);
         else if (!strcmp(argv[i], "-G") && i + 1 < argc) gran = strcmp(argv[++i], "COARSE") ? FINE : COARSE;
         else usage();
     }
@@ -85,6 +91,9 @@ int main(int argc, char **argv)
     if (NK == NV) streamingBCInitStaticExact(forest, G, 1, eAPT);
     else streamingBCInitStaticApproximate(forest, G, 1, eAPT, NK, roots);
     double t_static = now_s() - t0;
+    streamingBCGetBC(forest, NULL);
+    double sum0 = 0.0;
+    for (int64_t v = 0; v < NV; v++) sum0 += forest->totalBC[v];
 
     double t_upd = 0.0;
     if (window > 0) {
@@ -109,6 +118,19 @@ int main(int argc, char **argv)
             t_upd += now_s() - t0;
         }
     }
+    if (D > E) D = E;
+    double t_del = 0.0;
+    int64_t cases[3] = {0, 0, 0};
+    for (int64_t i = D - 1; i >= 0; i--) {
+        stinger_remove_edge(G, 0, src[i], dst[i]);
+        stinger_remove_edge(G, 0, dst[i], src[i]);
+        t0 = now_s();
+        StreamingExtraInfo sei = deleteEdgeStreamingBC(forest, G, src[i], dst[i], roots, NK, NV, 1, eAPT, BALANCE, gran);
+        t_del += now_s() - t0;
+        cases[0] += sei.sameLevel;
+        cases[1] += sei.adjacent;
+        cases[2] += sei.movement;
+    }
     streamingBCGetBC(forest, NULL);
     double sum = 0.0, mx = 0.0;
     for (int64_t v = 0; v < NV; v++) {
@@ -116,8 +138,10 @@ int main(int argc, char **argv)
         if (forest->totalBC[v] > mx) mx = forest->totalBC[v];
     }
     printf("{\"nv\": %ld, \"roots\": %ld, \"insertions\": %ld, \"static_s\": %.4f, \"insertions_per_s\": %.2f, "
-           "\"bc_sum\": %.10e, \"bc_max\": %.10e}\n",
-           (long)NV, (long)NK, (long)E, t_static, E / (t_upd > 0 ? t_upd : 1e-9), sum, mx);
+           "\"deletions\": %ld, \"deletions_per_s\": %.2f, \"same_level\": %ld, \"adjacent\": %ld, "
+           "\"movement\": %ld, \"bc_sum_static\": %.10e, \"bc_sum\": %.10e, \"bc_max\": %.10e}\n",
+           (long)NV, (long)NK, (long)E, t_static, E / (t_upd > 0 ? t_upd : 1e-9), (long)D,
+           D / (t_del > 0 ? t_del : 1e-9), (long)cases[0], (long)cases[1], (long)cases[2], sum0, sum, mx);
 
     streamingBCDeleteAuxilary(eAPT, 1, NV);
     if (NK == NV) streamingBCDeleteForestExact(&forest);

@@ -281,6 +281,35 @@ def test_c_driver_matches_oracle(window):
     assert abs(res["bc_max"] - bo.max()) <= 1e-9 * abs(bo.max())
 
 
+def test_c_driver_deletes_back_to_the_static_bc():
+    """The C driver inserts 24 edges and deletes them again (newest first)
+    through deleteEdgeStreamingBC: the case counts equal the oracle's and BC
+    returns to the static pass's within rounding."""
+    import json
+    import os
+    import subprocess
+    from sbc_testutil import GOLDEN, ROOT
+    exe = os.path.join(ROOT, "streamingbc_amd", "lib", "sbc_stream")
+    path = os.path.join(GOLDEN, "graphs", "jazz.graph")
+    out = subprocess.run([exe, "-g", path, "-e", "24", "-s", "3", "-d", "24"], check=True,
+                         capture_output=True, text=True, timeout=120)
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    hg = sbc.read_dimacs(path)
+    roots = np.arange(hg.nv)
+    u, v = hg.insertions(24, 13)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    for a, b in zip(u, v):
+        o.insert(int(a), int(b), O_FINE)
+    tot = np.zeros(3, dtype=np.int64)
+    for a, b in zip(u[::-1], v[::-1]):
+        tot += o.delete(int(a), int(b))
+    assert res["deletions"] == 24
+    assert (res["same_level"], res["adjacent"], res["movement"]) == tuple(int(t) for t in tot)
+    assert abs(res["bc_sum"] - res["bc_sum_static"]) <= 1e-9 * abs(res["bc_sum_static"])
+    assert abs(res["bc_max"] - o.bc().max()) <= 1e-9 * abs(o.bc().max())
+
+
 @pytest.mark.skip(reason="shared levels (idle workgroups taking chunks of another root's large BFS levels) "
                          "faulted the GPU twice; the path is out of the library until the cause is found "
                          "on the CPU, and SBC_ASSIST_* are ignored meanwhile")
