@@ -628,6 +628,12 @@ struct extraArraysPerThread {
     int64_t hubq_blocks = 0;
     int32_t* d_del = nullptr;   // edge deletion: 4 counts, then the affected root indices
     int64_t del_k = 0;
+    // whole-GPU rebuild of one root: a level's big vertices, their term offsets and terms
+    int32_t* d_big = nullptr;
+    int64_t big_cap = 0;
+    int64_t* d_toff = nullptr;
+    double* d_terms = nullptr;
+    int64_t terms_cap = 0;
     WinStage win;
     GraphCache gc;
     int refs = 1;
@@ -664,6 +670,8 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
     if (C->d_del) (void)hipFree(C->d_del);
+    for (void* q : {(void*)C->d_big, (void*)C->d_toff, (void*)C->d_terms})
+        if (q) (void)hipFree(q);
     for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
                     (void*)C->d_sheavy})
         if (q) (void)hipFree(q);
@@ -996,9 +1004,21 @@ static int32_t wide_rebuild_max()
 // The static pass of root ri spread over the whole GPU: host-driven, one
 // launch per BFS level (the frontier's end read back after each) and per
 // ascent level.  fr: one device counter.
-static sbcStatus rebuild_root_wide(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri,
-                                   int32_t* fr)
+static sbcStatus rebuild_root_wide(sbcForestDev* F, const stinger* G, extraArraysPerThread* C, const SbcParams& P,
+                                   int32_t ri, int32_t* fr)
 {
+    // room for every vertex with more than 16384 neighbours
+    const int64_t nbig_max = G->total_cap / 16384 + 2;
+    if (C->big_cap < nbig_max) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (C->d_big) HIPCHK(hipFree(C->d_big));
+        if (C->d_toff) HIPCHK(hipFree(C->d_toff));
+        C->d_big = nullptr;
+        C->d_toff = nullptr;
+        sbcStatus st;
+        if ((st = dalloc(&C->d_big, (size_t)nbig_max + 1)) || (st = dalloc(&C->d_toff, (size_t)nbig_max + 1))) return st;
+        C->big_cap = nbig_max;
+    }
     HIPCHK(sbc_launch_root_reset(&P, ri, fr, C->stream));
     std::vector<int32_t> lv = {0, 1};
     int32_t j = 0;
@@ -1014,8 +1034,37 @@ static sbcStatus rebuild_root_wide(sbcForestDev* F, extraArraysPerThread* C, con
         if (j + 1 >= F->lcap) return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", F->lcap);
         lv.push_back(ne);
     }
-    // levels 0..j, level L = list[lv[L], lv[L+1]); the deepest has no children
-    for (int32_t L = j - 1; L >= 0; --L) HIPCHK(sbc_launch_root_ascent(&P, ri, lv[L], lv[L + 1], L, C->stream));
+    // levels 0..j, level L = list[lv[L], lv[L+1]); the deepest has no children.
+    // Vertices with more than 16384 neighbours are left by the ascent launch
+    // and summed by sbc_launch_root_bigsum.
+    for (int32_t L = j - 1; L >= 0; --L) {
+        HIPCHK(sbc_launch_reset(C->d_big, 1, C->stream));
+        HIPCHK(sbc_launch_root_ascent(&P, ri, lv[L], lv[L + 1], L, C->d_big, C->stream));
+        int32_t nb = 0;
+        HIPCHK(hipMemcpyAsync(&nb, C->d_big, sizeof(nb), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (nb == 0) continue;
+        if (nb < 0 || nb > C->big_cap) return set_err(SBC_ERR_STATE, "rebuild: big vertex count %d out of range", nb);
+        std::vector<int32_t> ids((size_t)nb);
+        HIPCHK(hipMemcpyAsync(ids.data(), C->d_big + 1, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        std::vector<int64_t> toff((size_t)nb + 1, 0);
+        for (int32_t i = 0; i < nb; i++) {
+            if (ids[i] < 0 || ids[i] >= G->nv) return set_err(SBC_ERR_STATE, "rebuild: big vertex out of range");
+            toff[i + 1] = toff[i] + G->deg[ids[i]];
+        }
+        const int64_t T = toff[nb];
+        if (C->terms_cap < T) {
+            if (C->d_terms) HIPCHK(hipFree(C->d_terms));
+            C->d_terms = nullptr;
+            sbcStatus st;
+            if ((st = dalloc(&C->d_terms, (size_t)T))) return st;
+            C->terms_cap = T;
+        }
+        HIPCHK(hipMemcpyAsync(C->d_toff, toff.data(), sizeof(int64_t) * toff.size(), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(sbc_launch_root_bigsum(&P, ri, L, C->d_big, C->d_toff, nb, T, C->d_terms, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));  // toff is host stack data
+    }
     return SBC_OK;
 }
 
@@ -1087,7 +1136,7 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(hipStreamSynchronize(C->stream));
         if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
         for (int32_t ri : mr)
-            if ((st = rebuild_root_wide(F, C, P, ri, C->d_del + 4 + 2 * F->K))) return st;
+            if ((st = rebuild_root_wide(F, G, C, P, ri, C->d_del + 4 + 2 * F->K))) return st;
         if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
     } else if (h[3] > 0) {
         // movement roots: rebuilt by the static pass, one workgroup each

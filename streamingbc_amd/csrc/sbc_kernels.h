@@ -213,7 +213,12 @@ hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* list, const in
 hipError_t sbc_launch_root_reset(const SbcParams* P, int ri, int32_t* fr, hipStream_t st);
 hipError_t sbc_launch_root_bfs(const SbcParams* P, int ri, int fs, int fe, int j, int32_t* fr, hipStream_t st);
 hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, int b, int j, hipStream_t st);
-hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, hipStream_t st);
+// The ascent launch leaves vertices with more than 16384 neighbours in big
+// (count big[0], ids big[1..]); sbc_launch_root_bigsum then computes their
+// terms (offsets toff[0..n], nterms = toff[n]) and sums each in order.
+hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, int32_t* big, hipStream_t st);
+hipError_t sbc_launch_root_bigsum(const SbcParams* P, int ri, int L, const int32_t* big, const int64_t* toff, int n,
+                                  int64_t nterms, double* terms, hipStream_t st);
 // Marks x as a parent on level lev_parent for parked item wi of root ri (a
 // deletion's p once the level of s is done).
 hipError_t sbc_launch_wide_mark(const SbcParams* P, struct SbcWideItem* wi, int ri, int32_t x, int lev_parent,

@@ -2696,7 +2696,11 @@ __global__ void sbc_root_settle_kernel(SbcParams P, int ri, int a, int b, int j)
 }
 
 // Level L = list[a, b): delta pulled from the children in adjacency order.
-__global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L)
+// Vertices with more than ROOT_BIG neighbours are left to the two kernels
+// below (their positions in the level are appended to big[1..], count big[0]).
+constexpr int ROOT_BIG = 16384;
+
+__global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L, int32_t* big)
 {
     const size_t off = (size_t)ri * (size_t)P.nv;
     const int32_t* dist = P.dist + off;
@@ -2706,8 +2710,12 @@ __global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L)
     const int32_t childw = L + 1;
     for (int idx = a + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6); idx < b; idx += nw) {
         const int32_t k = P.list[off + idx];
-        const int64_t sk = vr[k].sigma;
         const SbcVInfo vi = P.vinfo[k];
+        if (vi.deg > ROOT_BIG) {
+            if (lane == 0) big[1 + atomicAdd(&big[0], 1)] = k;
+            continue;
+        }
+        const int64_t sk = vr[k].sigma;
         double acc = 0.0;
         // ROOT_UNROLL chunks per round: ids, levels, then the children's
         // records in flight together; the adds stay in adjacency order
@@ -2736,6 +2744,52 @@ __global__ void sbc_root_ascent_kernel(SbcParams P, int ri, int a, int b, int L)
         }
         if (lane == 0) vr[k].delta = acc;
     }
+}
+
+// Terms of the big vertices of level L (ids big[1..n], term offsets off[0..n]):
+// terms[off[i] + e] = vertex i's term for its e-th neighbour, 0.0 for a
+// neighbour that is not a child (adding +0.0 to a non-negative sum that
+// starts at +0.0 changes nothing), one thread per term over the whole GPU.
+__global__ void sbc_root_terms_kernel(SbcParams P, int ri, int L, const int32_t* big, const int64_t* toff, int n,
+                                      double* terms)
+{
+    const size_t off = (size_t)ri * (size_t)P.nv;
+    const int32_t* dist = P.dist + off;
+    const SbcVRec* vr = P.vr + off;
+    const int64_t T = toff[n];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (toff[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int32_t k = big[1 + lo];
+        const SbcVInfo vi = P.vinfo[k];
+        const int32_t c = P.col[vi.start + (int)(t - toff[lo])];
+        terms[t] = dist[c] == L + 1 ? dep(vr[k].sigma, vr[c].sigma, vr[c].delta) : 0.0;
+    }
+}
+
+// The big vertices' deltas: one wavefront per vertex stages its terms through
+// LDS and lane 0 adds them in adjacency order.
+__global__ __launch_bounds__(64) void sbc_root_bigsum_kernel(SbcParams P, int ri, const int32_t* big,
+                                                             const int64_t* toff, int n, const double* terms)
+{
+    __shared__ double buf[512];
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int lane = threadIdx.x;
+    const int64_t t0 = toff[i], t1 = toff[i + 1];
+    double acc = 0.0;
+    for (int64_t b0 = t0; b0 < t1; b0 += 512) {
+        const int m = (int)(t1 - b0 < 512 ? t1 - b0 : 512);
+        for (int j = lane; j < m; j += 64) buf[j] = terms[b0 + j];
+        __syncthreads();
+        if (lane == 0) acc = lds_chain_sum(acc, buf, m);
+        __syncthreads();
+    }
+    if (lane == 0) P.vr[(size_t)ri * (size_t)P.nv + big[1 + i]].delta = acc;
 }
 
 // BC[v] += sign * (sum over the listed roots list[0, *cnt) of delta_r[v],
@@ -2927,9 +2981,19 @@ extern "C" hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, 
     return hipGetLastError();
 }
 
-extern "C" hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, hipStream_t st)
+extern "C" hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, int32_t* big,
+                                             hipStream_t st)
 {
-    hipLaunchKernelGGL(sbc_root_ascent_kernel, dim3(root_grid(b - a, 4)), dim3(256), 0, st, *P, ri, a, b, L);
+    hipLaunchKernelGGL(sbc_root_ascent_kernel, dim3(root_grid(b - a, 4)), dim3(256), 0, st, *P, ri, a, b, L, big);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_root_bigsum(const SbcParams* P, int ri, int L, const int32_t* big, const int64_t* toff,
+                                             int n, int64_t nterms, double* terms, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_root_terms_kernel, dim3(root_grid(nterms, 256)), dim3(256), 0, st, *P, ri, L, big, toff, n,
+                       terms);
+    hipLaunchKernelGGL(sbc_root_bigsum_kernel, dim3(n), dim3(64), 0, st, *P, ri, big, toff, n, terms);
     return hipGetLastError();
 }
 

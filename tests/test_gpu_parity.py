@@ -724,3 +724,22 @@ def test_checkpoint_resume_is_bit_identical(tmp_path):
     assert np.array_equal(f.bc(), f2.bc())
     f2.close()
     g2.close()
+
+
+def test_wide_rebuild_of_a_root_with_a_big_hub(monkeypatch):
+    """Movement-case roots rebuilt over the whole GPU when a level holds a
+    vertex with more than 16384 neighbours: its terms are computed across the
+    GPU and summed in order by one wavefront, bit-identical to the oracle."""
+    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", "1000")
+    n = 20100
+    edges = [(1, i) for i in range(2, 20002)] + [(i, i + 1) for i in range(2, 300)] + [(20002, 20003), (20003, 5)]
+    hg = sbc.graph_from_edges(n, np.array([a for a, b in edges]), np.array([b for a, b in edges]))
+    roots = np.array([3, 7, 150, 299, 20003, 1, 9000, 12000], dtype=np.int64)
+    print(f"[delete] big hub nv={hg.nv}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    c = _apply_ops(f, g, ctx, o, [(0, 1, 9000), (0, 1, 250), (0, 20003, 5)])
+    assert c[:, 2].sum() > 0, "no movement-case root"
+    _compare_trees(f, o, "wide rebuild with a big hub")
+    assert_bc_close(f.bc(), o.bc())
