@@ -355,6 +355,11 @@ constexpr int SCAN_UNROLL = SBC_SCAN_UNROLL;
 #define SBC_WAVE_UNROLL 2
 #endif
 constexpr int WAVE_UNROLL = SBC_WAVE_UNROLL;
+// lowest ascent level whose hubs may be resolved by table lookups (level 0:
+// the root itself, whose sources are the processed level-1 vertices)
+#ifndef SBC_LOOKUP_MIN_LEV
+#define SBC_LOOKUP_MIN_LEV 0
+#endif
 // (source, hub) lookups whose first table slots one thread loads together
 #ifndef SBC_PROBE_UNROLL
 #define SBC_PROBE_UNROLL 2
@@ -1344,7 +1349,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const int32_t c = P.col[start + e];
             const uint8_t b = ld_sb(&X.sb[c]);
             if (sb_type(b) == 0) {
-                if (!(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
+                if (lev >= 1 && !(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
             } else {
                 probe_term(P, X, hs, bk, k, hi, lev, c, e);
             }
@@ -2059,7 +2064,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 load_hub_batch<TW>(P, *hs, hb, nh);
                 int64_t tl1 = 0;
                 if (tp && leader) tl1 = (int64_t)wall_clock64();
-                // split the batch: ascent-only indexed hubs on level 1 or 2
+                // split the batch: ascent-only indexed hubs on levels 0-2 (and 3)
                 // whose lists are much longer than the lookup work are
                 // handled together by table lookups; the rest are scanned
                 if (threadIdx.x == 0) hs->np = 0;
@@ -2105,7 +2110,8 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                         hs->hstart[h] = vi.start;
                     }
                     bool pk = false;
-                    if (P.hub && P.bucket && lev >= 1 && (lev <= 2 || (lev == 3 && src.nr > 0)) && hy > 0 && asc)
+                    if (P.hub && P.bucket && lev >= SBC_LOOKUP_MIN_LEV && (lev <= 2 || (lev == 3 && src.nr > 0)) && hy > 0 &&
+                        asc)
                         pk = cand < (long)hy;
                     atomicAdd(&ts.stat[0], 1ull);
                     atomicAdd(&ts.stat[1], (unsigned long long)len);

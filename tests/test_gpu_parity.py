@@ -743,3 +743,25 @@ def test_wide_rebuild_of_a_root_with_a_big_hub(monkeypatch):
     assert c[:, 2].sum() > 0, "no movement-case root"
     _compare_trees(f, o, "wide rebuild with a big hub")
     assert_bc_close(f.bc(), o.bc())
+
+
+def test_hub_roots_match_oracle():
+    """Roots that are themselves hubs with lookup tables: their own level-0
+    update is resolved by table lookups over the processed level-1 vertices."""
+    hg, deg = _hub_graph()
+    top = np.argsort(-deg)[:6]
+    roots = np.concatenate([top, hg.pick_roots(10, 5)]).astype(np.int64)
+    print(f"[hub roots] nv={hg.nv} roots={len(roots)} top degree {deg[top[0]]}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    u, v = hg.insertions(40, 21)
+    for a, b in zip(u[:12], v[:12]):
+        f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+        o.insert(int(a), int(b), O_FINE)
+    _compare_trees(f, o, "plain insertions, hub roots")
+    f.insert_batched(g, ctx, u[12:], v[12:], sbc.BatchConfig(window=7, block_threads=1024))
+    for a, b in zip(u[12:], v[12:]):
+        o.insert(int(a), int(b), O_FINE)
+    _compare_trees(f, o, "batched insertions, hub roots")
+    assert_bc_close(f.bc(), o.bc())
