@@ -1096,6 +1096,9 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
     int32_t h[4];
     HIPCHK(hipMemcpyAsync(h, C->d_del, sizeof(h), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
+    if (h[1] < 0 || h[2] < 0 || h[3] < 0 || (int64_t)h[1] + h[2] + h[3] != F->K)
+        return set_err(SBC_ERR_STATE, "deletion: case counts %d/%d/%d do not add up to %ld roots", h[1], h[2], h[3],
+                       (long)F->K);
     hipEvent_t e0;
     if (h[2] > 0) {
         // adjacent roots: the single-item update in deletion mode
