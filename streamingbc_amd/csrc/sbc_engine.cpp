@@ -634,6 +634,12 @@ struct extraArraysPerThread {
     int64_t* d_toff = nullptr;
     double* d_terms = nullptr;
     int64_t terms_cap = 0;
+    // whole-GPU rebuild of several roots: root list, list ends, segments, prefix
+    int32_t* d_mrl = nullptr;
+    int32_t* d_mfr = nullptr;
+    SbcRootSeg* d_mseg = nullptr;
+    int64_t* d_mpre = nullptr;
+    int64_t mroot_cap = 0;
     WinStage win;
     GraphCache gc;
     int refs = 1;
@@ -670,7 +676,8 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
     if (C->d_del) (void)hipFree(C->d_del);
-    for (void* q : {(void*)C->d_big, (void*)C->d_toff, (void*)C->d_terms})
+    for (void* q : {(void*)C->d_big, (void*)C->d_toff, (void*)C->d_terms, (void*)C->d_mrl, (void*)C->d_mfr,
+                    (void*)C->d_mseg, (void*)C->d_mpre})
         if (q) (void)hipFree(q);
     for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
                     (void*)C->d_sheavy})
@@ -1001,6 +1008,20 @@ static int32_t wide_rebuild_max()
     return e ? (int32_t)atoi(e) : kWideRebuildMax;
 }
 
+// More movement-case roots than that go to the several-roots whole-GPU
+// rebuild on graphs of at least this many adjacency slots (R-MAT scale 20:
+// one workgroup rebuilds a root in ~196 ms, the whole GPU in ~6.3 ms, and the
+// several-roots form shares each level's launches), else one workgroup per
+// root (Chung-Lu 2^15: 3.6 ms for ~41 roots against 5.4 ms level by level,
+// whose round trips per level dominate on its many thin levels).
+static const int64_t kWideGraphSlots = 4LL << 20;
+
+static int64_t wide_graph_min()
+{
+    const char* e = getenv("SBC_WIDE_GRAPH_MIN");
+    return e ? atoll(e) : kWideGraphSlots;
+}
+
 // The static pass of root ri spread over the whole GPU: host-driven, one
 // launch per BFS level (the frontier's end read back after each) and per
 // ascent level.  fr: one device counter.
@@ -1064,6 +1085,99 @@ static sbcStatus rebuild_root_wide(sbcForestDev* F, const stinger* G, extraArray
         HIPCHK(hipMemcpyAsync(C->d_toff, toff.data(), sizeof(int64_t) * toff.size(), hipMemcpyHostToDevice, C->stream));
         HIPCHK(sbc_launch_root_bigsum(&P, ri, L, C->d_big, C->d_toff, nb, T, C->d_terms, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));  // toff is host stack data
+    }
+    return SBC_OK;
+}
+
+// The static pass of roots mr[0..n) together over the whole GPU: every
+// launch takes one level of each root still growing (segments flattened by
+// prefix sums); the list ends are read back after each BFS level.
+static sbcStatus rebuild_roots_wide(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P,
+                                    const std::vector<int32_t>& mr)
+{
+    const int n = (int)mr.size();
+    if (C->mroot_cap < n) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        for (void* q : {(void*)C->d_mrl, (void*)C->d_mfr, (void*)C->d_mseg, (void*)C->d_mpre})
+            if (q) HIPCHK(hipFree(q));
+        C->d_mrl = C->d_mfr = nullptr;
+        C->d_mseg = nullptr;
+        C->d_mpre = nullptr;
+        sbcStatus st;
+        if ((st = dalloc(&C->d_mrl, (size_t)n)) || (st = dalloc(&C->d_mfr, (size_t)n)) ||
+            (st = dalloc(&C->d_mseg, (size_t)n)) || (st = dalloc(&C->d_mpre, (size_t)n + 1)))
+            return st;
+        C->mroot_cap = n;
+    }
+    HIPCHK(hipMemcpyAsync(C->d_mrl, mr.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(sbc_launch_roots_reset(&P, C->d_mrl, n, C->d_mfr, C->stream));
+    std::vector<std::vector<int32_t>> lv((size_t)n, std::vector<int32_t>{0, 1});
+    std::vector<int32_t> j((size_t)n, 0), who, ends;
+    std::vector<char> act((size_t)n, 1);
+    std::vector<SbcRootSeg> sg, sg2;
+    std::vector<int64_t> pre, pre2;
+    for (;;) {
+        sg.clear();
+        who.clear();
+        ends.clear();
+        pre.assign(1, 0);
+        for (int i = 0; i < n; i++) {
+            if (!act[i]) continue;
+            const int32_t a = lv[i][j[i]], b = lv[i][j[i] + 1];
+            sg.push_back(SbcRootSeg{mr[i], a, b, j[i]});
+            pre.push_back(pre.back() + (b - a));
+            who.push_back(i);
+            ends.push_back(b);  // the root's list end: where its next frontier starts
+        }
+        const int m = (int)sg.size();
+        if (m == 0) break;
+        HIPCHK(hipMemcpyAsync(C->d_mseg, sg.data(), sizeof(SbcRootSeg) * m, hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_mpre, pre.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_mfr, ends.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, C->stream));
+        HIPCHK(sbc_launch_roots_level(&P, 0, C->d_mseg, C->d_mpre, m, pre.back(), C->d_mfr, C->stream));
+        HIPCHK(hipMemcpyAsync(ends.data(), C->d_mfr, sizeof(int32_t) * m, hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        sg2.clear();
+        pre2.assign(1, 0);
+        for (int s2 = 0; s2 < m; s2++) {
+            const int i = who[s2];
+            const int32_t fe = sg[s2].b, ne = ends[s2];
+            if (ne < fe || ne > F->nv) return set_err(SBC_ERR_STATE, "rebuild: list end %d out of range", ne);
+            if (ne == fe) {
+                act[i] = 0;
+                continue;
+            }
+            sg2.push_back(SbcRootSeg{mr[i], fe, ne, j[i]});
+            pre2.push_back(pre2.back() + (ne - fe));
+            j[i]++;
+            if (j[i] + 1 >= F->lcap) return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", F->lcap);
+            lv[i].push_back(ne);
+        }
+        const int m2 = (int)sg2.size();
+        if (m2 == 0) continue;
+        HIPCHK(hipMemcpyAsync(C->d_mseg, sg2.data(), sizeof(SbcRootSeg) * m2, hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_mpre, pre2.data(), sizeof(int64_t) * (m2 + 1), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(sbc_launch_roots_level(&P, 1, C->d_mseg, C->d_mpre, m2, pre2.back(), nullptr, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));  // sg2/pre2 are refilled next round
+    }
+    // ascent: root i's levels 0..j[i], deepest first (its deepest has no children)
+    int32_t maxd = 0;
+    for (int i = 0; i < n; i++) maxd = j[i] > maxd ? j[i] : maxd;
+    for (int32_t step = 0; step < maxd; step++) {
+        sg.clear();
+        pre.assign(1, 0);
+        for (int i = 0; i < n; i++) {
+            const int32_t L = j[i] - 1 - step;
+            if (L < 0) continue;
+            sg.push_back(SbcRootSeg{mr[i], lv[i][L], lv[i][L + 1], L});
+            pre.push_back(pre.back() + (lv[i][L + 1] - lv[i][L]));
+        }
+        const int m = (int)sg.size();
+        if (m == 0) continue;
+        HIPCHK(hipMemcpyAsync(C->d_mseg, sg.data(), sizeof(SbcRootSeg) * m, hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_mpre, pre.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(sbc_launch_roots_level(&P, 2, C->d_mseg, C->d_mpre, m, pre.back(), nullptr, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
     }
     return SBC_OK;
 }
@@ -1140,6 +1254,15 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
         for (int32_t ri : mr)
             if ((st = rebuild_root_wide(F, G, C, P, ri, C->d_del + 4 + 2 * F->K))) return st;
+        if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+    } else if (h[3] > 0 && G->total_cap >= wide_graph_min()) {
+        // many movement roots: rebuilt together by the whole GPU, level by level
+        std::vector<int32_t> mr((size_t)h[3]);
+        HIPCHK(hipMemcpyAsync(mr.data(), C->d_del + 4 + F->K, sizeof(int32_t) * mr.size(), hipMemcpyDeviceToHost,
+                              C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
+        if ((st = rebuild_roots_wide(F, C, P, mr))) return st;
         if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
     } else if (h[3] > 0) {
         // movement roots: rebuilt by the static pass, one workgroup each

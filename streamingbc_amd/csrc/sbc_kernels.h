@@ -219,6 +219,18 @@ hipError_t sbc_launch_root_settle(const SbcParams* P, int ri, int a, int b, int 
 hipError_t sbc_launch_root_ascent(const SbcParams* P, int ri, int a, int b, int L, int32_t* big, hipStream_t st);
 hipError_t sbc_launch_root_bigsum(const SbcParams* P, int ri, int L, const int32_t* big, const int64_t* toff, int n,
                                   int64_t nterms, double* terms, hipStream_t st);
+// Rebuild of several roots over the whole GPU: segment i is root ri's list
+// range [a, b) on level lev; the roots' ranges are flattened by prefix sums.
+struct SbcRootSeg {
+    int32_t ri, a, b, lev;
+};
+// Resets roots rl[0..n) (fr[i] = 1); then per step kind 0 expands every
+// segment's frontier (root i appends at fr[i]), kind 1 settles new levels
+// (segment lev = the level before), kind 2 pulls a level's dependencies;
+// items = pre[n].
+hipError_t sbc_launch_roots_reset(const SbcParams* P, const int32_t* rl, int n, int32_t* fr, hipStream_t st);
+hipError_t sbc_launch_roots_level(const SbcParams* P, int kind, const SbcRootSeg* sg, const int64_t* pre, int n,
+                                  int64_t items, int32_t* fr, hipStream_t st);
 // Marks x as a parent on level lev_parent for parked item wi of root ri (a
 // deletion's p once the level of s is done).
 hipError_t sbc_launch_wide_mark(const SbcParams* P, struct SbcWideItem* wi, int ri, int32_t x, int lev_parent,

@@ -2798,6 +2798,165 @@ __global__ __launch_bounds__(64) void sbc_root_bigsum_kernel(SbcParams P, int ri
     if (lane == 0) P.vr[(size_t)ri * (size_t)P.nv + big[1 + i]].delta = acc;
 }
 
+// ------------------------------------------------------------ several roots, whole GPU
+//
+// The same rebuild for n roots at once: every launch takes one level of each
+// root, the roots' ranges flattened into one index space by the prefix
+// pre[0..n] (segment i: root sg[i].ri, list[sg[i].a, sg[i].b), level
+// sg[i].lev), so a level of many small roots fills the GPU as one large level
+// does.  Per root the per-vertex code is the single-root kernels', so the
+// trees are the same bits; root i appends its next frontier at fr[i].
+
+__device__ __forceinline__ int seg_of(const int64_t* pre, int n, int64_t w)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void sbc_roots_reset_kernel(SbcParams P, const int32_t* rl, int n, int32_t* fr)
+{
+    const int64_t tot = (int64_t)n * P.nv;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / P.nv);
+        const int x = (int)(t - (int64_t)i * P.nv);
+        const int ri = rl[i];
+        const size_t off = (size_t)ri * (size_t)P.nv;
+        const int32_t r = P.roots[ri];
+        const bool root = x == r;
+        P.dist[off + x] = root ? 0 : INF32;
+        P.sb[(size_t)ri * (size_t)P.sb_stride + x] = root ? sb_make(0, 0) : SB_INF;
+        P.vr[off + x].sigma = root ? 1 : SIGMA_INF;
+        P.vr[off + x].delta = 0.0;
+        if (x == 0) {
+            P.list[off] = r;
+            fr[i] = 1;
+        }
+    }
+}
+
+__global__ void sbc_roots_bfs_kernel(SbcParams P, const SbcRootSeg* sg, const int64_t* pre, int n, int32_t* fr)
+{
+    const int lane = lane_id();
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < pre[n]; w += nw) {
+        const int i = seg_of(pre, n, w);
+        const SbcRootSeg g = sg[i];
+        const size_t off = (size_t)g.ri * (size_t)P.nv;
+        int32_t* dist = P.dist + off;
+        SbcVRec* vr = P.vr + off;
+        int32_t* list = P.list + off;
+        const int32_t next = g.lev + 1;
+        const int32_t c = list[g.a + (int)(w - pre[i])];
+        const int64_t sc = vr[c].sigma;
+        const SbcVInfo vi = P.vinfo[c];
+        for (int e0 = 0; e0 < vi.deg; e0 += 64 * ROOT_UNROLL) {
+            int32_t k[ROOT_UNROLL], wk[ROOT_UNROLL];
+            bool app[ROOT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                const int e = e0 + u * 64 + lane;
+                k[u] = e < vi.deg ? P.col[vi.start + e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u)
+                wk[u] = k[u] >= 0 ? __hip_atomic_load(&dist[k[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                app[u] = false;
+                if (k[u] >= 0 && wk[u] == INF32) {
+                    const int32_t old = cas32_dev(&dist[k[u]], INF32, next);
+                    app[u] = old == INF32;
+                    wk[u] = app[u] ? next : old;
+                }
+            }
+            uint64_t m[ROOT_UNROLL];
+            int tot = 0;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                m[u] = ballot(app[u]);
+                tot += __popcll(m[u]);
+            }
+            if (tot) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&fr[i], tot);
+                base = __shfl(base, 0);
+#pragma unroll
+                for (int u = 0; u < ROOT_UNROLL; ++u) {
+                    if (app[u]) list[base + prefix_in_wave(m[u])] = k[u];
+                    base += __popcll(m[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u)
+                if (k[u] >= 0 && wk[u] == next)
+                    __hip_atomic_fetch_add((unsigned long long*)&vr[k[u]].dpath, (unsigned long long)sc,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// new levels: segment i is the new level g.lev + 1 of its root
+__global__ void sbc_roots_settle_kernel(SbcParams P, const SbcRootSeg* sg, const int64_t* pre, int n)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < pre[n]; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = seg_of(pre, n, t);
+        const SbcRootSeg g = sg[i];
+        const size_t off = (size_t)g.ri * (size_t)P.nv;
+        const int32_t k = P.list[off + g.a + (int)(t - pre[i])];
+        SbcVRec& r = P.vr[off + k];
+        r.sigma = r.dpath;
+        r.dpath = 0;
+        P.sb[(size_t)g.ri * (size_t)P.sb_stride + k] = sb_make(g.lev + 1, 0);
+    }
+}
+
+__global__ void sbc_roots_ascent_kernel(SbcParams P, const SbcRootSeg* sg, const int64_t* pre, int n)
+{
+    const int lane = lane_id();
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < pre[n]; w += nw) {
+        const int i = seg_of(pre, n, w);
+        const SbcRootSeg g = sg[i];
+        const size_t off = (size_t)g.ri * (size_t)P.nv;
+        const int32_t* dist = P.dist + off;
+        SbcVRec* vr = P.vr + off;
+        const int32_t childw = g.lev + 1;
+        const int32_t k = P.list[off + g.a + (int)(w - pre[i])];
+        const int64_t sk = vr[k].sigma;
+        const SbcVInfo vi = P.vinfo[k];
+        double acc = 0.0;
+        for (int e0 = 0; e0 < vi.deg; e0 += 64 * ROOT_UNROLL) {
+            int32_t c[ROOT_UNROLL];
+            bool child[ROOT_UNROLL];
+            double add[ROOT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                const int e = e0 + u * 64 + lane;
+                c[u] = e < vi.deg ? P.col[vi.start + e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) child[u] = c[u] >= 0 && dist[c[u]] == childw;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) add[u] = child[u] ? dep(sk, vr[c[u]].sigma, vr[c[u]].delta) : 0.0;
+#pragma unroll
+            for (int u = 0; u < ROOT_UNROLL; ++u) {
+                uint64_t m = ballot(child[u]);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    acc = acc + readlane_d(add[u], l);
+                }
+            }
+        }
+        if (lane == 0) vr[k].delta = acc;
+    }
+}
+
 // BC[v] += sign * (sum over the listed roots list[0, *cnt) of delta_r[v],
 // r != v): an edge deletion takes the roots it rebuilds out of BC before the
 // rebuild (sign -1) and puts them back after (+1).
@@ -3008,6 +3167,25 @@ extern "C" hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* lis
 {
     const int bs = 256;
     hipLaunchKernelGGL(sbc_bc_roots_kernel, dim3((P->nv + bs - 1) / bs), dim3(bs), 0, st, *P, list, cnt, sign);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_roots_reset(const SbcParams* P, const int32_t* rl, int n, int32_t* fr, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_roots_reset_kernel, dim3(root_grid((long)n * P->nv, 256)), dim3(256), 0, st, *P, rl, n, fr);
+    return hipGetLastError();
+}
+
+// kind 0: BFS (one wavefront per vertex), 1: settle (one thread), 2: ascent (one wavefront)
+extern "C" hipError_t sbc_launch_roots_level(const SbcParams* P, int kind, const SbcRootSeg* sg, const int64_t* pre,
+                                             int n, int64_t items, int32_t* fr, hipStream_t st)
+{
+    if (kind == 0)
+        hipLaunchKernelGGL(sbc_roots_bfs_kernel, dim3(root_grid(items, 4)), dim3(256), 0, st, *P, sg, pre, n, fr);
+    else if (kind == 1)
+        hipLaunchKernelGGL(sbc_roots_settle_kernel, dim3(root_grid(items, 256)), dim3(256), 0, st, *P, sg, pre, n);
+    else
+        hipLaunchKernelGGL(sbc_roots_ascent_kernel, dim3(root_grid(items, 4)), dim3(256), 0, st, *P, sg, pre, n);
     return hipGetLastError();
 }
 

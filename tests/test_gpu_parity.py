@@ -493,12 +493,26 @@ def _apply_ops(f, g, ctx, o, ops):
     return np.array(counts, dtype=np.int64)
 
 
-# movement-case roots: SBC_WIDE_REBUILD_MAX = 0 rebuilds them one workgroup
-# per root, a large value one root at a time over the whole GPU
-@pytest.mark.parametrize("wide", ["0", "1000"])
+# movement-case roots: SBC_WIDE_REBUILD_MAX = 1000 rebuilds them one root at a
+# time over the whole GPU; 0 with SBC_WIDE_GRAPH_MIN = 0 all together over the
+# whole GPU; 0 with the default graph-size threshold (above these graphs) one
+# workgroup per root
+REBUILD_MODES = {"single": ("1000", None), "multi": ("0", "0"), "workgroup": ("0", None)}
+
+
+def _rebuild_mode(monkeypatch, mode):
+    wide, gmin = REBUILD_MODES[mode]
+    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
+    if gmin is not None:
+        monkeypatch.setenv("SBC_WIDE_GRAPH_MIN", gmin)
+    else:
+        monkeypatch.delenv("SBC_WIDE_GRAPH_MIN", raising=False)
+
+
+@pytest.mark.parametrize("wide", sorted(REBUILD_MODES))
 @pytest.mark.parametrize("name", ["karate", "dolphins"])
 def test_delete_matches_oracle_and_reference(name, wide, monkeypatch):
-    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
+    _rebuild_mode(monkeypatch, wide)
     fx = np.load(os.path.join(GOLDEN, f"refdel_{name}.npz"))
     hg = sbc.read_dimacs(os.path.join(GOLDEN, "graphs", f"{name}.graph"))
     roots = np.arange(hg.nv)
@@ -517,9 +531,9 @@ def test_delete_matches_oracle_and_reference(name, wide, monkeypatch):
     assert np.array_equal(lv, fx["fine_level"]) and np.array_equal(sg, fx["fine_sigma"])
 
 
-@pytest.mark.parametrize("wide", ["0", "1000"])
+@pytest.mark.parametrize("wide", sorted(REBUILD_MODES))
 def test_delete_rmat10_matches_reference_hashes(wide, monkeypatch):
-    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
+    _rebuild_mode(monkeypatch, wide)
     fx = np.load(os.path.join(GOLDEN, "refdel_rmat10.npz"))
     hg = sbc.gen_rmat(10, 8192, seed=1, keep_lcc=True)
     assert hg.nv == int(fx["nv"])
