@@ -626,6 +626,7 @@ struct extraArraysPerThread {
     int64_t bucket_blocks = 0;
     int32_t* d_hubq = nullptr;  // per workgroup: hubs queued beyond the LDS queue
     int64_t hubq_blocks = 0;
+    int64_t capture_fallbacks = 0;  // batched calls whose graph capture failed and ran directly
     int32_t* d_del = nullptr;   // edge deletion: 4 counts, then the affected root indices
     int64_t del_k = 0;
     // whole-GPU rebuild of one root: a level's big vertices, their term offsets and terms
@@ -1743,7 +1744,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     A.d_vv = di + 20 * W;
     A.d_vi = C->win.d_vi;
 
-    const bool use_graph = cfg->use_graph != 0;
+    bool use_graph = cfg->use_graph != 0;
     GraphCache& gc = C->gc;
     bool need_capture = use_graph &&
         (!gc.exec || gc.forest != F || gc.g != G || gc.generation != G->generation || gc.nwb != nwb ||
@@ -1758,16 +1759,27 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         sbcStatus cst = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, false);
         hipGraph_t g = nullptr;
         hipError_t ee = hipStreamEndCapture(C->stream, &g);
-        if (cst) return cst;
-        if (ee != hipSuccess) return set_err(SBC_ERR_HIP, "graph capture: %s", hipGetErrorString(ee));
-        gc.graph = g;
-        HIPCHK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
-        gc.forest = F; gc.g = G; gc.generation = G->generation; gc.nwb = nwb; gc.grid = grid; gc.heavy = heavy;
-        gc.recompute = cfg->recompute_bc; gc.W = W; gc.has_order = has_order; gc.with_rec = 1;
-        gc.park_min = P.park_min;
-        gc.run_cap = P.run_cap;
-        gc.hub_batch = P.hub_batch;
-        gc.park_scan = P.park_scan;
+        if (cst != SBC_OK || ee != hipSuccess) {
+            // Nothing inside a capture runs, so this call's windows are
+            // launched directly instead (the same kernels, the same results).
+            // Seen when several host threads capture their own streams at
+            // once: a capture is occasionally invalidated.  A real launch
+            // error shows up again on the direct launch.
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            C->capture_fallbacks++;
+            use_graph = false;
+        } else {
+            gc.graph = g;
+            HIPCHK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+            gc.forest = F; gc.g = G; gc.generation = G->generation; gc.nwb = nwb; gc.grid = grid;
+            gc.heavy = heavy; gc.recompute = cfg->recompute_bc; gc.W = W; gc.has_order = has_order;
+            gc.with_rec = 1;
+            gc.park_min = P.park_min;
+            gc.run_cap = P.run_cap;
+            gc.hub_batch = P.hub_batch;
+            gc.park_scan = P.park_scan;
+        }
     }
 
     std::vector<int32_t> prev_touched;
