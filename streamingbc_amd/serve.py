@@ -48,9 +48,10 @@ class Resident:
 
 class Server:
     def __init__(self, budget_bytes: int, max_window: int = 128, block_threads: int = 1024, use_graph: bool = True,
-                 num_cu: int = 0):
+                 num_cu: int = 0, park_min: int = 0):
         self.budget = int(budget_bytes)
         self.max_window = int(max_window)
+        self.park_min = int(park_min)
         self.block_threads = block_threads
         self.use_graph = use_graph
         self.num_cu = num_cu
@@ -91,7 +92,8 @@ class Server:
         try:
             n = t.shape[0]
             i = 0
-            cfg = BatchConfig(window=self.max_window, block_threads=self.block_threads, use_graph=self.use_graph)
+            cfg = BatchConfig(window=self.max_window, block_threads=self.block_threads, use_graph=self.use_graph,
+                              park_min=self.park_min)
             while i < n:
                 now = time.perf_counter() - t0
                 if t[i] > now:
@@ -176,11 +178,13 @@ def main(argv=None) -> int:
     ap.add_argument("--rates", default="1000,10000,50000", help="aggregate arrival rates (insertions/s)")
     ap.add_argument("--seconds", type=float, default=4.0, help="arrival span per rate")
     ap.add_argument("--max-window", type=int, default=128)
+    ap.add_argument("--park-min", type=int, default=0,
+                    help="sbcBatchConfig.park_min (0: the library default, < 0: never hand items to the wide kernels)")
     ap.add_argument("--budget-gib", type=float, default=200.0)
     ap.add_argument("--graphs", default=",".join(f"{s}:{k}" for s, k in SERVE_GRAPHS))
     ap.add_argument("--seed", type=int, default=11)
     a = ap.parse_args(argv)
-    srv = Server(int(a.budget_gib * (1 << 30)), max_window=a.max_window)
+    srv = Server(int(a.budget_gib * (1 << 30)), max_window=a.max_window, park_min=a.park_min)
     specs = [tuple(int(x) for x in s.split(":")) for s in a.graphs.split(",")]
     t0 = time.perf_counter()
     hgs = {}
