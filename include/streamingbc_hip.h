@@ -11,10 +11,12 @@
  *   - `struct stinger` is an opaque device-resident CSR with per-vertex append
  *     slack (int32 neighbour ids, per-slot insertion sequence numbers) plus a
  *     host mirror of the adjacency, instead of STINGER's blocked lists;
- *   - `bcForest` keeps its public `totalBC` / `NV` fields, but `totalBC` is a
- *     host mirror refreshed by streamingBCGetBC(); the per-root trees live in
- *     HBM as structure-of-arrays (int32 dist, int64 sigma, FP64 delta) and are
- *     read with streamingBCGetTree();
+ *   - `bcForest` keeps its public `totalBC` / `NV` fields: `totalBC` is a host
+ *     mirror of the device BC that the section-1 entry points refresh before
+ *     they return (as the reference leaves BC there, src/streamingbc.c:211-218),
+ *     and streamingBCGetBC() refreshes after the sbc* and batched forms; the
+ *     per-root trees live in HBM as structure-of-arrays (int32 dist, int64
+ *     sigma, FP64 delta) and are read with streamingBCGetTree();
  *   - `extraArraysPerThread` is an opaque device work context (HIP stream,
  *     timing events, staging buffers).  Only element 0 of the array is used.
  *
@@ -68,7 +70,8 @@ typedef struct sbcForestDev sbcForestDev;
  * opaque device handle. */
 typedef struct {
     sbcForestDev *dev;
-    double *totalBC; /* host mirror, valid after streamingBCGetBC() */
+    double *totalBC; /* host mirror: refreshed by the section-1 update and
+                        static-pass calls and by streamingBCGetBC() */
     int64_t NV;
 } bcForest;
 typedef bcForest *bcForestPtr;
@@ -272,6 +275,12 @@ typedef struct {
 sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,
                                         const int64_t *u, const int64_t *v, int64_t count,
                                         const sbcBatchConfig *cfg, sbcItemRecord *records);
+
+/* Window graphs built and windows launched as graph replays by the batched
+ * calls of this context.  The graphs are built node by node, without stream
+ * capture, so nothing another host thread does can invalidate one; a build
+ * failure is an error of the batched call, never a silent direct launch. */
+sbcStatus streamingBCGraphCounts(extraArraysPerThread **eAPT, int64_t *built, int64_t *launched);
 
 /* Number of items of the last streamingBCInsertEdgesBatched call (or of the
  * last deletion) that were finished by the whole-GPU kernels

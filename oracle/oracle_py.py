@@ -188,6 +188,95 @@ class Oracle:
         return bool(oracle_lib().orc_sigma_overflowed(self.f))
 
 
+class OracleGroup:
+    """The oracle over a root list split into `parts` forests that share one
+    graph; each insertion's per-root updates run on host threads (ctypes
+    releases the GIL, and roots never touch each other's state).  Every
+    root's tree is the one the single-forest oracle computes, bit for bit;
+    BC is the sum of the parts' BC (another summation order, so it agrees
+    with the single forest within rounding, not bit for bit)."""
+
+    def __init__(self, rowptr, col, roots, parts=None):
+        from concurrent.futures import ThreadPoolExecutor
+        L = oracle_lib()
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int64)
+        self.roots = np.ascontiguousarray(roots, dtype=np.int64)
+        self.nv = self.rowptr.shape[0] - 1
+        self.k = self.roots.shape[0]
+        if parts is None:
+            parts = min(self.k, max(1, min(len(os.sched_getaffinity(0)), 16)))
+        bounds = np.linspace(0, self.k, parts + 1).astype(np.int64)
+        self.g = L.orc_graph_new(self.nv, _p64(self.rowptr), _p64(self.col))
+        self.chunks = []  # (first root index, root array, forest)
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            if b > a:
+                r = np.ascontiguousarray(self.roots[a:b])
+                self.chunks.append((int(a), r, L.orc_forest_new(self.nv, r.shape[0], _p64(r))))
+        self.pool = ThreadPoolExecutor(len(self.chunks))
+
+    def close(self):
+        L = oracle_lib()
+        if self.pool is not None:
+            self.pool.shutdown()
+            self.pool = None
+        for _, _, f in self.chunks:
+            L.orc_forest_free(f)
+        self.chunks = []
+        if self.g:
+            L.orc_graph_free(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _each(self, fn):
+        return list(self.pool.map(fn, self.chunks))
+
+    def static(self, order=PULL):
+        L = oracle_lib()
+        self._each(lambda c: L.orc_static(c[2], self.g, order))
+
+    def insert(self, u, v, mode=FINE, items=False):
+        L = oracle_lib()
+        r = L.orc_graph_insert(self.g, u, v)
+        if r != 1:
+            raise ValueError(f"edge ({u},{v}) not insertable: {r}")
+        rec = np.zeros(self.k, dtype=ITEM_DTYPE) if items else None
+
+        def one(c):
+            a, rr, f = c
+            p = rec[a:].ctypes.data if items else None
+            L.orc_insert(f, self.g, u, v, mode, p)
+
+        self._each(one)
+        return rec
+
+    def tree(self, ri):
+        for a, r, f in self.chunks:
+            if a <= ri < a + r.shape[0]:
+                lv = np.zeros(self.nv, dtype=np.int64)
+                sg = np.zeros(self.nv, dtype=np.int64)
+                dl = np.zeros(self.nv, dtype=np.float64)
+                oracle_lib().orc_get_tree(f, ri - a, _p64(lv), _p64(sg), _pd(dl))
+                return lv, sg, dl
+        raise IndexError(ri)
+
+    def bc(self):
+        acc = np.zeros(self.nv, dtype=np.float64)
+        for _, _, f in self.chunks:
+            out = np.zeros(self.nv, dtype=np.float64)
+            oracle_lib().orc_get_bc(f, _pd(out))
+            acc += out
+        return acc
+
+    def sigma_overflowed(self) -> bool:
+        return any(bool(oracle_lib().orc_sigma_overflowed(f)) for _, _, f in self.chunks)
+
+
 class Reference:
     """The reference library compiled from its own sources (oracle/_ref)."""
 

@@ -134,6 +134,7 @@ _sig("streamingBCSynchronize", C.c_int, [VPP])
 _sig("streamingBCLastParked", i64, [VPP])
 _sig("streamingBCComputeUnits", i32, [VPP])
 _sig("streamingBCWorkTotals", C.c_int, [VPP, i64, P64])
+_sig("streamingBCGraphCounts", C.c_int, [VPP, P64, P64])
 # cost model (include/sbc_costmodel.h)
 PF = C.POINTER(C.c_float)
 _sig("sbcCostModelNumParams", i32, [])
@@ -286,6 +287,14 @@ class Context:
         t = out.reshape(k, 8)
         return {"nonzero": t[:, 0], "processed": t[:, 1], "scanned": t[:, 2], "affected": t[:, 3],
                 "moved": t[:, 4], "max_affected": t[:, 5]}
+
+    def graph_counts(self):
+        """(window graphs built, windows launched as graph replays) of the
+        batched calls of this context."""
+        b = np.zeros(1, dtype=np.int64)
+        n = np.zeros(1, dtype=np.int64)
+        _check(lib.streamingBCGraphCounts(self._p, _p64(b), _p64(n)))
+        return int(b[0]), int(n[0])
 
     def last_parked(self) -> int:
         """Items of the last batched call finished by the whole-GPU kernels."""

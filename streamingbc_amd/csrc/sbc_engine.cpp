@@ -241,15 +241,17 @@ static sbcStatus graph_upload_full(stinger* G, hipStream_t st)
     if (!G->d_hub) {
         HIPCHK(hipMalloc(&G->d_hub, sizeof(int4) * G->nv));
         HIPCHK(hipMalloc(&G->d_htab, sizeof(int2) * (G->htab.size() ? G->htab.size() : 1)));
-        HIPCHK(hipMemcpy(G->d_hub, G->hub.data(), sizeof(int4) * G->nv, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(G->d_hub, G->hub.data(), sizeof(int4) * G->nv, hipMemcpyHostToDevice, st));
         if (!G->htab.empty())
-            HIPCHK(hipMemcpy(G->d_htab, G->htab.data(), sizeof(int2) * G->htab.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(G->d_htab, G->htab.data(), sizeof(int2) * G->htab.size(), hipMemcpyHostToDevice,
+                                  st));
     }
     std::vector<SbcVInfo> vi(G->nv);
     for (int64_t v = 0; v < G->nv; v++) vi[v] = vinfo_of(G, (int32_t)v);
-    HIPCHK(hipMemcpy(G->d_vinfo, vi.data(), sizeof(SbcVInfo) * G->nv, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(G->d_col, G->col.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(G->d_seq, G->seq.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(G->d_vinfo, vi.data(), sizeof(SbcVInfo) * G->nv, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(G->d_col, G->col.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(G->d_seq, G->seq.data(), sizeof(int32_t) * G->total_cap, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // vi is local; the mirrors may change after return
     G->full_dirty = false;
     G->p_slot.clear(); G->p_col.clear(); G->p_seq.clear(); G->p_v.clear();
     return hub_flush(G, st);
@@ -425,7 +427,41 @@ struct sbcForestDev {
     int32_t* d_list = nullptr;
     int32_t* d_lvl = nullptr;
     int32_t* d_err = nullptr;
+    // readouts: the forest's own stream, and the point on the last updating
+    // context's stream after which the forest is final
+    hipStream_t rs = nullptr;
+    hipEvent_t done = nullptr;
+    bool done_set = false;
 };
+
+static void legacy_refresh_bc(bcForest* forest, const char* where);
+
+// Records that the forest's last enqueued update ends at this point of st.
+static sbcStatus forest_mark(sbcForestDev* F, hipStream_t st)
+{
+    HIPCHK(hipEventRecord(F->done, st));
+    F->done_set = true;
+    return SBC_OK;
+}
+
+// Device -> host copy of forest data, ordered after the forest's last update
+// (on the forest's own stream: no device-wide synchronisation, so graphs that
+// other threads update meanwhile are not waited for).
+static sbcStatus forest_read(sbcForestDev* F, void* dst, const void* src, size_t bytes)
+{
+    if (F->done_set) HIPCHK(hipStreamWaitEvent(F->rs, F->done, 0));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, F->rs));
+    HIPCHK(hipStreamSynchronize(F->rs));
+    return SBC_OK;
+}
+
+static sbcStatus forest_write(sbcForestDev* F, void* dst, const void* src, size_t bytes)
+{
+    if (F->done_set) HIPCHK(hipStreamWaitEvent(F->rs, F->done, 0));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, F->rs));
+    HIPCHK(hipStreamSynchronize(F->rs));
+    return SBC_OK;
+}
 
 static int32_t lcap_for(int64_t nv) { return (int32_t)(nv + 2 < 4096 ? nv + 2 : 4096); }
 
@@ -458,9 +494,13 @@ extern "C" int64_t streamingBCResidentFootprint(const int64_t* rowptr, int64_t N
 static void forest_free_dev(sbcForestDev* F)
 {
     if (!F) return;
+    if (F->done_set) (void)hipEventSynchronize(F->done);
+    if (F->rs) (void)hipStreamSynchronize(F->rs);
     void* ptrs[] = {F->d_roots, F->d_dist, F->d_sb, F->d_vr, F->d_bc, F->d_list, F->d_lvl, F->d_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (F->done) (void)hipEventDestroy(F->done);
+    if (F->rs) (void)hipStreamDestroy(F->rs);
     delete F;
 }
 
@@ -504,13 +544,15 @@ static sbcStatus forest_create(int64_t NV, int64_t NK, const int64_t* roots, boo
         forest_free_dev(F);
         return st;
     }
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = hipMemcpy(F->d_roots, r32.data(), sizeof(int32_t) * NK, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(F->d_sb, 0, (size_t)NK * (size_t)F->sb_stride);
-    if (e == hipSuccess) e = hipMemset(F->d_vr, 0, sizeof(SbcVRec) * kn);
-    if (e == hipSuccess) e = hipMemset(F->d_bc, 0, sizeof(double) * NV);
-    if (e == hipSuccess) e = hipMemset(F->d_err, 0, sizeof(int32_t) * 4);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    hipError_t e = hipStreamCreateWithFlags(&F->rs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&F->done, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(F->d_roots, r32.data(), sizeof(int32_t) * NK, hipMemcpyHostToDevice, F->rs);
+    if (e == hipSuccess) e = hipMemsetAsync(F->d_sb, 0, (size_t)NK * (size_t)F->sb_stride, F->rs);
+    if (e == hipSuccess) e = hipMemsetAsync(F->d_vr, 0, sizeof(SbcVRec) * kn, F->rs);
+    if (e == hipSuccess) e = hipMemsetAsync(F->d_bc, 0, sizeof(double) * NV, F->rs);
+    if (e == hipSuccess) e = hipMemsetAsync(F->d_err, 0, sizeof(int32_t) * 4, F->rs);
+    if (e == hipSuccess) e = hipStreamSynchronize(F->rs);
     if (e != hipSuccess) {
         forest_free_dev(F);
         return set_err(SBC_ERR_HIP, "forest init: %s", hipGetErrorString(e));
@@ -626,7 +668,8 @@ struct extraArraysPerThread {
     int64_t bucket_blocks = 0;
     int32_t* d_hubq = nullptr;  // per workgroup: hubs queued beyond the LDS queue
     int64_t hubq_blocks = 0;
-    int64_t capture_fallbacks = 0;  // batched calls whose graph capture failed and ran directly
+    int64_t graphs_built = 0;    // window graphs built (one per change of launch shape or buffers)
+    int64_t graph_launches = 0;  // windows launched as a graph replay
     int32_t* d_del = nullptr;   // edge deletion: 4 counts, then the affected root indices
     int64_t del_k = 0;
     // whole-GPU rebuild of one root: a level's big vertices, their term offsets and terms
@@ -650,7 +693,8 @@ static sbcStatus ctx_init(extraArraysPerThread* C)
 {
     HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&C->d_qctr, sizeof(int32_t) * 4));
-    HIPCHK(hipMemset(C->d_qctr, 0, sizeof(int32_t) * 4));
+    HIPCHK(hipMemsetAsync(C->d_qctr, 0, sizeof(int32_t) * 4, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -773,8 +817,8 @@ extern "C" sbcStatus streamingBCResetStats(extraArraysPerThread** eAPT, int32_t 
     for (int k = 0; k < KSTAT_N; k++) { C->launches[k] = 0; C->total_ms[k] = 0; }
     C->timing = enable_timing != 0;
     if (C->d_wtot) {
+        HIPCHK(hipMemsetAsync(C->d_wtot, 0, sizeof(int64_t) * 8 * (size_t)C->wtot_k, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
-        HIPCHK(hipMemset(C->d_wtot, 0, sizeof(int64_t) * 8 * (size_t)C->wtot_k));
     }
     return SBC_OK;
 }
@@ -866,14 +910,19 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     return P;
 }
 
+// End of an update call on forest F: marks where the forest is final for
+// readouts and reports a kernel-side error.
 static sbcStatus check_kernel_err(sbcForestDev* F, extraArraysPerThread* C)
 {
+    sbcStatus st0 = forest_mark(F, C->stream);
+    if (st0) return st0;
     int32_t err = 0;
     HIPCHK(hipMemcpyAsync(&err, F->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     if (err & SBC_KERR_LEVELS) {
         int32_t z = 0;
-        HIPCHK(hipMemcpy(F->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(F->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipStreamSynchronize(C->stream));
         return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", F->lcap);
     }
     return SBC_OK;
@@ -908,6 +957,7 @@ extern "C" void streamingBCInitStaticExact(bcForest* forest, struct stinger* sti
 {
     (void)NT;
     if (!auxilary || run_static(forest, stingerGraph, auxilary[0]) != SBC_OK) legacy_abort("streamingBCInitStaticExact");
+    legacy_refresh_bc(forest, "streamingBCInitStaticExact");
 }
 
 extern "C" void streamingBCInitStaticApproximate(bcForest* forest, struct stinger* stingerGraph, int64_t NT,
@@ -925,6 +975,7 @@ extern "C" void streamingBCInitStaticApproximate(bcForest* forest, struct stinge
         }
     if (!auxilary || run_static(forest, stingerGraph, auxilary[0]) != SBC_OK)
         legacy_abort("streamingBCInitStaticApproximate");
+    legacy_refresh_bc(forest, "streamingBCInitStaticApproximate");
 }
 
 // ------------------------------------------------------------ plain update
@@ -978,6 +1029,7 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
     }
     if (!eAPT || run_insert_plain(forest, sStinger, eAPT[0], newU, newV, granularity) != SBC_OK)
         legacy_abort("insertEdgeStreamingBC");
+    legacy_refresh_bc(forest, "insertEdgeStreamingBC");
     return r;
 }
 
@@ -1298,6 +1350,7 @@ extern "C" StreamingExtraInfo deleteEdgeStreamingBC(bcForest* forest, struct sti
     }
     int64_t c[3];
     if (!eAPT || run_delete(forest, sStinger, eAPT[0], oldU, oldV, c) != SBC_OK) legacy_abort("deleteEdgeStreamingBC");
+    legacy_refresh_bc(forest, "deleteEdgeStreamingBC");
     r.sameLevel = c[0];
     r.adjacent = c[1];
     r.movement = c[2];
@@ -1341,10 +1394,18 @@ extern "C" sbcStatus streamingBCGetBC(bcForest* forest, double* out)
 {
     if (!forest) return set_err(SBC_ERR_ARG, "null forest");
     sbcForestDev* F = forest->dev;
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(forest->totalBC, F->d_bc, sizeof(double) * F->nv, hipMemcpyDeviceToHost));
+    sbcStatus st = forest_read(F, forest->totalBC, F->d_bc, sizeof(double) * F->nv);
+    if (st) return st;
     if (out) memcpy(out, forest->totalBC, sizeof(double) * F->nv);
     return SBC_OK;
+}
+
+// The reference's update calls leave BC in forest->totalBC, where its callers
+// read it (src/streamingbc.c:211-218; src/sbcMain_utils.c:488-548): the
+// legacy entry points refresh the host mirror before they return.
+static void legacy_refresh_bc(bcForest* forest, const char* where)
+{
+    if (streamingBCGetBC(forest, nullptr) != SBC_OK) legacy_abort(where);
 }
 
 extern "C" sbcStatus streamingBCGetTree(bcForest* forest, int64_t ri, int64_t* level, int64_t* sigma, double* delta)
@@ -1352,16 +1413,16 @@ extern "C" sbcStatus streamingBCGetTree(bcForest* forest, int64_t ri, int64_t* l
     if (!forest) return set_err(SBC_ERR_ARG, "null forest");
     sbcForestDev* F = forest->dev;
     if (ri < 0 || ri >= F->K) return set_err(SBC_ERR_ARG, "root index out of range");
-    HIPCHK(hipDeviceSynchronize());
     size_t off = (size_t)ri * (size_t)F->nv;
+    sbcStatus st;
     if (level) {
         std::vector<int32_t> d(F->nv);
-        HIPCHK(hipMemcpy(d.data(), F->d_dist + off, sizeof(int32_t) * F->nv, hipMemcpyDeviceToHost));
+        if ((st = forest_read(F, d.data(), F->d_dist + off, sizeof(int32_t) * F->nv))) return st;
         for (int64_t x = 0; x < F->nv; x++) level[x] = d[x] == INT32_MAX ? LONG_MAX : d[x];
     }
     if (sigma || delta) {
         std::vector<SbcVRec> rs((size_t)F->nv);
-        HIPCHK(hipMemcpy(rs.data(), F->d_vr + off, sizeof(SbcVRec) * F->nv, hipMemcpyDeviceToHost));
+        if ((st = forest_read(F, rs.data(), F->d_vr + off, sizeof(SbcVRec) * F->nv))) return st;
         for (int64_t x = 0; x < F->nv; x++) {
             if (sigma) sigma[x] = rs[x].sigma;
             if (delta) delta[x] = rs[x].delta;
@@ -1387,6 +1448,7 @@ extern "C" sbcStatus streamingBCRecomputeBC(bcForest* forest, extraArraysPerThre
     if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
     HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
     if ((st = ev_end(C, KSTAT_BC, e0))) return st;
+    if ((st = forest_mark(F, C->stream))) return st;
     HIPCHK(hipStreamSynchronize(C->stream));
     return SBC_OK;
 }
@@ -1703,7 +1765,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         if (C->d_wtot) HIPCHK(hipFree(C->d_wtot));
         C->d_wtot = nullptr;
         if ((st = dalloc(&C->d_wtot, (size_t)8 * F->K))) return st;
-        HIPCHK(hipMemset(C->d_wtot, 0, sizeof(int64_t) * 8 * (size_t)F->K));
+        HIPCHK(hipMemsetAsync(C->d_wtot, 0, sizeof(int64_t) * 8 * (size_t)F->K, C->stream));
         C->wtot_k = F->K;
         C->gc.with_rec = -1;  // the captured launch holds the old pointer
     }
@@ -1752,26 +1814,24 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
          gc.has_order != has_order || gc.with_rec != 1 || gc.park_min != P.park_min ||
          gc.run_cap != P.run_cap || gc.hub_batch != P.hub_batch || gc.park_scan != P.park_scan);
     if (need_capture) {
+        // the window's launch sequence is built node by node (no stream
+        // capture), so work that other host threads enqueue, allocate or
+        // synchronise meanwhile cannot invalidate it; a failure is an error
         if (gc.exec) { (void)hipGraphExecDestroy(gc.exec); gc.exec = nullptr; }
         if (gc.graph) { (void)hipGraphDestroy(gc.graph); gc.graph = nullptr; }
         HIPCHK(hipStreamSynchronize(C->stream));
-        HIPCHK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal));
-        sbcStatus cst = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, false);
         hipGraph_t g = nullptr;
-        hipError_t ee = hipStreamEndCapture(C->stream, &g);
-        if (cst != SBC_OK || ee != hipSuccess) {
-            // Nothing inside a capture runs, so this call's windows are
-            // launched directly instead (the same kernels, the same results).
-            // Seen when several host threads capture their own streams at
-            // once: a capture is occasionally invalidated.  A real launch
-            // error shows up again on the direct launch.
-            if (g) (void)hipGraphDestroy(g);
-            (void)hipGetLastError();
-            C->capture_fallbacks++;
-            use_graph = false;
-        } else {
+        HIPCHK(hipGraphCreate(&g, 0));
+        hipError_t ee = sbc_graph_window(g, &A, 4 * W, C->d_qctr, &P, nwb, grid, cfg->recompute_bc);
+        if (ee == hipSuccess) ee = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+        if (ee != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            gc.exec = nullptr;
+            return set_err(SBC_ERR_HIP, "building the window graph failed: %s", hipGetErrorString(ee));
+        }
+        {
             gc.graph = g;
-            HIPCHK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+            C->graphs_built++;
             gc.forest = F; gc.g = G; gc.generation = G->generation; gc.nwb = nwb; gc.grid = grid;
             gc.heavy = heavy; gc.recompute = cfg->recompute_bc; gc.W = W; gc.has_order = has_order;
             gc.with_rec = 1;
@@ -1846,6 +1906,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             hipEvent_t e0;
             if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
             HIPCHK(hipGraphLaunch(gc.exec, C->stream));
+            C->graph_launches++;
             // items parked by this window are finished inside the same timed interval
             if (P.park && (st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
             if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
@@ -1997,6 +2058,14 @@ extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger*
     return SBC_OK;
 }
 
+extern "C" sbcStatus streamingBCGraphCounts(extraArraysPerThread** eAPT, int64_t* built, int64_t* launched)
+{
+    if (!eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null context");
+    if (built) *built = eAPT[0]->graphs_built;
+    if (launched) *launched = eAPT[0]->graph_launches;
+    return SBC_OK;
+}
+
 extern "C" int64_t streamingBCLastParked(extraArraysPerThread** eAPT)
 {
     return (eAPT && eAPT[0]) ? eAPT[0]->last_parked : -1;
@@ -2007,8 +2076,8 @@ extern "C" sbcStatus streamingBCWorkTotals(extraArraysPerThread** eAPT, int64_t 
     if (!eAPT || !eAPT[0] || !out || K < 1) return set_err(SBC_ERR_ARG, "bad argument");
     extraArraysPerThread* C = eAPT[0];
     if (K > C->wtot_k) return set_err(SBC_ERR_ARG, "no work totals for %ld roots", (long)K);
+    HIPCHK(hipMemcpyAsync(out, C->d_wtot, sizeof(int64_t) * 8 * (size_t)K, hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    HIPCHK(hipMemcpy(out, C->d_wtot, sizeof(int64_t) * 8 * (size_t)K, hipMemcpyDeviceToHost));
     return SBC_OK;
 }
 
@@ -2160,6 +2229,7 @@ extern "C" StreamingExtraInfo insertVertexStreamingBC(bcForest* forest, struct s
     }
     if (insert_vertex(forest, sStinger, eAPT, src, adjacencyArray, adjacencySize) != SBC_OK)
         legacy_abort("insertVertexStreamingBC");
+    legacy_refresh_bc(forest, "insertVertexStreamingBC");
     return r;
 }
 
@@ -2179,6 +2249,7 @@ extern "C" StreamingExtraInfo deleteVertexStreamingBC(bcForest* forest, struct s
     int64_t c[3];
     if (delete_vertex(forest, sStinger, eAPT, src, adjacencyArray, cap, adjacencySize, c) != SBC_OK)
         legacy_abort("deleteVertexStreamingBC");
+    legacy_refresh_bc(forest, "deleteVertexStreamingBC");
     r.sameLevel = c[0];
     r.adjacent = c[1];
     r.movement = c[2];
@@ -2232,7 +2303,7 @@ extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stin
     if (!forest || !G || !path) return set_err(SBC_ERR_ARG, "null argument");
     const sbcForestDev* F = forest->dev;
     if (F->nv != G->nv) return set_err(SBC_ERR_ARG, "forest NV %ld != graph NV %ld", (long)F->nv, (long)G->nv);
-    HIPCHK(hipDeviceSynchronize());
+    sbcForestDev* Fw = forest->dev;
     FILE* fp = fopen(path, "wb");
     if (!fp) return set_err(SBC_ERR_ARG, "cannot open %s for writing", path);
     const int64_t nv = F->nv, K = F->K;
@@ -2252,10 +2323,9 @@ extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stin
     std::vector<SbcVRec> r((size_t)nv);
     for (int64_t ri = 0; ok && ri < K; ri++) {
         const size_t off = (size_t)ri * (size_t)nv;
-        if (hipMemcpy(d.data(), F->d_dist + off, 4 * (size_t)nv, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(b.data(), F->d_sb + (size_t)ri * (size_t)F->sb_stride, (size_t)nv, hipMemcpyDeviceToHost) !=
-                hipSuccess ||
-            hipMemcpy(r.data(), F->d_vr + off, sizeof(SbcVRec) * (size_t)nv, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (forest_read(Fw, d.data(), F->d_dist + off, 4 * (size_t)nv) ||
+            forest_read(Fw, b.data(), F->d_sb + (size_t)ri * (size_t)F->sb_stride, (size_t)nv) ||
+            forest_read(Fw, r.data(), F->d_vr + off, sizeof(SbcVRec) * (size_t)nv)) {
             fclose(fp);
             return set_err(SBC_ERR_HIP, "checkpoint: device copy failed");
         }
@@ -2263,7 +2333,7 @@ extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stin
              fwrite(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv;
     }
     std::vector<double> bc((size_t)nv);
-    if (ok && hipMemcpy(bc.data(), F->d_bc, 8 * (size_t)nv, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (ok && forest_read(Fw, bc.data(), F->d_bc, 8 * (size_t)nv)) {
         fclose(fp);
         return set_err(SBC_ERR_HIP, "checkpoint: device copy failed");
     }
@@ -2341,13 +2411,12 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
         const size_t off = (size_t)ri * (size_t)nv;
         ok = fread(d.data(), 4, (size_t)nv, fp) == (size_t)nv && fread(b.data(), 1, (size_t)nv, fp) == (size_t)nv &&
              fread(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv &&
-             hipMemcpy(F->d_dist + off, d.data(), 4 * (size_t)nv, hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(F->d_sb + (size_t)ri * (size_t)F->sb_stride, b.data(), (size_t)nv, hipMemcpyHostToDevice) ==
-                 hipSuccess &&
-             hipMemcpy(F->d_vr + off, r.data(), sizeof(SbcVRec) * (size_t)nv, hipMemcpyHostToDevice) == hipSuccess;
+             forest_write(F, F->d_dist + off, d.data(), 4 * (size_t)nv) == SBC_OK &&
+             forest_write(F, F->d_sb + (size_t)ri * (size_t)F->sb_stride, b.data(), (size_t)nv) == SBC_OK &&
+             forest_write(F, F->d_vr + off, r.data(), sizeof(SbcVRec) * (size_t)nv) == SBC_OK;
     }
     ok = ok && fread(f->totalBC, 8, (size_t)nv, fp) == (size_t)nv &&
-         hipMemcpy(F->d_bc, f->totalBC, 8 * (size_t)nv, hipMemcpyHostToDevice) == hipSuccess &&
+         forest_write(F, F->d_bc, f->totalBC, 8 * (size_t)nv) == SBC_OK &&
          fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkptEnd, 8) == 0;
     fclose(fp);
     if (!ok) {

@@ -238,4 +238,9 @@ hipError_t sbc_launch_wide_mark(const SbcParams* P, struct SbcWideItem* wi, int 
 hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out, hipStream_t st);
 hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
 hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);
+// Adds one window's launch sequence to graph g as kernel nodes in a chain:
+// append (when maxn_append > 0), queue reset of qctr[0..4), the window kernel
+// and, with recompute, the BC reduction.  No stream capture is involved.
+hipError_t sbc_graph_window(hipGraph_t g, const SbcAppend* A, int maxn_append, int32_t* qctr, const SbcParams* P,
+                            int nwb, int grid, int recompute);
 }

@@ -30,6 +30,7 @@
 // in wave order, which is again adjacency order.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "sbc_kernels.h"
 
@@ -3201,4 +3202,68 @@ extern "C" hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_
     const int bs = 256;
     hipLaunchKernelGGL(sbc_append_kernel, dim3((maxn + bs - 1) / bs), dim3(bs), 0, st, *A);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------ window graph
+//
+// The launch sequence of one window (append, queue reset, window kernel,
+// optional BC reduction) built node by node into a HIP graph, with each
+// node's arguments copied into the node, so no stream is ever in capture
+// mode: allocations, frees and synchronisations that other host threads
+// make meanwhile (one per resident graph in the serving loop) cannot
+// invalidate it.
+
+static hipError_t add_node(hipGraph_t g, hipGraphNode_t* prev, void* func, dim3 grid, dim3 block, void** args)
+{
+    hipKernelNodeParams kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.func = func;
+    kp.gridDim = grid;
+    kp.blockDim = block;
+    kp.kernelParams = args;
+    kp.sharedMemBytes = 0;
+    kp.extra = nullptr;
+    hipGraphNode_t n = nullptr;
+    hipError_t e = hipGraphAddKernelNode(&n, g, *prev ? prev : nullptr, *prev ? 1 : 0, &kp);
+    if (e == hipSuccess) *prev = n;
+    return e;
+}
+
+extern "C" hipError_t sbc_graph_window(hipGraph_t g, const SbcAppend* A, int maxn_append, int32_t* qctr,
+                                       const SbcParams* P, int nwb, int grid, int recompute)
+{
+    hipGraphNode_t prev = nullptr;
+    hipError_t e;
+    SbcAppend a = *A;
+    SbcParams p = *P;
+    if (maxn_append > 0) {
+        void* args[] = {&a};
+        e = add_node(g, &prev, (void*)sbc_append_kernel, dim3((maxn_append + 255) / 256), dim3(256), args);
+        if (e != hipSuccess) return e;
+    }
+    {
+        int n = 4;
+        void* args[] = {&qctr, &n};
+        e = add_node(g, &prev, (void*)sbc_reset_kernel, dim3(1), dim3(64), args);
+        if (e != hipSuccess) return e;
+    }
+    {
+        void* args[] = {&p};
+        void* fn = nullptr;
+        if (P->tprof && nwb == 16) fn = (void*)sbc_window_kernel<16, true>;
+        else if (nwb == 1) fn = (void*)sbc_window_kernel<1>;
+        else if (nwb == 2) fn = (void*)sbc_window_kernel<2>;
+        else if (nwb == 4) fn = (void*)sbc_window_kernel<4>;
+        else if (nwb == 8) fn = (void*)sbc_window_kernel<8>;
+        else if (nwb == 16) fn = (void*)sbc_window_kernel<16>;
+        else return hipErrorInvalidValue;
+        e = add_node(g, &prev, fn, dim3(grid), dim3(nwb * 64), args);
+        if (e != hipSuccess) return e;
+    }
+    if (recompute) {
+        void* args[] = {&p};
+        e = add_node(g, &prev, (void*)sbc_bc_reduce_kernel, dim3((P->nv + 255) / 256), dim3(256), args);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }

@@ -8,14 +8,18 @@
  * include/streamingbc_hip.h.
  *
  *   sbc_stream -g file.graph | -r SCALE  [-k ROOTS] [-e INSERTIONS] [-s SEED]
- *              [-G COARSE|FINE] [-w WINDOW] [-d DELETIONS]
+ *              [-G COARSE|FINE] [-w WINDOW] [-d DELETIONS] [-o OUT [-t ROOT]]
  *
  * -r generates an undirected R-MAT graph (16 draws per vertex, largest
  * component); -w > 0 applies the insertions with the batched windowed form;
  * -d then deletes the first DELETIONS inserted edges again, newest first.
  * Prints one JSON line: insertions/s, deletions/s and BC checksums (after the
  * static pass and at the end: equal within rounding when every insertion is
- * deleted again).
+ * deleted again).  With the reference's calls (no -w) BC is read from
+ * forest->totalBC directly, as the reference's callers do
+ * (src/sbcMain_utils.c:488-548).  -o writes, raw and native-endian, the NV
+ * doubles of BC, then level, sigma (int64) and delta (double) of root index
+ * ROOT (default 0), NV entries each.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdint.h>
@@ -37,13 +41,14 @@ static double now_s(void)
 static void usage(void)
 {
     fprintf(stderr, "usage: sbc_stream -g file.graph | -r SCALE [-k ROOTS] [-e INSERTIONS] [-s SEED] "
-                    "[-G COARSE|FINE] [-w WINDOW] [-d DELETIONS]\n");
+                    "[-G COARSE|FINE] [-w WINDOW] [-d DELETIONS] [-o OUT [-t ROOT]]\n");
     exit(2);
 }
 
 int main(int argc, char **argv)
 {
-    const char *path = NULL;
+    const char *path = NULL, *out = NULL;
+    int64_t tree_root = 0;
     int scale = 0, window = 0;
     int64_t NK = 0, E = 64, D = 0;
     uint64_t seed = 1;
@@ -56,6 +61,8 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-s") && i + 1 < argc) seed = strtoull(argv[++i], NULL, 10);
         else if (!strcmp(argv[i], "-w") && i + 1 < argc) window = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-d") && i + 1 < argc) D = atoll(argv[++i]);
+        else if (!strcmp(argv[i], "-o") && i + 1 < argc) out = argv[++i];
+        else if (!strcmp(argv[i], "-t") && i + 1 < argc) tree_root = atoll(argv[++i]);
         else if (!strcmp(argv[i], "-G") && i + 1 < argc) gran = strcmp(argv[++i], "COARSE") ? FINE : COARSE;
         else usage();
     }
@@ -91,7 +98,7 @@ int main(int argc, char **argv)
     if (NK == NV) streamingBCInitStaticExact(forest, G, 1, eAPT);
     else streamingBCInitStaticApproximate(forest, G, 1, eAPT, NK, roots);
     double t_static = now_s() - t0;
-    streamingBCGetBC(forest, NULL);
+    /* the static-pass calls leave BC in forest->totalBC */
     double sum0 = 0.0;
     for (int64_t v = 0; v < NV; v++) sum0 += forest->totalBC[v];
 
@@ -131,7 +138,8 @@ int main(int argc, char **argv)
         cases[1] += sei.adjacent;
         cases[2] += sei.movement;
     }
-    streamingBCGetBC(forest, NULL);
+    /* the batched form is an addition: it leaves BC on the device until read */
+    if (window > 0) streamingBCGetBC(forest, NULL);
     double sum = 0.0, mx = 0.0;
     for (int64_t v = 0; v < NV; v++) {
         sum += forest->totalBC[v];
@@ -142,6 +150,26 @@ int main(int argc, char **argv)
            "\"movement\": %ld, \"bc_sum_static\": %.10e, \"bc_sum\": %.10e, \"bc_max\": %.10e}\n",
            (long)NV, (long)NK, (long)E, t_static, E / (t_upd > 0 ? t_upd : 1e-9), (long)D,
            D / (t_del > 0 ? t_del : 1e-9), (long)cases[0], (long)cases[1], (long)cases[2], sum0, sum, mx);
+    if (out) {
+        if (tree_root < 0 || tree_root >= NK) tree_root = 0;
+        int64_t *lv = (int64_t *)malloc((size_t)NV * sizeof(int64_t));
+        int64_t *sg = (int64_t *)malloc((size_t)NV * sizeof(int64_t));
+        double *dl = (double *)malloc((size_t)NV * sizeof(double));
+        FILE *fp = fopen(out, "wb");
+        int ok = fp && lv && sg && dl && streamingBCGetTree(forest, tree_root, lv, sg, dl) == SBC_OK &&
+                 fwrite(forest->totalBC, sizeof(double), (size_t)NV, fp) == (size_t)NV &&
+                 fwrite(lv, sizeof(int64_t), (size_t)NV, fp) == (size_t)NV &&
+                 fwrite(sg, sizeof(int64_t), (size_t)NV, fp) == (size_t)NV &&
+                 fwrite(dl, sizeof(double), (size_t)NV, fp) == (size_t)NV;
+        if (fp && fclose(fp) != 0) ok = 0;
+        free(lv);
+        free(sg);
+        free(dl);
+        if (!ok) {
+            fprintf(stderr, "cannot write %s\n", out);
+            return 1;
+        }
+    }
 
     streamingBCDeleteAuxilary(eAPT, 1, NV);
     if (NK == NV) streamingBCDeleteForestExact(&forest);

@@ -256,29 +256,42 @@ def test_hub_lookup_path_rmat14(batched, run_cap, hub_batch, monkeypatch):
 
 
 @pytest.mark.parametrize("window", [0, 4])
-def test_c_driver_matches_oracle(window):
+def test_c_driver_matches_oracle(window, tmp_path):
     """The C driver (a reference-style caller of include/streamingbc_hip.h)
-    gives the oracle's BC on the same graph, roots and stream."""
+    gives the oracle's BC vector and trees on the same graph, roots and
+    stream.  With the reference's calls (window 0) it reads BC straight from
+    forest->totalBC after the last insertEdgeStreamingBC, as the reference's
+    callers do; the batched form reads it through streamingBCGetBC."""
     import json
     import os
     import subprocess
     from sbc_testutil import GOLDEN, ROOT
     exe = os.path.join(ROOT, "streamingbc_amd", "lib", "sbc_stream")
     path = os.path.join(GOLDEN, "graphs", "jazz.graph")
-    out = subprocess.run([exe, "-g", path, "-e", "24", "-s", "3", "-w", str(window)], check=True,
-                         capture_output=True, text=True, timeout=120)
+    dump = str(tmp_path / "bc_tree.bin")
+    ri = 17
+    out = subprocess.run([exe, "-g", path, "-e", "24", "-s", "3", "-w", str(window), "-o", dump, "-t", str(ri)],
+                         check=True, capture_output=True, text=True, timeout=120)
     res = json.loads(out.stdout.strip().splitlines()[-1])
     hg = sbc.read_dimacs(path)
-    roots = np.arange(hg.nv)
+    nv = hg.nv
+    raw = np.fromfile(dump, dtype=np.uint8)
+    assert raw.size == 32 * nv
+    bc = raw[:8 * nv].view(np.float64)
+    lv = raw[8 * nv:16 * nv].view(np.int64)
+    sg = raw[16 * nv:24 * nv].view(np.int64)
+    dl = raw[24 * nv:].view(np.float64)
+    roots = np.arange(nv)
     u, v = hg.insertions(24, 13)  # the driver draws its stream with seed + 10
     o = Oracle(hg.rowptr, hg.col, roots)
     o.static(PULL)
     for a, b in zip(u, v):
         o.insert(int(a), int(b), O_FINE)
-    bo = o.bc()
     assert res["insertions"] == 24
-    assert abs(res["bc_sum"] - bo.sum()) <= 1e-9 * abs(bo.sum())
-    assert abs(res["bc_max"] - bo.max()) <= 1e-9 * abs(bo.max())
+    assert_bc_close(bc, o.bc(), what="BC vector read by the C driver")
+    lo, so, do = o.tree(ri)
+    assert np.array_equal(lv, lo) and np.array_equal(sg, so)
+    assert np.array_equal(dl, do)
 
 
 def test_c_driver_deletes_back_to_the_static_bc():

@@ -88,3 +88,7 @@ def test_alone_vs_mix_bit_identical():
         for a, b in zip(srv.resident[name].f.trees(), alone[name][1]):
             assert np.array_equal(a, b), name
         assert srv.resident[name].applied == n
+        # every window ran as a replay of its graph (graphs are built without
+        # stream capture: no other thread's work can invalidate them)
+        built, launched = srv.resident[name].ctx.graph_counts()
+        assert built >= 1 and launched == len(srv.resident[name].windows), name
