@@ -116,7 +116,9 @@ void streamingBCInitStaticApproximate(bcForest *forest, struct stinger *stingerG
  * added with stinger_insert_edge in both directions.  loadBalancing is
  * accepted and ignored (every order gives identical trees);
  * granularity COARSE runs one wavefront per root, FINE one workgroup per
- * root.  NT is ignored. */
+ * root, and a FINE item whose descent frontier or ascent scan grows large is
+ * finished by kernels spanning the whole GPU.  NT is ignored.  forest->totalBC
+ * holds the updated BC on return. */
 StreamingExtraInfo insertEdgeStreamingBC(bcForest *forest, struct stinger *sStinger, int64_t newU,
                                          int64_t newV, int64_t *rootArrayForApproximation, int64_t NK,
                                          int64_t NV, int64_t NT, extraArraysPerThread **eAPT,
@@ -282,9 +284,11 @@ sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, ext
  * failure is an error of the batched call, never a silent direct launch. */
 sbcStatus streamingBCGraphCounts(extraArraysPerThread **eAPT, int64_t *built, int64_t *launched);
 
-/* Number of items of the last streamingBCInsertEdgesBatched call (or of the
- * last deletion) that were finished by the whole-GPU kernels
- * (sbcBatchConfig.park_min; deletions always hand large items over). */
+/* Number of items of the last streamingBCInsertEdgesBatched call, FINE
+ * insertEdgeStreamingBC / sbcInsertEdge call or deletion that were finished
+ * by the whole-GPU kernels (sbcBatchConfig.park_min for the batched form; the
+ * FINE single insertion and deletions hand items over at a frontier of 4096
+ * vertices or 512K adjacency entries to scan). */
 int64_t streamingBCLastParked(extraArraysPerThread **eAPT);
 
 /* Per-root work totals of the batched calls since the last
@@ -302,8 +306,12 @@ int32_t streamingBCComputeUnits(extraArraysPerThread **eAPT);
 sbcStatus streamingBCLastRecords(extraArraysPerThread **eAPT, sbcItemRecord *out, int64_t K);
 
 /* Kernel timing, accumulated since the last reset, from HIP events recorded
- * on the context's stream around each launch.  kernel: 0 update (plain and
- * windowed), 1 static pass, 2 BC recompute, 3 edge append. */
+ * on the context's stream around each launch.  kernel: 0 update (a window's
+ * graph -- append, queue reset, window kernel -- or a single insertion's or
+ * deletion's update kernel), 1 static pass (and deletion rebuilds), 2 BC
+ * recompute, 3 edge append, 4 the whole-GPU kernels that finish items handed
+ * over by an update launch (one interval per update launch that may hand
+ * items over). */
 sbcStatus streamingBCKernelStats(extraArraysPerThread **eAPT, int32_t kernel, int64_t *launches,
                                  double *total_ms);
 sbcStatus streamingBCResetStats(extraArraysPerThread **eAPT, int32_t enable_timing);

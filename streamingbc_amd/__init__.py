@@ -309,7 +309,7 @@ class Context:
         return out
 
 
-KSTAT_UPDATE, KSTAT_STATIC, KSTAT_BC, KSTAT_APPEND = 0, 1, 2, 3
+KSTAT_UPDATE, KSTAT_STATIC, KSTAT_BC, KSTAT_APPEND, KSTAT_WIDE = 0, 1, 2, 3, 4
 
 
 class Graph:

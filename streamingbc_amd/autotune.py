@@ -18,7 +18,7 @@ from dataclasses import asdict, dataclass, field
 
 import numpy as np
 
-from . import KSTAT_UPDATE, BatchConfig, Context, Forest, Graph
+from . import KSTAT_UPDATE, KSTAT_WIDE, BatchConfig, Context, Forest, Graph
 
 DEFAULT_CANDIDATES = (
     {"block_threads": 1024, "grid_blocks": 0},
@@ -86,7 +86,8 @@ def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int,
             forest.insert_batched(g, ctx, u[pos:pos + window], v[pos:pos + window], cfg)
             ctx.synchronize()
             _, ms = ctx.kernel_stats(KSTAT_UPDATE)
-            times[ci].append(ms)
+            _, wms = ctx.kernel_stats(KSTAT_WIDE)
+            times[ci].append(ms + wms)
             pos += window
             used += 1
     table = []

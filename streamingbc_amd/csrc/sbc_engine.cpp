@@ -602,7 +602,11 @@ extern "C" void streamingBCDeleteForestApproximate(bcForestPtr* deadForest, int6
 
 // ------------------------------------------------------------ context
 
-enum { KSTAT_UPDATE = 0, KSTAT_STATIC = 1, KSTAT_BC = 2, KSTAT_APPEND = 3, KSTAT_N = 4 };
+// Kernel timing classes: the update launch (a window's graph: append, queue
+// reset, window kernel; or a single insertion's window kernel), the static
+// pass, the BC reduction, edge appends, and the whole-GPU kernels that finish
+// handed-over items (with the resumed roots' launches) after an update launch.
+enum { KSTAT_UPDATE = 0, KSTAT_STATIC = 1, KSTAT_BC = 2, KSTAT_APPEND = 3, KSTAT_WIDE = 4, KSTAT_N = 5 };
 
 struct WinStage {
     // device window descriptor and its pinned host twin
@@ -630,8 +634,8 @@ struct extraArraysPerThread {
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[KSTAT_N];
-    int64_t launches[KSTAT_N] = {0, 0, 0, 0};
-    double total_ms[KSTAT_N] = {0, 0, 0, 0};
+    int64_t launches[KSTAT_N] = {0, 0, 0, 0, 0};
+    double total_ms[KSTAT_N] = {0, 0, 0, 0, 0};
     int32_t* d_qctr = nullptr;
     SbcItemRec* d_rec = nullptr;
     int64_t rec_cap = 0;       // entries
@@ -980,6 +984,10 @@ extern "C" void streamingBCInitStaticApproximate(bcForest* forest, struct stinge
 
 // ------------------------------------------------------------ plain update
 
+static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K);
+static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
+                              int64_t* total);
+
 static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerThread* C, int64_t u, int64_t v,
                                   uint32_t granularity)
 {
@@ -1002,6 +1010,15 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
         nwb = 16;
         P.heavy = (int32_t)F->K;
         grid = (int)(F->K < 2LL * C->num_cu ? F->K : 2LL * C->num_cu);
+        // items whose descent or ascent grows large are finished by the
+        // whole-GPU level kernels, with the batched form's thresholds
+        if ((st = ensure_park(C, F->K))) return st;
+        P.park = C->d_park;
+        P.parked = C->d_parked;
+        P.park_min = 4096;
+        P.park_scan = kParkScan;
+        const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
+        if (ps) P.park_scan = atoll(ps);
     } else {
         P.heavy = 0;
         int64_t g = (F->K + nwb - 1) / nwb;
@@ -1012,6 +1029,13 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
     if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
     HIPCHK(sbc_launch_window(&P, nwb, grid, C->stream));
     if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+    int64_t parked = 0;
+    if (P.park) {
+        if ((st = ev_begin(C, KSTAT_WIDE, &e0))) return st;
+        if ((st = finish_parked(F, C, P, nwb, grid, &parked))) return st;
+        if ((st = ev_end(C, KSTAT_WIDE, e0))) return st;
+    }
+    C->last_parked = parked;
     C->last_rec_n = F->K;
     return check_kernel_err(F, C);
 }
@@ -1289,9 +1313,11 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
         if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
         HIPCHK(sbc_launch_window(&Q, 16, grid, C->stream));
-        int64_t parked = 0;
-        if ((st = finish_parked(F, C, Q, 16, grid, &parked))) return st;
         if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+        int64_t parked = 0;
+        if ((st = ev_begin(C, KSTAT_WIDE, &e0))) return st;
+        if ((st = finish_parked(F, C, Q, 16, grid, &parked))) return st;
+        if ((st = ev_end(C, KSTAT_WIDE, e0))) return st;
         C->last_parked = parked;
     }
     // movement roots leave BC before their rebuild and return after it; the
@@ -1907,12 +1933,16 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
             HIPCHK(hipGraphLaunch(gc.exec, C->stream));
             C->graph_launches++;
-            // items parked by this window are finished inside the same timed interval
-            if (P.park && (st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
             if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
         } else {
             if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) return st;
-            if (P.park && (st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
+        }
+        // items this window handed over are finished before the next window
+        if (P.park) {
+            hipEvent_t e1;
+            if ((st = ev_begin(C, KSTAT_WIDE, &e1))) return st;
+            if ((st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
+            if ((st = ev_end(C, KSTAT_WIDE, e1))) return st;
         }
         // the captured BC reduction ran before the parked items were finished
         if (nparked > nparked_before && cfg->recompute_bc) HIPCHK(sbc_launch_bc_reduce(&P, C->stream));

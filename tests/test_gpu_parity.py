@@ -419,6 +419,31 @@ def test_items_handed_over_in_the_ascent_match_oracle(window, monkeypatch):
     assert_bc_close(f.bc(), o.bc())
 
 
+def test_plain_items_handed_over_match_oracle(monkeypatch):
+    """The plain FINE form hands large items to the whole-GPU level kernels
+    with the batched form's thresholds; with a tiny scan threshold most
+    nonzero items take that path.  Records, trees and BC stay the oracle's."""
+    monkeypatch.setenv("SBC_PARK_SCAN", "600")
+    hg = sbc.gen_rmat(12, 16 << 12, seed=9, keep_lcc=True)
+    roots = hg.pick_roots(48, 4)
+    u, v = hg.insertions(32, 23)
+    print(f"[plain handover] rmat12 nv={hg.nv} k={len(roots)} n={len(u)}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    parked = 0
+    for i, (a, b) in enumerate(zip(u, v)):
+        f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+        parked += ctx.last_parked()
+        rec_g = ctx.last_records(f.k)
+        rec_o = o.insert(int(a), int(b), O_FINE, items=True)
+        for key in ("kase", "affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec_g[key], rec_o[key]), f"insertion {i} {key}"
+    assert parked > 0, "no plain item was handed over"
+    _compare_trees(f, o, "plain hand-over")
+    assert_bc_close(f.bc(), o.bc())
+
+
 @pytest.mark.gpu
 def test_work_totals_equal_record_sums(monkeypatch):
     """The per-root work totals kept on the device equal the sums of the item
