@@ -237,7 +237,14 @@ def traffic_from(d, workload, bench_json, commands):
     fetch = _window_launches(_dispatches(_rows(os.path.join(d, "fetch"), "counter_collection.csv")))
     write = _window_launches(_dispatches(_rows(os.path.join(d, "write"), "counter_collection.csv")))
     tw = _window_launches(trace)
-    if not (len(fetch) == len(write) == len(tw)) or not fetch:
+    with open(bench_json) as fh:
+        line = [l for l in fh if l.startswith("{")][-1]
+    b = json.loads(line)
+    # the main forest's windows come first; later legs (the plain form's
+    # replay and insertions) may match the same launch pattern
+    n_main = b["timed_windows"][1] + 1
+    fetch, write, tw = fetch[:n_main], write[:n_main], tw[:n_main]
+    if not (len(fetch) == len(write) == len(tw) == n_main):
         raise SystemExit(f"window launches differ between runs: trace {len(tw)} fetch {len(fetch)} write {len(write)}")
 
     def copy_kib(sub, counter):
@@ -251,9 +258,6 @@ def traffic_from(d, workload, bench_json, commands):
     fk = [x["FETCH_SIZE"] for x in fetch]
     wk = [x["WRITE_SIZE"] for x in write]
     ns = [x["ns"] for x in tw]
-    with open(bench_json) as fh:
-        line = [l for l in fh if l.startswith("{")][-1]
-    b = json.loads(line)
     t0 = b["timed_windows"][0]
     t1 = t0 + b["steps"]
     out = {
