@@ -1017,8 +1017,10 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
         P.parked = C->d_parked;
         P.park_min = 4096;
         P.park_scan = kParkScan;
-        const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
+        const char* ps = getenv("SBC_PARK_SCAN");  // test hooks: the scan and frontier thresholds
         if (ps) P.park_scan = atoll(ps);
+        const char* pm = getenv("SBC_PARK_MIN");
+        if (pm && atoi(pm) > 0) P.park_min = atoi(pm);
     } else {
         P.heavy = 0;
         int64_t g = (F->K + nwb - 1) / nwb;
