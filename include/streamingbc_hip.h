@@ -246,6 +246,22 @@ typedef struct {
     int64_t moved;      /* vertices whose distance changed */
 } sbcItemRecord;
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  sbcInsertEdge with granularity COARSE whose only
+ * scratch -- the work-queue counters and the item records -- lives in a fixed
+ * per-GPU array of the library (about 640 KiB, held until the process ends):
+ * it calls no hipMalloc/hipFree and creates no event.  records (may be NULL)
+ * gets the K item records.  Refused before anything changes: more than 16384
+ * roots (SBC_ERR_ARG); an endpoint without append slack, or a graph whose
+ * device image must be uploaded again, both of which allocate (SBC_ERR_STATE:
+ * an allocating call such as sbcInsertEdge handles them); existing edges and
+ * self loops (SBC_ERR_EXISTS).  Every call on one GPU shares the array, which
+ * keeps the last call's contents: calls that overlap on one GPU (from several
+ * host threads) race and must be serialised by the caller. */
+sbcStatus sbcInsertEdgeNoAlloc(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT, int64_t u,
+                               int64_t v, sbcItemRecord *records);
+
 /* Batched persistent form: inserts edges (u[i], v[i]), i = 0..count-1, into
  * the graph and the forest.  The batch is cut into windows of `window`
  * insertions; each window is one persistent kernel launch (captured in a HIP

@@ -106,6 +106,7 @@ _sig("sbcGraphCreate", C.c_int, [i64, P64, P64, VPP])
 _sig("sbcForestCreate", C.c_int, [i64, i64, P64, C.POINTER(FP)])
 _sig("sbcInitStatic", C.c_int, [FP, VP, VPP])
 _sig("sbcInsertEdge", C.c_int, [FP, VP, VPP, i64, i64, u32])
+_sig("sbcInsertEdgeNoAlloc", C.c_int, [FP, VP, VPP, i64, i64, C.POINTER(ItemRecord)])
 _sig("sbcDeleteEdge", C.c_int, [FP, VP, VPP, i64, i64, C.POINTER(ExtraInfoC)])
 _sig("sbcInsertVertex", C.c_int, [FP, VP, VPP, i64, P64, i64])
 _sig("sbcCheckpointSave", C.c_int, [FP, VP, C.c_char_p])
@@ -421,6 +422,15 @@ class Forest:
     def insert_edge(self, g: Graph, ctx: Context, u: int, v: int, granularity: int = FINE):
         """Adds (u,v) to the graph and updates every root (the plain form)."""
         _check(lib.sbcInsertEdge(self._p, g._p, ctx._p, u, v, granularity))
+
+    def insert_edge_noalloc(self, g: Graph, ctx: Context, u: int, v: int, records: bool = False):
+        """sbcInsertEdgeNoAlloc: the plain COARSE insertion with its scratch in
+        the library's fixed per-GPU array (no device allocation); returns the
+        K item records when asked."""
+        rec = np.zeros(self.k, dtype=RECORD_DTYPE) if records else None
+        rp = rec.ctypes.data_as(C.POINTER(ItemRecord)) if records else None
+        _check(lib.sbcInsertEdgeNoAlloc(self._p, g._p, ctx._p, u, v, rp))
+        return rec
 
     def update_inserted(self, g: Graph, ctx: Context, u: int, v: int, granularity: int = FINE):
         """insertEdgeStreamingBC for an edge already added with Graph.insert_edge."""

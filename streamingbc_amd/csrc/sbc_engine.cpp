@@ -2043,6 +2043,60 @@ extern "C" sbcStatus sbcInsertEdge(bcForest* forest, struct stinger* G, extraArr
     return run_insert_plain(forest, G, eAPT[0], u, v, granularity);
 }
 
+// ------------------------------------------------------------ no-allocation form
+//
+// sbcInsertEdge with granularity COARSE (one wavefront per root), for callers
+// that cannot allocate device memory: its only scratch -- the work-queue
+// counters and the item records -- lives in the library's fixed per-GPU
+// array (sbc_noalloc_scratch), and it refuses, before changing anything,
+// whatever would need an allocation: more roots than the array holds, an
+// endpoint without append slack (a relayout), a graph whose device image must
+// be uploaded again.  No timing events are created.
+
+extern "C" sbcStatus sbcInsertEdgeNoAlloc(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT, int64_t u,
+                                          int64_t v, sbcItemRecord* records)
+{
+    if (!forest || !G || !eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null argument");
+    extraArraysPerThread* C = eAPT[0];
+    sbcForestDev* F = forest->dev;
+    if (!F->initialized) return set_err(SBC_ERR_STATE, "static pass has not run");
+    if (F->nv != G->nv) return set_err(SBC_ERR_ARG, "forest NV %ld != graph NV %ld", (long)F->nv, (long)G->nv);
+    if (F->K > SBC_NOALLOC_MAX_ROOTS)
+        return set_err(SBC_ERR_ARG, "%ld roots: the fixed scratch holds %d", (long)F->K, SBC_NOALLOC_MAX_ROOTS);
+    if (u < 0 || v < 0 || u >= G->nv || v >= G->nv) return set_err(SBC_ERR_ARG, "vertex id out of range");
+    if (u == v) return set_err(SBC_ERR_EXISTS, "self loop");
+    if (graph_has_edge(G, (int32_t)u, (int32_t)v)) return set_err(SBC_ERR_EXISTS, "edge (%ld,%ld) exists", (long)u, (long)v);
+    if (G->full_dirty || !G->d_vinfo)
+        return set_err(SBC_ERR_STATE, "the graph's device image must be uploaded again (an allocating call does that)");
+    if (G->deg[u] == G->cap[u] || G->deg[v] == G->cap[v])
+        return set_err(SBC_ERR_STATE, "no append slack left at vertex %ld: a relayout allocates",
+                       (long)(G->deg[u] == G->cap[u] ? u : v));
+    int32_t* qctr = nullptr;
+    SbcItemRec* rec = nullptr;
+    HIPCHK(sbc_noalloc_scratch(&qctr, &rec));
+    stinger_insert_edge(G, 0, u, v, 0, 0);
+    stinger_insert_edge(G, 0, v, u, 0, 0);
+    sbcStatus st = graph_flush(G, C->stream);
+    if (st) return st;
+    SbcParams P = base_params(F, G, C);
+    P.bucket = nullptr;
+    P.hubq = nullptr;
+    P.qctr = qctr;
+    P.rec = rec;
+    P.u0 = (int32_t)u;
+    P.v0 = (int32_t)v;
+    P.heavy = 0;
+    const int nwb = 4;
+    const int64_t g = (F->K + nwb - 1) / nwb;
+    const int grid = (int)(g < 4LL * C->num_cu ? g : 4LL * C->num_cu);
+    HIPCHK(sbc_launch_reset(qctr, 4, C->stream));
+    HIPCHK(sbc_launch_window(&P, nwb, grid, C->stream));
+    static_assert(sizeof(sbcItemRecord) == sizeof(SbcItemRec), "record layout");
+    if (records)
+        HIPCHK(hipMemcpyAsync(records, rec, sizeof(SbcItemRec) * (size_t)F->K, hipMemcpyDeviceToHost, C->stream));
+    return check_kernel_err(F, C);
+}
+
 // ------------------------------------------------------------ cost-model features
 
 extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger* G, extraArraysPerThread** eAPT,

@@ -238,6 +238,11 @@ hipError_t sbc_launch_wide_mark(const SbcParams* P, struct SbcWideItem* wi, int 
 hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, int32_t v, int32_t* out, hipStream_t st);
 hipError_t sbc_launch_reset(int32_t* q, int n, hipStream_t st);
 hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_t st);
+// Device address of this GPU's fixed scratch (the current device's): the
+// queue counters (4 int32) and SBC_NOALLOC_MAX_ROOTS item records.  Resolved
+// once per device; allocates nothing.
+#define SBC_NOALLOC_MAX_ROOTS 16384
+hipError_t sbc_noalloc_scratch(int32_t** qctr, SbcItemRec** rec);
 // Adds one window's launch sequence to graph g as kernel nodes in a chain:
 // append (when maxn_append > 0), queue reset of qctr[0..4), the window kernel
 // and, with recompute, the BC reduction.  No stream capture is involved.

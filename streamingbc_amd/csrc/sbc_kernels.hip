@@ -32,6 +32,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "sbc_kernels.h"
 
 #define INF32 0x7fffffff
@@ -3202,6 +3204,44 @@ extern "C" hipError_t sbc_launch_append(const SbcAppend* A, int maxn, hipStream_
     const int bs = 256;
     hipLaunchKernelGGL(sbc_append_kernel, dim3((maxn + bs - 1) / bs), dim3(bs), 0, st, *A);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------ fixed scratch
+//
+// Device scratch of the entry point that allocates nothing
+// (sbcInsertEdgeNoAlloc): the work-queue counters, then one item record per
+// root.  One copy per GPU, held until the process ends; every call on a GPU
+// shares it, so calls that overlap on one GPU race.  Host code never uses the
+// array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device, below.
+
+constexpr size_t SBC_NOALLOC_QCTR_BYTES = 256;
+static __device__ __align__(256) unsigned char scratch[SBC_NOALLOC_QCTR_BYTES +
+                                                       (size_t)SBC_NOALLOC_MAX_ROOTS * sizeof(SbcItemRec)];
+
+#define SBC_MAX_DEVICES 64
+static void* g_scratch_ptr[SBC_MAX_DEVICES];
+static int g_scratch_err[SBC_MAX_DEVICES];
+static std::once_flag g_scratch_once[SBC_MAX_DEVICES];
+
+extern "C" hipError_t sbc_noalloc_scratch(int32_t** qctr, SbcItemRec** rec)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= SBC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void* p = nullptr;
+        // the current device is dev: its own copy of the array
+        hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_err[dev] = (int)r;
+        g_scratch_ptr[dev] = r == hipSuccess ? p : nullptr;
+    });
+    if (g_scratch_err[dev] != (int)hipSuccess) return (hipError_t)g_scratch_err[dev];
+    unsigned char* base = (unsigned char*)g_scratch_ptr[dev];
+    *qctr = (int32_t*)base;
+    *rec = (SbcItemRec*)(base + SBC_NOALLOC_QCTR_BYTES);
+    return hipSuccess;
 }
 
 // ------------------------------------------------------------ window graph
