@@ -6,10 +6,11 @@ Default workload (the headline configuration): undirected R-MAT scale 20
 deduplicated, seed 1), K = 256 roots drawn from the largest component (seed
 7), random non-adjacent insertions inside it (seed 11), applied in windows of
 128 by the batched persistent kernel.  One step = one window.  The stream is
-cut into 30 untimed leading windows (the cost model's training data under
---schedule mlp), the warmup windows and the timed windows; the plain form is
-then timed on exactly the insertions of the timed windows, on a second forest
-brought to the same state.
+cut into 30 untimed leading windows (on the first 16 the autotuner picks team
+size, heavy/light split and hand-over thresholds; the rest are the cost
+model's training data under --schedule mlp), the warmup windows and the
+timed windows; the plain form is then timed on exactly the insertions of the
+timed windows, on a second forest brought to the same state.
 
 Printed JSON line (rank 0):
   value         insertions/s of the batched form over the timed steps (inputs
@@ -357,7 +358,9 @@ def main():
                     help="untimed leading windows (the cost model's training data under --schedule mlp)")
     ap.add_argument("--light-max", type=float, default=0.0,
                     help="--schedule mlp: roots predicted below this many affected vertices per window run light")
-    ap.add_argument("--autotune", action="store_true", help="tune block size on extra warmup windows first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-parameter search (team size, heavy/light split, hand-over thresholds) "
+                         "that otherwise runs on the first untimed leading windows")
     ap.add_argument("--park-min", type=int, default=0,
                     help="descent frontier size at which an item is finished by whole-GPU kernels "
                          "(0: library default, <0: never)")
@@ -399,10 +402,10 @@ def main():
     kind, kw, K, W, desc = WORKLOADS[args.workload]
     if args.window:
         W = args.window
-    from streamingbc_amd.autotune import DEFAULT_CANDIDATES
-    n_tune = 2 * len(DEFAULT_CANDIDATES) if args.autotune else 0
-    n_train = args.train_windows
-    n_ins = (n_tune + n_train + args.warmup + args.steps) * W
+    from streamingbc_amd.autotune import tuning_windows
+    # the search takes its windows from the untimed leading ones
+    n_train = max(args.train_windows, 0 if args.no_autotune else tuning_windows())
+    n_ins = (n_train + args.warmup + args.steps) * W
     hg, roots, u, v, K, _, desc, t_gen = make_workload(args.workload, n_ins)
     print(f"[bench] {args.workload}: nv={hg.nv} directed slots={hg.ne} K={K} W={W} "
           f"insertions={len(u)} gen={t_gen:.1f}s", file=sys.stderr, flush=True)
@@ -421,12 +424,16 @@ def main():
         cfg.heavy_roots = args.heavy
     pos = 0
     tune = None
-    if args.autotune:
+    if not args.no_autotune:
         from streamingbc_amd.autotune import autotune
-        res, used = autotune(f, g, ctx, u, v, W, hg.ne, base=cfg)
+        res, used = autotune(f, g, ctx, u, v, W, hg.ne, base=cfg, cache=os.environ.get("SBC_AUTOTUNE_CACHE"))
         pos += used
         cfg.block_threads, cfg.grid_blocks = res.block_threads, res.grid_blocks
-        tune = {"block_threads": res.block_threads, "grid_blocks": res.grid_blocks, "table": res.table}
+        cfg.park_min, cfg.park_scan = res.park_min, res.park_scan
+        if args.heavy < -1:
+            cfg.heavy_roots = res.heavy_roots
+        tune = {"block_threads": res.block_threads, "heavy_roots": res.heavy_roots, "park_min": res.park_min,
+                "park_scan": res.park_scan, "windows": used // W, "from_cache": res.from_cache, "table": res.table}
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
 
     from streamingbc_amd.schedule import LearnedSchedule, StaticSchedule
@@ -440,15 +447,16 @@ def main():
         c = sbc.BatchConfig(window=W, block_threads=cfg.block_threads, grid_blocks=cfg.grid_blocks,
                             use_graph=cfg.use_graph, heavy_roots=cfg.heavy_roots if plan.heavy_roots < 0
                             else plan.heavy_roots, root_order=plan.root_order, park_min=cfg.park_min,
-                            use_context_order=plan.device_order)
+                            use_context_order=plan.device_order, park_scan=cfg.park_scan)
         rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or sched.needs_records)
         parked[0] += ctx.last_parked()
         sched.observe(plan, X, rec)
         return rec
 
     parked = [0]
-    # training windows (cost model) and warmup windows, untimed
-    for _ in range(n_train + args.warmup):
+    # the rest of the leading windows (the cost model's training data) and the
+    # warmup windows, untimed
+    for _ in range(n_train - pos // W + args.warmup):
         window(pos, pos + W)
         pos += W
     ctx.synchronize()

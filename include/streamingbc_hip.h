@@ -214,9 +214,11 @@ sbcStatus sbcDeleteVertex(bcForest *forest, struct stinger *G, extraArraysPerThr
  * graph (slot layout with slack, insertion sequence numbers, lookup tables)
  * and every root's tree, state bytes and BC to one file after the device is
  * idle; sbcCheckpointLoad rebuilds a graph and forest from it that continue
- * bit-identically.  A file that is not a checkpoint of this format, or is
- * truncated, is SBC_ERR_ARG before any device allocation.  Host memory: one
- * root's arrays at a time. */
+ * bit-identically.  Every section carries a checksum; a file that is not a
+ * checkpoint of this format, is truncated, fails a checksum or holds a graph
+ * or tree out of range (list bounds, levels, uncommitted update state) is
+ * SBC_ERR_ARG before any device allocation.  Host memory: one root's arrays
+ * at a time. */
 sbcStatus sbcCheckpointSave(const bcForest *forest, const struct stinger *G, const char *path);
 sbcStatus sbcCheckpointLoad(const char *path, bcForest **forest, struct stinger **G);
 
@@ -283,11 +285,13 @@ typedef struct {
     int32_t heavy_roots;    /* roots run in whole-workgroup mode (-1: all) */
     const int32_t *root_order; /* NULL: root-array order */
     int32_t park_min;       /* an item whose descent frontier reaches this many vertices, whose descent
-                               scans 512K adjacency entries, or one of whose ascent levels leaves more
-                               than 512K entries of hubs to scan, is finished by kernels spanning the
-                               whole GPU (0: 4096, < 0: never) */
+                               scans park_scan adjacency entries, or one of whose ascent levels leaves
+                               more than park_scan entries of hubs to scan, is finished by kernels
+                               spanning the whole GPU (0: 4096, < 0: never) */
     int32_t use_context_order; /* 1: root order left in the context by streamingBCScheduleWindow
                                   (sbc_costmodel.h); root_order must be NULL */
+    int64_t park_scan;      /* adjacency entries left to scan at which a descent or an ascent level is
+                               handed to the whole-GPU kernels (0: 512K) */
 } sbcBatchConfig;
 
 sbcStatus streamingBCInsertEdgesBatched(bcForest *forest, struct stinger *G, extraArraysPerThread **eAPT,

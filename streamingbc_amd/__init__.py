@@ -68,7 +68,7 @@ class ExtraInfoC(C.Structure):
 class BatchConfigC(C.Structure):
     _fields_ = [("window", i32), ("block_threads", i32), ("grid_blocks", i32), ("use_graph", i32),
                 ("recompute_bc", i32), ("heavy_roots", i32), ("root_order", C.POINTER(C.c_int32)),
-                ("park_min", i32), ("use_context_order", i32)]
+                ("park_min", i32), ("use_context_order", i32), ("park_scan", i64)]
 
 
 RECORD_DTYPE = np.dtype([("root_index", np.int32), ("insertion", np.int32), ("kase", np.int32),
@@ -388,6 +388,7 @@ class BatchConfig:
     root_order: np.ndarray | None = None
     park_min: int = 0  # 0: library default, < 0: never hand an item to the whole-GPU kernels
     use_context_order: bool = False  # root order left by schedule_window (device)
+    park_scan: int = 0  # 0: library default (512K adjacency entries)
 
 
 class Forest:
@@ -470,7 +471,7 @@ class Forest:
         order = None
         c = BatchConfigC(cfg.window, cfg.block_threads, cfg.grid_blocks, 1 if cfg.use_graph else 0,
                          1 if cfg.recompute_bc else 0, cfg.heavy_roots, None, cfg.park_min,
-                         1 if cfg.use_context_order else 0)
+                         1 if cfg.use_context_order else 0, cfg.park_scan)
         if cfg.root_order is not None:
             order = np.ascontiguousarray(cfg.root_order, dtype=np.int32)
             c.root_order = order.ctypes.data_as(C.POINTER(C.c_int32))

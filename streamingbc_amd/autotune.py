@@ -1,14 +1,23 @@
 """Online autotuner for the windowed update's launch parameters.
 
-The first windows of a stream are run under each candidate launch shape in
-turn (round-robin, ``reps`` windows each), the device time of each window's
-update kernel is recorded, and the candidate with the lowest median time per
-window is kept for the rest of the stream.  Tuning windows are real work: the
-launch shape never changes results (trees and BC are bit-identical under every
-shape), so nothing is thrown away or replayed.
+The search runs in stages over the parameters that change the launch but
+never the results (trees and BC are bit-identical under every setting):
 
-Results are cached in a JSON file keyed by (vertices, edges, roots, window)
-so a later stream on the same graph shape skips the sweep.
+1. team size: threads per workgroup (1024, 512, 256: one root per workgroup
+   team, or one per wavefront for the light phase);
+2. heavy/light split: how many roots run as whole-workgroup teams (all, or
+   all but the last quarter of the queue, which then run one wavefront each);
+3. hand-over thresholds: the descent frontier size (``park_min``) and the
+   adjacency entries left to scan (``park_scan``) at which an item is
+   finished by the whole-GPU level kernels.
+
+Within a stage the candidates take the stream's leading windows in turn
+(round-robin, ``reps`` windows each); each window's device time is the
+window-kernel launch plus the hand-over kernels it caused, and the candidate
+with the lowest median keeps its value for the next stage.  Tuning windows
+are real work: nothing is thrown away or replayed.
+
+Results are cached in a JSON file keyed by (vertices, edges, roots, window).
 """
 from __future__ import annotations
 
@@ -20,11 +29,12 @@ import numpy as np
 
 from . import KSTAT_UPDATE, KSTAT_WIDE, BatchConfig, Context, Forest, Graph
 
-DEFAULT_CANDIDATES = (
-    {"block_threads": 1024, "grid_blocks": 0},
-    {"block_threads": 512, "grid_blocks": 0},
-    {"block_threads": 256, "grid_blocks": 0},
-)
+BLOCKS = (1024, 512, 256)
+# (park_min, park_scan): the library default first
+HANDOVER = ((4096, 512 * 1024), (2048, 256 * 1024), (8192, 1024 * 1024))
+
+# kept for callers that size the tuning budget: the first stage's candidates
+DEFAULT_CANDIDATES = tuple({"block_threads": b, "grid_blocks": 0} for b in BLOCKS)
 
 
 @dataclass
@@ -34,9 +44,13 @@ class TuneResult:
     from_cache: bool
     windows_used: int
     table: list = field(default_factory=list)
+    heavy_roots: int = -1
+    park_min: int = 0
+    park_scan: int = 0
 
     def config(self, window: int, **kw) -> BatchConfig:
-        return BatchConfig(window=window, block_threads=self.block_threads, grid_blocks=self.grid_blocks, **kw)
+        return BatchConfig(window=window, block_threads=self.block_threads, grid_blocks=self.grid_blocks,
+                           heavy_roots=self.heavy_roots, park_min=self.park_min, park_scan=self.park_scan, **kw)
 
 
 def cache_key(nv: int, ne: int, k: int, window: int) -> str:
@@ -61,45 +75,76 @@ def _store(path, key, entry):
     os.replace(tmp, path)
 
 
-def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int, candidates=DEFAULT_CANDIDATES,
-             reps: int = 2, cache: str | None = None, base: BatchConfig | None = None):
+def tuning_windows(reps: int = 2) -> int:
+    """Windows a full search consumes at most."""
+    return reps * (len(BLOCKS) + 2 + len(HANDOVER))
+
+
+def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int, reps: int = 2,
+             cache: str | None = None, base: BatchConfig | None = None, candidates=None):
     """Tunes on the leading windows of (u, v), applying them.  Returns
     (TuneResult, number of insertions consumed)."""
     key = cache_key(forest.nv, ne, forest.k, window)
     hit = _load(cache).get(key)
     if hit is not None:
-        return TuneResult(hit["block_threads"], hit["grid_blocks"], True, 0, hit.get("table", [])), 0
+        return TuneResult(hit["block_threads"], hit.get("grid_blocks", 0), True, 0, hit.get("table", []),
+                          hit.get("heavy_roots", -1), hit.get("park_min", 0), hit.get("park_scan", 0)), 0
     base = base or BatchConfig(window=window)
     u = np.asarray(u, dtype=np.int64)
     v = np.asarray(v, dtype=np.int64)
-    times = {i: [] for i in range(len(candidates))}
+    K = forest.k
+    cur = {"block_threads": 1024, "grid_blocks": 0, "heavy_roots": -1, "park_min": 0, "park_scan": 0}
+    table = []
     pos = 0
     used = 0
-    for rep in range(reps):
-        for ci, cand in enumerate(candidates):
-            if pos + window > u.shape[0]:
-                break
-            cfg = BatchConfig(window=window, block_threads=cand["block_threads"], grid_blocks=cand["grid_blocks"],
-                              use_graph=base.use_graph, recompute_bc=base.recompute_bc,
-                              heavy_roots=base.heavy_roots, root_order=base.root_order)
-            ctx.reset_stats(True)
-            forest.insert_batched(g, ctx, u[pos:pos + window], v[pos:pos + window], cfg)
-            ctx.synchronize()
-            _, ms = ctx.kernel_stats(KSTAT_UPDATE)
-            _, wms = ctx.kernel_stats(KSTAT_WIDE)
-            times[ci].append(ms + wms)
-            pos += window
-            used += 1
-    table = []
-    for ci, cand in enumerate(candidates):
-        if times[ci]:
-            table.append(dict(cand, ms_per_window=float(np.median(times[ci])), windows=len(times[ci])))
-    if not table:
-        c = candidates[0]
-        return TuneResult(c["block_threads"], c["grid_blocks"], False, 0, []), 0
-    best = min(table, key=lambda r: r["ms_per_window"])
-    res = TuneResult(best["block_threads"], best["grid_blocks"], False, used, table)
-    _store(cache, key, {"block_threads": res.block_threads, "grid_blocks": res.grid_blocks, "table": table})
+
+    def run_stage(stage, cands):
+        nonlocal pos, used
+        times = [[] for _ in cands]
+        for rep in range(reps):
+            for ci, cand in enumerate(cands):
+                if pos + window > u.shape[0]:
+                    break
+                # a candidate several times slower than the best first window
+                # is not run again (one-wavefront teams on hub-heavy graphs)
+                firsts = [t[0] for t in times if t]
+                if rep > 0 and times[ci] and times[ci][0] > 4.0 * min(firsts):
+                    continue
+                c = dict(cur, **cand)
+                cfg = BatchConfig(window=window, block_threads=c["block_threads"], grid_blocks=c["grid_blocks"],
+                                  use_graph=base.use_graph, recompute_bc=base.recompute_bc,
+                                  heavy_roots=c["heavy_roots"], root_order=base.root_order,
+                                  park_min=c["park_min"], park_scan=c["park_scan"])
+                ctx.reset_stats(True)
+                forest.insert_batched(g, ctx, u[pos:pos + window], v[pos:pos + window], cfg)
+                ctx.synchronize()
+                _, ms = ctx.kernel_stats(KSTAT_UPDATE)
+                _, wms = ctx.kernel_stats(KSTAT_WIDE)
+                times[ci].append(ms + wms)
+                pos += window
+                used += 1
+        best, best_ms = None, None
+        for cand, t in zip(cands, times):
+            if not t:
+                continue
+            med = float(np.median(t))
+            table.append(dict(cand, stage=stage, ms_per_window=round(med, 4), windows=len(t)))
+            if best_ms is None or med < best_ms:
+                best, best_ms = cand, med
+        if best is not None:
+            cur.update(best)
+
+    if candidates is not None:
+        run_stage("given", list(candidates))
+    else:
+        run_stage("team size", [{"block_threads": b} for b in BLOCKS])
+        if cur["block_threads"] > 64:
+            run_stage("heavy/light split", [{"heavy_roots": -1}, {"heavy_roots": K - K // 4}])
+            run_stage("hand-over thresholds", [{"park_min": a, "park_scan": b} for a, b in HANDOVER])
+    res = TuneResult(cur["block_threads"], cur["grid_blocks"], False, used, table, cur["heavy_roots"],
+                     cur["park_min"], cur["park_scan"])
+    _store(cache, key, {k: getattr(res, k) for k in ("block_threads", "grid_blocks", "heavy_roots", "park_min",
+                                                     "park_scan", "table")})
     return res, pos
 
 

@@ -1818,7 +1818,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         // across the GPU; the rest of the root's window then runs in a launch
         // of its own (headline workload: 2M entries 10.65K insertions/s, 1M
         // 10.89K, 512K 11.49K, 256K 11.26K, 128K 10.59K)
-        P.park_scan = kParkScan;
+        P.park_scan = cfg->park_scan > 0 ? cfg->park_scan : kParkScan;
         const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
         if (ps) P.park_scan = atoll(ps);
     }
@@ -2368,20 +2368,63 @@ extern "C" sbcStatus sbcDeleteVertex(bcForest* forest, struct stinger* G, extraA
 //
 // One file per resident graph and its forest (the reference's only precedent
 // is STINGER's own save/open, src/stinger/stinger.c:1808-2038).  Layout,
-// little-endian: "SBCCKPT1", int64 {version 1, nv, K, exact, initialized,
-// slot count, table slot count, next sequence number}, roots[K] (int64); the
-// graph's host mirror: start/deg/deg0/cap[nv] (int32), col/seq[slots] (int32),
-// hub[nv] (int4), htab[table slots] (int2); per root: dist[nv] (int32), state
-// bytes[nv], records[nv] (32 B); BC[nv] (double); "SBCEND01".  Loading checks
-// the header and the exact file size before any device allocation.
+// little-endian: "SBCCKPT2", int64 {version 2, nv, K, exact, initialized,
+// slot count, table slot count, next sequence number}; then sections, each
+// followed by the uint64 checksum of its bytes (ckpt_hash): roots[K] (int64);
+// the graph's host mirror: start/deg/deg0/cap[nv] (int32), col/seq[slots]
+// (int32), hub[nv] (int4), htab[table slots] (int2); per root: dist[nv]
+// (int32), state bytes[nv], records[nv] (32 B); BC[nv] (double); then
+// "SBCEND02".  Loading checks the header, the exact file size, every
+// section's checksum and the ranges of the graph and the trees (list bounds,
+// levels below the level capacity or unreachable, committed state bytes, a
+// root at level 0) in a first pass, before any device allocation.
 
-static const char kCkptMagic[8] = {'S', 'B', 'C', 'C', 'K', 'P', 'T', '1'};
-static const char kCkptEnd[8] = {'S', 'B', 'C', 'E', 'N', 'D', '0', '1'};
+static const char kCkptMagic[8] = {'S', 'B', 'C', 'C', 'K', 'P', 'T', '2'};
+static const char kCkptEnd[8] = {'S', 'B', 'C', 'E', 'N', 'D', '0', '2'};
+
+// Streaming 64-bit checksum over a byte sequence: 8-byte little-endian words
+// (the tail zero-padded), each xored in, multiplied by the FNV-64 prime and
+// folded; the length closes it.
+struct CkptHash {
+    uint64_t h = 0x9E3779B97F4A7C15ULL, len = 0;
+    uint8_t buf[8];
+    int nb = 0;
+    void word(uint64_t w)
+    {
+        h ^= w;
+        h *= 0x100000001B3ULL;
+        h ^= h >> 32;
+    }
+    void add(const void* p, size_t n)
+    {
+        const uint8_t* b = (const uint8_t*)p;
+        len += n;
+        while (nb && n) {
+            buf[nb++] = *b++;
+            n--;
+            if (nb == 8) { uint64_t w; memcpy(&w, buf, 8); word(w); nb = 0; }
+        }
+        for (; n >= 8; n -= 8, b += 8) { uint64_t w; memcpy(&w, b, 8); word(w); }
+        while (n) { buf[nb++] = *b++; n--; }
+    }
+    uint64_t fin()
+    {
+        if (nb) {
+            memset(buf + nb, 0, (size_t)(8 - nb));
+            uint64_t w;
+            memcpy(&w, buf, 8);
+            word(w);
+            nb = 0;
+        }
+        word(len);
+        return h;
+    }
+};
 
 static int64_t ckpt_size(int64_t nv, int64_t K, int64_t slots, int64_t tslots)
 {
-    return 8 + 8 * 8 + 8 * K + 16 * nv + 8 * slots + 16 * nv + 8 * tslots +
-           K * nv * (4 + 1 + (int64_t)sizeof(SbcVRec)) + 8 * nv + 8;
+    return 8 + 8 * 8 + (8 * K + 8) + (16 * nv + 8 * slots + 16 * nv + 8 * tslots + 8) +
+           K * (nv * (4 + 1 + (int64_t)sizeof(SbcVRec)) + 8) + (8 * nv + 8) + 8;
 }
 
 extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stinger* G, const char* path)
@@ -2393,16 +2436,24 @@ extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stin
     FILE* fp = fopen(path, "wb");
     if (!fp) return set_err(SBC_ERR_ARG, "cannot open %s for writing", path);
     const int64_t nv = F->nv, K = F->K;
-    const int64_t hdr[8] = {1, nv, K, F->exact ? 1 : 0, F->initialized ? 1 : 0, G->total_cap,
+    const int64_t hdr[8] = {2, nv, K, F->exact ? 1 : 0, F->initialized ? 1 : 0, G->total_cap,
                             (int64_t)G->htab.size(), G->seq_next};
+    CkptHash ck;
+    auto put = [&](const void* p, size_t sz, size_t n) {
+        ck.add(p, sz * n);
+        return fwrite(p, sz, n, fp) == n;
+    };
+    auto seal = [&]() {
+        const uint64_t c = ck.fin();
+        ck = CkptHash();
+        return fwrite(&c, 8, 1, fp) == 1;
+    };
     bool ok = fwrite(kCkptMagic, 1, 8, fp) == 8 && fwrite(hdr, 8, 8, fp) == 8 &&
-              fwrite(F->roots.data(), 8, (size_t)K, fp) == (size_t)K;
+              put(F->roots.data(), 8, (size_t)K) && seal();
     for (const std::vector<int32_t>* a : {&G->start, &G->deg, &G->deg0, &G->cap})
-        ok = ok && fwrite(a->data(), 4, (size_t)nv, fp) == (size_t)nv;
-    ok = ok && fwrite(G->col.data(), 4, (size_t)G->total_cap, fp) == (size_t)G->total_cap &&
-         fwrite(G->seq.data(), 4, (size_t)G->total_cap, fp) == (size_t)G->total_cap &&
-         fwrite(G->hub.data(), sizeof(int4), (size_t)nv, fp) == (size_t)nv &&
-         fwrite(G->htab.data(), sizeof(int2), G->htab.size(), fp) == G->htab.size();
+        ok = ok && put(a->data(), 4, (size_t)nv);
+    ok = ok && put(G->col.data(), 4, (size_t)G->total_cap) && put(G->seq.data(), 4, (size_t)G->total_cap) &&
+         put(G->hub.data(), sizeof(int4), (size_t)nv) && put(G->htab.data(), sizeof(int2), G->htab.size()) && seal();
     // per root, through one host buffer of nv records
     std::vector<int32_t> d((size_t)nv);
     std::vector<uint8_t> b((size_t)nv);
@@ -2415,18 +2466,34 @@ extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stin
             fclose(fp);
             return set_err(SBC_ERR_HIP, "checkpoint: device copy failed");
         }
-        ok = fwrite(d.data(), 4, (size_t)nv, fp) == (size_t)nv && fwrite(b.data(), 1, (size_t)nv, fp) == (size_t)nv &&
-             fwrite(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv;
+        ok = put(d.data(), 4, (size_t)nv) && put(b.data(), 1, (size_t)nv) &&
+             put(r.data(), sizeof(SbcVRec), (size_t)nv) && seal();
     }
     std::vector<double> bc((size_t)nv);
     if (ok && forest_read(Fw, bc.data(), F->d_bc, 8 * (size_t)nv)) {
         fclose(fp);
         return set_err(SBC_ERR_HIP, "checkpoint: device copy failed");
     }
-    ok = ok && fwrite(bc.data(), 8, (size_t)nv, fp) == (size_t)nv && fwrite(kCkptEnd, 1, 8, fp) == 8;
+    ok = ok && put(bc.data(), 8, (size_t)nv) && seal() && fwrite(kCkptEnd, 1, 8, fp) == 8;
     if (fclose(fp) != 0) ok = false;
     if (!ok) return set_err(SBC_ERR_ARG, "checkpoint: writing %s failed", path);
     return SBC_OK;
+}
+
+// Reads n items of sz bytes into p and adds them to the running checksum.
+static bool ckpt_get(FILE* fp, CkptHash& ck, void* p, size_t sz, size_t n)
+{
+    if (fread(p, sz, n, fp) != n) return false;
+    ck.add(p, sz * n);
+    return true;
+}
+
+static bool ckpt_check(FILE* fp, CkptHash& ck)
+{
+    uint64_t c = 0;
+    const bool ok = fread(&c, 8, 1, fp) == 1 && c == ck.fin();
+    ck = CkptHash();
+    return ok;
 }
 
 extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, struct stinger** graph)
@@ -2438,7 +2505,7 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
     int64_t hdr[8];
     bool ok = fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(hdr, 8, 8, fp) == 8;
     const int64_t nv = ok ? hdr[1] : 0, K = ok ? hdr[2] : 0, slots = ok ? hdr[5] : 0, tslots = ok ? hdr[6] : 0;
-    ok = ok && hdr[0] == 1 && nv >= 1 && nv < INT32_MAX / 4 && K >= 1 && K <= nv && slots >= 0 &&
+    ok = ok && hdr[0] == 2 && nv >= 1 && nv < INT32_MAX / 4 && K >= 1 && K <= nv && slots >= 0 &&
          slots < INT32_MAX && tslots >= 0 && tslots < INT32_MAX && hdr[7] >= 1 && hdr[7] < INT32_MAX;
     if (ok) {
         // the exact size, before anything is allocated
@@ -2460,13 +2527,20 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
     G->seq.resize((size_t)slots);
     G->hub.resize((size_t)nv);
     G->htab.resize((size_t)tslots);
-    ok = fread(roots.data(), 8, (size_t)K, fp) == (size_t)K;
+    auto fail = [&](const char* why) {
+        fclose(fp);
+        delete G;
+        return set_err(SBC_ERR_ARG, "%s: %s", path, why);
+    };
+    CkptHash ck;
+    if (!ckpt_get(fp, ck, roots.data(), 8, (size_t)K) || !ckpt_check(fp, ck)) return fail("root section fails its checksum");
+    ok = true;
     for (std::vector<int32_t>* a : {&G->start, &G->deg, &G->deg0, &G->cap})
-        ok = ok && fread(a->data(), 4, (size_t)nv, fp) == (size_t)nv;
-    ok = ok && fread(G->col.data(), 4, (size_t)slots, fp) == (size_t)slots &&
-         fread(G->seq.data(), 4, (size_t)slots, fp) == (size_t)slots &&
-         fread(G->hub.data(), sizeof(int4), (size_t)nv, fp) == (size_t)nv &&
-         fread(G->htab.data(), sizeof(int2), (size_t)tslots, fp) == (size_t)tslots;
+        ok = ok && ckpt_get(fp, ck, a->data(), 4, (size_t)nv);
+    ok = ok && ckpt_get(fp, ck, G->col.data(), 4, (size_t)slots) && ckpt_get(fp, ck, G->seq.data(), 4, (size_t)slots) &&
+         ckpt_get(fp, ck, G->hub.data(), sizeof(int4), (size_t)nv) &&
+         ckpt_get(fp, ck, G->htab.data(), sizeof(int2), (size_t)tslots) && ckpt_check(fp, ck);
+    if (!ok) return fail("graph section fails its checksum");
     // the graph arrays must describe lists inside the slots and tables inside htab
     for (int64_t v = 0; ok && v < nv; v++) {
         const int4 h = G->hub[v];
@@ -2476,11 +2550,26 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
     }
     for (int64_t i = 0; ok && i < K; i++) ok = roots[(size_t)i] >= 0 && roots[(size_t)i] < nv;
     for (int64_t e = 0; ok && e < slots; e++) ok = G->col[(size_t)e] >= 0 && G->col[(size_t)e] < nv;
-    if (!ok) {
-        fclose(fp);
-        delete G;
-        return set_err(SBC_ERR_ARG, "%s: inconsistent graph in checkpoint", path);
+    if (!ok) return fail("inconsistent graph in checkpoint");
+    // first pass over the trees: checksums and ranges, nothing allocated yet
+    const long long trees_at = (long long)ftello(fp);
+    const int32_t lcap = lcap_for(nv);
+    std::vector<int32_t> d((size_t)nv);
+    std::vector<uint8_t> b((size_t)nv);
+    std::vector<SbcVRec> r((size_t)nv);
+    for (int64_t ri = 0; ri < K; ri++) {
+        if (!ckpt_get(fp, ck, d.data(), 4, (size_t)nv) || !ckpt_get(fp, ck, b.data(), 1, (size_t)nv) ||
+            !ckpt_get(fp, ck, r.data(), sizeof(SbcVRec), (size_t)nv) || !ckpt_check(fp, ck))
+            return fail("a tree section fails its checksum");
+        bool good = d[(size_t)roots[(size_t)ri]] == 0;
+        for (int64_t x = 0; good && x < nv; x++)
+            good = (d[(size_t)x] == INT32_MAX || (d[(size_t)x] >= 0 && d[(size_t)x] < lcap)) && (b[(size_t)x] & 3) == 0 &&
+                   r[(size_t)x].dpath == 0 && r[(size_t)x].ndel == 0.0;
+        if (!good) return fail("a tree holds a level out of range, a root off level 0 or uncommitted state");
     }
+    std::vector<double> bc((size_t)nv);
+    if (!ckpt_get(fp, ck, bc.data(), 8, (size_t)nv) || !ckpt_check(fp, ck)) return fail("BC section fails its checksum");
+    if (fread(magic, 1, 8, fp) != 8 || memcmp(magic, kCkptEnd, 8) != 0) return fail("end marker missing");
     G->full_dirty = true;
     bcForest* f = nullptr;
     sbcStatus st = forest_create(nv, K, roots.data(), hdr[3] != 0, &f);
@@ -2490,25 +2579,23 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
         return st;
     }
     sbcForestDev* F = f->dev;
-    std::vector<int32_t> d((size_t)nv);
-    std::vector<uint8_t> b((size_t)nv);
-    std::vector<SbcVRec> r((size_t)nv);
+    ok = fseeko(fp, (off_t)trees_at, SEEK_SET) == 0;
     for (int64_t ri = 0; ok && ri < K; ri++) {
         const size_t off = (size_t)ri * (size_t)nv;
+        uint64_t c;
         ok = fread(d.data(), 4, (size_t)nv, fp) == (size_t)nv && fread(b.data(), 1, (size_t)nv, fp) == (size_t)nv &&
-             fread(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv &&
+             fread(r.data(), sizeof(SbcVRec), (size_t)nv, fp) == (size_t)nv && fread(&c, 8, 1, fp) == 1 &&
              forest_write(F, F->d_dist + off, d.data(), 4 * (size_t)nv) == SBC_OK &&
              forest_write(F, F->d_sb + (size_t)ri * (size_t)F->sb_stride, b.data(), (size_t)nv) == SBC_OK &&
              forest_write(F, F->d_vr + off, r.data(), sizeof(SbcVRec) * (size_t)nv) == SBC_OK;
     }
-    ok = ok && fread(f->totalBC, 8, (size_t)nv, fp) == (size_t)nv &&
-         forest_write(F, F->d_bc, f->totalBC, 8 * (size_t)nv) == SBC_OK &&
-         fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkptEnd, 8) == 0;
     fclose(fp);
+    memcpy(f->totalBC, bc.data(), 8 * (size_t)nv);
+    ok = ok && forest_write(F, F->d_bc, f->totalBC, 8 * (size_t)nv) == SBC_OK;
     if (!ok) {
         forest_destroy(&f);
         delete G;
-        return set_err(SBC_ERR_ARG, "%s: reading the forest failed", path);
+        return set_err(SBC_ERR_HIP, "%s: uploading the forest failed", path);
     }
     F->initialized = hdr[4] != 0;
     *forest = f;

@@ -4,7 +4,11 @@ forest, context and HIP stream, fed by a stream of edge insertions.
 * Admission: a graph is admitted only while the sum of resident footprints
   (``streamingBCResidentFootprint``: forest + device CSR + update scratch)
   stays within the HBM budget; otherwise ``admit`` raises ``SbcError`` with
-  status ``SBC_ERR_BUDGET`` before touching the device.
+  status ``SBC_ERR_BUDGET`` before touching the device, and ``submit`` queues
+  the graph instead: queued graphs are admitted, first come first served,
+  while ``serve`` runs, as soon as graphs whose streams have ended are evicted
+  (``evict_when_done``) and their memory is free; their insertions wait
+  meanwhile and the wait counts in their latency.
 * Continuous batching: one host worker per resident graph takes every
   insertion that has arrived and not yet been applied (up to ``max_window``)
   and applies them as one window; work for different graphs runs
@@ -22,6 +26,7 @@ Run ``python -m streamingbc_amd.serve`` for the Chung-Lu serving workload.
 from __future__ import annotations
 
 import argparse
+import collections
 import json
 import threading
 import time
@@ -31,6 +36,14 @@ import numpy as np
 
 from . import (SBC_ERR_ARG, SBC_ERR_BUDGET, SBC_ERR_EXISTS, BatchConfig, Context, Forest, Graph, HostGraph, SbcError,
                gen_chung_lu, resident_footprint)
+
+
+@dataclass
+class Pending:
+    name: str
+    hg: HostGraph
+    roots: np.ndarray
+    footprint: int
 
 
 @dataclass
@@ -56,6 +69,8 @@ class Server:
         self.use_graph = use_graph
         self.num_cu = num_cu
         self.resident: dict[str, Resident] = {}
+        self.pending: collections.deque = collections.deque()
+        self.admitted_at: dict[str, float] = {}   # seconds into serve() when a queued graph was admitted
 
     @property
     def used(self) -> int:
@@ -81,6 +96,24 @@ class Server:
         r = Resident(name, hg, roots, fp, ctx, g, f)
         self.resident[name] = r
         return r
+
+    def submit(self, name: str, hg: HostGraph, roots) -> bool:
+        """Admits the graph now if it fits, else queues it for admission
+        during ``serve``.  Returns True if admitted now.  A graph larger than
+        the whole budget is refused (SBC_ERR_BUDGET)."""
+        roots = np.ascontiguousarray(roots, dtype=np.int64)
+        if roots.ndim != 1 or roots.size == 0:
+            raise SbcError(SBC_ERR_ARG, "roots must be a non-empty 1-d array")
+        if name in self.resident or any(p.name == name for p in self.pending):
+            raise SbcError(SBC_ERR_EXISTS, f"graph {name!r} is already resident or queued")
+        fp = self.footprint(hg, roots.size)
+        if fp > self.budget:
+            raise SbcError(SBC_ERR_BUDGET, f"graph {name!r} needs {fp} B; the budget is {self.budget} B")
+        if not self.pending and self.used + fp <= self.budget:
+            self.admit(name, hg, roots)
+            return True
+        self.pending.append(Pending(name, hg, roots, fp))
+        return False
 
     def evict(self, name: str) -> None:
         r = self.resident.pop(name)
@@ -111,30 +144,75 @@ class Server:
         except BaseException as e:  # reported by serve()
             err.append((r.name, e))
 
-    def serve(self, arrivals: dict) -> dict:
+    def serve(self, arrivals: dict, evict_when_done: bool = False, results: dict | None = None) -> dict:
         """arrivals: name -> (t_seconds sorted ascending, u, v) for resident
-        graphs.  Blocks until every insertion is applied."""
+        or queued graphs.  Blocks until every insertion is applied.  With
+        evict_when_done a graph is evicted as soon as its stream is applied
+        (its deterministic BC kept in ``results`` when given), which admits
+        queued graphs that then fit."""
+        queued = {p.name for p in self.pending}
         for name in arrivals:
-            if name not in self.resident:
-                raise SbcError(SBC_ERR_ARG, f"graph {name!r} is not resident")
+            if name not in self.resident and name not in queued:
+                raise SbcError(SBC_ERR_ARG, f"graph {name!r} is neither resident nor queued")
         lats = {}
         err = []
         threads = []
-        t0 = time.perf_counter()
+        windows = {}
+        lock = threading.Lock()
+        streams = {}
         for name, (t, u, v) in arrivals.items():
             t = np.asarray(t, dtype=np.float64)
             lats[name] = np.zeros(t.shape[0])
-            th = threading.Thread(target=self._worker,
-                                  args=(self.resident[name], t, np.asarray(u, np.int64), np.asarray(v, np.int64), t0,
-                                        lats[name], err))
+            streams[name] = (t, np.asarray(u, np.int64), np.asarray(v, np.int64))
+        t0 = time.perf_counter()
+
+        def run(name):
+            r = self.resident[name]
+            t, u, v = streams[name]
+            self._worker(r, t, u, v, t0, lats[name], err)
+            with lock:
+                windows[name] = list(r.windows)
+                if results is not None:
+                    results[name] = self.bc(name)
+                if evict_when_done:
+                    self.evict(name)
+                    admit_ready()
+
+        def start(name):
+            th = threading.Thread(target=run, args=(name,))
             th.start()
             threads.append(th)
-        for th in threads:
-            th.join()
+
+        def admit_ready():
+            # first come, first served: the head of the queue waits for room
+            while self.pending and self.used + self.pending[0].footprint <= self.budget:
+                p = self.pending.popleft()
+                try:
+                    self.admit(p.name, p.hg, p.roots)
+                except BaseException as e:  # reported after the join
+                    err.append((p.name, e))
+                    continue
+                self.admitted_at[p.name] = time.perf_counter() - t0
+                if p.name in streams:
+                    start(p.name)
+
+        with lock:
+            for name in arrivals:
+                if name in self.resident:
+                    start(name)
+        while True:
+            with lock:
+                live = [th for th in threads if th.is_alive()]
+                if not live:
+                    break
+            live[0].join()
         wall = time.perf_counter() - t0
         if err:
             raise err[0][1]
-        return summarize(lats, wall, {n: self.resident[n].windows for n in arrivals})
+        left = [n for n in arrivals if n not in windows]
+        if left:
+            raise SbcError(SBC_ERR_BUDGET, f"graphs {left} were never admitted (evict_when_done frees memory)")
+        return summarize(lats, wall, windows)
 
     def bc(self, name: str) -> np.ndarray:
         r = self.resident[name]

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 sbc = pytest.importorskip("streamingbc_amd")
-from streamingbc_amd.autotune import DEFAULT_CANDIDATES, _store, autotune, cache_key  # noqa: E402
+from streamingbc_amd.autotune import DEFAULT_CANDIDATES, HANDOVER, _store, autotune, cache_key, tuning_windows  # noqa: E402
 
 
 class _NoDevice:
@@ -40,7 +40,7 @@ def test_tuning_windows_are_real_work(tmp_path):
     hg = sbc.gen_rmat(12, 16 << 12, seed=3)
     roots = hg.pick_roots(64, 2)
     W = 8
-    u, v = hg.insertions(W * 10, 5)
+    u, v = hg.insertions(W * 20, 5)
 
     def fresh():
         ctx = sbc.Context(hg.nv)
@@ -52,8 +52,11 @@ def test_tuning_windows_are_real_work(tmp_path):
     ctx, g, f = fresh()
     path = str(tmp_path / "tune.json")
     res, used = autotune(f, g, ctx, u, v, W, hg.ne, reps=2, cache=path)
-    assert used == 2 * len(DEFAULT_CANDIDATES) * W and not res.from_cache
-    assert len(res.table) == len(DEFAULT_CANDIDATES) and all(r["ms_per_window"] > 0 for r in res.table)
+    assert used <= tuning_windows(2) * W and used % W == 0 and not res.from_cache
+    stages = [r["stage"] for r in res.table]
+    assert stages == ["team size"] * 3 + ["heavy/light split"] * 2 + ["hand-over thresholds"] * 3
+    assert all(r["ms_per_window"] > 0 for r in res.table)
+    assert res.heavy_roots in (-1, 64 - 16) and (res.park_min, res.park_scan) in [(0, 0)] + list(HANDOVER)
     f.insert_batched(g, ctx, u[used:], v[used:], res.config(W))
     f.recompute_bc(ctx)
     ctx2, g2, f2 = fresh()
@@ -65,3 +68,4 @@ def test_tuning_windows_are_real_work(tmp_path):
     # second call: served from the cache
     res2, used2 = autotune(f, g, ctx, u, v, W, hg.ne, cache=path)
     assert res2.from_cache and used2 == 0 and res2.block_threads == res.block_threads
+    assert (res2.heavy_roots, res2.park_min, res2.park_scan) == (res.heavy_roots, res.park_min, res.park_scan)

@@ -323,9 +323,10 @@ def test_c_driver_deletes_back_to_the_static_bc():
     assert abs(res["bc_max"] - o.bc().max()) <= 1e-9 * abs(o.bc().max())
 
 
-@pytest.mark.skip(reason="shared levels (idle workgroups taking chunks of another root's large BFS levels) "
-                         "faulted the GPU twice; the path is out of the library until the cause is found "
-                         "on the CPU, and SBC_ASSIST_* are ignored meanwhile")
+@pytest.mark.skip(reason="kept as a record: the shared-levels path (idle workgroups taking chunks of another root's "
+                         "large BFS levels) faulted the GPU twice with its cause unfound; its code is no longer in "
+                         "the library and the SBC_ASSIST_* switches it set do nothing, so this test exercises only "
+                         "the default path")
 @pytest.mark.parametrize("knobs", [("16", "8", "32"), ("96", "64", "256")])
 def test_shared_levels_match_oracle(knobs, monkeypatch):
     """Levels above a small size are split into chunks taken by idle
@@ -749,20 +750,28 @@ def test_vertex_delete_and_insert_match_oracle():
 
 def test_checkpoint_resume_is_bit_identical(tmp_path):
     """A checkpoint taken after insertions, deletions (one rebuilding a hub's
-    lookup table) and a batched window restores trees and BC exactly, and the
-    resumed forest stays bit-identical to the original under further updates."""
+    lookup table) and a batched window restores trees and BC exactly; the
+    resumed forest stays bit-identical to the original under further updates,
+    and both equal the oracle, which runs the same operations from the start
+    without any checkpoint."""
     hg, deg = _hub_graph()
     roots = hg.pick_roots(24, 9)
     hub = int(np.argmax(deg))
     nb = hg.col[hg.rowptr[hub]:hg.rowptr[hub + 1]]
     print(f"[checkpoint] nv={hg.nv} k={len(roots)}", flush=True)
     ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
     u, v = hg.insertions(48, 12)
     for a, b in zip(u[:4], v[:4]):
         f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
-    f.delete_edge(g, ctx, hub, int(nb[2]))
-    f.delete_edge(g, ctx, int(u[0]), int(v[0]))
+        o.insert(int(a), int(b), O_FINE)
+    for a, b in ((hub, int(nb[2])), (int(u[0]), int(v[0]))):
+        f.delete_edge(g, ctx, a, b)
+        o.delete(a, b)
     f.insert_batched(g, ctx, u[4:20], v[4:20], sbc.BatchConfig(window=8, block_threads=1024))
+    for a, b in zip(u[4:20], v[4:20]):
+        o.insert(int(a), int(b), O_FINE)
     path = str(tmp_path / "state.ckpt")
     sbc.checkpoint_save(f, g, path)
     f2, g2 = sbc.checkpoint_load(path)
@@ -779,8 +788,54 @@ def test_checkpoint_resume_is_bit_identical(tmp_path):
     for x, y in zip(f.trees(), f2.trees()):
         assert np.array_equal(x, y)
     assert np.array_equal(f.bc(), f2.bc())
+    for a, b in zip(u[20:40], v[20:40]):
+        o.insert(int(a), int(b), O_FINE)
+    o.delete(hub, int(nb[5]))
+    o.insert(int(u[40]), int(v[40]), O_FINE)
+    _compare_trees(f2, o, "resumed forest vs the oracle")
+    assert_bc_close(f2.bc(), o.bc(), what="resumed BC vs the oracle")
     f2.close()
     g2.close()
+
+
+def test_batched_relayout_inside_one_window():
+    """One window appends more edges to a vertex than its append slack holds:
+    the batched call lays the graph out again up front (fresh slack for every
+    vertex, the device image uploaded again) and the results stay the
+    oracle's; later windows and plain insertions use the new layout."""
+    hg = sbc.gen_chung_lu(4096, 2.3, 8.0, seed=7, keep_lcc=True)
+    roots = hg.pick_roots(32, 3)
+    deg = hg.degree()
+    x = int(np.flatnonzero(deg == 3)[0])
+    nbx = set(hg.col[hg.rowptr[x]:hg.rowptr[x + 1]].tolist())
+    lcc = np.flatnonzero(hg.largest_component())
+    far = [int(y) for y in lcc if y != x and int(y) not in nbx][:12]
+    room = 4 + 3 // 16
+    assert len(far) > room
+    u = np.array([x] * len(far), dtype=np.int64)
+    v = np.array(far, dtype=np.int64)
+    print(f"[relayout] vertex {x} degree 3, slack {room}, {len(far)} insertions in one window", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    rec = f.insert_batched(g, ctx, u, v, sbc.BatchConfig(window=16, block_threads=1024), records=True)
+    assert g.degree(x) == 3 + len(far)
+    for i, (a, b) in enumerate(zip(u, v)):
+        ro = o.insert(int(a), int(b), O_FINE, items=True)
+        for key in ("kase", "affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec[i][key], ro[key]), f"insertion {i} {key}"
+    _compare_trees(f, o, "after the relayout window")
+    u2, v2 = hg.insertions(40, 9)
+    keep = [(a, b) for a, b in zip(u2, v2) if a != x and b != x][:24]
+    f.insert_batched(g, ctx, [a for a, _ in keep[:16]], [b for _, b in keep[:16]],
+                     sbc.BatchConfig(window=8, block_threads=1024))
+    for a, b in keep[:16]:
+        o.insert(int(a), int(b), O_FINE)
+    for a, b in keep[16:]:
+        f.insert_edge(g, ctx, int(a), int(b), sbc.FINE)
+        o.insert(int(a), int(b), O_FINE)
+    _compare_trees(f, o, "after later windows")
+    assert_bc_close(f.bc(), o.bc())
 
 
 def test_wide_rebuild_of_a_root_with_a_big_hub(monkeypatch):

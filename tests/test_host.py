@@ -130,10 +130,69 @@ def test_remove_edge_return_codes():
     g.close()
 
 
+def _ckpt_hash(data: bytes) -> int:
+    """The checkpoint's section checksum: 8-byte little-endian words (tail
+    zero-padded) xored in, times the FNV-64 prime, folded; length last."""
+    M = (1 << 64) - 1
+    h = 0x9E3779B97F4A7C15
+
+    def word(h, w):
+        h = ((h ^ w) * 0x100000001B3) & M
+        return h ^ (h >> 32)
+
+    pad = data + bytes((-len(data)) % 8)
+    for w in np.frombuffer(pad, dtype="<u8").tolist():
+        h = word(h, w)
+    return word(h, len(data))
+
+
+def _ckpt_file(dist_override=None, flip=None):
+    """A complete checkpoint of the path 1-2-3 (phantom vertex 0), one root
+    (vertex 1), written by hand in the documented layout."""
+    nv, K = 4, 1
+    deg = np.array([0, 1, 2, 1], np.int32)
+    cap = deg + 4
+    start = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int32)
+    slots = int(cap.sum())
+    col = np.zeros(slots, np.int32)
+    col[start[1]] = 2
+    col[start[2]:start[2] + 2] = [1, 3]
+    col[start[3]] = 2
+    seq = np.zeros(slots, np.int32)
+    hub = np.zeros((nv, 4), np.int32)
+    roots = np.array([1], np.int64)
+    dist = np.array([0x7FFFFFFF, 0, 1, 2], np.int32) if dist_override is None else dist_override
+    sb = np.array([0x80, 0, 1 << 2, 2 << 2], np.uint8)
+    rec = np.zeros((nv, 4), np.float64)
+    recb = bytearray(rec.tobytes())
+    sig = np.array([np.iinfo(np.int64).max, 1, 1, 1], np.int64)
+    dlt = np.array([0.0, 2.0, 1.0, 0.0])
+    for x in range(nv):
+        recb[32 * x:32 * x + 8] = sig[x].tobytes()
+        recb[32 * x + 8:32 * x + 16] = dlt[x].tobytes()
+    bc = np.array([0.0, 0.0, 1.0, 0.0])
+    hdr = np.array([2, nv, K, 0, 1, slots, 0, 1], np.int64)
+    secs = [roots.tobytes(),
+            start.tobytes() + deg.tobytes() + deg.tobytes() + cap.tobytes() + col.tobytes() + seq.tobytes()
+            + hub.tobytes(),
+            dist.tobytes() + sb.tobytes() + bytes(recb),
+            bc.tobytes()]
+    out = bytearray(b"SBCCKPT2" + hdr.tobytes())
+    for sec in secs:
+        out += sec + np.array([_ckpt_hash(sec)], "<u8").tobytes()
+    out += b"SBCEND02"
+    if flip is not None:
+        out[flip] ^= 0x10
+    return bytes(out)
+
+
 def test_checkpoint_load_rejects_bad_files(tmp_path):
-    """sbcCheckpointLoad checks the header and the exact file size before any
-    device allocation: a missing file, a wrong magic or a truncated body is
-    SBC_ERR_ARG."""
+    """sbcCheckpointLoad checks the header, the exact file size, every
+    section's checksum and the ranges of the graph and trees before any
+    device allocation: a missing file, a wrong magic, a truncated body, a
+    flipped bit anywhere after the header, or a level beyond the level
+    capacity is SBC_ERR_ARG.  A sound file passes those checks and only then
+    reaches the device (here without a GPU: any status but SBC_ERR_ARG)."""
     p = tmp_path / "state.ckpt"
     with pytest.raises(sbc.SbcError) as e:
         sbc.checkpoint_load(str(p))
@@ -142,8 +201,28 @@ def test_checkpoint_load_rejects_bad_files(tmp_path):
     with pytest.raises(sbc.SbcError) as e:
         sbc.checkpoint_load(str(p))
     assert e.value.status == sbc.SBC_ERR_ARG
-    hdr = np.array([1, 35, 35, 1, 1, 200, 0, 1], dtype=np.int64)
-    p.write_bytes(b"SBCCKPT1" + hdr.tobytes() + bytes(1000))
+    hdr = np.array([2, 35, 35, 1, 1, 200, 0, 1], dtype=np.int64)
+    p.write_bytes(b"SBCCKPT2" + hdr.tobytes() + bytes(1000))
     with pytest.raises(sbc.SbcError) as e:
         sbc.checkpoint_load(str(p))
     assert e.value.status == sbc.SBC_ERR_ARG
+    good = _ckpt_file()
+    # a flipped bit in each section (roots, graph, tree, BC) fails a checksum
+    for at in (72 + 3, 72 + 8 + 8 + 20, len(good) - 8 - 8 - 32 - 100, len(good) - 8 - 8 - 3):
+        p.write_bytes(_ckpt_file(flip=at))
+        with pytest.raises(sbc.SbcError) as e:
+            sbc.checkpoint_load(str(p))
+        assert e.value.status == sbc.SBC_ERR_ARG and "checksum" in str(e.value), at
+    # a level beyond the level capacity, with valid checksums
+    p.write_bytes(_ckpt_file(dist_override=np.array([0x7FFFFFFF, 0, 1, 9999], np.int32)))
+    with pytest.raises(sbc.SbcError) as e:
+        sbc.checkpoint_load(str(p))
+    assert e.value.status == sbc.SBC_ERR_ARG and "range" in str(e.value)
+    p.write_bytes(good)
+    try:
+        f, g = sbc.checkpoint_load(str(p))
+    except sbc.SbcError as e:
+        assert e.status != sbc.SBC_ERR_ARG, str(e)
+    else:  # a GPU is present: the file loads
+        f.close()
+        g.close()

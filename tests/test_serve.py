@@ -92,3 +92,47 @@ def test_alone_vs_mix_bit_identical():
         # stream capture: no other thread's work can invalidate them)
         built, launched = srv.resident[name].ctx.graph_counts()
         assert built >= 1 and launched == len(srv.resident[name].windows), name
+
+
+def test_submit_refuses_a_graph_above_the_whole_budget():
+    hg = sbc.gen_rmat(10, 8192, seed=1)
+    need = sbc.resident_footprint(hg, 16, 256)
+    srv = Server(budget_bytes=need - 1, num_cu=256)
+    with pytest.raises(sbc.SbcError) as e:
+        srv.submit("a", hg, np.arange(16))
+    assert e.value.status == sbc.SBC_ERR_BUDGET
+    assert srv.used == 0 and not srv.resident and not srv.pending
+
+
+@pytest.mark.gpu
+def test_queued_graph_admitted_while_serving():
+    """Three graphs, a budget that holds two: the third is queued and
+    admitted during serve() once a graph whose stream has ended is evicted;
+    every graph ends bit-identical to the same graph served alone."""
+    gs = _graphs()
+    names = list(gs)
+    fps = {n: sbc.resident_footprint(hg, len(r), 0) for n, (hg, r) in gs.items()}
+    budget = fps[names[1]] + fps[names[2]]
+    assert fps[names[0]] + fps[names[1]] <= budget < sum(fps.values())
+    counts = {names[0]: 24, names[1]: 64, names[2]: 64}
+    arr = {}
+    for i, (name, (hg, roots)) in enumerate(gs.items()):
+        u, v = hg.insertions(counts[name], 17)
+        arr[name] = (poisson_arrivals(2000.0, counts[name], 5 + i), u, v)
+    alone = {}
+    for name, (hg, roots) in gs.items():
+        srv = Server(budget_bytes=8 << 30, max_window=16)
+        srv.admit(name, hg, roots)
+        srv.serve({name: arr[name]})
+        alone[name] = srv.bc(name)
+        srv.evict(name)
+    srv = Server(budget_bytes=budget, max_window=16)
+    got = [srv.submit(name, hg, roots) for name, (hg, roots) in gs.items()]
+    assert got == [True, True, False] and len(srv.pending) == 1
+    results = {}
+    res = srv.serve(arr, evict_when_done=True, results=results)
+    print(f"[serve] queued graph admitted at {srv.admitted_at.get(names[2])} s; p99 {res['p99_ms']} ms", flush=True)
+    assert names[2] in srv.admitted_at and not srv.pending and not srv.resident and srv.used == 0
+    assert res["insertions"] == sum(counts.values())
+    for name in gs:
+        assert np.array_equal(results[name], alone[name]), name
