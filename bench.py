@@ -352,8 +352,10 @@ def main():
                     help="insertions timed in the plain form (-1: every insertion of the timed windows, 0: none)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference update time per granularity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--schedule", default="static", choices=["static", "mlp"],
-                    help="root order per window: identity, or by cost-model prediction (trained on --train-windows)")
+    ap.add_argument("--schedule", default="mlp", choices=["static", "mlp"],
+                    help="root order per window: by cost-model prediction (trained on the leading windows; with at "
+                         "most one team per compute unit every order is equivalent and only report windows are "
+                         "predicted), or identity")
     ap.add_argument("--train-windows", type=int, default=TRAIN_WINDOWS,
                     help="untimed leading windows (the cost model's training data under --schedule mlp)")
     ap.add_argument("--light-max", type=float, default=0.0,
@@ -438,7 +440,8 @@ def main():
 
     from streamingbc_amd.schedule import LearnedSchedule, StaticSchedule
     if args.schedule == "mlp":
-        sched = LearnedSchedule(train_after=max(1, n_train), light_max=args.light_max)
+        # it learns from the leading windows the autotuner left
+        sched = LearnedSchedule(train_after=max(1, n_train - pos // W), light_max=args.light_max)
     else:
         sched = StaticSchedule()
 
@@ -448,9 +451,9 @@ def main():
                             use_graph=cfg.use_graph, heavy_roots=cfg.heavy_roots if plan.heavy_roots < 0
                             else plan.heavy_roots, root_order=plan.root_order, park_min=cfg.park_min,
                             use_context_order=plan.device_order, park_scan=cfg.park_scan)
-        rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or sched.needs_records)
+        rec = f.insert_batched(g, ctx, u[a:b], v[a:b], c, records=records or plan.records)
         parked[0] += ctx.last_parked()
-        sched.observe(plan, X, rec)
+        sched.observe(plan, X, rec, ctx, K)
         return rec
 
     parked = [0]

@@ -17,7 +17,9 @@
  *   5 log1p(deg s)/10          6 log1p(deg p)/10
  *   7 log1p(sigma[p])/20       8 log1p(sigma[s])/20
  *   9 log1p(delta[s])/10       10 log1p(delta[p])/10
- *   11 root history (caller-supplied per-root value, e.g. recent mean label)/10
+ *   11 root history (a per-root value, e.g. the decayed mean label of the
+ *      root's recent items)/10: caller-supplied, or kept on the device by
+ *      streamingBCSetRootHistory / streamingBCUpdateRootHistory
  */
 #ifndef SBC_COSTMODEL_H
 #define SBC_COSTMODEL_H
@@ -77,6 +79,16 @@ sbcStatus streamingBCScheduleWindow(bcForest *forest, struct stinger *G, extraAr
                                     sbcCostModel *model, const int64_t *u, const int64_t *v, int64_t count,
                                     const float *root_hist, float light_max, float item_overhead,
                                     int32_t *heavy_out, float *pred_out);
+/* (root_hist NULL in streamingBCScheduleWindow: the context's device history
+ * below when it holds K roots, else zeros.) */
+
+/* The history feature kept on the device, so scheduling a window copies no
+ * item records: SetRootHistory uploads K starting values (e.g. those learned
+ * while collecting training data); UpdateRootHistory folds the last batched
+ * call's last window in: hist[r] = decay * hist[r] + (1 - decay) * mean over
+ * the window's items of log1p(affected). */
+sbcStatus streamingBCSetRootHistory(extraArraysPerThread **eAPT, int64_t K, const float *hist);
+sbcStatus streamingBCUpdateRootHistory(extraArraysPerThread **eAPT, int64_t K, float decay);
 
 #ifdef __cplusplus
 }

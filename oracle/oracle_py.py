@@ -61,6 +61,8 @@ def oracle_lib():
             ("orc_get_bc", None, [VP, PD]),
             ("orc_set_bc", None, [VP, PD]),
             ("orc_sigma_overflowed", C.c_int, [VP]),
+            ("orc_balance_work", None, [i64, P64, P64, i64, P64, i64, P64, P64, P64]),
+            ("orc_edge_counts", None, [i64, P64, P64, P64, P64, P64]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -86,6 +88,7 @@ def ref_lib():
             ("ref_delete", C.c_double, [VP, i64, i64, i64, i64, P64]),
             ("ref_get_tree", None, [VP, i64, P64, P64, PD]),
             ("ref_get_bc", None, [VP, PD]),
+            ("ref_get_edge_counts", None, [VP, i64, P64, P64]),
             ("ref_counters", None, [VP, P64]),
             ("ref_free", None, [VP]),
         ]:
@@ -186,6 +189,33 @@ class Oracle:
 
     def sigma_overflowed(self) -> bool:
         return bool(oracle_lib().orc_sigma_overflowed(self.f))
+
+
+def edge_counts(rowptr, col, levels):
+    """(edgesAbove, edgesBelow) of one tree from its levels (orc_edge_counts)."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int64)
+    levels = np.ascontiguousarray(levels, dtype=np.int64)
+    nv = levels.shape[0]
+    a = np.zeros(nv, dtype=np.int64)
+    b = np.zeros(nv, dtype=np.int64)
+    oracle_lib().orc_edge_counts(nv, _p64(rowptr), _p64(col), _p64(levels), _p64(a), _p64(b))
+    return a, b
+
+
+def balance_work(rowptr, col, levels, u, v):
+    """The reference's BALANCE estimate per root summed over a window
+    (orc_balance_work): levels is (K, nv) int64 with LONG_MAX unreachable."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int64)
+    levels = np.ascontiguousarray(levels, dtype=np.int64)
+    u = np.ascontiguousarray(u, dtype=np.int64)
+    v = np.ascontiguousarray(v, dtype=np.int64)
+    K, nv = levels.shape
+    out = np.zeros(K, dtype=np.int64)
+    oracle_lib().orc_balance_work(nv, _p64(rowptr), _p64(col), K, _p64(levels), u.shape[0], _p64(u), _p64(v),
+                                  _p64(out))
+    return out
 
 
 class OracleGroup:
@@ -338,6 +368,12 @@ class Reference:
         out = np.zeros(self.nv, dtype=np.float64)
         ref_lib().ref_get_bc(self.h, _pd(out))
         return out
+
+    def edge_counts(self, ri):
+        a = np.zeros(self.nv, dtype=np.int64)
+        b = np.zeros(self.nv, dtype=np.int64)
+        ref_lib().ref_get_edge_counts(self.h, ri, _p64(a), _p64(b))
+        return a, b
 
     def counters(self):
         out = np.zeros(4, dtype=np.int64)

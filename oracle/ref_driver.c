@@ -122,6 +122,17 @@ void ref_get_tree(void *h, int64_t ri, int64_t *level, int64_t *sigma, double *d
     }
 }
 
+/* edgesAbove / edgesBelow of root index ri's tree (src/streamingbc_aux.h:21-33) */
+void ref_get_edge_counts(void *h, int64_t ri, int64_t *above, int64_t *below)
+{
+    ref_ctx *c = (ref_ctx *)h;
+    bcTree *t = c->F->forest[c->roots[ri]];
+    for (int64_t v = 0; v < c->NV; v++) {
+        above[v] = t->vArr[v].edgesAbove;
+        below[v] = t->vArr[v].edgesBelow;
+    }
+}
+
 void ref_get_bc(void *h, double *bc)
 {
     ref_ctx *c = (ref_ctx *)h;

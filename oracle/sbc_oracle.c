@@ -694,3 +694,70 @@ void orc_insert(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int mod
     }
     fold_bc(f);
 }
+
+/* ------------------------------------------------------------------ BALANCE */
+
+/* The reference's BALANCE work estimate (src/streamingbc.c:98-122) summed
+ * over a window of insertions, for ordering roots in scheduling comparisons.
+ * Per root r, from r's levels (levels[r * nv + x], LONG_MAX unreachable) over
+ * the CSR graph (rowptr, col): edgesAbove[k] = sum over parents p of
+ * (edgesAbove[p] + 1) in increasing level order and edgesBelow[k] = sum over
+ * children c of (edgesBelow[c] + 1) in decreasing level order, as the static
+ * pass computes them (src/streamingbc_init.c:112-169); then work[r] = sum over
+ * the window's insertions (u, v) with different endpoint levels of
+ * 2 * edgesBelow[s] + edgesAbove[s], s the deeper endpoint. */
+void orc_edge_counts(int64_t nv, const int64_t *rowptr, const int64_t *col, const int64_t *L, int64_t *above,
+                     int64_t *below)
+{
+    int64_t maxl = 0;
+    for (int64_t x = 0; x < nv; x++)
+        if (L[x] != INF_LEVEL && L[x] > maxl) maxl = L[x];
+    int64_t *cnt = (int64_t *)calloc((size_t)(maxl + 2), sizeof(int64_t));
+    int64_t *order = (int64_t *)malloc((size_t)nv * sizeof(int64_t));
+    for (int64_t x = 0; x < nv; x++)
+        if (L[x] != INF_LEVEL) cnt[L[x] + 1]++;
+    for (int64_t l = 0; l <= maxl; l++) cnt[l + 1] += cnt[l];
+    for (int64_t x = 0; x < nv; x++) {
+        above[x] = 0;
+        below[x] = 0;
+        if (L[x] != INF_LEVEL) order[cnt[L[x]]++] = x;
+    }
+    const int64_t m = cnt[maxl];
+    for (int64_t i = 0; i < m; i++) {
+        int64_t k = order[i];
+        for (int64_t e = rowptr[k]; e < rowptr[k + 1]; e++) {
+            int64_t p = col[e];
+            if (L[p] != INF_LEVEL && L[p] == L[k] - 1) above[k] += above[p] + 1;
+        }
+    }
+    for (int64_t i = m - 1; i >= 0; i--) {
+        int64_t k = order[i];
+        for (int64_t e = rowptr[k]; e < rowptr[k + 1]; e++) {
+            int64_t c = col[e];
+            if (L[c] != INF_LEVEL && L[c] == L[k] + 1) below[k] += below[c] + 1;
+        }
+    }
+    free(order);
+    free(cnt);
+}
+
+void orc_balance_work(int64_t nv, const int64_t *rowptr, const int64_t *col, int64_t K, const int64_t *levels,
+                      int64_t n, const int64_t *u, const int64_t *v, int64_t *work)
+{
+    int64_t *above = (int64_t *)malloc((size_t)nv * sizeof(int64_t));
+    int64_t *below = (int64_t *)malloc((size_t)nv * sizeof(int64_t));
+    for (int64_t r = 0; r < K; r++) {
+        const int64_t *L = levels + (size_t)r * (size_t)nv;
+        orc_edge_counts(nv, rowptr, col, L, above, below);
+        int64_t w = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int64_t lu = L[u[i]], lv = L[v[i]];
+            if (lu == lv) continue;
+            int64_t s = (lu == INF_LEVEL || (lv != INF_LEVEL && lu > lv)) ? u[i] : v[i];
+            w += 2 * below[s] + above[s];
+        }
+        work[r] = w;
+    }
+    free(above);
+    free(below);
+}

@@ -103,6 +103,19 @@ void orc_set_bc(orc_forest *f, const double *bc);
 /* 1 if any sigma addition overflowed int64 since the forest was created. */
 int orc_sigma_overflowed(const orc_forest *f);
 
+/* BALANCE estimate of each of K roots for a window of n insertions, from the
+ * roots' levels (K x nv, LONG_MAX unreachable) over a CSR graph: the sum over
+ * the window of 2 * edgesBelow[s] + edgesAbove[s] of the deeper endpoint s
+ * (src/streamingbc.c:98-122; src/streamingbc_init.c:112-169).  Used only to
+ * order roots in scheduling comparisons. */
+void orc_balance_work(int64_t nv, const int64_t *rowptr, const int64_t *col, int64_t K, const int64_t *levels,
+                      int64_t n, const int64_t *u, const int64_t *v, int64_t *work);
+/* edgesAbove / edgesBelow of one tree given its levels L[nv], as the static
+ * pass computes them (src/streamingbc_init.c:112-169): pinned against the
+ * reference's own arrays after its static pass. */
+void orc_edge_counts(int64_t nv, const int64_t *rowptr, const int64_t *col, const int64_t *L, int64_t *above,
+                     int64_t *below);
+
 #ifdef __cplusplus
 }
 #endif

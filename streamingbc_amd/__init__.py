@@ -150,6 +150,8 @@ _sig("sbcCostModelPredictFP32", C.c_int, [VP, PF, i64, PF])
 _sig("streamingBCWindowFeatures", C.c_int, [FP, VP, VPP, P64, P64, i64, PF, PF])
 _sig("streamingBCScheduleWindow", C.c_int, [FP, VP, VPP, VP, P64, P64, i64, PF, C.c_float, C.c_float,
                                             C.POINTER(C.c_int32), PF])
+_sig("streamingBCSetRootHistory", C.c_int, [VPP, i64, PF])
+_sig("streamingBCUpdateRootHistory", C.c_int, [VPP, i64, C.c_float])
 
 
 def _check(st):
@@ -495,6 +497,13 @@ class Forest:
         _check(lib.streamingBCGetTree(self._p, ri, _p64(lv), _p64(sg), _pd(dl)))
         return lv, sg, dl
 
+    def levels(self) -> np.ndarray:
+        """(K, nv) levels of every root's tree (LONG_MAX: unreachable)."""
+        lv = np.zeros((self.k, self.nv), dtype=np.int64)
+        for ri in range(self.k):
+            _check(lib.streamingBCGetTree(self._p, ri, _p64(lv[ri]), None, None))
+        return lv
+
     def trees(self):
         lv = np.zeros((self.k, self.nv), dtype=np.int64)
         sg = np.zeros((self.k, self.nv), dtype=np.int64)
@@ -598,11 +607,24 @@ def window_features(forest: Forest, g: Graph, ctx: Context, u, v, root_hist=None
     return X
 
 
+def set_root_history(ctx: Context, hist) -> None:
+    """Uploads the per-root history feature to the context's device copy."""
+    h = _f32(hist)
+    _check(lib.streamingBCSetRootHistory(ctx._p, h.shape[0], _pf(h)))
+
+
+def update_root_history(ctx: Context, k: int, decay: float) -> None:
+    """Folds the last batched window's item records into the device history
+    (no records copied)."""
+    _check(lib.streamingBCUpdateRootHistory(ctx._p, k, decay))
+
+
 def schedule_window(forest: Forest, g: Graph, ctx: Context, model: CostModel, u, v, root_hist=None,
                     light_max: float = 0.0, item_overhead: float = 4.0, predictions: bool = False):
     """Device scheduling of the next window (streamingBCScheduleWindow):
     leaves the root order in the context for a batched call with
-    use_context_order=True.  Returns (heavy_roots, predictions (n, K) or None)."""
+    use_context_order=True.  root_hist None: the context's device history.
+    Returns (heavy_roots, predictions (n, K) or None)."""
     u = np.ascontiguousarray(u, dtype=np.int64)
     v = np.ascontiguousarray(v, dtype=np.int64)
     n = u.shape[0]

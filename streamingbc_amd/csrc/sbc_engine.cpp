@@ -657,6 +657,9 @@ struct extraArraysPerThread {
     float* d_scost = nullptr;
     int32_t* d_sheavy = nullptr;
     int64_t sched_items = 0, sched_k = 0, sched_cnt = 0;
+    float* d_hist = nullptr;     // per-root history feature kept on the device (streamingBCSetRootHistory)
+    int64_t hist_k = 0;
+    int64_t last_batch_items = 0;  // insertions of the last window of the last batched call
     bool order_ready = false;
     const void* order_forest = nullptr;  // forest the scheduled order belongs to
     int64_t order_cap = 0;
@@ -729,7 +732,7 @@ static void ctx_free(extraArraysPerThread* C)
                     (void*)C->d_mseg, (void*)C->d_mpre})
         if (q) (void)hipFree(q);
     for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
-                    (void*)C->d_sheavy})
+                    (void*)C->d_sheavy, (void*)C->d_hist})
         if (q) (void)hipFree(q);
     if (C->win.d_desc) (void)hipFree(C->win.d_desc);
     if (C->win.d_ints) (void)hipFree(C->win.d_ints);
@@ -1877,6 +1880,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     std::vector<char>& mark = G->h_mark;  // cleared again at the end of each window's loop body
     for (int64_t w0 = 0; w0 < count; w0 += W) {
         const int n = (int)((count - w0) < W ? (count - w0) : W);
+        C->last_batch_items = n;  // the device records hold this window's items
         const int32_t seq0 = G->seq_next;
         // deg before this window = always-visible prefix for this window
         touched.clear();
@@ -2144,6 +2148,35 @@ extern "C" sbcStatus streamingBCWindowFeatures(bcForest* forest, struct stinger*
     return SBC_OK;
 }
 
+extern "C" sbcStatus streamingBCSetRootHistory(extraArraysPerThread** eAPT, int64_t K, const float* hist)
+{
+    if (!eAPT || !eAPT[0] || K < 1 || !hist) return set_err(SBC_ERR_ARG, "bad argument");
+    extraArraysPerThread* C = eAPT[0];
+    if (C->hist_k != K) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (C->d_hist) HIPCHK(hipFree(C->d_hist));
+        C->d_hist = nullptr;
+        C->hist_k = 0;
+        sbcStatus st = dalloc(&C->d_hist, (size_t)K);
+        if (st) return st;
+        C->hist_k = K;
+    }
+    HIPCHK(hipMemcpyAsync(C->d_hist, hist, sizeof(float) * (size_t)K, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return SBC_OK;
+}
+
+extern "C" sbcStatus streamingBCUpdateRootHistory(extraArraysPerThread** eAPT, int64_t K, float decay)
+{
+    if (!eAPT || !eAPT[0] || K < 1) return set_err(SBC_ERR_ARG, "bad argument");
+    extraArraysPerThread* C = eAPT[0];
+    if (C->hist_k != K) return set_err(SBC_ERR_STATE, "no device root history of %ld roots", (long)K);
+    const int64_t n = C->last_batch_items;
+    if (n < 1 || !C->d_rec || C->rec_cap < n * K) return set_err(SBC_ERR_STATE, "no batched window to learn from");
+    HIPCHK(sbc_launch_hist_update(C->d_rec, (int)n, (int)K, decay, C->d_hist, C->stream));
+    return SBC_OK;
+}
+
 extern "C" sbcStatus streamingBCGraphCounts(extraArraysPerThread** eAPT, int64_t* built, int64_t* launched)
 {
     if (!eAPT || !eAPT[0]) return set_err(SBC_ERR_ARG, "null context");
@@ -2222,7 +2255,7 @@ extern "C" sbcStatus streamingBCScheduleWindow(bcForest* forest, struct stinger*
     A.vr = F->d_vr;
     A.u = C->d_suv;
     A.v = C->d_suv + count;
-    A.hist = root_hist ? C->d_shist : nullptr;
+    A.hist = root_hist ? C->d_shist : (C->hist_k == F->K ? C->d_hist : nullptr);
     A.X = C->d_sX;
     A.nv = (int32_t)F->nv;
     A.K = (int32_t)F->K;

@@ -188,6 +188,7 @@ struct sbcCostModel;
 
 extern "C" {
 hipError_t sbc_launch_features(const SbcFeatArgs* A, hipStream_t st);
+hipError_t sbc_launch_hist_update(const SbcItemRec* rec, int count, int K, float decay, float* hist, hipStream_t st);
 hipError_t sbc_cost_schedule(sbcCostModel* m, const float* dX, int n, int K, float light_max, float overhead,
                              float* d_pred, float* d_cost, int32_t* d_order, int32_t* d_heavy, hipStream_t st);
 hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc, int grid,

@@ -270,6 +270,26 @@ extern "C" hipError_t sbc_launch_features(const SbcFeatArgs* A, hipStream_t st)
     return hipGetLastError();
 }
 
+// Per-root history feature, kept on the device: hist[r] = decay * hist[r] +
+// (1 - decay) * mean over the window's items of log1p(affected), from the
+// item records of the last batched call (rec[i * K + r]).
+__global__ void sbc_hist_kernel(const SbcItemRec* __restrict__ rec, int count, int K, float decay, float* hist)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= K) return;
+    float acc = 0.0f;
+    for (int i = 0; i < count; ++i) acc += log1pf((float)rec[(size_t)i * K + r].affected);
+    hist[r] = decay * hist[r] + (1.0f - decay) * (acc / (float)count);
+}
+
+extern "C" hipError_t sbc_launch_hist_update(const SbcItemRec* rec, int count, int K, float decay, float* hist,
+                                             hipStream_t st)
+{
+    if (count <= 0 || K <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sbc_hist_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, rec, count, K, decay, hist);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- model object
 
 struct sbcCostModel {

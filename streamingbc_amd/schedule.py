@@ -21,7 +21,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import BatchConfig, CostModel, Context, Forest, Graph, schedule_window, window_features
+from . import (BatchConfig, CostModel, Context, Forest, Graph, schedule_window, set_root_history, update_root_history,
+               window_features)
 
 # Fixed per-item work (in affected-vertex units) added to each prediction:
 # even an item that changes nothing reads its endpoints and syncs its team.
@@ -34,6 +35,7 @@ class WindowPlan:
     heavy_roots: int
     predicted: np.ndarray | None = None  # (n, K) predicted log1p(affected)
     device_order: bool = False           # order left on the device by schedule_window
+    records: bool = False                # the schedule reads this window's item records
 
 
 class StaticSchedule:
@@ -45,7 +47,7 @@ class StaticSchedule:
     def plan(self, forest, g, ctx, u, v):
         return WindowPlan(None, -1), None
 
-    def observe(self, plan, X, records) -> None:
+    def observe(self, plan, X, records, ctx=None, k=None) -> None:
         pass
 
 
@@ -68,7 +70,10 @@ class LearnedSchedule:
     hist_decay: float = 0.5
     mfma: bool = True
     name: str = "mlp"
-    needs_records: bool = True  # observe() reads every window's item records (root history feature)
+    # the host path reads every window's item records (root history feature);
+    # the device path keeps that feature on the device once trained and reads
+    # records only for its training windows and the report windows
+    needs_records: bool = True
     model: CostModel | None = None
     _X: list = field(default_factory=list)
     _y: list = field(default_factory=list)
@@ -92,14 +97,18 @@ class LearnedSchedule:
         return window_features(forest, g, ctx, u, v, self._hist)
 
     def _predict_only(self, forest, g, ctx, u, v):
-        _, pred = schedule_window(forest, g, ctx, self.model, u, v, self._hist, 0.0, ITEM_OVERHEAD, predictions=True)
+        _, pred = schedule_window(forest, g, ctx, self.model, u, v, None, 0.0, ITEM_OVERHEAD, predictions=True)
         return pred
+
+    def _device(self, forest) -> bool:
+        return self.trained and self.device and self.mfma and forest.k <= 4096
 
     def plan(self, forest: Forest, g: Graph, ctx: Context, u, v):
         """Returns (WindowPlan, features of the window)."""
         if self._hist is None:
             self._hist = np.zeros(forest.k, dtype=np.float32)
-        if self.trained and self.device and self.mfma and forest.k <= 4096:
+        self._ctx = ctx
+        if self._device(forest):
             self._sched_windows = getattr(self, "_sched_windows", 0) + 1
             want = (self._sched_windows - 1) % self.eval_every == 0
             if self.light_max <= 0 and forest.k <= ctx.num_cu:
@@ -108,21 +117,32 @@ class LearnedSchedule:
                 # time, so only report windows are predicted
                 if not want:
                     return WindowPlan(None, -1, None), None
-                return WindowPlan(None, -1, self._predict_only(forest, g, ctx, u, v)), None
-            heavy, pred = schedule_window(forest, g, ctx, self.model, u, v, self._hist, self.light_max,
+                return WindowPlan(None, -1, self._predict_only(forest, g, ctx, u, v), records=True), None
+            heavy, pred = schedule_window(forest, g, ctx, self.model, u, v, None, self.light_max,
                                           ITEM_OVERHEAD, predictions=want)
-            return WindowPlan(None, heavy, pred, device_order=True), None
+            return WindowPlan(None, heavy, pred, device_order=True, records=want), None
         X = self._features(forest, g, ctx, u, v)
         if not self.trained:
-            return WindowPlan(None, -1, None), X
+            return WindowPlan(None, -1, None, records=True), X
         pred = self.model.predict(X, mfma=self.mfma)  # (n, K)
         work = np.expm1(np.maximum(pred, 0.0)).sum(axis=0) + ITEM_OVERHEAD * pred.shape[0]
         order = np.argsort(-work, kind="stable").astype(np.int32)
         heavy = int(np.count_nonzero(work >= self.light_max)) if self.light_max > 0 else -1
-        return WindowPlan(order, heavy, pred), X
+        return WindowPlan(order, heavy, pred, records=True), X
 
-    def observe(self, plan: WindowPlan, X: np.ndarray, records: np.ndarray) -> None:
-        """records: (n, K) item records of the window just run."""
+    def observe(self, plan: WindowPlan, X: np.ndarray, records: np.ndarray | None, ctx=None, k=None) -> None:
+        """records: (n, K) item records of the window just run (may be None
+        on the device path when the plan did not ask for them)."""
+        if self.trained and getattr(self, "_hist_on_device", False):
+            # the history feature lives on the device: fold the window in there
+            update_root_history(ctx if ctx is not None else self._ctx, k if k is not None else self._hist.shape[0],
+                                self.hist_decay)
+            if records is not None and plan.predicted is not None:
+                y = np.log1p(records["affected"].astype(np.float64)).astype(np.float32)
+                self.eval_pred.append(plan.predicted.copy())
+                self.eval_true.append(y.copy())
+            self._windows += 1
+            return
         y = np.log1p(records["affected"].astype(np.float64)).astype(np.float32)
         if plan.predicted is not None:
             self.eval_pred.append(plan.predicted.copy())
@@ -135,6 +155,9 @@ class LearnedSchedule:
         self._windows += 1
         if not self.trained and self._windows >= self.train_after:
             self.fit()
+            if self.device and self.mfma and self._hist.shape[0] <= 4096 and getattr(self, "_ctx", None) is not None:
+                set_root_history(self._ctx, self._hist)
+                self._hist_on_device = True
 
     def fit(self) -> float:
         X = np.concatenate(self._X)
@@ -200,5 +223,5 @@ def run_stream(forest: Forest, g: Graph, ctx: Context, u, v, window: int, schedu
         rec = forest.insert_batched(g, ctx, uu, vv, cfg, records=True)
         times.append(time.perf_counter() - t0)
         recs.append(rec)
-        schedule.observe(plan, X, rec)
+        schedule.observe(plan, X, rec, ctx, forest.k)
     return np.array(times), np.concatenate(recs) if recs else None

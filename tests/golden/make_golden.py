@@ -21,6 +21,9 @@ Fixtures (inputs and the reference's outputs, nothing else):
                     COARSE and FINE at NT=1, NO_BALANCE: BC after the static pass
                     and after every operation, the (sameLevel, adjacent,
                     movement) counts each deletion returns, final level/sigma/delta
+  refedges_<graph>.npz  karate / dolphins, exact: the reference's edgesAbove and
+                    edgesBelow of every root's tree after its static pass (the
+                    inputs of its BALANCE estimate), with the levels they hang on
   refdel_rmat10.npz R-MAT scale 10 as above, exact, 64 deletions (seed 9), COARSE:
                     BC after every 8th deletion, the counts, SHA-256 of (level,
                     sigma) of all roots after every deletion, final level/sigma
@@ -187,9 +190,25 @@ def main():
     print(f"ref_rmat10.npz: nv={hg.nv} insertions={len(u)}")
 
 
+def main_edges():
+    for name in ("karate", "dolphins"):
+        hg = sbc.read_dimacs(os.path.join(HERE, "graphs", f"{name}.graph"))
+        roots = np.arange(hg.nv)
+        ref = Reference(hg.rowptr, hg.col, roots, nt=1)
+        ref.static()
+        lv, _, _ = ref.trees()
+        ab = [ref.edge_counts(ri) for ri in range(hg.nv)]
+        ref.close()
+        np.savez_compressed(os.path.join(HERE, f"refedges_{name}.npz"), level=lv,
+                            above=np.array([a for a, _ in ab]), below=np.array([b for _, b in ab]))
+        print(f"refedges_{name}.npz: nv={hg.nv}")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["delete"]:
         main_delete()
+    elif sys.argv[1:] == ["edges"]:
+        main_edges()
     else:
         main()
         main_delete()

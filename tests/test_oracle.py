@@ -410,3 +410,28 @@ def test_planted_delete_error_is_caught():
     for x, y in zip(bad, saved):
         x[ri] = y[ri]
     assert not (np.array_equal(bad[0], lv) and np.array_equal(bad[1], sg) and np.array_equal(bad[2], dl))
+
+
+@pytest.mark.parametrize("name", ["karate", "dolphins"])
+def test_edge_counts_match_reference_golden(name):
+    """orc_edge_counts (the inputs of the BALANCE estimate used as a scheduling
+    comparator) equals the reference's edgesAbove / edgesBelow of every root's
+    tree after its static pass (fixtures written by the reference)."""
+    from oracle_py import edge_counts
+    fx = np.load(os.path.join(GOLDEN, f"refedges_{name}.npz"))
+    hg = sbc.read_dimacs(os.path.join(GOLDEN, "graphs", f"{name}.graph"))
+    for ri in range(hg.nv):
+        a, b = edge_counts(hg.rowptr, hg.col, fx["level"][ri])
+        assert np.array_equal(a, fx["above"][ri]) and np.array_equal(b, fx["below"][ri]), ri
+
+
+def test_balance_work_closed_form():
+    """On the path 1-2-3-4 (phantom 0) rooted at 1, edgesAbove[k] = k - 1 and
+    edgesBelow[k] = 4 - k; the window (1,3),(2,4) has the deeper endpoints 3
+    and 4, so the estimate is (2*1 + 2) + (2*0 + 3) = 7; same-level
+    endpoints add nothing."""
+    from oracle_py import balance_work
+    hg = sbc.graph_from_edges(5, np.array([1, 2, 3]), np.array([2, 3, 4]))
+    lv = np.array([[np.iinfo(np.int64).max, 0, 1, 2, 3]], dtype=np.int64)
+    w = balance_work(hg.rowptr, hg.col, lv, [1, 2, 3], [3, 4, 3])
+    assert w.tolist() == [7]
