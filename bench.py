@@ -441,7 +441,7 @@ def main():
     from streamingbc_amd.schedule import LearnedSchedule, StaticSchedule
     if args.schedule == "mlp":
         # it learns from the leading windows the autotuner left
-        sched = LearnedSchedule(train_after=max(1, n_train - pos // W), light_max=args.light_max)
+        sched = LearnedSchedule(train_after=max(1, n_train - pos // W), light_max=args.light_max, eval_every=8)
     else:
         sched = StaticSchedule()
 
