@@ -284,12 +284,19 @@ def traffic_from(d, workload, bench_json, commands):
     tb = np.array([(fk[i] * ff + wk[i] * wf) * 1024.0 for i in range(t0, min(t1, len(fk)))])
     tus = np.array(ns[t0:t1]) / 1e3
     md = [f"# sbc_window_kernel on {WORKLOADS[workload][4]} (MI355X, 1 GPU)", "",
-          "Every command runs from the repository root after `cd /tmp && export TMPDIR=/tmp` (as a",
-          "shell line: `cd /tmp && export TMPDIR=/tmp && cd -`).", "",
+          "Commands run from the repository root with TMPDIR=/tmp and SBC_AUTOTUNE_CACHE pointing at one",
+          "file, so every run uses the launch parameters the first run's autotuner chose.  The runs use the",
+          "identity root order (`--schedule static`): with K <= compute units every order runs the same",
+          "kernel work, and the default learned order only adds report predictions.", "",
           "## Kernel trace", "", "    " + commands[0], "",
           "| kernel | dispatches | total ms | avg us |", "|---|---|---|---|"]
     for k, (c, t) in top:
         md.append(f"| {k} | {c} | {t / 1e6:.2f} | {t / 1e3 / c:,.1f} |")
+    main_us = np.array(ns) / 1e3
+    md.append(f"| sbc_window_kernel<16, false>, window launches of the main forest | {len(ns)} | "
+              f"{main_us.sum() / 1e3:.2f} | {main_us.mean():,.1f} |")
+    md += ["", "(The other window-kernel dispatches are the plain form's insertions, its replay of the leading",
+           "windows and the launches that resume roots after a hand-over.)"]
     md += ["",
            f"Window-kernel main launches of the timed windows {t0}..{t1 - 1} (the bench's own timed range in "
            f"the same command): rocprof average {tus.mean():,.1f} us; the bench's HIP events around each "

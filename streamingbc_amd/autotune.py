@@ -12,10 +12,12 @@ never the results (trees and BC are bit-identical under every setting):
    finished by the whole-GPU level kernels.
 
 Within a stage the candidates take the stream's leading windows in turn
-(round-robin, ``reps`` windows each); each window's device time is the
-window-kernel launch plus the hand-over kernels it caused, and the candidate
-with the lowest median keeps its value for the next stage.  Tuning windows
-are real work: nothing is thrown away or replayed.
+(round-robin, ``reps`` windows each; the threshold stage takes one more);
+each window's device time is the window-kernel launch plus the hand-over
+kernels it caused.  Windows differ in their work, so a candidate replaces the
+stage's first one (the current setting) only if its median is lower by more
+than ``margin``; otherwise the current setting stays.  Tuning windows are
+real work: nothing is thrown away or replayed.
 
 Results are cached in a JSON file keyed by (vertices, edges, roots, window).
 """
@@ -77,11 +79,11 @@ def _store(path, key, entry):
 
 def tuning_windows(reps: int = 2) -> int:
     """Windows a full search consumes at most."""
-    return reps * (len(BLOCKS) + 2 + len(HANDOVER))
+    return reps * (len(BLOCKS) + 2) + (reps + 1) * len(HANDOVER)
 
 
 def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int, reps: int = 2,
-             cache: str | None = None, base: BatchConfig | None = None, candidates=None):
+             cache: str | None = None, base: BatchConfig | None = None, candidates=None, margin: float = 0.07):
     """Tunes on the leading windows of (u, v), applying them.  Returns
     (TuneResult, number of insertions consumed)."""
     key = cache_key(forest.nv, ne, forest.k, window)
@@ -98,10 +100,10 @@ def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int,
     pos = 0
     used = 0
 
-    def run_stage(stage, cands):
+    def run_stage(stage, cands, nrep=reps):
         nonlocal pos, used
         times = [[] for _ in cands]
-        for rep in range(reps):
+        for rep in range(nrep):
             for ci, cand in enumerate(cands):
                 if pos + window > u.shape[0]:
                     break
@@ -124,12 +126,14 @@ def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int,
                 pos += window
                 used += 1
         best, best_ms = None, None
-        for cand, t in zip(cands, times):
+        for ci, (cand, t) in enumerate(zip(cands, times)):
             if not t:
                 continue
             med = float(np.median(t))
             table.append(dict(cand, stage=stage, ms_per_window=round(med, 4), windows=len(t)))
-            if best_ms is None or med < best_ms:
+            if best is None:
+                best, best_ms = cand, med          # the current setting
+            elif med < best_ms * (1.0 - margin):
                 best, best_ms = cand, med
         if best is not None:
             cur.update(best)
@@ -140,7 +144,7 @@ def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int,
         run_stage("team size", [{"block_threads": b} for b in BLOCKS])
         if cur["block_threads"] > 64:
             run_stage("heavy/light split", [{"heavy_roots": -1}, {"heavy_roots": K - K // 4}])
-            run_stage("hand-over thresholds", [{"park_min": a, "park_scan": b} for a, b in HANDOVER])
+            run_stage("hand-over thresholds", [{"park_min": a, "park_scan": b} for a, b in HANDOVER], reps + 1)
     res = TuneResult(cur["block_threads"], cur["grid_blocks"], False, used, table, cur["heavy_roots"],
                      cur["park_min"], cur["park_scan"])
     _store(cache, key, {k: getattr(res, k) for k in ("block_threads", "grid_blocks", "heavy_roots", "park_min",
