@@ -49,7 +49,7 @@ struct SbcBucketEnt {
     double sub;
 };
 
-#define SBC_TRACE_SLOTS 72
+#define SBC_TRACE_SLOTS 80
 #define SBC_HUBQ_CAP 8192   // hubs per workgroup and pass beyond the LDS queue (global overflow queue)
 
 // Path count and dependency of one vertex in one root's tree, with the
@@ -102,7 +102,9 @@ struct SbcParams {
                           // 7) its time; team-sorted buckets: staging, prefix, placing and summing time, terms,
                           // buckets; per ascent level 0..7: lookup time, scan time, scanned hubs, candidates;
                           // level-2 light passes: vertices, batched vertices and edges, leftover vertices and
-                          // edges, rounds, light time, hub-pass time
+                          // edges, rounds, light time, hub-pass time; lookup pass sub-steps: setup, source
+                          // rounds (loads, scans, staging), probes, hubs' inserted edges, 8-term sorts,
+                          // wavefront sorts
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)

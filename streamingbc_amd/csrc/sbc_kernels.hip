@@ -1225,6 +1225,8 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     const int tid = threadIdx.x, lane = lane_id(), twave = tid >> 6;
     constexpr int NT = TW * 64;
     SbcBucketEnt* bk = P.bucket + (size_t)blockIdx.x * SBC_BUCKET_SLOTS;
+    int64_t tq = 0;  // trace: sub-step clock
+    if (tp && tid == 0) tq = (int64_t)wall_clock64();
     for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
     for (int i = tid; i < np; i += NT) hs.bcnt[i] = 0;
     if (tid == 0) {
@@ -1243,6 +1245,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         hs.phv[h] = hi;
     }
     __syncthreads();
+    if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[72] += x - tq; tq = x; }
     const int nb = src.below.size(), ndl = src.dB - src.dA;
     const int nsrc = src.size();
     for (int r0 = 0; r0 < nsrc; r0 += NT) {
@@ -1284,6 +1287,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             ls.eflag[toff] = (i >= nb + ndl) ? 1 : 0;
         }
         __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[73] += x - tq; tq = x; }
         // edges of scan-mode sources
         for (int e = tid; e < ne; e += NT) {
             int lo = 0, hi = nsc;
@@ -1342,6 +1346,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             }
         }
         __syncthreads();
+        if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[74] += x - tq; tq = x; }
     }
     // inserted edges of the hubs (a few each), one thread per hub
     for (int hi = tid; hi < np; hi += NT) {
@@ -1360,6 +1365,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     }
     if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
     __syncthreads();
+    if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[75] += x - tq; tq = x; }
     if (tp && tid == 0) tp[12] -= (int64_t)wall_clock64();  // trace: sort-and-sum time
     // buckets of at most 8 terms: eight hubs per wavefront, an 8-lane group
     // each (bitonic sort of the group's keys, the group leader sums in key
@@ -1417,6 +1423,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         }
     }
     __syncthreads();
+    if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[76] += x - tq; tq = x; }
     // one wavefront per larger bucket (kinds 1 and 4)
     for (int h0 = 0; h0 < np; h0 += 64) {
         const int hl = h0 + lane;
@@ -1460,6 +1467,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         }
     }
     __syncthreads();
+    if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[77] += x - tq; tq = x; }
     // buckets of more than SBC_BUCKET terms (kind 3), one hub at a time by
     // the whole team: the entries' positions are set in an LDS bitmap over
     // the hub's list, each entry's rank is the number of bits below its own
