@@ -937,6 +937,18 @@ static sbcStatus check_kernel_err(sbcForestDev* F, extraArraysPerThread* C)
 
 // ------------------------------------------------------------ static pass
 
+// The static pass over P's roots (all, or the list in P.sroots), one
+// workgroup per root: the LDS-bitmap form when the vertex set fits its
+// bitmap, else the per-vertex form.  SBC_STATIC_FORM=vertex forces the
+// per-vertex form (both give the same trees).
+static hipError_t launch_static(const SbcParams& P, int grid, hipStream_t st)
+{
+    const char* f = getenv("SBC_STATIC_FORM");
+    const bool vertex = f && strcmp(f, "vertex") == 0;
+    if (!vertex && P.nv <= sbc_static_bm_max_nv()) return sbc_launch_static_bm(&P, grid, st);
+    return sbc_launch_static(&P, 16, grid, st);
+}
+
 static sbcStatus run_static(bcForest* forest, stinger* G, extraArraysPerThread* C)
 {
     if (!forest || !G || !C) return set_err(SBC_ERR_ARG, "null argument");
@@ -949,7 +961,7 @@ static sbcStatus run_static(bcForest* forest, stinger* G, extraArraysPerThread* 
     hipEvent_t e0;
     if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
     int grid = (int)(F->K < 4LL * C->num_cu ? F->K : 4LL * C->num_cu);
-    HIPCHK(sbc_launch_static(&P, 16, grid, C->stream));
+    HIPCHK(launch_static(P, grid, C->stream));
     if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
     if ((st = ev_begin(C, KSTAT_BC, &e0))) return st;
     HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
@@ -1356,7 +1368,7 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
         if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
         int grid = (int)(h[3] < 4LL * C->num_cu ? h[3] : 4LL * C->num_cu);
-        HIPCHK(sbc_launch_static(&Q, 16, grid, C->stream));
+        HIPCHK(launch_static(Q, grid, C->stream));
         if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
     }
     if (h[3] > 0) HIPCHK(sbc_launch_bc_roots(&P, mlist, C->d_del + 3, 1.0, C->stream));

@@ -201,6 +201,10 @@ hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, i
                                   int grid, hipStream_t st);
 hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);
 hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t st);
+// The static pass with an LDS bitmap of each root's settled levels (same
+// trees); only for graphs of at most sbc_static_bm_max_nv() vertices.
+hipError_t sbc_launch_static_bm(const SbcParams* P, int grid, hipStream_t st);
+int64_t sbc_static_bm_max_nv();
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
 // BC[v] += sign * sum over root indices list[0, *cnt) of delta_r[v] (r != v).
 hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* list, const int32_t* cnt, double sign,

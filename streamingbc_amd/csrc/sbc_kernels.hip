@@ -2596,6 +2596,424 @@ __global__ __launch_bounds__(NWB * 64) void sbc_static_kernel(SbcParams P)
     }
 }
 
+// ------------------------------------------------------------ static pass, LDS level bitmap
+//
+// The same tree as sbc_static_kernel, for graphs whose vertex set fits an LDS
+// bitmap (one bit per vertex).  Two changes of execution, none of arithmetic:
+// - the bitmap holds "level <= L" for the level L being expanded (BFS) or
+//   ascended; in an undirected graph a neighbour of a level-L vertex sits on
+//   level L-1, L or L+1, so a clear bit means "next level" (BFS: the CAS on
+//   dist and the added paths) or "child" (ascent), and the five neighbours in
+//   six that are neither never touch the per-root arrays in HBM;
+// - a wavefront takes 64 vertices of the level at a time, one per lane, and
+//   spreads their adjacency entries over its lanes (64 entries a chunk, the
+//   owner of each entry found by a binary search over the lanes' degree
+//   prefix), so low-degree vertices do not leave lanes idle and 64 vertices'
+//   loads are in flight per wavefront instead of one.
+// Ascent: a chunk's child terms are packed in lane order into the
+// wavefront's LDS row; each owner lane adds its own segment's terms to its
+// running sum in order, so every vertex's delta is the sequential sum over
+// its children in adjacency order -- the same bits as sbc_static_kernel.
+// The order of the frontier lists differs (they are scratch).
+
+// 64-entry chunks whose loads one wavefront issues together
+#ifndef SBC_SBM_UNROLL
+#define SBC_SBM_UNROLL 4
+#endif
+constexpr int SBM_UNROLL = SBC_SBM_UNROLL;
+
+struct SbmBatch {
+    int32_t v;      // this lane's vertex (0 when past the level's end)
+    int32_t base;   // adjacency start of the vertex minus its entries' first index in the batch
+    int32_t incl;   // inclusive prefix of the batch's degrees up to this lane
+    int32_t excl;   // exclusive prefix
+    int64_t sigma;  // the vertex's path count
+};
+
+__device__ __forceinline__ SbmBatch sbm_batch(const SbcParams& P, const int32_t* list, const SbcVRec* vr, int idx,
+                                              int end)
+{
+    SbmBatch b;
+    const bool have = idx < end;
+    b.v = have ? list[idx] : 0;
+    int deg = 0, start = 0;
+    b.sigma = 0;
+    if (have) {
+        const SbcVInfo vi = P.vinfo[b.v];
+        deg = vi.deg; // the static pass sees the whole graph
+        start = vi.start;
+        b.sigma = vr[b.v].sigma;
+    }
+    const int lane = lane_id();
+    int incl = deg;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    b.incl = incl;
+    b.excl = incl - deg;
+    b.base = start - b.excl;
+    return b;
+}
+
+// Lane index of the owner of batch entry e (0 <= e < total): the number of
+// lanes whose inclusive prefix is at most e.
+__device__ __forceinline__ int sbm_owner(int incl, int e)
+{
+    int c = 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        if (__shfl(incl, c + s - 1) <= e) c += s;
+    }
+    return c;
+}
+
+__device__ __forceinline__ bool sbm_test(const uint32_t* bm, int32_t k) { return (bm[k >> 5] >> (k & 31)) & 1u; }
+
+// this lane's partial sum added to *ctr once per wavefront
+__device__ __forceinline__ void sbm_wave_add(unsigned long long* ctr, unsigned long long x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d);
+    if (lane_id() == 0 && x) atomicAdd(ctr, x);
+}
+
+// A level is pulled (below) when its unvisited vertices' neighbour entries
+// number fewer than this many times its own.
+#ifndef SBC_SBM_PULL_FACTOR
+#define SBC_SBM_PULL_FACTOR 8
+#endif
+
+union SbmRow {
+    double d[64];             // ascent: a chunk's child terms in lane order
+    unsigned long long s[64]; // pulled BFS level: each lane's vertex's added paths
+};
+
+template <int NWB, int BMW>
+__global__ __launch_bounds__(NWB * 64) void sbc_static_bm_kernel(SbcParams P)
+{
+    __shared__ uint32_t bm[BMW];
+    __shared__ SbmRow row[NWB];
+    __shared__ unsigned long long found[NWB];
+    // 0: list end, 1: next batch of the level, 2: next block of vertex ids,
+    // 3: popped queue index
+    __shared__ int cnt[4];
+    // total degree of the level being settled (slot (j+1)&1) and of the one
+    // being expanded (slot j&1); the graph's adjacency entries
+    __shared__ unsigned long long dsum[2], dall;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const bool leader = threadIdx.x == 0;
+    const int nv = P.nv, nwords = (nv + 31) >> 5;
+    const int nq = P.sroots ? *P.scnt : P.K;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (leader) dall = 0ull;
+    __syncthreads();
+    {
+        unsigned long long d = 0;
+        for (int x = threadIdx.x; x < nv; x += NWB * 64) d += (unsigned long long)P.vinfo[x].deg;
+        sbm_wave_add(&dall, d);
+    }
+    for (;;) {
+        if (leader) cnt[3] = atomicAdd(&P.qctr[0], 1);
+        __syncthreads();
+        const int qi = cnt[3];
+        const unsigned long long all_deg = dall;
+        __syncthreads();
+        if (qi >= nq) break;
+        const int ri = P.sroots ? P.sroots[qi] : qi;
+        const size_t off = (size_t)ri * (size_t)nv;
+        int32_t* dist = P.dist + off;
+        uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
+        SbcVRec* vr = P.vr + off;
+        int32_t* list = P.list + off;
+        int32_t* lvl = P.lvl + (size_t)ri * 2 * P.lcap;
+        const int32_t r = P.roots[ri];
+        for (int x = threadIdx.x; x < nv; x += NWB * 64) {
+            st_rel32(&dist[x], INF32);
+            sb[x] = SB_INF;
+            vr[x].sigma = SIGMA_INF;
+            vr[x].delta = 0.0;
+        }
+        for (int w = threadIdx.x; w < nwords; w += NWB * 64) bm[w] = 0u;
+        __syncthreads();
+        if (leader) {
+            st_rel32(&dist[r], 0);
+            sb[r] = sb_make(0, 0);
+            vr[r].sigma = 1;
+            list[0] = r;
+            lvl[0] = 0;
+            lvl[1] = 1;
+            bm[r >> 5] |= 1u << (r & 31);
+            cnt[0] = 1;
+            cnt[1] = 0;
+            cnt[2] = 0;
+            dsum[0] = (unsigned long long)P.vinfo[r].deg;
+        }
+        __syncthreads();
+        // BFS: level j = list[fs, fe); bm = levels <= j.  Three ways to find
+        // level j+1, the same levels and path counts from each:
+        // - pull, when the unvisited vertices have fewer neighbour entries
+        //   than level j: every unvisited vertex (bit clear) scans its list;
+        //   a neighbour with its bit set is on level j (one on a lower level
+        //   would put the vertex on level <= j), so the vertex is on level
+        //   j+1 iff it has one, and its path count is the sum of theirs;
+        // - push with stores, when level j has at least nv/8 neighbour
+        //   entries: plain stores of the new level and path adds whose
+        //   results nobody waits for, then one pass over dist in vertex order
+        //   finds and settles the new level;
+        // - push with CAS: new vertices are claimed by CAS and appended as
+        //   they are found.
+        int fs = 0, fe = 1, j = 0;
+        bool overflow = false;
+        unsigned long long rem = all_deg; // neighbour entries of unvisited vertices
+        while (fs < fe) {
+            const int32_t next = j + 1;
+            const unsigned long long fdeg = dsum[j & 1];
+            rem -= fdeg;
+            const bool pull = rem < (unsigned long long)SBC_SBM_PULL_FACTOR * fdeg;
+            const bool scan = !pull && fdeg * 8ull >= (unsigned long long)nv;
+            if (leader) dsum[(j + 1) & 1] = 0ull;
+            unsigned long long deg = 0; // this thread's share of level j+1's degree
+            if (pull) {
+                for (;;) {
+                    int x0 = 0;
+                    if (lane == 0) x0 = atomicAdd(&cnt[2], 64);
+                    x0 = __shfl(x0, 0);
+                    if (x0 >= nv) break;
+                    const int32_t x = x0 + lane;
+                    const bool have = x < nv && !sbm_test(bm, x);
+                    int vdeg = 0, start = 0;
+                    if (have) {
+                        const SbcVInfo vi = P.vinfo[x];
+                        vdeg = vi.deg;
+                        start = vi.start;
+                    }
+                    int incl = vdeg;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const int t = __shfl_up(incl, d);
+                        if (lane >= d) incl += t;
+                    }
+                    const int32_t base = start - (incl - vdeg);
+                    const int total = __shfl(incl, 63);
+                    if (total == 0) continue;
+                    row[wave].s[lane] = 0ull;
+                    if (lane == 0) found[wave] = 0ull;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (int e0 = 0; e0 < total; e0 += 64 * SBM_UNROLL) {
+                        int32_t k[SBM_UNROLL];
+                        int o[SBM_UNROLL];
+                        bool t[SBM_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < SBM_UNROLL; ++u) {
+                            const int e = e0 + u * 64 + lane;
+                            t[u] = e < total;
+                            o[u] = sbm_owner(incl, t[u] ? e : total - 1);
+                            const int32_t b = __shfl(base, o[u]);
+                            k[u] = t[u] ? P.col[b + e] : 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < SBM_UNROLL; ++u) {
+                            if (t[u] && sbm_test(bm, k[u])) {
+                                atomicAdd(&row[wave].s[o[u]], (unsigned long long)vr[k[u]].sigma);
+                                atomicOr(&found[wave], 1ull << o[u]);
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const bool f = (found[wave] >> lane) & 1ull;
+                    if (f) {
+                        st_rel32(&dist[x], next);
+                        vr[x].sigma = (int64_t)row[wave].s[lane];
+                        sb[x] = sb_make(j + 1, 0);
+                        deg += (unsigned long long)vdeg;
+                    }
+                    const int pos = wave_append(f, &cnt[0]);
+                    if (pos >= 0) list[pos] = x;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            } else {
+                for (;;) {
+                    int b0 = 0;
+                    if (lane == 0) b0 = atomicAdd(&cnt[1], 64);
+                    b0 = __shfl(b0, 0);
+                    if (b0 >= fe) break;
+                    const SbmBatch B = sbm_batch(P, list, vr, b0 + lane, fe);
+                    const int total = __shfl(B.incl, 63);
+                    for (int e0 = 0; e0 < total; e0 += 64 * SBM_UNROLL) {
+                        int32_t k[SBM_UNROLL];
+                        int64_t sg[SBM_UNROLL];
+                        bool t[SBM_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < SBM_UNROLL; ++u) {
+                            const int e = e0 + u * 64 + lane;
+                            t[u] = e < total;
+                            // every lane takes part in the shuffles
+                            const int o = sbm_owner(B.incl, t[u] ? e : total - 1);
+                            const int32_t b = __shfl(B.base, o);
+                            sg[u] = __shfl(B.sigma, o);
+                            k[u] = t[u] ? P.col[b + e] : 0;
+                        }
+                        if (scan) {
+#pragma unroll
+                            for (int u = 0; u < SBM_UNROLL; ++u) {
+                                if (t[u] && !sbm_test(bm, k[u])) {
+                                    // level j+1 or unseen: every writer stores the same level
+                                    st_rel32(&dist[k[u]], next);
+                                    add64_wg(&vr[k[u]].dpath, sg[u]);
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < SBM_UNROLL; ++u) {
+                                bool app = false;
+                                if (t[u] && !sbm_test(bm, k[u])) {
+                                    // level j+1 or unseen: the CAS and the added paths
+                                    app = cas32_wg(&dist[k[u]], INF32, next) == INF32;
+                                    add64_wg(&vr[k[u]].dpath, sg[u]);
+                                }
+                                const int pos = wave_append(app, &cnt[0]);
+                                if (pos >= 0) list[pos] = k[u];
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (scan) {
+                // the new level in vertex order, settled as it is found
+                for (int x0 = wave * 64; x0 < nv; x0 += NWB * 64) {
+                    const int32_t x = x0 + lane;
+                    const bool m = x < nv && ld_acq32(&dist[x]) == next;
+                    if (m) {
+                        vr[x].sigma = ld_acq64(&vr[x].dpath);
+                        sb[x] = sb_make(j + 1, 0);
+                        st_rel64(&vr[x].dpath, 0);
+                        atomicOr(&bm[x >> 5], 1u << (x & 31));
+                        deg += (unsigned long long)P.vinfo[x].deg;
+                    }
+                    const int pos = wave_append(m, &cnt[0]);
+                    if (pos >= 0) list[pos] = x;
+                }
+                __syncthreads();
+            }
+            const int nfe = cnt[0];
+            if (pull) {
+                for (int x = fe + threadIdx.x; x < nfe; x += NWB * 64) {
+                    const int32_t k = list[x];
+                    atomicOr(&bm[k >> 5], 1u << (k & 31));
+                }
+            } else if (!scan) {
+                for (int x = fe + threadIdx.x; x < nfe; x += NWB * 64) {
+                    const int32_t k = list[x];
+                    vr[k].sigma = ld_acq64(&vr[k].dpath);
+                    sb[k] = sb_make(j + 1, 0);
+                    st_rel64(&vr[k].dpath, 0);
+                    atomicOr(&bm[k >> 5], 1u << (k & 31));
+                    deg += (unsigned long long)P.vinfo[k].deg;
+                }
+            }
+            sbm_wave_add(&dsum[(j + 1) & 1], deg);
+            j++;
+            if (j + 1 >= P.lcap) { overflow = true; break; }
+            if (leader) {
+                lvl[j + 1] = nfe;
+                cnt[1] = fe;
+                cnt[2] = 0;
+            }
+            fs = fe;
+            fe = nfe;
+            __syncthreads();
+        }
+        if (overflow) {
+            if (leader) atomicOr(P.err, SBC_KERR_LEVELS);
+            continue;
+        }
+        // levels 0..j-1, level L = list[lvl[L] .. lvl[L+1]); bm = every level
+        for (int L = j - 1; L >= 0; --L) {
+            if (L + 1 < j) {
+                // bm = levels <= L: take level L+1 out
+                const int a1 = lvl[L + 1], b1 = lvl[L + 2];
+                for (int x = a1 + threadIdx.x; x < b1; x += NWB * 64) {
+                    const int32_t k = list[x];
+                    atomicAnd(&bm[k >> 5], ~(1u << (k & 31)));
+                }
+            }
+            const int a = lvl[L], b = lvl[L + 1];
+            if (leader) cnt[1] = a;
+            __syncthreads();
+            for (;;) {
+                int b0 = 0;
+                if (lane == 0) b0 = atomicAdd(&cnt[1], 64);
+                b0 = __shfl(b0, 0);
+                if (b0 >= b) break;
+                const SbmBatch B = sbm_batch(P, list, vr, b0 + lane, b);
+                const int total = __shfl(B.incl, 63);
+                double acc = 0.0; // this lane's vertex
+                for (int e0 = 0; e0 < total; e0 += 64 * SBM_UNROLL) {
+                    int32_t k[SBM_UNROLL];
+                    int64_t sk[SBM_UNROLL];
+                    bool c[SBM_UNROLL];
+#pragma unroll
+                    for (int u = 0; u < SBM_UNROLL; ++u) {
+                        const int e = e0 + u * 64 + lane;
+                        c[u] = e < total;
+                        const int o = sbm_owner(B.incl, c[u] ? e : total - 1);
+                        const int32_t base = __shfl(B.base, o);
+                        sk[u] = __shfl(B.sigma, o);
+                        k[u] = c[u] ? P.col[base + e] : 0;
+                    }
+                    int64_t sc[SBM_UNROLL];
+                    double dc[SBM_UNROLL];
+#pragma unroll
+                    for (int u = 0; u < SBM_UNROLL; ++u) {
+                        c[u] = c[u] && !sbm_test(bm, k[u]);
+                        sc[u] = 1;
+                        dc[u] = 0.0;
+                        if (c[u]) {
+                            sc[u] = vr[k[u]].sigma;
+                            dc[u] = vr[k[u]].delta;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < SBM_UNROLL; ++u) {
+                        const int E = e0 + u * 64;
+                        if (E >= total) break;
+                        const uint64_t cm = ballot(c[u]);
+                        if (cm == 0) continue;
+                        if (c[u]) row[wave].d[__popcll(cm & below)] = dep(sk[u], sc[u], dc[u]);
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        // this lane's vertex's entries in the chunk: [s0, s1) - E
+                        const int s0 = B.excl > E ? B.excl : E;
+                        const int s1 = B.incl < E + 64 ? B.incl : E + 64;
+                        if (s0 < s1) {
+                            const int ls = s0 - E, le = s1 - E;
+                            const uint64_t mlo = ls == 0 ? 0ull : (~0ull >> (64 - ls));
+                            const uint64_t mhi = le == 64 ? ~0ull : (~0ull >> (64 - le));
+                            const int f = __popcll(cm & mlo), n = __popcll(cm & mhi) - f;
+                            acc = lds_chain_sum(acc, &row[wave].d[f], n);
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
+                }
+                if (b0 + lane < b) vr[B.v].delta = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ------------------------------------------------------------ one root, whole GPU
 //
 // The static pass of a single root spread over the whole GPU (an edge
@@ -3107,6 +3525,24 @@ extern "C" hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, h
     case 16: hipLaunchKernelGGL(sbc_static_kernel<16>, dim3(grid), dim3(1024), 0, st, *P); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// LDS bitmap sizes (32-bit words) of the two instantiations
+constexpr int SBM_WORDS_SMALL = 8448;  // up to 270,336 vertices
+constexpr int SBM_WORDS_LARGE = 36864; // up to 1,179,648 vertices
+
+extern "C" int64_t sbc_static_bm_max_nv() { return (int64_t)SBM_WORDS_LARGE * 32; }
+
+extern "C" hipError_t sbc_launch_static_bm(const SbcParams* P, int grid, hipStream_t st)
+{
+    const int64_t nwords = ((int64_t)P->nv + 31) / 32;
+    if (nwords <= SBM_WORDS_SMALL)
+        hipLaunchKernelGGL((sbc_static_bm_kernel<16, SBM_WORDS_SMALL>), dim3(grid), dim3(1024), 0, st, *P);
+    else if (nwords <= SBM_WORDS_LARGE)
+        hipLaunchKernelGGL((sbc_static_bm_kernel<16, SBM_WORDS_LARGE>), dim3(grid), dim3(1024), 0, st, *P);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -63,6 +63,55 @@ def test_static_matches_oracle(name):
     assert_bc_close(f.bc(), op.bc())
 
 
+def _static_trees(hg, roots, form, monkeypatch):
+    if form == "vertex":
+        monkeypatch.setenv("SBC_STATIC_FORM", "vertex")
+    else:
+        monkeypatch.delenv("SBC_STATIC_FORM", raising=False)
+    ctx, g, f = _engine(hg, roots)
+    out = f.trees(), f.bc()
+    f.close()
+    g.close()
+    ctx.close()
+    return out
+
+
+# nv 16K and 2.5K (small LDS bitmap), 524K (large bitmap), 1.2M (past the
+# bitmap: both forms are the per-vertex kernel)
+@pytest.mark.parametrize("name", ["rmat14", "chunglu_multi", "rmat19", "sparse_1_2m"])
+def test_static_bitmap_form_matches_vertex_form(name, monkeypatch):
+    if name == "rmat14":
+        hg = sbc.gen_rmat(14, 16 << 14, seed=2, keep_lcc=False)
+        roots = hg.pick_roots(64, 3)
+    elif name == "chunglu_multi":
+        # several components; roots also outside the largest one
+        hg = sbc.gen_chung_lu(2500, 2.3, 2.0, seed=4, keep_lcc=False)
+        roots = hg.pick_roots(48, 5, from_lcc=False)
+    elif name == "rmat19":
+        hg = sbc.gen_rmat(19, 16 << 19, seed=1, keep_lcc=False)
+        roots = hg.pick_roots(16, 7)
+    else:
+        n = 1_200_000
+        rng = np.random.default_rng(9)
+        eu = rng.integers(1, 2000, 6000)
+        ev = rng.integers(1, 2000, 6000)
+        keep = eu != ev
+        hg = sbc.graph_from_edges(n, eu[keep], ev[keep])
+        roots = hg.pick_roots(8, 1)
+    print(f"[static forms] {name} nv={hg.nv} ne={hg.ne} k={len(roots)}", flush=True)
+    (lb, sb_, db), bcb = _static_trees(hg, roots, "bitmap", monkeypatch)
+    (lv, sv, dv), bcv = _static_trees(hg, roots, "vertex", monkeypatch)
+    assert np.array_equal(lb, lv) and np.array_equal(sb_, sv)
+    bad = np.argwhere(db != dv)
+    assert bad.size == 0, f"delta differs at {bad[:5]}"
+    assert np.array_equal(bcb, bcv)
+    if hg.nv < 20000:
+        o = Oracle(hg.rowptr, hg.col, roots)
+        o.static(PULL)
+        lo, so, do = o.trees()
+        assert np.array_equal(lb, lo) and np.array_equal(sb_, so) and np.array_equal(db, do)
+
+
 @pytest.mark.parametrize("gran", [sbc.FINE, sbc.COARSE])
 def test_plain_insert_karate(gran):
     hg = sbc.read_dimacs(karate_path())
