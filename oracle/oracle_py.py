@@ -23,6 +23,7 @@ VP = C.c_void_p
 
 PUSH, PULL = 0, 1
 COARSE, FINE = 0, 1
+MOVE_REBUILD, MOVE_IN_PLACE = 0, 1
 
 ITEM_DTYPE = np.dtype([("kase", np.int32), ("gap", np.int32), ("processed", np.int64), ("scanned", np.int64),
                        ("affected", np.int64), ("moved", np.int64), ("descent", np.int64)])
@@ -51,6 +52,7 @@ def oracle_lib():
             ("orc_graph_insert", C.c_int, [VP, i64, i64]),
             ("orc_graph_remove", C.c_int, [VP, i64, i64]),
             ("orc_delete", None, [VP, VP, i64, i64, P64]),
+            ("orc_delete_mode", None, [VP, VP, i64, i64, P64, C.c_int]),
             ("orc_graph_degree", i64, [VP, i64]),
             ("orc_forest_new", VP, [i64, i64, P64]),
             ("orc_forest_free", None, [VP]),
@@ -144,14 +146,16 @@ class Oracle:
             raise ValueError(f"edge ({u},{v}) not insertable: {r}")
         return self.update(u, v, mode, items)
 
-    def delete(self, u, v):
+    def delete(self, u, v, move=MOVE_REBUILD):
         """Removes (u,v) and runs the deletion update; returns the root counts
-        (sameLevel, adjacent, movement)."""
+        (sameLevel, adjacent, movement).  move: movement-case roots rebuilt by
+        the static pull pass (MOVE_REBUILD) or updated in place
+        (MOVE_IN_PLACE)."""
         L = oracle_lib()
         if L.orc_graph_remove(self.g, u, v) != 1:
             raise ValueError(f"edge ({u},{v}) not present")
         out = np.zeros(3, dtype=np.int64)
-        L.orc_delete(self.f, self.g, u, v, _p64(out))
+        L.orc_delete_mode(self.f, self.g, u, v, _p64(out), move)
         return out
 
     def tree(self, ri):

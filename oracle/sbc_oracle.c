@@ -288,10 +288,184 @@ static void fine_update(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s
  *  - Movement case (moveDownTreeBrandes, 880-1401): s moves down or out of
  *    reach; the root's tree is recomputed by the static pull pass over g, the
  *    tree that case ends in, and the change of each dependency added to BC. */
+/* ------------------------------------------------------------------ movement case in place */
+
+/* The movement case of deleting (s, p) from root ri's tree (p one level above
+ * s and s left without a parent), updated in place to the static pull tree
+ * of the new graph: the rule the device's in-place update follows (stated
+ * here, not taken from the reference, whose moveDownTreeBrandes,
+ * src/streamingbc_internal.c:880-1401, moves the subtree down in its own
+ * summation order).  Levels and path counts come out as the static pass's;
+ * a delta is recomputed by the pull sum over its children in adjacency order
+ * exactly when its level, its path count, its children or their counts or
+ * deltas changed, so it equals the static pass's delta whenever the tree
+ * before the deletion was the static pass's.  flg (zero) and lst are nv
+ * scratch entries; flg is zero again on return. */
+enum { MV_LIST = 1, MV_SEEN = 2, MV_MOVER = 4, MV_SIGSEED = 8, MV_SIGPROP = 16, MV_SIGCH = 32, MV_DEL = 64,
+       MV_DONE = 128 };
+
+static void mv_flag(int64_t *flg, int64_t *lst, int64_t *n, int64_t y, int64_t fl)
+{
+    if (!(flg[y] & MV_LIST)) lst[(*n)++] = y;
+    flg[y] |= fl | MV_LIST;
+}
+
+static void move_in_place(orc_forest *f, const orc_graph *g, int64_t ri, int64_t s, int64_t p, int64_t *flg,
+                          int64_t *lst)
+{
+    int64_t nv = f->nv, r = f->roots[ri];
+    int64_t *L = f->level + (size_t)ri * nv, *S = f->sigma + (size_t)ri * nv;
+    double *D = f->delta + (size_t)ri * nv;
+    int64_t n = 0;
+    mv_flag(flg, lst, &n, s, MV_SEEN | MV_MOVER);
+    /* movers, old level by old level */
+    for (int64_t lv = L[s];; lv++) {
+        int64_t t = n;
+        for (int64_t i = 0; i < t; i++) {
+            int64_t w = lst[i];
+            if (!(flg[w] & MV_MOVER) || L[w] != lv) continue;
+            for (int64_t e = 0; e < g->deg[w]; e++) {
+                int64_t y = g->adj[w][e];
+                if (L[y] == lv + 1) mv_flag(flg, lst, &n, y, MV_SEEN);
+            }
+        }
+        int more = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int64_t y = lst[i];
+            if (!(flg[y] & MV_SEEN) || (flg[y] & (MV_MOVER | MV_SIGSEED)) || L[y] != lv + 1) continue;
+            int keep = 0;
+            for (int64_t e = 0; e < g->deg[y] && !keep; e++) {
+                int64_t z = g->adj[y][e];
+                if (L[z] == lv && !(flg[z] & MV_MOVER)) keep = 1;
+            }
+            flg[y] |= keep ? MV_SIGSEED : MV_MOVER;
+            if (!keep) more = 1;
+        }
+        if (!more) break;
+    }
+    /* new levels: 1 + the lowest level among non-mover neighbours, then
+     * breadth-first among the movers */
+    int64_t kmin = INF_LEVEL;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t w = lst[i];
+        if (!(flg[w] & MV_MOVER)) continue;
+        int64_t best = INF_LEVEL;
+        for (int64_t e = 0; e < g->deg[w]; e++) {
+            int64_t z = g->adj[w][e];
+            if (!(flg[z] & MV_MOVER) && L[z] != INF_LEVEL && L[z] + 1 < best) best = L[z] + 1;
+        }
+        L[w] = best;
+        if (best < kmin) kmin = best;
+    }
+    for (int64_t k = kmin; k != INF_LEVEL; k++) {
+        int more = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int64_t w = lst[i];
+            if (!(flg[w] & MV_MOVER)) continue;
+            if (L[w] != k) {
+                if (L[w] > k && L[w] != INF_LEVEL) more = 1;
+                continue;
+            }
+            for (int64_t e = 0; e < g->deg[w]; e++) {
+                int64_t z = g->adj[w][e];
+                if ((flg[z] & MV_MOVER) && L[z] > k + 1) {
+                    L[z] = k + 1;
+                    more = 1;
+                }
+            }
+        }
+        if (!more) break;
+    }
+    /* path counts, downwards */
+    int64_t lo = INF_LEVEL, hi = -1, ld = L[p];
+    int64_t t0 = n;
+    for (int64_t i = 0; i < t0; i++) {
+        int64_t x = lst[i];
+        if (L[x] == INF_LEVEL || !(flg[x] & (MV_MOVER | MV_SIGSEED))) continue;
+        if (L[x] < lo) lo = L[x];
+        if (L[x] > hi) hi = L[x];
+        if (!(flg[x] & MV_MOVER)) continue;
+        for (int64_t e = 0; e < g->deg[x]; e++) {
+            int64_t y = g->adj[x][e];
+            if (L[y] == L[x] + 1 && !(flg[y] & MV_MOVER)) {
+                mv_flag(flg, lst, &n, y, MV_SIGSEED);
+                if (L[x] + 1 > hi) hi = L[x] + 1;
+            }
+        }
+    }
+    for (int64_t lv = lo; lv <= hi; lv++) {
+        int64_t t = n;
+        for (int64_t i = 0; i < t; i++) {
+            int64_t x = lst[i];
+            int64_t fx = flg[x];
+            if ((fx & MV_DONE) || !(fx & (MV_MOVER | MV_SIGSEED | MV_SIGPROP)) || L[x] != lv) continue;
+            int64_t sg = 0;
+            for (int64_t e = 0; e < g->deg[x]; e++) {
+                int64_t y = g->adj[x][e];
+                if (L[y] == lv - 1) sg = (int64_t)((uint64_t)sg + (uint64_t)S[y]);
+            }
+            int changed = sg != S[x] || (fx & MV_MOVER);
+            S[x] = sg;
+            flg[x] |= MV_DONE | (changed ? MV_SIGCH : 0);
+            if (!changed) continue;
+            if (lv > ld) ld = lv;
+            for (int64_t e = 0; e < g->deg[x]; e++) {
+                int64_t c = g->adj[x][e];
+                if (L[c] == lv + 1) {
+                    mv_flag(flg, lst, &n, c, MV_SIGPROP);
+                    if (lv + 1 > hi) hi = lv + 1;
+                }
+            }
+        }
+    }
+    /* deltas, upwards */
+    mv_flag(flg, lst, &n, p, MV_DEL);
+    for (int64_t lv = ld; lv >= 0; lv--) {
+        int64_t t = n;
+        for (int64_t i = 0; i < t; i++) {
+            int64_t x = lst[i];
+            int64_t fx = flg[x];
+            if (!(fx & (MV_SIGCH | MV_DEL)) || L[x] != lv) continue;
+            double acc = 0.0;
+            for (int64_t e = 0; e < g->deg[x]; e++) {
+                int64_t c = g->adj[x][e];
+                if (L[c] == lv + 1) acc += dep(S[x], S[c], D[c]);
+            }
+            int changed = memcmp(&acc, &D[x], sizeof(double)) != 0;
+            if (changed) {
+                if (x != r) f->bcpart[x] += acc - D[x];
+                D[x] = acc;
+            }
+            if (!changed && !(fx & MV_SIGCH)) continue;
+            for (int64_t e = 0; e < g->deg[x]; e++) {
+                int64_t q = g->adj[x][e];
+                if (L[q] == lv - 1) mv_flag(flg, lst, &n, q, MV_DEL);
+            }
+        }
+    }
+    /* out-of-reach movers; scratch */
+    for (int64_t i = 0; i < n; i++) {
+        int64_t x = lst[i];
+        if (L[x] == INF_LEVEL) {
+            S[x] = INF_LEVEL;
+            if (D[x] != 0.0) f->bcpart[x] -= D[x];
+            D[x] = 0.0;
+        }
+        flg[x] = 0;
+    }
+}
+
 void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3])
+{
+    orc_delete_mode(f, g, u, v, counts, ORC_MOVE_REBUILD);
+}
+
+void orc_delete_mode(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3], int move)
 {
     int64_t nv = f->nv;
     counts[0] = counts[1] = counts[2] = 0;
+    int64_t *flg = move == ORC_MOVE_IN_PLACE ? (int64_t *)calloc((size_t)nv, sizeof(int64_t)) : NULL;
+    int64_t *lst = move == ORC_MOVE_IN_PLACE ? (int64_t *)malloc((size_t)nv * sizeof(int64_t)) : NULL;
     double *old = (double *)malloc((size_t)nv * sizeof(double));
     for (int64_t ri = 0; ri < f->K; ri++) {
         int64_t r = f->roots[ri];
@@ -310,6 +484,9 @@ void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t
             orc_item it;
             memset(&it, 0, sizeof(it));
             fine_update(f, g, ri, s, p, 0, 1, &it);
+        } else if (move == ORC_MOVE_IN_PLACE) {
+            counts[2]++;
+            move_in_place(f, g, ri, s, p, flg, lst);
         } else {
             counts[2]++;
             memcpy(old, D, (size_t)nv * sizeof(double));
@@ -319,6 +496,8 @@ void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t
         }
     }
     free(old);
+    free(flg);
+    free(lst);
     fold_bc(f);
 }
 

@@ -25,7 +25,10 @@
  *                               (src/streamingbc_internal.c:607-868) as the FINE update with
  *                               negated added paths, the shallower endpoint dropping the
  *                               deeper one's old term; the movement case (880-1401) by the
- *                               static pull pass over the post-deletion graph
+ *                               static pull pass over the post-deletion graph, or, in
+ *                               orc_delete_mode(ORC_MOVE_IN_PLACE), by the device's in-place
+ *                               rule (movers, their new levels, path counts downwards, deltas
+ *                               upwards where an input changed), which is not the reference's
  *   orc_graph_remove            stinger_remove_edge x2 as called by DeleteEdge
  *                               (src/sbcMain_utils.c:473-481; src/stinger/stinger.c:1055-1090)
  *   orc_graph_insert            stinger_insert_edge x2 as called by InsertEdge
@@ -44,7 +47,10 @@
  *   - delete: case counts equal and level/sigma bit-exact, delta/BC within
  *     1e-9, against the reference's deleteEdgeStreamingBC (golden fixtures
  *     written by the compiled reference), and equal to a static recompute of
- *     the post-deletion graph.
+ *     the post-deletion graph; in place: bit-exact equal to the rebuild on
+ *     every movement-case root whose tree before the deletion was the static
+ *     pass's, level/sigma bit-exact and delta within 1e-12 after any mix of
+ *     updates, and the reference's fixtures as above.
  *
  * Conventions: level and sigma are int64 with LONG_MAX for unreachable
  * vertices, delta and BC are double, BC excludes each tree's own root.
@@ -93,8 +99,13 @@ void orc_static(orc_forest *f, const orc_graph *g, int order);
 /* The edge (u,v) must already be in g.  items: K records or NULL. */
 void orc_insert(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int mode, orc_item *items);
 /* The edge (u,v) must already be removed from g.  counts: roots of the
- * same-level, adjacent and movement cases. */
+ * same-level, adjacent and movement cases.  Movement-case roots are rebuilt
+ * by the static pull pass (orc_delete, ORC_MOVE_REBUILD) or updated in place
+ * (ORC_MOVE_IN_PLACE): the same levels and path counts, and a delta
+ * recomputed by the pull sum exactly where one of its inputs changed. */
+enum { ORC_MOVE_REBUILD = 0, ORC_MOVE_IN_PLACE = 1 };
 void orc_delete(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3]);
+void orc_delete_mode(orc_forest *f, const orc_graph *g, int64_t u, int64_t v, int64_t counts[3], int move);
 
 void orc_get_tree(const orc_forest *f, int64_t ri, int64_t *level, int64_t *sigma, double *delta);
 void orc_set_tree(orc_forest *f, int64_t ri, const int64_t *level, const int64_t *sigma, const double *delta);

@@ -1337,9 +1337,37 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         if ((st = ev_end(C, KSTAT_WIDE, e0))) return st;
         C->last_parked = parked;
     }
-    // movement roots leave BC before their rebuild and return after it; the
-    // update kernel adds the adjacent roots' changes itself (as insertions do)
+    // movement roots: on graphs below the several-roots threshold, updated
+    // in place by one workgroup each (BC by the deltas that change); on
+    // larger ones rebuilt by the static pass over the whole GPU, leaving BC
+    // before the rebuild and returning after it -- there a deletion's
+    // in-place update recomputes the deltas of hundreds to thousands of
+    // ancestors with thousands of neighbours each, more than one workgroup
+    // does faster than the whole GPU rebuilds the root (R-MAT scale 20:
+    // 25 ms against 6.3).  SBC_DELETE_MOVE=rebuild or in_place (test hooks)
+    // forces either.  The update kernel adds the adjacent roots' changes
+    // itself (as insertions do).
     int32_t* mlist = C->d_del + 4 + F->K;
+    const char* dm = getenv("SBC_DELETE_MOVE");
+    const bool in_place = dm ? strcmp(dm, "in_place") == 0 : G->total_cap < wide_graph_min();
+    if (h[3] > 0 && in_place) {
+        SbcParams Q = P;
+        Q.u0 = (int32_t)u;
+        Q.v0 = (int32_t)v;
+        Q.sroots = mlist;
+        Q.scnt = C->d_del + 3;
+        HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
+        if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
+        int grid = (int)(h[3] < 2LL * C->num_cu ? h[3] : 2LL * C->num_cu);
+        HIPCHK(sbc_launch_delmove(&Q, grid, C->stream));
+        if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
+        C->last_rec_n = 0;
+        if ((st = check_kernel_err(F, C))) return st;
+        counts[0] = h[1];
+        counts[1] = h[2];
+        counts[2] = h[3];
+        return SBC_OK;
+    }
     if (h[3] > 0) HIPCHK(sbc_launch_bc_roots(&P, mlist, C->d_del + 3, -1.0, C->stream));
     if (h[3] > 0 && h[3] <= wide_rebuild_max()) {
         // few movement roots: each rebuilt by the whole GPU

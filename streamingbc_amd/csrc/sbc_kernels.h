@@ -206,6 +206,10 @@ hipError_t sbc_launch_static(const SbcParams* P, int nwb, int grid, hipStream_t 
 hipError_t sbc_launch_static_bm(const SbcParams* P, int grid, hipStream_t st);
 int64_t sbc_static_bm_max_nv();
 hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
+// Movement-case roots of deleting (P->u0, P->v0), root indices
+// P->sroots[0, *P->scnt): each tree updated in place to the static pass's
+// tree of the new graph, BC by the changed deltas (P->bc_atomic).
+hipError_t sbc_launch_delmove(const SbcParams* P, int grid, hipStream_t st);
 // BC[v] += sign * sum over root indices list[0, *cnt) of delta_r[v] (r != v).
 hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* list, const int32_t* cnt, double sign,
                                hipStream_t st);

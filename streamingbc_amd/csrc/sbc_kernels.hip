@@ -3446,6 +3446,486 @@ __global__ void sbc_delete_classify_kernel(SbcParams P, int32_t u, int32_t v, in
     }
 }
 
+// ------------------------------------------------------------ deletion, movement case
+//
+// The movement-case roots of deleting (u0, v0) (already out of the graph),
+// one workgroup each (root indices sroots[0, *scnt)), updated in place to
+// the tree the static pass would build on the new graph, bit for bit:
+// levels and path counts are exact, and a vertex's delta is recomputed --
+// by the static pass's own sum over its children in adjacency order --
+// exactly when one of its inputs (its level, its path count, its children,
+// their path counts or their deltas) changed, so every other delta already
+// has the static pass's bits.  The reference moves the subtree down instead
+// (moveDownTreeBrandes, src/streamingbc_internal.c:880-1401).
+//  A  movers (old levels, level by level from the deeper endpoint s): a
+//     child of a mover moves iff every parent it had is a mover; a child
+//     that keeps a parent loses one (its path count is recomputed);
+//  B  new levels of the movers: 1 + the lowest level among their non-mover
+//     neighbours, then a breadth-first pass among the movers (a mover left
+//     without any is now out of reach);
+//  C  path counts, level by level downwards: movers, vertices that lost or
+//     gained a parent, and the children of every vertex whose count changed
+//     pull the sum of their parents' counts;
+//  D  deltas, level by level upwards: the vertices of C whose count or level
+//     changed, p (which lost s), and the parents of every vertex whose
+//     delta, count or level changed;
+//  E  state bytes, out-of-reach movers, scratch.
+// Every vertex touched is listed once in the root's list (flags in the
+// dpath scratch word), so the list never outgrows its nv entries.  BC
+// changes by new - old for every delta that changes.
+
+#define DM_LIST 1ll     // in the touched list
+#define DM_SEEN 2ll     // A: a child of a mover, being decided
+#define DM_MOVER 4ll    // level changes
+#define DM_SIGSEED 8ll  // lost or gained a parent: path count recomputed
+#define DM_SIGPROP 16ll // a parent's path count changed: recomputed
+#define DM_SIGCH 32ll   // path count or level changed
+#define DM_DEL 64ll     // delta recomputed: a child's term changed
+#define DM_DONE 128ll   // path count recomputed
+
+__device__ __forceinline__ int64_t or64_wg(int64_t* p, int64_t v)
+{
+    return (int64_t)__hip_atomic_fetch_or((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED, WG_SCOPE);
+}
+
+__device__ __forceinline__ int32_t min32_wg(int32_t* p, int32_t v)
+{
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, WG_SCOPE);
+}
+
+// Flags y and lists it the first time it is flagged; the whole wavefront
+// calls this once per chunk.
+__device__ __forceinline__ void dm_flag(bool want, int32_t y, int64_t flags, SbcVRec* vr, int32_t* list, int* tail)
+{
+    bool app = false;
+    if (want) app = !(or64_wg(&vr[y].dpath, flags | DM_LIST) & DM_LIST);
+    const int pos = wave_append(app, tail);
+    if (pos >= 0) list[pos] = y;
+}
+
+__device__ __forceinline__ int32_t wave_min32(int32_t x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const int32_t t = __shfl_xor(x, d);
+        x = t < x ? t : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = (int64_t)((uint64_t)x + (uint64_t)__shfl_xor(x, d));
+    return x;
+}
+
+// Vertices with more neighbours than this are handled by the whole workgroup
+// in C and D (at most DM_BIGQ a level; more go to single wavefronts): one
+// wavefront would need at least 256 rounds of loads for them, while the
+// workgroup handles them one at a time.
+constexpr int DM_BIG = 16384;
+constexpr int DM_BIGQ = 255;
+// chunks per wavefront in one segment of a whole-workgroup delta sum
+constexpr int DM_SEGU = 4;
+
+// Queues x for the whole-workgroup pass (lane 0 takes the slot); false when
+// the queue is full and the calling wavefront handles x itself.
+__device__ __forceinline__ bool dm_queue_big(int32_t x, int* big)
+{
+    int b = 0;
+    if (lane_id() == 0) {
+        b = atomicAdd(&big[0], 1);
+        if (b < DM_BIGQ) big[1 + b] = x;
+    }
+    return __shfl(b, 0) < DM_BIGQ;
+}
+
+// Flags (and lists) the neighbours of a vertex on level lv: chunks w, w +
+// step, ... of its list (one wavefront: 0, 1; the whole workgroup: wave,
+// NWB).  hi, when given, is raised to lv if any neighbour was flagged.
+__device__ __forceinline__ void dm_mark_level(const SbcParams& P, const SbcVInfo& vi, int w, int step, int32_t lv,
+                                              int64_t flag, const int32_t* dist, SbcVRec* vr, int32_t* list, int* tail,
+                                              int* hi)
+{
+    bool any = false;
+    for (int e0 = w * 64; e0 < vi.deg; e0 += step * 64) {
+        const int e = e0 + lane_id();
+        int32_t y = 0;
+        bool m = false;
+        if (e < vi.deg) {
+            y = P.col[vi.start + e];
+            m = ld_acq32(&dist[y]) == lv;
+        }
+        any |= m;
+        dm_flag(m, y, flag, vr, list, tail);
+    }
+    if (hi && ballot(any) != 0 && lane_id() == 0) atomicMax(hi, lv);
+}
+
+template <int NWB>
+__global__ __launch_bounds__(NWB * 64) void sbc_delmove_kernel(SbcParams P)
+{
+    // 0: touched-list end, 1 and 2: the last round of A and of B that found
+    // more work (a round writes its own number: nothing to reset), 3: popped
+    // queue index
+    __shared__ int cnt[4];
+    // 0, 1: level range of pending path counts; 2: highest level of D
+    __shared__ int lv[4];
+    // whole-workgroup vertices of a level: count, then the vertices
+    __shared__ int big[1 + DM_BIGQ];
+    __shared__ double dmrow[NWB][DM_SEGU * 64];
+    __shared__ int dmcnt[NWB];
+    __shared__ unsigned long long bsum;
+    __shared__ int bflag;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const bool leader = threadIdx.x == 0;
+    const int nv = P.nv;
+    const int nq = *P.scnt;
+    for (;;) {
+        if (leader) cnt[3] = atomicAdd(&P.qctr[0], 1);
+        __syncthreads();
+        const int qi = cnt[3];
+        __syncthreads();
+        if (qi >= nq) break;
+        const int ri = P.sroots[qi];
+        const size_t off = (size_t)ri * (size_t)nv;
+        int32_t* dist = P.dist + off;
+        uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
+        SbcVRec* vr = P.vr + off;
+        int32_t* list = P.list + off;
+        const int32_t r = P.roots[ri];
+        const int32_t du = ld_acq32(&dist[P.u0]), dv = ld_acq32(&dist[P.v0]);
+        const int32_t s = du > dv ? P.u0 : P.v0, p = du > dv ? P.v0 : P.u0;
+        const int32_t ls = du > dv ? du : dv, lp = du > dv ? dv : du;
+        __syncthreads();
+        if (leader) {
+            list[0] = s;
+            st_rel64(&vr[s].dpath, DM_LIST | DM_SEEN | DM_MOVER);
+            cnt[0] = 1;
+            cnt[1] = -1;
+            cnt[2] = -1;
+        }
+        __syncthreads();
+
+        // A: movers, old level by old level
+        for (int L = ls;; ++L) {
+            const int t1 = cnt[0];
+            for (int i = wave; i < t1; i += NWB) {
+                const int32_t w = list[i];
+                if (!(ld_acq64(&vr[w].dpath) & DM_MOVER) || ld_acq32(&dist[w]) != L) continue;
+                const SbcVInfo vi = P.vinfo[w];
+                for (int e0 = 0; e0 < vi.deg; e0 += 64) {
+                    const int e = e0 + lane;
+                    int32_t y = 0;
+                    bool c = false;
+                    if (e < vi.deg) {
+                        y = P.col[vi.start + e];
+                        c = ld_acq32(&dist[y]) == L + 1;
+                    }
+                    dm_flag(c, y, DM_SEEN, vr, list, &cnt[0]);
+                }
+            }
+            __syncthreads();
+            const int t2 = cnt[0];
+            for (int i = wave; i < t2; i += NWB) {
+                const int32_t y = list[i];
+                const int64_t fy = ld_acq64(&vr[y].dpath);
+                if (!(fy & DM_SEEN) || (fy & (DM_MOVER | DM_SIGSEED)) || ld_acq32(&dist[y]) != L + 1) continue;
+                const SbcVInfo vi = P.vinfo[y];
+                bool keep = false;
+                for (int e0 = 0; e0 < vi.deg && !keep; e0 += 64) {
+                    const int e = e0 + lane;
+                    bool k = false;
+                    if (e < vi.deg) {
+                        const int32_t z = P.col[vi.start + e];
+                        if (ld_acq32(&dist[z]) == L) k = !(ld_acq64(&vr[z].dpath) & DM_MOVER);
+                    }
+                    keep = ballot(k) != 0;
+                }
+                if (lane == 0) {
+                    or64_wg(&vr[y].dpath, keep ? DM_SIGSEED : DM_MOVER);
+                    if (!keep) cnt[1] = L;
+                }
+            }
+            __syncthreads();
+            const bool more = cnt[1] == L;
+            __syncthreads();
+            if (!more) break;
+        }
+
+        // B: new levels of the movers
+        if (leader) lv[0] = INF32;
+        __syncthreads();
+        {
+            const int t1 = cnt[0];
+            for (int i = wave; i < t1; i += NWB) {
+                const int32_t w = list[i];
+                if (!(ld_acq64(&vr[w].dpath) & DM_MOVER)) continue;
+                const SbcVInfo vi = P.vinfo[w];
+                int32_t best = INF32;
+                for (int e = lane; e < vi.deg; e += 64) {
+                    const int32_t z = P.col[vi.start + e];
+                    const int32_t dz = ld_acq32(&dist[z]);
+                    // a mover's dist may already hold its key: movers are skipped
+                    if (dz != INF32 && dz + 1 < best && !(ld_acq64(&vr[z].dpath) & DM_MOVER)) best = dz + 1;
+                }
+                best = wave_min32(best);
+                if (lane == 0) {
+                    st_rel32(&dist[w], best);
+                    if (best != INF32) atomicMin(&lv[0], best);
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = lv[0]; k != INF32; ++k) {
+            const int t1 = cnt[0];
+            for (int i = wave; i < t1; i += NWB) {
+                const int32_t w = list[i];
+                if (!(ld_acq64(&vr[w].dpath) & DM_MOVER)) continue;
+                const int32_t dw = ld_acq32(&dist[w]);
+                if (dw != k) {
+                    if (dw > k && dw != INF32 && lane == 0) cnt[2] = k;
+                    continue;
+                }
+                const SbcVInfo vi = P.vinfo[w];
+                bool relaxed = false;
+                for (int e = lane; e < vi.deg; e += 64) {
+                    const int32_t z = P.col[vi.start + e];
+                    if (ld_acq32(&dist[z]) > k + 1 && (ld_acq64(&vr[z].dpath) & DM_MOVER)) {
+                        min32_wg(&dist[z], k + 1);
+                        relaxed = true;
+                    }
+                }
+                if (ballot(relaxed) != 0 && lane == 0) cnt[2] = k;
+            }
+            __syncthreads();
+            const bool more = cnt[2] == k;
+            __syncthreads();
+            if (!more) break;
+        }
+
+        // C: path counts, downwards.  Vertices gaining a mover as a parent
+        // join the seeds first.
+        if (leader) {
+            lv[0] = INF32;
+            lv[1] = -1;
+            lv[2] = lp;
+        }
+        __syncthreads();
+        {
+            const int t1 = cnt[0];
+            for (int i = wave; i < t1; i += NWB) {
+                const int32_t x = list[i];
+                const int64_t fx = ld_acq64(&vr[x].dpath);
+                const int32_t dx = ld_acq32(&dist[x]);
+                if (dx == INF32 || !(fx & (DM_MOVER | DM_SIGSEED))) continue;
+                if (lane == 0) {
+                    atomicMin(&lv[0], dx);
+                    atomicMax(&lv[1], dx);
+                }
+                if (!(fx & DM_MOVER)) continue;
+                const SbcVInfo vi = P.vinfo[x];
+                bool any = false;
+                for (int e0 = 0; e0 < vi.deg; e0 += 64) {
+                    const int e = e0 + lane;
+                    int32_t y = 0;
+                    bool g = false;
+                    if (e < vi.deg) {
+                        y = P.col[vi.start + e];
+                        g = ld_acq32(&dist[y]) == dx + 1 && !(ld_acq64(&vr[y].dpath) & DM_MOVER);
+                    }
+                    any |= g;
+                    dm_flag(g, y, DM_SIGSEED, vr, list, &cnt[0]);
+                }
+                if (ballot(any) != 0 && lane == 0) atomicMax(&lv[1], dx + 1);
+            }
+        }
+        __syncthreads();
+        for (int L = lv[0];;) {
+            const int hi = lv[1];
+            if (leader) big[0] = 0;
+            __syncthreads();
+            if (L > hi) break;
+            const int t1 = cnt[0];
+            for (int i = wave; i < t1; i += NWB) {
+                const int32_t x = list[i];
+                const int64_t fx = ld_acq64(&vr[x].dpath);
+                if ((fx & DM_DONE) || !(fx & (DM_MOVER | DM_SIGSEED | DM_SIGPROP)) || ld_acq32(&dist[x]) != L) continue;
+                const SbcVInfo vi = P.vinfo[x];
+                if (vi.deg > DM_BIG && dm_queue_big(x, big)) continue;
+                int64_t part = 0;
+                for (int e = lane; e < vi.deg; e += 64) {
+                    const int32_t y = P.col[vi.start + e];
+                    if (ld_acq32(&dist[y]) == L - 1) part = (int64_t)((uint64_t)part + (uint64_t)vr[y].sigma);
+                }
+                const int64_t sig = wave_sum64(part);
+                const bool changed = sig != vr[x].sigma || (fx & DM_MOVER);
+                if (lane == 0) {
+                    vr[x].sigma = sig;
+                    or64_wg(&vr[x].dpath, DM_DONE | (changed ? DM_SIGCH : 0ll));
+                    if (changed) atomicMax(&lv[2], L);
+                }
+                if (changed) dm_mark_level(P, vi, 0, 1, L + 1, DM_SIGPROP, dist, vr, list, &cnt[0], &lv[1]);
+            }
+            __syncthreads();
+            // vertices with many neighbours: the whole workgroup
+            const int nb = big[0] < DM_BIGQ ? big[0] : DM_BIGQ;
+            for (int bi = 0; bi < nb; ++bi) {
+                const int32_t x = big[1 + bi];
+                const SbcVInfo vi = P.vinfo[x];
+                if (leader) bsum = 0ull;
+                __syncthreads();
+                int64_t part = 0;
+                for (int e = threadIdx.x; e < vi.deg; e += NWB * 64) {
+                    const int32_t y = P.col[vi.start + e];
+                    if (ld_acq32(&dist[y]) == L - 1) part = (int64_t)((uint64_t)part + (uint64_t)vr[y].sigma);
+                }
+                part = wave_sum64(part);
+                if (lane == 0) atomicAdd(&bsum, (unsigned long long)part);
+                __syncthreads();
+                if (leader) {
+                    const int64_t fx = ld_acq64(&vr[x].dpath);
+                    const int64_t sig = (int64_t)bsum;
+                    const bool changed = sig != vr[x].sigma || (fx & DM_MOVER);
+                    vr[x].sigma = sig;
+                    or64_wg(&vr[x].dpath, DM_DONE | (changed ? DM_SIGCH : 0ll));
+                    if (changed) atomicMax(&lv[2], L);
+                    bflag = changed;
+                }
+                __syncthreads();
+                if (bflag) dm_mark_level(P, vi, wave, NWB, L + 1, DM_SIGPROP, dist, vr, list, &cnt[0], &lv[1]);
+                __syncthreads();
+            }
+            ++L;
+        }
+
+        // D: deltas, upwards from the deepest level C changed (p at least).
+        // A vertex with more than DM_BIG neighbours is summed by the whole
+        // workgroup after the level's others: in segments of NWB x DM_SEGU
+        // chunks, each wavefront packs its chunks' child terms in order into
+        // its LDS row and one lane adds the rows in order -- the same
+        // sequence of additions as one wavefront's sum.
+        if (leader) {
+            const int64_t fp = or64_wg(&vr[p].dpath, DM_LIST | DM_DEL);
+            if (!(fp & DM_LIST)) list[cnt[0]++] = p;
+        }
+        for (int L = lv[2]; L >= 0; --L) {
+            if (leader) big[0] = 0;
+            __syncthreads();
+            const int t1 = cnt[0];
+            for (int i = wave; i < t1; i += NWB) {
+                const int32_t x = list[i];
+                const int64_t fx = ld_acq64(&vr[x].dpath);
+                if (!(fx & (DM_SIGCH | DM_DEL)) || ld_acq32(&dist[x]) != L) continue;
+                const SbcVInfo vi = P.vinfo[x];
+                if (vi.deg > DM_BIG && dm_queue_big(x, big)) continue;
+                const int64_t sx = vr[x].sigma;
+                double acc = 0.0;
+                for (int e0 = 0; e0 < vi.deg; e0 += 64 * DM_SEGU) {
+                    bool ch[DM_SEGU];
+                    double add[DM_SEGU];
+#pragma unroll
+                    for (int u = 0; u < DM_SEGU; ++u) {
+                        const int e = e0 + u * 64 + lane;
+                        ch[u] = false;
+                        add[u] = 0.0;
+                        if (e < vi.deg) {
+                            const int32_t c = P.col[vi.start + e];
+                            if (ld_acq32(&dist[c]) == L + 1) {
+                                ch[u] = true;
+                                add[u] = dep(sx, vr[c].sigma, vr[c].delta);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < DM_SEGU; ++u) acc = wave_ordered_add(acc, ch[u], add[u], false, 0.0);
+                }
+                const double od = vr[x].delta;
+                const bool changed = __double_as_longlong(acc) != __double_as_longlong(od);
+                if (lane == 0 && changed) {
+                    vr[x].delta = acc;
+                    if (P.bc_atomic && x != r) atomicAdd(&P.bc[x], acc - od);
+                }
+                if (changed || (fx & DM_SIGCH)) dm_mark_level(P, vi, 0, 1, L - 1, DM_DEL, dist, vr, list, &cnt[0], nullptr);
+            }
+            __syncthreads();
+            const int nb = big[0] < DM_BIGQ ? big[0] : DM_BIGQ;
+            for (int bi = 0; bi < nb; ++bi) {
+                const int32_t x = big[1 + bi];
+                const SbcVInfo vi = P.vinfo[x];
+                const int64_t sx = vr[x].sigma;
+                const int nch = (vi.deg + 63) >> 6;
+                double acc = 0.0; // thread 0's
+                for (int s0 = 0; s0 < nch; s0 += NWB * DM_SEGU) {
+                    bool ch[DM_SEGU];
+                    double add[DM_SEGU];
+#pragma unroll
+                    for (int u = 0; u < DM_SEGU; ++u) {
+                        const int e = (s0 + wave * DM_SEGU + u) * 64 + lane;
+                        ch[u] = false;
+                        add[u] = 0.0;
+                        if (e < vi.deg) {
+                            const int32_t c = P.col[vi.start + e];
+                            if (ld_acq32(&dist[c]) == L + 1) {
+                                ch[u] = true;
+                                add[u] = dep(sx, vr[c].sigma, vr[c].delta);
+                            }
+                        }
+                    }
+                    int n = 0;
+#pragma unroll
+                    for (int u = 0; u < DM_SEGU; ++u) {
+                        const uint64_t m = ballot(ch[u]);
+                        if (ch[u]) dmrow[wave][n + __popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = add[u];
+                        n += __popcll(m);
+                    }
+                    if (lane == 0) dmcnt[wave] = n;
+                    __syncthreads();
+                    if (leader)
+                        for (int w = 0; w < NWB; ++w) acc = lds_chain_sum(acc, dmrow[w], dmcnt[w]);
+                    __syncthreads();
+                }
+                if (leader) {
+                    const int64_t fx = ld_acq64(&vr[x].dpath);
+                    const double od = vr[x].delta;
+                    const bool changed = __double_as_longlong(acc) != __double_as_longlong(od);
+                    if (changed) {
+                        vr[x].delta = acc;
+                        if (P.bc_atomic && x != r) atomicAdd(&P.bc[x], acc - od);
+                    }
+                    bflag = changed || (fx & DM_SIGCH);
+                }
+                __syncthreads();
+                if (bflag) dm_mark_level(P, vi, wave, NWB, L - 1, DM_DEL, dist, vr, list, &cnt[0], nullptr);
+                __syncthreads();
+            }
+        }
+
+        // E: state bytes, out-of-reach movers, scratch
+        {
+            const int t1 = cnt[0];
+            for (int i = threadIdx.x; i < t1; i += NWB * 64) {
+                const int32_t x = list[i];
+                const int32_t dx = ld_acq32(&dist[x]);
+                if (dx == INF32) {
+                    sb[x] = SB_INF;
+                    vr[x].sigma = SIGMA_INF;
+                    const double od = vr[x].delta;
+                    if (od != 0.0) {
+                        if (P.bc_atomic) atomicAdd(&P.bc[x], -od);
+                        vr[x].delta = 0.0;
+                    }
+                } else {
+                    sb[x] = sb_make(dx, 0);
+                }
+                st_rel64(&vr[x].dpath, 0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Clears the work-queue counters (a kernel, not a memset, so it can sit in
 // a captured graph).
 __global__ void sbc_reset_kernel(int32_t* q, int n)
@@ -3558,6 +4038,12 @@ extern "C" hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, 
 {
     const int bs = 256;
     hipLaunchKernelGGL(sbc_delete_classify_kernel, dim3((P->K + bs - 1) / bs), dim3(bs), 0, st, *P, u, v, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_delmove(const SbcParams* P, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_delmove_kernel<16>, dim3(grid), dim3(1024), 0, st, *P);
     return hipGetLastError();
 }
 

@@ -6,10 +6,12 @@ adjacency order, with FP contraction off); BC within 1e-9 relative (floor
 1e-6 x max|BC|) because the device adds per-root BC changes with atomics in
 arbitrary order.
 """
+import os
+
 import numpy as np
 import pytest
 
-from oracle_py import COARSE as O_COARSE, FINE as O_FINE, PULL, PUSH, Oracle
+from oracle_py import COARSE as O_COARSE, FINE as O_FINE, MOVE_IN_PLACE, MOVE_REBUILD, PULL, PUSH, Oracle
 from sbc_testutil import LONG_MAX, assert_bc_close, karate_path
 
 pytestmark = pytest.mark.gpu
@@ -35,6 +37,14 @@ def _compare_trees(f, o, what, delta_exact=True):
         assert bad.size == 0, f"{what}: delta differs at {bad[:5]} gpu={dl[tuple(bad[0])]!r} orc={do[tuple(bad[0])]!r}"
     else:
         assert_bc_close(dl, do, what=f"{what} delta")
+
+
+def _odel(o, a, b):
+    """The oracle's deletion with the movement case the device runs: in place
+    by default (every graph here is below the size from which the device
+    rebuilds instead), rebuilt under SBC_DELETE_MOVE=rebuild."""
+    move = MOVE_REBUILD if os.environ.get("SBC_DELETE_MOVE") == "rebuild" else MOVE_IN_PLACE
+    return o.delete(a, b, move)
 
 
 def _rmat_c1():
@@ -365,7 +375,7 @@ def test_c_driver_deletes_back_to_the_static_bc():
         o.insert(int(a), int(b), O_FINE)
     tot = np.zeros(3, dtype=np.int64)
     for a, b in zip(u[::-1], v[::-1]):
-        tot += o.delete(int(a), int(b))
+        tot += _odel(o, int(a), int(b))
     assert res["deletions"] == 24
     assert (res["same_level"], res["adjacent"], res["movement"]) == tuple(int(t) for t in tot)
     assert abs(res["bc_sum"] - res["bc_sum_static"]) <= 1e-9 * abs(res["bc_sum_static"])
@@ -557,14 +567,14 @@ def test_hub_lookup_level3_leaf_roots():
 
 
 # ------------------------------------------------------------ edge deletion
-# The device deletion rebuilds each affected root with the static pass, so
-# after any mix of insertions and deletions the per-root trees equal the
-# oracle's bit for bit (same adjacency order on both sides: removal keeps the
-# order, insertion appends); BC is folded again from the trees, so it agrees
-# within 1e-9 with the oracle's incremental BC and the reference's fixtures.
+# The device deletion brings each movement-case root to the tree the static
+# pass gives on the new graph (updated in place, or rebuilt), so after any mix
+# of insertions and deletions the per-root trees equal the oracle's bit for
+# bit (same adjacency order on both sides: removal keeps the order, insertion
+# appends); BC agrees within 1e-9 with the oracle's incremental BC and the
+# reference's fixtures.
 
 import hashlib
-import os
 
 from sbc_testutil import GOLDEN
 
@@ -579,22 +589,30 @@ def _apply_ops(f, g, ctx, o, ops):
             counts.append((-1, -1, -1))
         else:
             c = f.delete_edge(g, ctx, a, b)
-            co = o.delete(a, b)
+            co = _odel(o, a, b)
             got = (c["sameLevel"], c["adjacent"], c["movement"])
             assert got == tuple(int(x) for x in co), f"case counts of deleting ({a},{b}): {got} vs {tuple(co)}"
             counts.append(got)
     return np.array(counts, dtype=np.int64)
 
 
-# movement-case roots: SBC_WIDE_REBUILD_MAX = 1000 rebuilds them one root at a
-# time over the whole GPU; 0 with SBC_WIDE_GRAPH_MIN = 0 all together over the
-# whole GPU; 0 with the default graph-size threshold (above these graphs) one
-# workgroup per root
-REBUILD_MODES = {"single": ("1000", None), "multi": ("0", "0"), "workgroup": ("0", None)}
+# movement-case roots: updated in place (the default below 4M adjacency
+# slots), or, with SBC_DELETE_MOVE=rebuild, rebuilt by the static pass: SBC_WIDE_REBUILD_MAX =
+# 1000 rebuilds them one root at a time over the whole GPU; 0 with
+# SBC_WIDE_GRAPH_MIN = 0 all together over the whole GPU; 0 with the default
+# graph-size threshold (above these graphs) one workgroup per root
+REBUILD_MODES = {"incremental": (None, None), "single": ("1000", None), "multi": ("0", "0"),
+                 "workgroup": ("0", None)}
 
 
 def _rebuild_mode(monkeypatch, mode):
     wide, gmin = REBUILD_MODES[mode]
+    if wide is None:
+        for k in ("SBC_WIDE_REBUILD_MAX", "SBC_WIDE_GRAPH_MIN"):
+            monkeypatch.delenv(k, raising=False)
+        monkeypatch.setenv("SBC_DELETE_MOVE", "in_place")
+        return
+    monkeypatch.setenv("SBC_DELETE_MOVE", "rebuild")
     monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
     if gmin is not None:
         monkeypatch.setenv("SBC_WIDE_GRAPH_MIN", gmin)
@@ -622,6 +640,58 @@ def test_delete_matches_oracle_and_reference(name, wide, monkeypatch):
     _compare_trees(f, o, f"{name} after deletions")
     lv, sg, _ = f.trees()
     assert np.array_equal(lv, fx["fine_level"]) and np.array_equal(sg, fx["fine_sigma"])
+
+
+def _graph_edges(hg, n, seed):
+    rng = np.random.default_rng(seed)
+    src = np.repeat(np.arange(hg.nv, dtype=np.int64), np.diff(hg.rowptr))
+    out, seen = [], set()
+    while len(out) < n:
+        e = int(rng.integers(hg.ne))
+        a, b = int(src[e]), int(hg.col[e])
+        if (min(a, b), max(a, b)) not in seen:
+            seen.add((min(a, b), max(a, b)))
+            out.append((a, b))
+    return out
+
+
+@pytest.mark.parametrize("name", ["chunglu_sparse", "rmat12"])
+def test_delete_movement_in_place_matches_oracle(name, monkeypatch):
+    """Many movement-case roots (subtrees cut off, moved down, hubs above
+    them), updated in place: trees bit-identical to the oracle's static
+    rebuild after every deletion batch, and the same trees as the rebuild."""
+    if name == "chunglu_sparse":
+        hg = sbc.gen_chung_lu(4096, 2.3, 3.0, seed=6, keep_lcc=True)
+        roots = hg.pick_roots(64, 2)
+    else:
+        hg = sbc.gen_rmat(12, 8 << 12, seed=4, keep_lcc=True)
+        roots = hg.pick_roots(48, 3)
+    dels = _graph_edges(hg, 48, 21)
+    print(f"[delete] in place {name} nv={hg.nv} k={len(roots)} deletions={len(dels)}", flush=True)
+    _rebuild_mode(monkeypatch, "incremental")
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    moved = 0
+    for i in range(0, len(dels), 8):
+        c = _apply_ops(f, g, ctx, o, [(0, a, b) for a, b in dels[i:i + 8]])
+        moved += int(c[:, 2].sum())
+        _compare_trees(f, o, f"{name} after deletion {i + 7}")
+        assert_bc_close(f.bc(), o.bc(), what=f"BC after deletion {i + 7}")
+    assert moved > 20, f"only {moved} movement-case roots"
+    f.recompute_bc(ctx)
+    assert_bc_close(f.bc(), o.bc(), what="folded BC")
+    # the rebuild path on the same deletions: the same levels and path
+    # counts; deltas that no input of theirs touched keep the rounding of
+    # earlier updates in place, so they agree within rounding
+    _rebuild_mode(monkeypatch, "workgroup")
+    ctx2, g2, f2 = _engine(hg, roots)
+    for a, b in dels:
+        f2.delete_edge(g2, ctx2, a, b)
+    (l1, s1, d1), (l2, s2, d2) = f.trees(), f2.trees()
+    assert np.array_equal(l1, l2) and np.array_equal(s1, s2)
+    assert_bc_close(d1, d2, what="delta in place vs rebuilt")
+    assert_bc_close(f.bc(), f2.bc(), what="BC in place vs rebuilt")
 
 
 @pytest.mark.parametrize("wide", sorted(REBUILD_MODES))
@@ -703,7 +773,7 @@ def test_delete_disconnects_and_legacy_form():
         assert bc[k] == 2 * (k - 1) * (4 - k)
     assert g.remove_edge(2, 3) == 1 and g.remove_edge(3, 2) == 1
     c = f.update_deleted(g, ctx, 2, 3)
-    co = o.delete(2, 3)
+    co = _odel(o, 2, 3)
     assert (c["sameLevel"], c["adjacent"], c["movement"]) == tuple(int(x) for x in co)
     _compare_trees(f, o, "legacy deletion")
     before = f.trees()
@@ -775,7 +845,7 @@ def test_vertex_delete_and_insert_match_oracle():
     assert list(removed) == nb and g.degree(1) == 0
     tot = np.zeros(3, dtype=np.int64)
     for x in nb:
-        tot += o.delete(1, x)
+        tot += _odel(o, 1, x)
     assert (c["sameLevel"], c["adjacent"], c["movement"]) == tuple(int(t) for t in tot)
     _compare_trees(f, o, "after deleting vertex 1")
     # 16 consecutive deletions take the most central vertex's BC (and its
@@ -817,7 +887,7 @@ def test_checkpoint_resume_is_bit_identical(tmp_path):
         o.insert(int(a), int(b), O_FINE)
     for a, b in ((hub, int(nb[2])), (int(u[0]), int(v[0]))):
         f.delete_edge(g, ctx, a, b)
-        o.delete(a, b)
+        _odel(o, a, b)
     f.insert_batched(g, ctx, u[4:20], v[4:20], sbc.BatchConfig(window=8, block_threads=1024))
     for a, b in zip(u[4:20], v[4:20]):
         o.insert(int(a), int(b), O_FINE)
@@ -839,7 +909,7 @@ def test_checkpoint_resume_is_bit_identical(tmp_path):
     assert np.array_equal(f.bc(), f2.bc())
     for a, b in zip(u[20:40], v[20:40]):
         o.insert(int(a), int(b), O_FINE)
-    o.delete(hub, int(nb[5]))
+    _odel(o, hub, int(nb[5]))
     o.insert(int(u[40]), int(v[40]), O_FINE)
     _compare_trees(f2, o, "resumed forest vs the oracle")
     assert_bc_close(f2.bc(), o.bc(), what="resumed BC vs the oracle")
@@ -887,11 +957,13 @@ def test_batched_relayout_inside_one_window():
     assert_bc_close(f.bc(), o.bc())
 
 
-def test_wide_rebuild_of_a_root_with_a_big_hub(monkeypatch):
-    """Movement-case roots rebuilt over the whole GPU when a level holds a
-    vertex with more than 16384 neighbours: its terms are computed across the
-    GPU and summed in order by one wavefront, bit-identical to the oracle."""
-    monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", "1000")
+@pytest.mark.parametrize("mode", ["single", "incremental"])
+def test_wide_rebuild_of_a_root_with_a_big_hub(mode, monkeypatch):
+    """Movement-case roots when a level holds a vertex with more than 16384
+    neighbours: rebuilt over the whole GPU (its terms computed across the GPU
+    and summed in order by one wavefront), or updated in place (the hub's
+    delta recomputed by one wavefront); bit-identical to the oracle."""
+    _rebuild_mode(monkeypatch, mode)
     n = 20100
     edges = [(1, i) for i in range(2, 20002)] + [(i, i + 1) for i in range(2, 300)] + [(20002, 20003), (20003, 5)]
     hg = sbc.graph_from_edges(n, np.array([a for a, b in edges]), np.array([b for a, b in edges]))

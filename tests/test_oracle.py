@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import streamingbc_amd as sbc
-from oracle_py import COARSE, FINE, PULL, PUSH, Oracle
+from oracle_py import COARSE, FINE, MOVE_IN_PLACE, MOVE_REBUILD, PULL, PUSH, Oracle
 from sbc_testutil import GOLDEN, LONG_MAX, assert_bc_close
 
 GRAPH_DIR = os.path.join(GOLDEN, "graphs")
@@ -363,6 +363,90 @@ def test_delete_equals_static_recompute():
     assert np.array_equal(lv, ls) and np.array_equal(sg, ss)
     assert_bc_close(dl, ds, what="delta")
     assert_bc_close(o.bc(), s.bc())
+
+
+def _edges_of(hg):
+    src = np.repeat(np.arange(hg.nv), np.diff(hg.rowptr))
+    return [(int(a), int(b)) for a, b in zip(src, hg.col) if a < b]
+
+
+@pytest.mark.parametrize("name", ["chunglu_sparse", "rmat11"])
+def test_delete_in_place_equals_rebuild_from_static_trees(name):
+    """The in-place movement case gives the rebuild's trees bit for bit when
+    the trees before the deletion are the static pass's: for each deleted
+    edge, two fresh forests, one per mode."""
+    if name == "chunglu_sparse":
+        hg = sbc.gen_chung_lu(1500, 2.3, 3.0, seed=4, keep_lcc=True)
+    else:
+        hg = sbc.gen_rmat(11, 8 << 11, seed=3, keep_lcc=True)
+    roots = hg.pick_roots(40, 5)
+    edges = _edges_of(hg)
+    rng = np.random.default_rng(8)
+    moved = 0
+    for i in rng.choice(len(edges), 25, replace=False):
+        a, b = edges[i]
+        ob = Oracle(hg.rowptr, hg.col, roots)
+        ob.static(PULL)
+        oi = Oracle(hg.rowptr, hg.col, roots)
+        oi.static(PULL)
+        cb = ob.delete(a, b, MOVE_REBUILD)
+        ci = oi.delete(a, b, MOVE_IN_PLACE)
+        assert np.array_equal(cb, ci)
+        moved += int(ci[2])
+        for x, y in zip(ob.trees(), oi.trees()):
+            assert np.array_equal(x, y), f"deleting ({a},{b})"
+        assert_bc_close(oi.bc(), ob.bc(), what=f"BC after deleting ({a},{b})")
+    assert moved >= 10, f"only {moved} movement-case roots"
+
+
+def test_delete_in_place_after_updates_is_close_to_rebuild():
+    """After a mix of insertions and deletions, in place and rebuilt agree on
+    levels and path counts exactly and on deltas and BC within rounding."""
+    hg = sbc.gen_chung_lu(1500, 2.3, 3.0, seed=7, keep_lcc=True)
+    roots = hg.pick_roots(40, 6)
+    edges = _edges_of(hg)
+    rng = np.random.default_rng(9)
+    dels = [edges[i] for i in rng.choice(len(edges), 60, replace=False)]
+    u, v = hg.insertions(30, 4)
+    ob = Oracle(hg.rowptr, hg.col, roots)
+    oi = Oracle(hg.rowptr, hg.col, roots)
+    ob.static(PULL)
+    oi.static(PULL)
+    moved = 0
+    for i, (a, b) in enumerate(dels):
+        if i < len(u):
+            ob.insert(int(u[i]), int(v[i]), FINE)
+            oi.insert(int(u[i]), int(v[i]), FINE)
+        cb = ob.delete(a, b, MOVE_REBUILD)
+        ci = oi.delete(a, b, MOVE_IN_PLACE)
+        assert np.array_equal(cb, ci)
+        moved += int(ci[2])
+    assert moved >= 20
+    (lb, sb_, db), (li, si, di) = ob.trees(), oi.trees()
+    assert np.array_equal(lb, li) and np.array_equal(sb_, si)
+    assert_bc_close(di, db, what="delta")
+    assert_bc_close(oi.bc(), ob.bc())
+
+
+@pytest.mark.parametrize("name", ["karate", "dolphins"])
+def test_oracle_delete_in_place_matches_reference_golden(name):
+    """The in-place movement case against the reference's own deletions
+    (FINE): case counts exact, levels and path counts exact, BC and delta
+    within 1e-9."""
+    g = np.load(os.path.join(GOLDEN, f"refdel_{name}.npz"))
+    hg = load_graph(name)
+    o = Oracle(hg.rowptr, hg.col, np.arange(hg.nv))
+    o.static(PUSH)
+    for i, (op, a, b) in enumerate(g["ops"]):
+        if op == 1:
+            o.insert(int(a), int(b), FINE)
+        else:
+            c = o.delete(int(a), int(b), MOVE_IN_PLACE)
+            assert np.array_equal(c, g["fine_counts"][i]), f"case counts of operation {i}"
+        assert_bc_close(o.bc(), g["fine_bc"][i + 1], what=f"BC after operation {i}")
+    lv, sg, dl = o.trees()
+    assert np.array_equal(lv, g["fine_level"]) and np.array_equal(sg, g["fine_sigma"])
+    assert_bc_close(dl, g["fine_delta"], what="delta")
 
 
 def test_delete_closed_forms():
