@@ -563,24 +563,18 @@ def main():
         t_del = time.perf_counter() - td
         sn, sms = ctx.kernel_stats(sbc.KSTAT_STATIC)
         un, ums = ctx.kernel_stats(sbc.KSTAT_UPDATE)
-        # movement-case roots are updated in place below 4M adjacency slots,
-        # rebuilt by the static pass over the whole GPU above
-        in_place = hg.ne < (4 << 20)
-        rebuilt = 0 if in_place else int(cases[2])
-        # the static pass of a rebuilt root reads every adjacency entry with
-        # the level of its far end (8 B) and writes level, state and record of
-        # every vertex (4 + 1 + 16 B)
-        sbytes = rebuilt * (8.0 * hg.ne + 21.0 * hg.nv)
+        # movement-case roots are updated in place: one workgroup each below
+        # 4M adjacency slots, the delta pass over the whole GPU above
+        wide_delta = hg.ne >= (4 << 20)
         deletion = {"value": round(len(dels) / t_del, 3), "unit": "deletions/s", "deletions": len(dels),
                     "ms_p50": round(1e3 * float(np.percentile(dlat, 50)), 3),
                     "cases": {"sameLevel": int(cases[0]), "adjacent": int(cases[1]), "movement": int(cases[2])},
                     "update_kernel_ms_avg": round(ums / max(un, 1), 3),
                     "movement_kernel_ms_avg": round(sms / max(sn, 1), 3),
-                    "rebuild_GBs": round(sbytes / (sms * 1e-3) / 1e9, 1) if sms > 0 and rebuilt else None,
                     "method": "adjacent-case roots: update kernel with negated added paths; movement-case "
-                              "roots: " + ("updated in place, one workgroup each" if in_place else
-                                           "rebuilt by the static pass over the whole GPU")
-                              + "; BC updated by the same changes"}
+                              "roots: updated in place (" + ("delta pass over the whole GPU, one launch per level"
+                                                            if wide_delta else "one workgroup each")
+                              + "); BC updated by the same changes"}
         print(f"[bench] deletions: {deletion['value']:.2f} del/s, cases {deletion['cases']}",
               file=sys.stderr, flush=True)
 

@@ -1290,7 +1290,7 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         HIPCHK(hipStreamSynchronize(C->stream));
         if (C->d_del) HIPCHK(hipFree(C->d_del));
         C->d_del = nullptr;
-        if ((st = dalloc(&C->d_del, (size_t)(8 + 2 * F->K)))) return st;
+        if ((st = dalloc(&C->d_del, (size_t)(8 + 5 * F->K)))) return st;
         C->del_k = F->K;
     }
     if ((st = ensure_rec(C, F->K))) return st;
@@ -1337,29 +1337,43 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         if ((st = ev_end(C, KSTAT_WIDE, e0))) return st;
         C->last_parked = parked;
     }
-    // movement roots: on graphs below the several-roots threshold, updated
-    // in place by one workgroup each (BC by the deltas that change); on
-    // larger ones rebuilt by the static pass over the whole GPU, leaving BC
-    // before the rebuild and returning after it -- there a deletion's
-    // in-place update recomputes the deltas of hundreds to thousands of
-    // ancestors with thousands of neighbours each, more than one workgroup
-    // does faster than the whole GPU rebuilds the root (R-MAT scale 20:
-    // 25 ms against 6.3).  SBC_DELETE_MOVE=rebuild or in_place (test hooks)
-    // forces either.  The update kernel adds the adjacent roots' changes
-    // itself (as insertions do).
+    // movement roots: updated in place (BC by the deltas that change), one
+    // workgroup each; on graphs of at least the several-roots threshold the
+    // delta pass -- the deltas of every ancestor of p, hundreds to thousands
+    // of vertices with thousands of neighbours each at R-MAT scale 20, 25 ms
+    // a deletion on one workgroup -- runs over the whole GPU, one launch per
+    // level.  SBC_DELETE_MOVE (test hooks): in_place (one workgroup through),
+    // in_place_wide (the delta pass over the GPU) or rebuild (the static pass
+    // rebuilds the roots, leaving BC before the rebuild and returning after
+    // it).  The update kernel adds the adjacent roots' changes itself (as
+    // insertions do).
     int32_t* mlist = C->d_del + 4 + F->K;
     const char* dm = getenv("SBC_DELETE_MOVE");
-    const bool in_place = dm ? strcmp(dm, "in_place") == 0 : G->total_cap < wide_graph_min();
-    if (h[3] > 0 && in_place) {
+    const bool rebuild = dm && strcmp(dm, "rebuild") == 0;
+    const bool wide_delta = dm ? strcmp(dm, "in_place_wide") == 0 : G->total_cap >= wide_graph_min();
+    if (h[3] > 0 && !rebuild) {
         SbcParams Q = P;
         Q.u0 = (int32_t)u;
         Q.v0 = (int32_t)v;
         Q.sroots = mlist;
         Q.scnt = C->d_del + 3;
+        if (wide_delta) Q.dm_out = C->d_del + 8 + 2 * F->K;
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
         if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
         int grid = (int)(h[3] < 2LL * C->num_cu ? h[3] : 2LL * C->num_cu);
         HIPCHK(sbc_launch_delmove(&Q, grid, C->stream));
+        if (wide_delta) {
+            std::vector<int32_t> out((size_t)2 * h[3]);
+            HIPCHK(hipMemcpyAsync(out.data(), Q.dm_out, sizeof(int32_t) * out.size(), hipMemcpyDeviceToHost,
+                                  C->stream));
+            HIPCHK(hipStreamSynchronize(C->stream));
+            int32_t top = 0;
+            for (int32_t q = 0; q < h[3]; q++) top = out[2 * q + 1] > top ? out[2 * q + 1] : top;
+            if (top >= F->lcap) return set_err(SBC_ERR_STATE, "deletion: delta pass from level %d", top);
+            const int lgrid = (int)C->num_cu;
+            for (int32_t L = top; L >= 0; L--) HIPCHK(sbc_launch_delmove_level(&Q, L, lgrid, C->stream));
+            HIPCHK(sbc_launch_delmove_finish(&Q, 4 * (int)C->num_cu, C->stream));
+        }
         if ((st = ev_end(C, KSTAT_STATIC, e0))) return st;
         C->last_rec_n = 0;
         if ((st = check_kernel_err(F, C))) return st;

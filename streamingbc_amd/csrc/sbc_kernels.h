@@ -131,6 +131,13 @@ struct SbcParams {
     // s loses p's paths (added paths -sigma[p]), and p, marked as a parent
     // once s's level is done, pulls from its old dependency less s's old term
     int32_t del;
+    // in-place deletion, movement case: when set, sbc_delmove_kernel stops
+    // after the path counts and leaves, per movement root q, its touched-list
+    // length in dm_out[2q] and the highest level of its delta pass in
+    // dm_out[2q + 1]; the delta levels and the rest are then done over the
+    // whole GPU (sbc_delmove_level_kernel, sbc_delmove_finish_kernel), with
+    // the list lengths frozen for each level launch in dm_out[2K + q]
+    int32_t* dm_out;
 };
 
 // A parked item (one per root slot), in global memory.  cnt/stat have the
@@ -210,6 +217,11 @@ hipError_t sbc_launch_bc_reduce(const SbcParams* P, hipStream_t st);
 // P->sroots[0, *P->scnt): each tree updated in place to the static pass's
 // tree of the new graph, BC by the changed deltas (P->bc_atomic).
 hipError_t sbc_launch_delmove(const SbcParams* P, int grid, hipStream_t st);
+// With P->dm_out set: the delta pass of level L of every movement root, over
+// the whole GPU (after sbc_launch_delmove, levels from the highest down),
+// then the state bytes, out-of-reach movers and scratch.
+hipError_t sbc_launch_delmove_level(const SbcParams* P, int L, int grid, hipStream_t st);
+hipError_t sbc_launch_delmove_finish(const SbcParams* P, int grid, hipStream_t st);
 // BC[v] += sign * sum over root indices list[0, *cnt) of delta_r[v] (r != v).
 hipError_t sbc_launch_bc_roots(const SbcParams* P, const int32_t* list, const int32_t* cnt, double sign,
                                hipStream_t st);

@@ -3809,6 +3809,15 @@ __global__ __launch_bounds__(NWB * 64) void sbc_delmove_kernel(SbcParams P)
         if (leader) {
             const int64_t fp = or64_wg(&vr[p].dpath, DM_LIST | DM_DEL);
             if (!(fp & DM_LIST)) list[cnt[0]++] = p;
+            if (P.dm_out) {
+                P.dm_out[2 * qi] = cnt[0];
+                P.dm_out[2 * qi + 1] = lv[2];
+            }
+        }
+        if (P.dm_out) {
+            // D and E over the whole GPU
+            __syncthreads();
+            continue;
         }
         for (int L = lv[2]; L >= 0; --L) {
             if (leader) big[0] = 0;
@@ -3923,6 +3932,131 @@ __global__ __launch_bounds__(NWB * 64) void sbc_delmove_kernel(SbcParams P)
             }
         }
         __syncthreads();
+    }
+}
+
+// The delta pass of level L of every movement root over the whole GPU (the
+// D step above, spread over every workgroup: one wavefront per vertex, 4
+// chunks of loads in flight, device-scope flags and list appends since
+// several workgroups work on one root).  Entries appended meanwhile are the
+// next level's.
+__device__ __forceinline__ int64_t or64_dev(int64_t* p, int64_t v)
+{
+    return (int64_t)__hip_atomic_fetch_or((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(1024) void sbc_delmove_level_kernel(SbcParams P, int L)
+{
+    const int lane = lane_id();
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+    const int nq = *P.scnt;
+    const int nv = P.nv;
+    const int32_t* lim = P.dm_out + 2 * P.K; // list lengths as the launch began
+    for (int q = 0; q < nq; ++q) {
+        if (L > P.dm_out[2 * q + 1]) continue;
+        // every wavefront takes the same length: entries appended meanwhile
+        // belong to the next level, and one read before its writer's store
+        // is visible would name a vertex twice
+        const int t1 = lim[q];
+        const int ri = P.sroots[q];
+        const size_t off = (size_t)ri * (size_t)nv;
+        const int32_t* dist = P.dist + off;
+        SbcVRec* vr = P.vr + off;
+        int32_t* list = P.list + off;
+        const int32_t r = P.roots[ri];
+        for (int i = gw; i < t1; i += nw) {
+            const int32_t x = list[i];
+            const int64_t fx = vr[x].dpath;
+            if (!(fx & (DM_SIGCH | DM_DEL)) || dist[x] != L) continue;
+            const SbcVInfo vi = P.vinfo[x];
+            const int64_t sx = vr[x].sigma;
+            double acc = 0.0;
+            for (int e0 = 0; e0 < vi.deg; e0 += 64 * DM_SEGU) {
+                bool ch[DM_SEGU];
+                double add[DM_SEGU];
+#pragma unroll
+                for (int u = 0; u < DM_SEGU; ++u) {
+                    const int e = e0 + u * 64 + lane;
+                    ch[u] = false;
+                    add[u] = 0.0;
+                    if (e < vi.deg) {
+                        const int32_t c = P.col[vi.start + e];
+                        if (dist[c] == L + 1) {
+                            ch[u] = true;
+                            add[u] = dep(sx, vr[c].sigma, vr[c].delta);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < DM_SEGU; ++u) acc = wave_ordered_add(acc, ch[u], add[u], false, 0.0);
+            }
+            const double od = vr[x].delta;
+            const bool changed = __double_as_longlong(acc) != __double_as_longlong(od);
+            if (lane == 0 && changed) {
+                vr[x].delta = acc;
+                if (P.bc_atomic && x != r) atomicAdd(&P.bc[x], acc - od);
+            }
+            if (!changed && !(fx & DM_SIGCH)) continue;
+            for (int e0 = 0; e0 < vi.deg; e0 += 64) {
+                const int e = e0 + lane;
+                int32_t y = 0;
+                bool app = false;
+                if (e < vi.deg) {
+                    y = P.col[vi.start + e];
+                    if (dist[y] == L - 1) app = !(or64_dev(&vr[y].dpath, DM_DEL | DM_LIST) & DM_LIST);
+                }
+                const uint64_t m = ballot(app);
+                if (m) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&P.dm_out[2 * q], __popcll(m));
+                    base = __shfl(base, 0);
+                    if (app) list[base + prefix_in_wave(m)] = y;
+                }
+            }
+        }
+    }
+}
+
+// Freezes every movement root's list length for the next level launch.
+__global__ void sbc_delmove_snap_kernel(SbcParams P)
+{
+    const int nq = *P.scnt;
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) P.dm_out[2 * P.K + q] = P.dm_out[2 * q];
+}
+
+// E of every movement root over the whole GPU: state bytes, out-of-reach
+// movers, scratch.
+__global__ void sbc_delmove_finish_kernel(SbcParams P)
+{
+    const int nq = *P.scnt;
+    const int nv = P.nv;
+    const int gt = (int)(blockIdx.x * blockDim.x + threadIdx.x), nt = (int)(gridDim.x * blockDim.x);
+    for (int q = 0; q < nq; ++q) {
+        const int t1 = P.dm_out[2 * q];
+        const int ri = P.sroots[q];
+        const size_t off = (size_t)ri * (size_t)nv;
+        const int32_t* dist = P.dist + off;
+        uint8_t* sb = P.sb + (size_t)ri * (size_t)P.sb_stride;
+        SbcVRec* vr = P.vr + off;
+        const int32_t* list = P.list + off;
+        for (int i = gt; i < t1; i += nt) {
+            const int32_t x = list[i];
+            const int32_t dx = dist[x];
+            if (dx == INF32) {
+                sb[x] = SB_INF;
+                vr[x].sigma = SIGMA_INF;
+                const double od = vr[x].delta;
+                if (od != 0.0) {
+                    if (P.bc_atomic) atomicAdd(&P.bc[x], -od);
+                    vr[x].delta = 0.0;
+                }
+            } else {
+                sb[x] = sb_make(dx, 0);
+            }
+            vr[x].dpath = 0;
+        }
     }
 }
 
@@ -4044,6 +4178,19 @@ extern "C" hipError_t sbc_launch_delete_classify(const SbcParams* P, int32_t u, 
 extern "C" hipError_t sbc_launch_delmove(const SbcParams* P, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(sbc_delmove_kernel<16>, dim3(grid), dim3(1024), 0, st, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_delmove_level(const SbcParams* P, int L, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_delmove_snap_kernel, dim3(1), dim3(256), 0, st, *P);
+    hipLaunchKernelGGL(sbc_delmove_level_kernel, dim3(grid), dim3(1024), 0, st, *P, L);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t sbc_launch_delmove_finish(const SbcParams* P, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(sbc_delmove_finish_kernel, dim3(grid), dim3(256), 0, st, *P);
     return hipGetLastError();
 }
 

@@ -601,8 +601,8 @@ def _apply_ops(f, g, ctx, o, ops):
 # 1000 rebuilds them one root at a time over the whole GPU; 0 with
 # SBC_WIDE_GRAPH_MIN = 0 all together over the whole GPU; 0 with the default
 # graph-size threshold (above these graphs) one workgroup per root
-REBUILD_MODES = {"incremental": (None, None), "single": ("1000", None), "multi": ("0", "0"),
-                 "workgroup": ("0", None)}
+REBUILD_MODES = {"incremental": (None, None), "incremental_wide": (None, None), "single": ("1000", None),
+                 "multi": ("0", "0"), "workgroup": ("0", None)}
 
 
 def _rebuild_mode(monkeypatch, mode):
@@ -610,7 +610,8 @@ def _rebuild_mode(monkeypatch, mode):
     if wide is None:
         for k in ("SBC_WIDE_REBUILD_MAX", "SBC_WIDE_GRAPH_MIN"):
             monkeypatch.delenv(k, raising=False)
-        monkeypatch.setenv("SBC_DELETE_MOVE", "in_place")
+        # one workgroup per root throughout, or the delta pass over the GPU
+        monkeypatch.setenv("SBC_DELETE_MOVE", "in_place_wide" if mode == "incremental_wide" else "in_place")
         return
     monkeypatch.setenv("SBC_DELETE_MOVE", "rebuild")
     monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
@@ -655,8 +656,9 @@ def _graph_edges(hg, n, seed):
     return out
 
 
+@pytest.mark.parametrize("mode", ["incremental", "incremental_wide"])
 @pytest.mark.parametrize("name", ["chunglu_sparse", "rmat12"])
-def test_delete_movement_in_place_matches_oracle(name, monkeypatch):
+def test_delete_movement_in_place_matches_oracle(name, mode, monkeypatch):
     """Many movement-case roots (subtrees cut off, moved down, hubs above
     them), updated in place: trees bit-identical to the oracle's static
     rebuild after every deletion batch, and the same trees as the rebuild."""
@@ -667,8 +669,8 @@ def test_delete_movement_in_place_matches_oracle(name, monkeypatch):
         hg = sbc.gen_rmat(12, 8 << 12, seed=4, keep_lcc=True)
         roots = hg.pick_roots(48, 3)
     dels = _graph_edges(hg, 48, 21)
-    print(f"[delete] in place {name} nv={hg.nv} k={len(roots)} deletions={len(dels)}", flush=True)
-    _rebuild_mode(monkeypatch, "incremental")
+    print(f"[delete] in place {name} {mode} nv={hg.nv} k={len(roots)} deletions={len(dels)}", flush=True)
+    _rebuild_mode(monkeypatch, mode)
     ctx, g, f = _engine(hg, roots)
     o = Oracle(hg.rowptr, hg.col, roots)
     o.static(PULL)
@@ -957,7 +959,7 @@ def test_batched_relayout_inside_one_window():
     assert_bc_close(f.bc(), o.bc())
 
 
-@pytest.mark.parametrize("mode", ["single", "incremental"])
+@pytest.mark.parametrize("mode", ["single", "incremental", "incremental_wide"])
 def test_wide_rebuild_of_a_root_with_a_big_hub(mode, monkeypatch):
     """Movement-case roots when a level holds a vertex with more than 16384
     neighbours: rebuilt over the whole GPU (its terms computed across the GPU
