@@ -49,7 +49,7 @@ struct SbcBucketEnt {
     double sub;
 };
 
-#define SBC_TRACE_SLOTS 80
+#define SBC_TRACE_SLOTS 112
 #define SBC_HUBQ_CAP 8192   // hubs per workgroup and pass beyond the LDS queue (global overflow queue)
 
 // Path count and dependency of one vertex in one root's tree, with the
@@ -104,7 +104,9 @@ struct SbcParams {
                           // level-2 light passes: vertices, batched vertices and edges, leftover vertices and
                           // edges, rounds, light time, hub-pass time; lookup pass sub-steps: setup, source
                           // rounds (loads, scans, staging), probes, hubs' inserted edges, 8-term sorts,
-                          // wavefront sorts
+                          // wavefront sorts; per ascent level 0..3 (deeper: 3) the time of its phases: light pass, hub set-up,
+                          // lookups, hub counters, hub scans, end of level; level-2 light vertices past the batch: with the source
+                          // set vertices, chunks, chunks with members, members, time; without it vertices, chunks, time
     int32_t* err;
     int32_t bc_atomic;
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)

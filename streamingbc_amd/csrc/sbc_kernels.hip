@@ -418,6 +418,9 @@ struct alignas(16) HubShared {
     int wcnt[16];         // terms per wavefront run (above capacity: overflow)
     uint8_t sovf[HUB_CAP];
     int runlen[16];
+    // a team scan's terms per wavefront; during a lookup pass (which uses no
+    // team scan) entry hi of the flattened rows holds lookup hub hi's old
+    // dependency (radd) and sigma (rsub, int64 bits)
     double radd[16][RUN_CAP];
     double rsub[16][RUN_CAP];
     uint8_t rflag[16][RUN_CAP];
@@ -702,12 +705,19 @@ __device__ __forceinline__ KState kstate(const SbcParams& P, const Item& X, int3
 template <typename TS>
 __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const Item& X, int32_t k, int32_t start,
                                                       int len, const KState& ks, int lev, TS& ts,
-                                                      const HubShared* srcset = nullptr)
+                                                      const HubShared* srcset = nullptr, int64_t* tp = nullptr)
 {
     const int lane = lane_id();
     double acc = ks.base;
     const int nch = (len + 63) >> 6;
+    // trace: vertices, chunks, chunks with members, members, time (source-set
+    // form); vertices, chunks, time (scan form)
+    const int64_t tv0 = (tp && lane == 0) ? (int64_t)wall_clock64() : 0;
     if (srcset) {
+        if (tp && lane == 0) {
+            atomicAdd((unsigned long long*)&tp[104], 1ull);
+            atomicAdd((unsigned long long*)&tp[105], (unsigned long long)nch);
+        }
         // neighbour ids of up to 8 chunks in flight, then chunk by chunk:
         // only members of the level's source set can give a term or need a
         // mark, and a chunk without one is skipped
@@ -722,7 +732,12 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
                 const bool ok = cc[u] >= 0 && shash_has(*srcset, cc[u]);
-                if (ballot(ok) == 0) continue;
+                const uint64_t bm = ballot(ok);
+                if (tp && lane == 0 && bm != 0) {
+                    atomicAdd((unsigned long long*)&tp[106], 1ull);
+                    atomicAdd((unsigned long long*)&tp[107], (unsigned long long)__popcll(bm));
+                }
+                if (bm == 0) continue;
                 const uint8_t ww = ok ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
                 bool appU;
                 const Term t = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok, cc[u], ww, appU);
@@ -732,7 +747,12 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
             }
         }
         if (lane == 0) st_reld(&X.vr[k].ndel, acc);
+        if (tp && lane == 0) atomicAdd((unsigned long long*)&tp[108], (unsigned long long)((int64_t)wall_clock64() - tv0));
         return;
+    }
+    if (tp && lane == 0) {
+        atomicAdd((unsigned long long*)&tp[109], 1ull);
+        atomicAdd((unsigned long long*)&tp[110], (unsigned long long)nch);
     }
     for (int ch = 0; ch < nch; ch += WAVE_UNROLL) {
         int32_t cc[WAVE_UNROLL];
@@ -765,6 +785,7 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
         for (int u = 0; u < WAVE_UNROLL; ++u) acc = wave_ordered_add(acc, t[u].child, t[u].add, t[u].hsub, t[u].sub);
     }
     if (lane == 0) st_reld(&X.vr[k].ndel, acc);
+    if (tp && lane == 0) atomicAdd((unsigned long long*)&tp[111], (unsigned long long)((int64_t)wall_clock64() - tv0));
 }
 
 template <typename TS>
@@ -1056,6 +1077,40 @@ __device__ __forceinline__ void probe_term(const SbcParams& P, const Item& X, Hu
     }
 }
 
+static_assert(16 * RUN_CAP >= HUB_CAP, "hub bases and sigmas fit the run rows");
+// Source data of a lookup pass kept in LDS by the source round (slot = the
+// thread that loaded it): ls.eflag = term kind (0 none, 1 child, 2 moved
+// vertex on the level; +4: the source is s), ls.sk_old = old sigma, ls.sk_new
+// = new sigma, ls.base = old delta, ls.eadd = new delta.  The hub's sigma is
+// hs.rsub[hi] (int64 bits) and its old dependency hs.radd[hi].
+__device__ __forceinline__ int64_t probe_hub_sigma(const HubShared& hs, int hi)
+{
+    return __builtin_bit_cast(int64_t, (&hs.rsub[0][0])[hi]);
+}
+__device__ __forceinline__ double probe_hub_base(const HubShared& hs, int hi)
+{
+    return (&hs.radd[0][0])[hi];
+}
+__device__ __forceinline__ void probe_term_pre(const Item& X, HubShared& hs, const LightShared& ls, SbcBucketEnt* bk,
+                                               int hi, int own, int pos)
+{
+    const int code = ls.eflag[own];
+    if ((code & 3) == 0) return;
+    const int64_t sk = probe_hub_sigma(hs, hi);
+    const bool child = (code & 3) == 1;
+    const bool hsub = !child || !(hs.plist[hi] == X.p && (code & 4));
+    const double add = child ? dep(sk, ls.sk_new[own], ls.eadd[own]) : 0.0;
+    const double sub = hsub ? dep(sk, ls.sk_old[own], ls.base[own]) : 0.0;
+    const int slot = atomicAdd(&hs.bcnt[hi], 1);
+    if (slot < hs.bstride) {
+        SbcBucketEnt& e = bk[(size_t)hi * hs.bstride + slot];
+        e.key = pos;
+        e.flag = (child ? 1 : 0) | (hsub ? 2 : 0);
+        e.add = add;
+        e.sub = sub;
+    }
+}
+
 __device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32_t x, int lev_parent)
 {
     const uint8_t b = ld_sb(&X.sb[x]);
@@ -1109,8 +1164,25 @@ __device__ __forceinline__ void two_hop(const SbcParams& P, const Item& X, int32
     int total;
     const int off = block_exscan<TW>(lx, ls.scan, total);
     const bool fits = len_r <= NT && total <= TWOHOP_CAP;
-    if (fits)
-        for (int e = 0; e < lx; ++e) hs.two[off + e] = P.col[xs + e];
+    // every thread copies entries: entry i belongs to the last neighbour
+    // whose offset is at most i (a neighbour with no entries shares its
+    // offset with the next one)
+    if (fits && tid < len_r) {
+        ls.off[tid] = off;
+        ls.start[tid] = xs;
+    }
+    __syncthreads();
+    if (fits) {
+        for (int i = tid; i < total; i += NT) {
+            int lo = 0, hi = len_r;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (ls.off[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            hs.two[i] = P.col[ls.start[lo] + (i - ls.off[lo])];
+        }
+    }
     if (tid == 0) hs.ntwo = fits ? total : -1;
     __syncthreads();
 }
@@ -1240,6 +1312,8 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         const int4 hb = P.hub[k];
         hs.poff[hi] = hb.x;
         hs.plog[hi] = (uint8_t)hb.z;
+        (&hs.radd[0][0])[hi] = asc_base(P, X, k);
+        (&hs.rsub[0][0])[hi] = __builtin_bit_cast(double, X.vr[k].sigma);
         int h = phash_slot(k);
         while (atomicCAS(&hs.phk[h], 0, k + 1) != 0) h = (h + 1) & (PHASH - 1);
         hs.phv[h] = hi;
@@ -1248,6 +1322,23 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[72] += x - tq; tq = x; }
     const int nb = src.below.size(), ndl = src.dB - src.dA;
     const int nsrc = src.size();
+    // inserted edges of the hubs (a few each), one thread per hub, beside the
+    // first source round (they share no data; its barriers cover them)
+    for (int hi = tid; hi < np; hi += NT) {
+        const int32_t k = hs.plist[hi];
+        const int len = hs.plen[hi];
+        const int start = hs.pstart[hi];
+        for (int e = P.hub[k].y; e < len; ++e) {
+            const int32_t c = P.col[start + e];
+            const uint8_t b = ld_sb(&X.sb[c]);
+            if (sb_type(b) == 0) {
+                if (lev >= 1 && !(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
+            } else {
+                probe_term(P, X, hs, bk, k, hi, lev, c, e);
+            }
+        }
+    }
+    if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
     for (int r0 = 0; r0 < nsrc; r0 += NT) {
         // one source per thread; short lists are scanned, long ones looked up
         const int i = r0 + tid;
@@ -1267,6 +1358,32 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             start = vi.start;
             scan_mode = !done && len <= 2 * np;
             table_mode = !done && !scan_mode;
+            // the source's state, kept in LDS so that the probes load
+            // nothing of it (slot = this thread)
+            int code = 0;
+            if (i < nb + ndl) {
+                // a processed vertex one level down or a descent vertex on the
+                // level: its term inputs, whichever hub it meets
+                const uint8_t bw = ld_sb(&X.sb[w]);
+                const int64_t so = X.vr[w].sigma;
+                const int64_t dp = ld_acq64(&X.vr[w].dpath);
+                const double nd = ld_accd(&X.vr[w].ndel);
+                const double od = X.vr[w].delta;
+                const int ct = sb_type(bw);
+                const int rel = sb_rel(bw, lev);
+                if (!(bw & SB_INF) && ct != 0) {
+                    if (rel == 1) code = 1;
+                    else if (rel == 0 && ct == 2) code = 2;
+                }
+                if (w == X.s) code |= 4;
+                ls.sk_old[tid] = so;
+                ls.sk_new[tid] = ct == 3 ? so : (int64_t)((uint64_t)dp + (uint64_t)(ct == 1 ? so : 0));
+                ls.base[tid] = od;
+                ls.eadd[tid] = nd;
+            }
+            ls.eflag[tid] = (uint8_t)code;
+            ls.k[tid] = w;
+            ls.isD[tid] = (i >= nb + ndl) ? 1 : 0;  // 1: parent candidate
         }
         // edge offsets of scan-mode sources, slots of table-mode sources and
         // compact scan-mode slots (number of scan-mode threads before), in
@@ -1276,16 +1393,11 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                                           nt, nsc);
         const int eoff = sc.x, toff = sc.y, sslot = sc.z;
         if (scan_mode) {
-            ls.k[tid] = w;
             ls.start[tid] = start;
             ls.off[tid] = eoff;
-            ls.isD[tid] = (i >= nb + ndl) ? 1 : 0;  // 1: parent candidate
             ls.len[sslot] = tid;                    // owner thread of slot
         }
-        if (table_mode) {
-            ls.ovf[toff] = w;
-            ls.eflag[toff] = (i >= nb + ndl) ? 1 : 0;
-        }
+        if (table_mode) ls.ovf[toff] = tid;
         __syncthreads();
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[73] += x - tq; tq = x; }
         // edges of scan-mode sources
@@ -1305,7 +1417,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                 mark_parent(X, ts, ws, lev - 1);
             } else {
                 const int pos = hub_find(P.htab + hs.poff[hi2], hs.plog[hi2], ws);
-                if (pos >= 0) probe_term(P, X, hs, bk, hs.plist[hi2], hi2, lev, ws, pos);
+                if (pos >= 0) probe_term_pre(X, hs, ls, bk, hi2, own, pos);
             }
         }
         // table-mode sources against every hub: PROBE_UNROLL (source, hub)
@@ -1323,13 +1435,14 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                 if (t < npairs) {
                     si[u] = t / np;
                     hi2[u] = t - si[u] * np;
-                    e[u] = P.htab[hs.poff[hi2[u]] + hub_slot(ls.ovf[si[u]], hs.plog[hi2[u]])];
+                    si[u] = ls.ovf[si[u]];  // the source's slot
+                    e[u] = P.htab[hs.poff[hi2[u]] + hub_slot(ls.k[si[u]], hs.plog[hi2[u]])];
                 }
             }
 #pragma unroll
             for (int u = 0; u < PROBE_UNROLL; ++u) {
                 if (si[u] < 0) continue;
-                const int32_t ws = ls.ovf[si[u]];
+                const int32_t ws = ls.k[si[u]];
                 int pos;
                 if (e[u].x == ws + 1) {
                     pos = e[u].y;
@@ -1341,30 +1454,14 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                                         (hub_slot(ws, lg) + 1u) & ((1u << lg) - 1u));
                 }
                 if (pos < 0) continue;
-                if (ls.eflag[si[u]]) mark_parent(X, ts, ws, lev - 1);
-                else probe_term(P, X, hs, bk, hs.plist[hi2[u]], hi2[u], lev, ws, pos);
+                if (ls.isD[si[u]]) mark_parent(X, ts, ws, lev - 1);
+                else probe_term_pre(X, hs, ls, bk, hi2[u], si[u], pos);
             }
         }
         __syncthreads();
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[74] += x - tq; tq = x; }
     }
-    // inserted edges of the hubs (a few each), one thread per hub
-    for (int hi = tid; hi < np; hi += NT) {
-        const int32_t k = hs.plist[hi];
-        const int len = hs.plen[hi];
-        const int start = hs.pstart[hi];
-        for (int e = P.hub[k].y; e < len; ++e) {
-            const int32_t c = P.col[start + e];
-            const uint8_t b = ld_sb(&X.sb[c]);
-            if (sb_type(b) == 0) {
-                if (lev >= 1 && !(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
-            } else {
-                probe_term(P, X, hs, bk, k, hi, lev, c, e);
-            }
-        }
-    }
-    if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
-    __syncthreads();
+    if (nsrc == 0) __syncthreads();
     if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[75] += x - tq; tq = x; }
     if (tp && tid == 0) tp[12] -= (int64_t)wall_clock64();  // trace: sort-and-sum time
     // buckets of at most 8 terms: eight hubs per wavefront, an 8-lane group
@@ -1386,7 +1483,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                     ad = e.add;
                     sb2 = e.sub;
                 }
-                if (gl == 0) base = asc_base(P, X, hs.plist[hi]);
+                if (gl == 0) base = probe_hub_base(hs, hi);
             }
             for (int k2 = 2; k2 <= 8; k2 <<= 1) {
                 for (int jj = k2 >> 1; jj > 0; jj >>= 1) {
@@ -1439,7 +1536,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const SbcBucketEnt* b = bk + (size_t)hi * hs.bstride;
             double acc;
             if (hs.kind[hi] == 1) {
-                acc = bucket_sum(bucket_lane(b, n), asc_base(P, X, hs.plist[hi]));
+                acc = bucket_sum(bucket_lane(b, n), probe_hub_base(hs, hi));
             } else {
                 int key[4], idx[4];
 #pragma unroll
@@ -1449,7 +1546,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                     idx[j] = e;
                 }
                 wave_sort256(key, idx);
-                acc = asc_base(P, X, hs.plist[hi]);
+                acc = probe_hub_base(hs, hi);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     int fl = 0;
@@ -1544,7 +1641,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane == 0) st_reld(&X.vr[k].ndel, lds_chain_sum(asc_base(P, X, k), row, nval));
+            if (lane == 0) st_reld(&X.vr[k].ndel, lds_chain_sum(probe_hub_base(hs, hi), row, nval));
         }
         __syncthreads();
         if (tp && tid == 0) tp[27] += (int64_t)wall_clock64() - tk;
@@ -1744,7 +1841,8 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
             ks.sk_new = ls.sk_new[sl];
             ks.base = ls.base[sl];
             const int l2 = ls.len[sl];
-            ascent_vertex_wave_ks(P, X, ls.k[sl], ls.start[sl], l2, ks, lev, ts, use_set ? &hs : nullptr);
+            ascent_vertex_wave_ks(P, X, ls.k[sl], ls.start[sl], l2, ks, lev, ts, use_set ? &hs : nullptr,
+                                  lev == 2 ? tp : nullptr);
             my_proc += 1;
             my_scan += (uint32_t)l2;
         }
@@ -2043,8 +2141,17 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         const int nd = dB - dA, ns = sB - sA, nu = uB - uA;
         const int tot = nd + ns + nu;
         if (tot == 0) break;
-        int64_t tl0 = 0;
-        if (tp && leader) { tl0 = (int64_t)wall_clock64(); tp[11] += 1; }
+        int64_t tl0 = 0, tph = 0;
+        if (tp && leader) { tl0 = (int64_t)wall_clock64(); tph = tl0; tp[11] += 1; }
+        // trace: phases of the level (light pass, hub set-up, lookups, hub
+        // counters, hub scans, end of level), levels 0..3 (deeper: 3)
+        auto phase = [&](int ph) {
+            if (tp && leader) {
+                const int64_t x = (int64_t)wall_clock64();
+                tp[80 + 6 * (lev < 3 ? lev : 3) + ph] += x - tph;
+                tph = x;
+            }
+        };
         if constexpr (TW > 1) {
             // levels 1 and 2: every vertex one level up is known (the root, its
             // neighbours), so the level's source set can be built when the
@@ -2069,6 +2176,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 tp[8] += (int64_t)wall_clock64() - tl0;
                 if (lev == 2) tp[70] += (int64_t)wall_clock64() - tl0;
             }
+            phase(0);
             const int nall = hubs_queued(P, *hs);
             for (int hb = 0; hb < nall; hb += hub_batch(P)) {
                 const int nh = min(nall - hb, hub_batch(P));
@@ -2138,7 +2246,9 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                 }
                 __syncthreads();
                 const int np = hs->np;
+                phase(1);
                 if (np > 0) ascent_hubs_probe<TW>(P, X, np, lev, ts, *hs, *ls, src, tp);
+                phase(2);
                 if (tp && leader) {
                     const int64_t t = (int64_t)wall_clock64();
                     tp[9] += t - tl1;
@@ -2175,6 +2285,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     }
                 }
                 __syncthreads();
+                phase(3);
                 if (hb == 0 && P.park && hs->svol >= P.park_scan && hs->n <= hub_batch(P) + (P.hubq ? SBC_HUBQ_CAP : 0)) {
                     // the pass's hubs left to scan are too much for one workgroup:
                     // take back the pass's hub counts (the wide kernels count the
@@ -2203,6 +2314,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     tp[40 + (lev < 7 ? lev : 7)] += t - tl1;
                     tp[48 + (lev < 7 ? lev : 7)] += nsh;
                 }
+                phase(4);
             }
             if (leader) { hs->n = 0; hs->svol = 0; }
             __syncthreads();
@@ -2221,6 +2333,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
             tp[16 + ls7] += (int64_t)wall_clock64() - tl0;
             if (lev == 2) tp[71] += (int64_t)wall_clock64() - tl0;
         }
+        phase(5);
         below = here;
         uA = uB;
         uB = ts.cnt[2];
