@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "sbc_kernels.h"
 
@@ -337,7 +338,15 @@ __device__ __forceinline__ double dep(int64_t sa, int64_t sb, double d)
 struct TeamShared {
     int cnt[4];                 // 0: |D|, 1: |S|, 2: end of U list, 3: pop result
     unsigned long long stat[4]; // processed, scanned, moved, adjacency entries scanned by the descent
+    unsigned long long tr[12];  // trace counters of the item (LDS atomics), copied to the trace at its end
 };
+
+// Trace counter i of the item (LDS atomic); wide kernels keep no trace.
+template <typename TS>
+__device__ __forceinline__ void tr_add(TS& ts, int i, unsigned long long v)
+{
+    if constexpr (std::is_same<TS, TeamShared>::value) atomicAdd(&ts.tr[i], v);
+}
 
 // ------------------------------------------------------------ update item
 
@@ -372,6 +381,8 @@ constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by o
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
 constexpr int PHASH = 2 * HUB_CAP;      // LDS hash: vertex -> number in the level's lookup list
+constexpr int SBITS_LOG = 14;           // bits of the membership filter beside the hash
+constexpr int SBITS_WORDS = (1 << SBITS_LOG) / 32;
 constexpr int TWOHOP_CAP = 1024;        // two-hop set of the root kept for level-3 lookups
 // Lookup buckets: a pass's SBC_BUCKET_SLOTS term slots are shared out evenly
 // between its lookup hubs, at most TEAM_SORT_CAP per hub; a bucket of up to
@@ -407,6 +418,7 @@ struct alignas(16) HubShared {
     int phk[PHASH];       // hash keys (vertex + 1, 0 = empty)
     int phv[PHASH];       // hash values (number in plist)
     int pidx[HUB_CAP];    // queued hub -> number in plist, or -1
+    uint32_t sbits[SBITS_WORDS];  // membership filter of phk's keys (sbit_set)
     uint8_t kind[HUB_CAP];
     int ntwo;                 // entries of two (-1: the root's two-hop set is too large)
     int32_t two[TWOHOP_CAP];  // the root's two-hop set: parent candidates of level-3 hubs
@@ -427,6 +439,20 @@ struct alignas(16) HubShared {
 };
 
 __device__ __forceinline__ int phash_slot(int32_t v) { return (int)(((uint32_t)v * 0x9E3779B1u) >> 21) & (PHASH - 1); }
+// A bit per value of a second hash of the vertex id, set for every member
+// of phk: a clear bit rules a vertex out with one LDS load, without the
+// probe sequence (one hash measured faster than two or three).
+__device__ __forceinline__ uint32_t sbit_of(int32_t v) { return ((uint32_t)v * 0x85EBCA6Bu) >> (32 - SBITS_LOG); }
+__device__ __forceinline__ void sbit_set(HubShared& hs, int32_t v)
+{
+    const uint32_t b = sbit_of(v);
+    atomicOr(&hs.sbits[b >> 5], 1u << (b & 31));
+}
+__device__ __forceinline__ bool sbit_test(const HubShared& hs, int32_t v)
+{
+    const uint32_t b = sbit_of(v);
+    return (hs.sbits[b >> 5] >> (b & 31)) & 1u;
+}
 
 // Membership set of an ascent level's possible sources, in hs.phk: every
 // vertex that can give a light vertex of the level a term (the processed
@@ -437,6 +463,7 @@ constexpr int SHASH_MAX = PHASH / 2;
 
 __device__ __forceinline__ void shash_insert(HubShared& hs, int32_t v)
 {
+    sbit_set(hs, v);
     int h = phash_slot(v);
     for (;;) {
         const int old = atomicCAS(&hs.phk[h], 0, v + 1);
@@ -447,6 +474,7 @@ __device__ __forceinline__ void shash_insert(HubShared& hs, int32_t v)
 
 __device__ __forceinline__ bool shash_has(const HubShared& hs, int32_t v)
 {
+    if (!sbit_test(hs, v)) return false;
     int h = phash_slot(v);
     for (;;) {
         const int key = hs.phk[h];
@@ -715,8 +743,8 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     const int64_t tv0 = (tp && lane == 0) ? (int64_t)wall_clock64() : 0;
     if (srcset) {
         if (tp && lane == 0) {
-            atomicAdd((unsigned long long*)&tp[104], 1ull);
-            atomicAdd((unsigned long long*)&tp[105], (unsigned long long)nch);
+            tr_add(ts, 4, 1ull);
+            tr_add(ts, 5, (unsigned long long)nch);
         }
         // neighbour ids of up to 8 chunks in flight, then chunk by chunk:
         // only members of the level's source set can give a term or need a
@@ -734,8 +762,8 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
                 const bool ok = cc[u] >= 0 && shash_has(*srcset, cc[u]);
                 const uint64_t bm = ballot(ok);
                 if (tp && lane == 0 && bm != 0) {
-                    atomicAdd((unsigned long long*)&tp[106], 1ull);
-                    atomicAdd((unsigned long long*)&tp[107], (unsigned long long)__popcll(bm));
+                    tr_add(ts, 6, 1ull);
+                    tr_add(ts, 7, (unsigned long long)__popcll(bm));
                 }
                 if (bm == 0) continue;
                 const uint8_t ww = ok ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
@@ -747,12 +775,12 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
             }
         }
         if (lane == 0) st_reld(&X.vr[k].ndel, acc);
-        if (tp && lane == 0) atomicAdd((unsigned long long*)&tp[108], (unsigned long long)((int64_t)wall_clock64() - tv0));
+        if (tp && lane == 0) tr_add(ts, 8, (unsigned long long)((int64_t)wall_clock64() - tv0));
         return;
     }
     if (tp && lane == 0) {
-        atomicAdd((unsigned long long*)&tp[109], 1ull);
-        atomicAdd((unsigned long long*)&tp[110], (unsigned long long)nch);
+        tr_add(ts, 9, 1ull);
+        tr_add(ts, 10, (unsigned long long)nch);
     }
     for (int ch = 0; ch < nch; ch += WAVE_UNROLL) {
         int32_t cc[WAVE_UNROLL];
@@ -785,7 +813,7 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
         for (int u = 0; u < WAVE_UNROLL; ++u) acc = wave_ordered_add(acc, t[u].child, t[u].add, t[u].hsub, t[u].sub);
     }
     if (lane == 0) st_reld(&X.vr[k].ndel, acc);
-    if (tp && lane == 0) atomicAdd((unsigned long long*)&tp[111], (unsigned long long)((int64_t)wall_clock64() - tv0));
+    if (tp && lane == 0) tr_add(ts, 11, (unsigned long long)((int64_t)wall_clock64() - tv0));
 }
 
 template <typename TS>
@@ -1121,6 +1149,7 @@ __device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32
 
 __device__ __forceinline__ int phash_find(const HubShared& hs, int32_t v)
 {
+    if (!sbit_test(hs, v)) return -1;
     int h = phash_slot(v);
     for (;;) {
         const int key = hs.phk[h];
@@ -1300,6 +1329,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
     int64_t tq = 0;  // trace: sub-step clock
     if (tp && tid == 0) tq = (int64_t)wall_clock64();
     for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
+    for (int i = tid; i < SBITS_WORDS; i += NT) hs.sbits[i] = 0;
     for (int i = tid; i < np; i += NT) hs.bcnt[i] = 0;
     if (tid == 0) {
         const int share = (SBC_BUCKET_SLOTS / np) & ~63;
@@ -1317,6 +1347,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
         int h = phash_slot(k);
         while (atomicCAS(&hs.phk[h], 0, k + 1) != 0) h = (h + 1) & (PHASH - 1);
         hs.phv[h] = hi;
+        sbit_set(hs, k);
     }
     __syncthreads();
     if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[72] += x - tq; tq = x; }
@@ -1768,6 +1799,7 @@ __device__ __forceinline__ void ascent_light_level(const SbcParams& P, const Ite
         if (use_set && !set_built) {
             set_built = true;
             for (int i = tid; i < PHASH; i += NT) hs.phk[i] = 0;
+            for (int i = tid; i < SBITS_WORDS; i += NT) hs.sbits[i] = 0;
             __syncthreads();
             const int nbl = ss.below.size(), ndl = ss.dB - ss.dA;
             for (int i = tid; i < nsrc; i += NT) {
@@ -2009,6 +2041,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         const int64_t t0 = (int64_t)wall_clock64();
         tp[0] = t0; tp[1] = t0; tp[2] = t0; tp[3] = t0;
         for (int i = 4; i < SBC_TRACE_SLOTS; ++i) tp[i] = 0;
+        for (int i = 0; i < 12; ++i) ts.tr[i] = 0;
     }
     if (gap == 0) {
         if (rec && leader) {
@@ -2268,8 +2301,8 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     const bool looked_up = hi >= 0 && hs->kind[hi] != 2;
                     const int len = hs->hlen[h];
                     if (tp) {
-                        atomicAdd((unsigned long long*)&tp[looked_up ? 4 : 5], 1ull);
-                        atomicAdd((unsigned long long*)&tp[looked_up ? 6 : 7], (unsigned long long)len);
+                        atomicAdd(&ts.tr[looked_up ? 0 : 1], 1ull);
+                        atomicAdd(&ts.tr[looked_up ? 2 : 3], (unsigned long long)len);
                     }
                     if (!looked_up) {
                         const int j = atomicAdd(&hs->ns, 1);
@@ -2380,7 +2413,11 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         atomicAdd(&ts.stat[2], my_moved);
     }
     team_sync<TW>();
-    if (tp && leader) tp[3] = (int64_t)wall_clock64();
+    if (tp && leader) {
+        tp[3] = (int64_t)wall_clock64();
+        for (int i = 0; i < 4; ++i) tp[4 + i] = (int64_t)ts.tr[i];
+        for (int i = 4; i < 12; ++i) tp[100 + i] = (int64_t)ts.tr[i];
+    }
     if (rec && leader) {
         rec->root_index = ri;
         rec->insertion = item;
@@ -2395,6 +2432,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     team_sync<TW>();
     return false;
 }
+
 
 // Persistent window kernel.  Phase 1: whole-workgroup teams pop root slots
 // [0, heavy) of the queue order; phase 2: each wavefront pops slots
