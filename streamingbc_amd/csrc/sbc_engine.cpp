@@ -89,6 +89,31 @@ struct stinger {
 // Vertices with more initial neighbours than this get a sorted index (the
 // update kernel looks neighbours up in it instead of scanning hub lists).
 static const int32_t kHubIndexMin = 512;
+// Lookup table of a high-degree vertex: 2^logT int2 slots {neighbour + 1,
+// position}, at most a quarter used, in buckets of 8 slots (64 bytes, one
+// cache line, the table 8-slot aligned); a neighbour takes the first free
+// slot of bucket hash(w) or of the buckets after it.  A device lookup reads
+// a whole bucket at once and stops at a bucket with a free slot.
+static inline int hub_table_log(int64_t d)
+{
+    int logT = 3;
+    while ((1LL << logT) < 4 * d) logT++;
+    return logT;
+}
+static inline void hub_table_put(int2* tab, int logT, int32_t w, int32_t pos)
+{
+    const uint32_t mask = (1u << (logT - 3)) - 1u;
+    uint32_t b = ((uint32_t)w * 0x9E3779B1u) >> (32 - (logT - 3));
+    for (;;) {
+        int2* bk = tab + ((size_t)b << 3);
+        for (int i = 0; i < 8; i++)
+            if (bk[i].x == 0) {
+                bk[i] = int2{w + 1, pos};
+                return;
+            }
+        b = (b + 1) & mask;
+    }
+}
 static const int64_t kBucketHubs = 512; // HUB_CAP of the kernels
 static const int64_t kParkScan = 512 * 1024; // adjacency entries left to scan that hand an ascent level over
 
@@ -130,18 +155,11 @@ static sbcStatus graph_init(stinger* G, int64_t nv, const int64_t* rowptr, const
     for (int64_t v = 0; v < nv; v++) {
         int32_t d = G->deg[v];
         if (d <= kHubIndexMin || d >= (1 << 22)) continue;
-        int logT = 1;
-        while ((1 << logT) < 2 * d) logT++;
-        const uint32_t mask = (1u << logT) - 1u;
-        int64_t off = (int64_t)G->htab.size();
-        if (off + (1 << logT) >= INT32_MAX) break;
-        G->htab.resize(off + (1 << logT), int2{0, 0});
-        for (int32_t i = 0; i < d; i++) {
-            const int32_t w = G->col[G->start[v] + i];
-            uint32_t h = ((uint32_t)w * 0x9E3779B1u) >> (32 - logT);
-            while (G->htab[off + h].x != 0) h = (h + 1) & mask;
-            G->htab[off + h] = int2{w + 1, i};
-        }
+        const int logT = hub_table_log(d);
+        const int64_t off = ((int64_t)G->htab.size() + 7) & ~(int64_t)7;
+        if (off + (1LL << logT) >= INT32_MAX) break;
+        G->htab.resize(off + ((size_t)1 << logT), int2{0, 0});
+        for (int32_t i = 0; i < d; i++) hub_table_put(&G->htab[off], logT, G->col[G->start[v] + i], i);
         G->hub[v] = int4{(int32_t)off, d, logT, 0};
     }
     G->full_dirty = true;
@@ -374,12 +392,7 @@ static void graph_remove_at(stinger* G, int32_t v, int32_t p)
         const uint32_t mask = (1u << hb.z) - 1u;
         int2* tab = &G->htab[hb.x];
         for (uint32_t i = 0; i <= mask; i++) tab[i] = int2{0, 0};
-        for (int32_t i = 0; i < n; i++) {
-            const int32_t w = G->col[s0 + i];
-            uint32_t h = ((uint32_t)w * 0x9E3779B1u) >> (32 - hb.z);
-            while (tab[h].x != 0) h = (h + 1) & mask;
-            tab[h] = int2{w + 1, i};
-        }
+        for (int32_t i = 0; i < n; i++) hub_table_put(tab, hb.z, G->col[s0 + i], i);
         hb.y = n;
         G->p_hub.push_back(v);
     }
@@ -480,9 +493,7 @@ extern "C" int64_t streamingBCResidentFootprint(const int64_t* rowptr, int64_t N
         const int64_t d = rowptr[v + 1] - rowptr[v];
         slots += d + slack_for((int32_t)d);
         if (d > kHubIndexMin && d < (1 << 22)) {
-            int64_t t = 2;
-            while (t < 2 * d) t <<= 1;
-            htab += t;
+            htab += ((int64_t)1 << hub_table_log(d)) + 7;
         }
     }
     int64_t graph = slots * 8 + NV * (int64_t)sizeof(SbcVInfo) + NV * 16 + htab * 8;
@@ -2633,7 +2644,8 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
         const int4 h = G->hub[v];
         ok = G->start[v] >= 0 && G->cap[v] >= 0 && G->deg[v] >= 0 && G->deg0[v] >= 0 && G->deg0[v] <= G->deg[v] &&
              G->deg[v] <= G->cap[v] && (int64_t)G->start[v] + G->cap[v] <= slots &&
-             (h.y == 0 || (h.z > 0 && h.z < 31 && h.x >= 0 && (int64_t)h.x + (1LL << h.z) <= tslots && h.y <= G->deg[v]));
+             (h.y == 0 || (h.z >= 3 && h.z < 31 && h.x >= 0 && (h.x & 7) == 0 &&
+                           (int64_t)h.x + (1LL << h.z) <= tslots && h.y <= G->deg[v]));
     }
     for (int64_t i = 0; ok && i < K; i++) ok = roots[(size_t)i] >= 0 && roots[(size_t)i] < nv;
     for (int64_t e = 0; ok && e < slots; e++) ok = G->col[(size_t)e] >= 0 && G->col[(size_t)e] < nv;

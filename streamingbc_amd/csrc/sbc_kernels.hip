@@ -209,6 +209,20 @@ __device__ __forceinline__ uint8_t cas_sb(uint8_t* p, uint8_t expect, uint8_t wa
     }
 }
 
+#ifdef SBC_EXP_MARKOR
+// Marks a vertex as an ascent-only parent (type 3) with one atomic OR of
+// its type bits: its state byte must have type 0 or 3 and already hold the
+// parent level (a neighbour one level up that no descent touched).  True if
+// this call set the type.
+__device__ __forceinline__ bool mark_sb3(uint8_t* p)
+{
+    unsigned int* w = (unsigned int*)((uintptr_t)p & ~(uintptr_t)3);
+    const int sh = (int)((uintptr_t)p & 3) * 8;
+    const unsigned int old = __hip_atomic_fetch_or(w, 3u << sh, __ATOMIC_RELAXED, WG_SCOPE);
+    return ((old >> sh) & 3u) == 0u;
+}
+#endif
+
 // Number of leading slots of a vertex visible to an item with sequence q:
 // slots below deg0 are always visible; appended slots carry increasing
 // sequence numbers, so the visible ones form a prefix.
@@ -373,10 +387,7 @@ constexpr int WAVE_UNROLL = SBC_WAVE_UNROLL;
 #define SBC_LOOKUP_MIN_LEV 0
 #endif
 // (source, hub) lookups whose first table slots one thread loads together
-#ifndef SBC_PROBE_UNROLL
-#define SBC_PROBE_UNROLL 2
-#endif
-constexpr int PROBE_UNROLL = SBC_PROBE_UNROLL;
+
 constexpr int HUB_CAP = 512;   // hubs queued per level pass (more: scanned by one wave)
 constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 
@@ -690,7 +701,11 @@ __device__ __forceinline__ Term ascent_lane(const SbcParams& P, const Item& X, i
             t.sub = dep(sk_old, X.vr[c].sigma, X.vr[c].delta);
         }
     } else if (lev >= 1 && rel == -1) {
+#ifdef SBC_EXP_MARKOR
+        appU = mark_sb3(&X.sb[c]);
+#else
         appU = (cas_sb(&X.sb[c], bc, sb_make(lev - 1, 3)) == bc);
+#endif
     }
     return t;
 }
@@ -1051,24 +1066,42 @@ __device__ __forceinline__ void ascent_hubs_flat(const SbcParams& P, const Item&
 }
 
 // Position of w in the indexed (initial) part of a vertex's adjacency list,
-// looked up in the vertex's open-addressing table, or -1.
-__device__ __forceinline__ uint32_t hub_slot(int32_t w, int logT) { return ((uint32_t)w * 0x9E3779B1u) >> (32 - logT); }
-
-// The same search from slot h on (linear probing).
-__device__ __forceinline__ int hub_find_from(const int2* __restrict__ tab, int logT, int32_t w, uint32_t h)
+// looked up in the vertex's table, or -1.  A table is 2^logT slots {key + 1,
+// position}, at most a quarter used, in buckets of 8 slots (64 bytes, one
+// load) that fill in order: w is in bucket hash(w), or in a later one when
+// every bucket from hash(w) on is full, so a bucket with a free slot ends
+// the search.
+__device__ __forceinline__ uint32_t hub_bucket(int32_t w, int logT)
 {
-    const uint32_t mask = (1u << logT) - 1u;
+    return ((uint32_t)w * 0x9E3779B1u) >> (32 - (logT - 3));
+}
+// w + 1 among the bucket's 8 keys: its position, -1 (a free slot: absent)
+// or -2 (full without it: the next bucket)
+__device__ __forceinline__ int bucket_match(const int4& a, const int4& b, const int4& c, const int4& d, int32_t key)
+{
+    const int k[8] = {a.x, a.z, b.x, b.z, c.x, c.z, d.x, d.z};
+    const int p[8] = {a.y, a.w, b.y, b.w, c.y, c.w, d.y, d.w};
+    int r = -2;
+#pragma unroll
+    for (int i = 7; i >= 0; --i) {
+        if (k[i] == 0) r = -1;
+        if (k[i] == key) r = p[i];
+    }
+    return r;
+}
+__device__ __forceinline__ int hub_find_bucket(const int2* __restrict__ tab, int logT, int32_t w, uint32_t b)
+{
+    const uint32_t mask = (1u << (logT - 3)) - 1u;
     for (;;) {
-        const int2 e = tab[h];
-        if (e.x == w + 1) return e.y;
-        if (e.x == 0) return -1;
-        h = (h + 1) & mask;
+        const int4* q = reinterpret_cast<const int4*>(tab + ((size_t)b << 3));
+        const int r = bucket_match(q[0], q[1], q[2], q[3], w + 1);
+        if (r != -2) return r;
+        b = (b + 1) & mask;
     }
 }
-
 __device__ __forceinline__ int hub_find(const int2* __restrict__ tab, int logT, int32_t w)
 {
-    return hub_find_from(tab, logT, w, hub_slot(w, logT));
+    return hub_find_bucket(tab, logT, w, hub_bucket(w, logT));
 }
 
 // Processed-vertex ranges of one ascent level: descent frontier list[dA..dB),
@@ -1139,11 +1172,18 @@ __device__ __forceinline__ void probe_term_pre(const Item& X, HubShared& hs, con
     }
 }
 
+// x is on level lev_parent and of type 0 or 3 (callers: neighbours one
+// level up of a processed vertex, parent candidates no descent touched)
 __device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32_t x, int lev_parent)
 {
+#ifdef SBC_EXP_MARKOR
+    (void)lev_parent;
+    if (mark_sb3(&X.sb[x])) X.list[atomicAdd(&ts.cnt[2], 1)] = x;
+#else
     const uint8_t b = ld_sb(&X.sb[x]);
     if (sb_type(b) == 0 && cas_sb(&X.sb[x], b, sb_make(lev_parent, 3)) == b)
         X.list[atomicAdd(&ts.cnt[2], 1)] = x;
+#endif
 }
 
 
@@ -1451,43 +1491,24 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                 if (pos >= 0) probe_term_pre(X, hs, ls, bk, hi2, own, pos);
             }
         }
-        // table-mode sources against every hub: PROBE_UNROLL (source, hub)
-        // pairs per thread, whose first table slots are loaded together
+        // table-mode sources against every hub, one (source, hub) pair per
+        // thread: the whole bucket of the source in the hub's table in one
+        // load (64 bytes), the next bucket only if that one is full
         const int npairs = nt * np;
-        for (int t0 = tid; t0 < npairs; t0 += PROBE_UNROLL * NT) {
-            int si[PROBE_UNROLL], hi2[PROBE_UNROLL];
-            int2 e[PROBE_UNROLL];
-#pragma unroll
-            for (int u = 0; u < PROBE_UNROLL; ++u) {
-                const int t = t0 + u * NT;
-                si[u] = -1;
-                hi2[u] = 0;
-                e[u] = int2{0, 0};
-                if (t < npairs) {
-                    si[u] = t / np;
-                    hi2[u] = t - si[u] * np;
-                    si[u] = ls.ovf[si[u]];  // the source's slot
-                    e[u] = P.htab[hs.poff[hi2[u]] + hub_slot(ls.k[si[u]], hs.plog[hi2[u]])];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < PROBE_UNROLL; ++u) {
-                if (si[u] < 0) continue;
-                const int32_t ws = ls.k[si[u]];
-                int pos;
-                if (e[u].x == ws + 1) {
-                    pos = e[u].y;
-                } else if (e[u].x == 0) {
-                    pos = -1;
-                } else {
-                    const int lg = hs.plog[hi2[u]];
-                    pos = hub_find_from(P.htab + hs.poff[hi2[u]], lg, ws,
-                                        (hub_slot(ws, lg) + 1u) & ((1u << lg) - 1u));
-                }
-                if (pos < 0) continue;
-                if (ls.isD[si[u]]) mark_parent(X, ts, ws, lev - 1);
-                else probe_term_pre(X, hs, ls, bk, hi2[u], si[u], pos);
-            }
+        for (int t = tid; t < npairs; t += NT) {
+            const int sj = t / np;
+            const int hi2 = t - sj * np;
+            const int si = ls.ovf[sj];  // the source's slot
+            const int32_t ws = ls.k[si];
+            const int lg = hs.plog[hi2];
+            const int2* tab = P.htab + hs.poff[hi2];
+            const uint32_t b0 = hub_bucket(ws, lg);
+            const int4* q = reinterpret_cast<const int4*>(tab + ((size_t)b0 << 3));
+            int pos = bucket_match(q[0], q[1], q[2], q[3], ws + 1);
+            if (pos == -2) pos = hub_find_bucket(tab, lg, ws, (b0 + 1u) & ((1u << (lg - 3)) - 1u));
+            if (pos < 0) continue;
+            if (ls.isD[si]) mark_parent(X, ts, ws, lev - 1);
+            else probe_term_pre(X, hs, ls, bk, hi2, si, pos);
         }
         __syncthreads();
         if (tp && tid == 0) { const int64_t x = (int64_t)wall_clock64(); tp[74] += x - tq; tq = x; }
