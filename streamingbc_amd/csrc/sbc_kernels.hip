@@ -209,7 +209,6 @@ __device__ __forceinline__ uint8_t cas_sb(uint8_t* p, uint8_t expect, uint8_t wa
     }
 }
 
-#ifdef SBC_EXP_MARKOR
 // Marks a vertex as an ascent-only parent (type 3) with one atomic OR of
 // its type bits: its state byte must have type 0 or 3 and already hold the
 // parent level (a neighbour one level up that no descent touched).  True if
@@ -221,7 +220,6 @@ __device__ __forceinline__ bool mark_sb3(uint8_t* p)
     const unsigned int old = __hip_atomic_fetch_or(w, 3u << sh, __ATOMIC_RELAXED, WG_SCOPE);
     return ((old >> sh) & 3u) == 0u;
 }
-#endif
 
 // Number of leading slots of a vertex visible to an item with sequence q:
 // slots below deg0 are always visible; appended slots carry increasing
@@ -701,11 +699,7 @@ __device__ __forceinline__ Term ascent_lane(const SbcParams& P, const Item& X, i
             t.sub = dep(sk_old, X.vr[c].sigma, X.vr[c].delta);
         }
     } else if (lev >= 1 && rel == -1) {
-#ifdef SBC_EXP_MARKOR
         appU = mark_sb3(&X.sb[c]);
-#else
-        appU = (cas_sb(&X.sb[c], bc, sb_make(lev - 1, 3)) == bc);
-#endif
     }
     return t;
 }
@@ -1172,18 +1166,13 @@ __device__ __forceinline__ void probe_term_pre(const Item& X, HubShared& hs, con
     }
 }
 
-// x is on level lev_parent and of type 0 or 3 (callers: neighbours one
-// level up of a processed vertex, parent candidates no descent touched)
-__device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32_t x, int lev_parent)
+// Lists x as an ascent-only parent unless it is already marked.  x is one
+// level above the level being ascended and of type 0 or 3 (callers: a
+// neighbour one level up of a processed vertex, a parent candidate that no
+// descent touched, the root, a deletion's shallower endpoint).
+__device__ __forceinline__ void mark_parent(const Item& X, TeamShared& ts, int32_t x)
 {
-#ifdef SBC_EXP_MARKOR
-    (void)lev_parent;
     if (mark_sb3(&X.sb[x])) X.list[atomicAdd(&ts.cnt[2], 1)] = x;
-#else
-    const uint8_t b = ld_sb(&X.sb[x]);
-    if (sb_type(b) == 0 && cas_sb(&X.sb[x], b, sb_make(lev_parent, 3)) == b)
-        X.list[atomicAdd(&ts.cnt[2], 1)] = x;
-#endif
 }
 
 
@@ -1403,13 +1392,13 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const int32_t c = P.col[start + e];
             const uint8_t b = ld_sb(&X.sb[c]);
             if (sb_type(b) == 0) {
-                if (lev >= 1 && !(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c, lev - 1);
+                if (lev >= 1 && !(b & SB_INF) && sb_rel(b, lev) == -1) mark_parent(X, ts, c);
             } else {
                 probe_term(P, X, hs, bk, k, hi, lev, c, e);
             }
         }
     }
-    if (lev == 1 && tid == 0) mark_parent(X, ts, X.r, 0);
+    if (lev == 1 && tid == 0) mark_parent(X, ts, X.r);
     for (int r0 = 0; r0 < nsrc; r0 += NT) {
         // one source per thread; short lists are scanned, long ones looked up
         const int i = r0 + tid;
@@ -1485,7 +1474,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             const int hi2 = phash_find(hs, c);
             if (hi2 < 0) continue;
             if (ls.isD[own]) {
-                mark_parent(X, ts, ws, lev - 1);
+                mark_parent(X, ts, ws);
             } else {
                 const int pos = hub_find(P.htab + hs.poff[hi2], hs.plog[hi2], ws);
                 if (pos >= 0) probe_term_pre(X, hs, ls, bk, hi2, own, pos);
@@ -1507,7 +1496,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             int pos = bucket_match(q[0], q[1], q[2], q[3], ws + 1);
             if (pos == -2) pos = hub_find_bucket(tab, lg, ws, (b0 + 1u) & ((1u << (lg - 3)) - 1u));
             if (pos < 0) continue;
-            if (ls.isD[si]) mark_parent(X, ts, ws, lev - 1);
+            if (ls.isD[si]) mark_parent(X, ts, ws);
             else probe_term_pre(X, hs, ls, bk, hi2, si, pos);
         }
         __syncthreads();
@@ -2380,7 +2369,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
         }
         // a deletion's p lost its child s: it is a parent to update even if
         // no scan of this level reaches it (its pull starts from asc_base)
-        if (P.del && lev == base && leader) mark_parent(X, ts, X.p, lev - 1);
+        if (P.del && lev == base && leader) mark_parent(X, ts, X.p);
         team_sync<TW>();
         if (tp && leader) {
             const int ls7 = lev < 7 ? lev : 7;
