@@ -430,6 +430,8 @@ struct alignas(16) HubShared {
     uint32_t sbits[SBITS_WORDS];  // membership filter of phk's keys (sbit_set)
     uint8_t kind[HUB_CAP];
     int ntwo;                 // entries of two (-1: the root's two-hop set is too large)
+    int two_ok;               // two/ntwo hold the current root's set (cleared when a root is popped and
+                              // when an item's edge ends at the root or at one of its neighbours)
     int32_t two[TWOHOP_CAP];  // the root's two-hop set: parent candidates of level-3 hubs
     // flat scan of the staged hubs: segment i = j + w holds the terms of hub
     // j found by wavefront w, entries [seg_b, seg_e) of w's run
@@ -2042,6 +2044,10 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     X.q = (P.win == nullptr) ? INF32 : P.win->seq0 + item;
 
     const int32_t du = X.dist[u], dv = X.dist[v];
+    // an inserted edge at the root or at one of its neighbours changes the
+    // root's two-hop set
+    if constexpr (TW > 1)
+        if (leader && (du <= 1 || dv <= 1)) hs->two_ok = 0;
     int64_t gap;
     if (du == INF32 && dv == INF32) gap = 0;
     else gap = (int64_t)du - (int64_t)dv;
@@ -2175,7 +2181,6 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     LevelSets below{0, 0, 0, 0, 0, 0};
     SbcVInfo vr = P.vinfo[X.r];
     const int len_r = visible_len(P, vr, X.q);
-    bool two_done = false;  // two-hop parent candidates computed for this item
     for (int lev = base + nf - 1; lev >= 0; --lev) {
         const int jj = lev - base;
         int dA = 0, dB = 0, sA = 0, sB = 0;
@@ -2241,9 +2246,11 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     src.pc = P.col + vr.start;
                     src.nr = len_r;
                 } else if (lev == 3 && P.hub && P.bucket) {
-                    if (!two_done) {
+                    // the root's two-hop set, computed once while the edges
+                    // that the items add leave it as it is
+                    if (!hs->two_ok) {
                         two_hop<TW>(P, X, vr.start, len_r, *hs, *ls);
-                        two_done = true;
+                        if (threadIdx.x == 0) hs->two_ok = 1;
                     }
                     if (hs->ntwo >= 0) {
                         src.pc = hs->two;
@@ -2459,6 +2466,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
     // phase 1: heavy
     for (;;) {
         if (threadIdx.x == 0) ts[0].cnt[3] = atomicAdd(&P.qctr[0], 1);
+        if (threadIdx.x == 0) hs.two_ok = 0;
         __syncthreads();
         const int t = ts[0].cnt[3];
         __syncthreads();
