@@ -1280,17 +1280,18 @@ __device__ __forceinline__ double shfl_d(double x, int src)
 // element e = j * 64 + lane in key[j], idx[j]: a bitonic network whose
 // strides below 64 exchange between lanes and whose strides of 64 and 128
 // exchange between a lane's registers.  Unused elements carry INF32 keys.
-__device__ __forceinline__ void wave_sort256(int (&key)[4], int (&idx)[4])
+template <int J>
+__device__ __forceinline__ void wave_sort_keys(int (&key)[J], int (&idx)[J])
 {
     const int lane = lane_id();
 #pragma unroll
-    for (int k2 = 2; k2 <= 256; k2 <<= 1) {
+    for (int k2 = 2; k2 <= 64 * J; k2 <<= 1) {
 #pragma unroll
         for (int d = k2 >> 1; d > 0; d >>= 1) {
             if (d >= 64) {
                 const int dj = d >> 6;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < J; ++j) {
                     const int pj = j ^ dj;
                     if (pj > j) {
                         const bool up = ((j * 64 + lane) & k2) == 0;
@@ -1303,7 +1304,7 @@ __device__ __forceinline__ void wave_sort256(int (&key)[4], int (&idx)[4])
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < J; ++j) {
                     const int pk = __shfl_xor(key[j], d), pi = __shfl_xor(idx[j], d);
                     const bool up = ((j * 64 + lane) & k2) == 0;
                     const bool lower = (lane & d) == 0;
@@ -1317,12 +1318,47 @@ __device__ __forceinline__ void wave_sort256(int (&key)[4], int (&idx)[4])
 
 // base + the bucket's terms in position order: a wavefront bitonic sort of
 // (key, lane), then each lane takes the terms of the lane it sorted to.
+// base + the terms of a bucket of n <= 64 J entries in key order: J keys
+// per lane sorted by one wavefront, then the entries in that order
+template <int J>
+__device__ __forceinline__ double bucket_sum_sorted(const SbcBucketEnt* b, int n, double base)
+{
+    const int lane = lane_id();
+    int key[J], idx[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int e = j * 64 + lane;
+        key[j] = e < n ? b[e].key : INF32;
+        idx[j] = e;
+    }
+    wave_sort_keys<J>(key, idx);
+    double acc = base;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        int fl = 0;
+        double ad = 0.0, sb2 = 0.0;
+        if (key[j] != INF32) {
+            const SbcBucketEnt& e = b[idx[j]];
+            fl = e.flag;
+            ad = e.add;
+            sb2 = e.sub;
+        }
+        acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
+    }
+    return acc;
+}
+
 __device__ __forceinline__ double bucket_sum(const BucketLane& e, double base)
 {
     const int lane = lane_id();
     int key = e.key;
     int idx = lane;
-    for (int k2 = 2; k2 <= 64; k2 <<= 1) {
+    // a network over the first m lanes, m the power of two that holds the
+    // terms (the other lanes hold the largest key)
+    const uint64_t vm = ballot(key != INF32);
+    const int nn = __popcll(vm);
+    const int m = nn <= 16 ? 16 : (nn <= 32 ? 32 : 64);
+    for (int k2 = 2; k2 <= m; k2 <<= 1) {
         for (int j = k2 >> 1; j > 0; j >>= 1) {
             const int pk = __shfl_xor(key, j), pi = __shfl_xor(idx, j);
             const bool up = (lane & k2) == 0;
@@ -1528,26 +1564,35 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
                 }
                 if (gl == 0) base = probe_hub_base(hs, hi);
             }
-            for (int k2 = 2; k2 <= 8; k2 <<= 1) {
-                for (int jj = k2 >> 1; jj > 0; jj >>= 1) {
-                    const int pk = __shfl_xor(key, jj), pi = __shfl_xor(idx, jj);
-                    const bool up = k2 == 8 || (lane & k2) == 0;
-                    const bool lower = (lane & jj) == 0;
-                    const bool take = lower ? ((pk < key) == up) : ((pk > key) == up);
-                    if (take) { key = pk; idx = pi; }
-                }
-            }
-            const int src = gb | idx;
-            const int sfl = __shfl(fl, src);
-            const double sad = shfl_d(ad, src), ssb = shfl_d(sb2, src);
+            // the wavefront's largest small bucket (n is the same in a group)
+            int nm = small ? n : 0;
+            for (int o = 8; o < 64; o <<= 1) nm = max(nm, __shfl_xor(nm, o));
             double acc = base;
-            for (int q = 0; q < 8; ++q) {
-                const int f2 = __shfl(sfl, gb + q);
-                const double a2 = shfl_d(sad, gb + q), b2 = shfl_d(ssb, gb + q);
-                if (gl == 0 && q < n) {
-                    if (f2 & 1) acc = acc + a2;
-                    if (f2 & 2) acc = acc - b2;
+            if (nm >= 2) {
+                for (int k2 = 2; k2 <= 8; k2 <<= 1) {
+                    for (int jj = k2 >> 1; jj > 0; jj >>= 1) {
+                        const int pk = __shfl_xor(key, jj), pi = __shfl_xor(idx, jj);
+                        const bool up = k2 == 8 || (lane & k2) == 0;
+                        const bool lower = (lane & jj) == 0;
+                        const bool take = lower ? ((pk < key) == up) : ((pk > key) == up);
+                        if (take) { key = pk; idx = pi; }
+                    }
                 }
+                const int src = gb | idx;
+                const int sfl = __shfl(fl, src);
+                const double sad = shfl_d(ad, src), ssb = shfl_d(sb2, src);
+                for (int q = 0; q < nm; ++q) {
+                    const int f2 = __shfl(sfl, gb + q);
+                    const double a2 = shfl_d(sad, gb + q), b2 = shfl_d(ssb, gb + q);
+                    if (gl == 0 && q < n) {
+                        if (f2 & 1) acc = acc + a2;
+                        if (f2 & 2) acc = acc - b2;
+                    }
+                }
+            } else if (gl == 0 && n == 1) {
+                // one term: the group leader's own entry
+                if (fl & 1) acc = acc + ad;
+                if (fl & 2) acc = acc - sb2;
             }
             if (small && gl == 0) st_reld(&X.vr[hs.plist[hi]].ndel, acc);
             if (hi < np && gl == 0) {
@@ -1581,27 +1626,7 @@ __device__ __forceinline__ void ascent_hubs_probe(const SbcParams& P, const Item
             if (hs.kind[hi] == 1) {
                 acc = bucket_sum(bucket_lane(b, n), probe_hub_base(hs, hi));
             } else {
-                int key[4], idx[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int e = j * 64 + lane;
-                    key[j] = e < n ? b[e].key : INF32;
-                    idx[j] = e;
-                }
-                wave_sort256(key, idx);
-                acc = probe_hub_base(hs, hi);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    int fl = 0;
-                    double ad = 0.0, sb2 = 0.0;
-                    if (key[j] != INF32) {
-                        const SbcBucketEnt& e = b[idx[j]];
-                        fl = e.flag;
-                        ad = e.add;
-                        sb2 = e.sub;
-                    }
-                    acc = wave_ordered_add(acc, (fl & 1) != 0, ad, (fl & 2) != 0, sb2);
-                }
+                acc = bucket_sum_sorted<4>(b, n, probe_hub_base(hs, hi));
             }
             if (lane == 0) st_reld(&X.vr[hs.plist[hi]].ndel, acc);
         }
