@@ -551,7 +551,7 @@ struct Item {
 // moved c.
 // TS: the owner of the list counters cnt[0..1] (a team's LDS copy, or a
 // parked item's global one).
-template <typename TS>
+template <typename TS, int SCAN_UNROLL = ::SCAN_UNROLL>
 __device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X, int32_t c, const SbcVInfo& vi,
                                                int len, int cb, int ce, int32_t oc, int32_t lc, int64_t dc,
                                                bool cmoved, TS& ts)
@@ -741,7 +741,7 @@ __device__ __forceinline__ KState kstate(const SbcParams& P, const Item& X, int3
 
 // One wavefront computes the new delta of k, summing the terms in adjacency
 // order lane by lane.
-template <typename TS>
+template <typename TS, int WAVE_UNROLL = ::WAVE_UNROLL>
 __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const Item& X, int32_t k, int32_t start,
                                                       int len, const KState& ks, int lev, TS& ts,
                                                       const HubShared* srcset = nullptr, int64_t* tp = nullptr)
@@ -827,11 +827,11 @@ __device__ __forceinline__ void ascent_vertex_wave_ks(const SbcParams& P, const 
     if (tp && lane == 0) tr_add(ts, 11, (unsigned long long)((int64_t)wall_clock64() - tv0));
 }
 
-template <typename TS>
+template <typename TS, int WAVE_UNROLL = ::WAVE_UNROLL>
 __device__ __forceinline__ void ascent_vertex_wave(const SbcParams& P, const Item& X, int32_t k, const SbcVInfo& vi, int len,
                                    int lev, TS& ts)
 {
-    ascent_vertex_wave_ks(P, X, k, vi.start, len, kstate(P, X, k), lev, ts);
+    ascent_vertex_wave_ks<TS, WAVE_UNROLL>(P, X, k, vi.start, len, kstate(P, X, k), lev, ts);
 }
 
 // The whole team computes the new delta of hub k: wave w scans a contiguous
@@ -2514,6 +2514,13 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
 }
 
 // ------------------------------------------------------------ parked items
+
+// Chunks of a vertex's list whose loads one wavefront of the wide kernels
+// issues together (their registers do not count against the window kernel's)
+#ifndef SBC_WIDE_UNROLL
+#define SBC_WIDE_UNROLL 8
+#endif
+constexpr int WIDE_UNROLL = SBC_WIDE_UNROLL;
 //
 // A parked item is finished by kernels that span the whole GPU, one launch
 // per BFS level; the launch boundaries order the levels and make each
@@ -2565,7 +2572,7 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_kernel(SbcParams P, 
         const int len = visible_len(P, vi, X.q);
         const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
         const int64_t dc = ld_acq64(&X.vr[c].dpath);
-        descent_chunks(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, w);
+        descent_chunks<SbcWideItem, WIDE_UNROLL>(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, w);
         my_proc += 1;
         my_scan += (unsigned long long)len;
         my_moved += cmoved ? 1 : 0;
@@ -2594,7 +2601,7 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_kernel(SbcParams P, S
         const int32_t k = here.at(X, i);
         const SbcVInfo vi = P.vinfo[k];
         const int len = visible_len(P, vi, X.q);
-        ascent_vertex_wave(P, X, k, vi, len, lev, w);
+        ascent_vertex_wave<SbcWideItem, WIDE_UNROLL>(P, X, k, vi, len, lev, w);
         // hubs_only: the level's light vertices were counted by the team
         if (!hubs_only || len > HUB_MIN) {
             my_proc += 1;
