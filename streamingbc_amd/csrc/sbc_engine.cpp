@@ -2466,19 +2466,19 @@ extern "C" sbcStatus sbcDeleteVertex(bcForest* forest, struct stinger* G, extraA
 //
 // One file per resident graph and its forest (the reference's only precedent
 // is STINGER's own save/open, src/stinger/stinger.c:1808-2038).  Layout,
-// little-endian: "SBCCKPT2", int64 {version 2, nv, K, exact, initialized,
+// little-endian: "SBCCKPT3", int64 {version 3, nv, K, exact, initialized,
 // slot count, table slot count, next sequence number}; then sections, each
 // followed by the uint64 checksum of its bytes (ckpt_hash): roots[K] (int64);
 // the graph's host mirror: start/deg/deg0/cap[nv] (int32), col/seq[slots]
 // (int32), hub[nv] (int4), htab[table slots] (int2); per root: dist[nv]
 // (int32), state bytes[nv], records[nv] (32 B); BC[nv] (double); then
-// "SBCEND02".  Loading checks the header, the exact file size, every
+// "SBCEND03" (version 3: hub tables in 8-slot buckets).  Loading checks the header, the exact file size, every
 // section's checksum and the ranges of the graph and the trees (list bounds,
 // levels below the level capacity or unreachable, committed state bytes, a
 // root at level 0) in a first pass, before any device allocation.
 
-static const char kCkptMagic[8] = {'S', 'B', 'C', 'C', 'K', 'P', 'T', '2'};
-static const char kCkptEnd[8] = {'S', 'B', 'C', 'E', 'N', 'D', '0', '2'};
+static const char kCkptMagic[8] = {'S', 'B', 'C', 'C', 'K', 'P', 'T', '3'};
+static const char kCkptEnd[8] = {'S', 'B', 'C', 'E', 'N', 'D', '0', '3'};
 
 // Streaming 64-bit checksum over a byte sequence: 8-byte little-endian words
 // (the tail zero-padded), each xored in, multiplied by the FNV-64 prime and
@@ -2534,7 +2534,7 @@ extern "C" sbcStatus sbcCheckpointSave(const bcForest* forest, const struct stin
     FILE* fp = fopen(path, "wb");
     if (!fp) return set_err(SBC_ERR_ARG, "cannot open %s for writing", path);
     const int64_t nv = F->nv, K = F->K;
-    const int64_t hdr[8] = {2, nv, K, F->exact ? 1 : 0, F->initialized ? 1 : 0, G->total_cap,
+    const int64_t hdr[8] = {3, nv, K, F->exact ? 1 : 0, F->initialized ? 1 : 0, G->total_cap,
                             (int64_t)G->htab.size(), G->seq_next};
     CkptHash ck;
     auto put = [&](const void* p, size_t sz, size_t n) {
@@ -2603,7 +2603,7 @@ extern "C" sbcStatus sbcCheckpointLoad(const char* path, bcForest** forest, stru
     int64_t hdr[8];
     bool ok = fread(magic, 1, 8, fp) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(hdr, 8, 8, fp) == 8;
     const int64_t nv = ok ? hdr[1] : 0, K = ok ? hdr[2] : 0, slots = ok ? hdr[5] : 0, tslots = ok ? hdr[6] : 0;
-    ok = ok && hdr[0] == 2 && nv >= 1 && nv < INT32_MAX / 4 && K >= 1 && K <= nv && slots >= 0 &&
+    ok = ok && hdr[0] == 3 && nv >= 1 && nv < INT32_MAX / 4 && K >= 1 && K <= nv && slots >= 0 &&
          slots < INT32_MAX && tslots >= 0 && tslots < INT32_MAX && hdr[7] >= 1 && hdr[7] < INT32_MAX;
     if (ok) {
         // the exact size, before anything is allocated

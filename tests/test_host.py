@@ -171,16 +171,16 @@ def _ckpt_file(dist_override=None, flip=None):
         recb[32 * x:32 * x + 8] = sig[x].tobytes()
         recb[32 * x + 8:32 * x + 16] = dlt[x].tobytes()
     bc = np.array([0.0, 0.0, 1.0, 0.0])
-    hdr = np.array([2, nv, K, 0, 1, slots, 0, 1], np.int64)
+    hdr = np.array([3, nv, K, 0, 1, slots, 0, 1], np.int64)
     secs = [roots.tobytes(),
             start.tobytes() + deg.tobytes() + deg.tobytes() + cap.tobytes() + col.tobytes() + seq.tobytes()
             + hub.tobytes(),
             dist.tobytes() + sb.tobytes() + bytes(recb),
             bc.tobytes()]
-    out = bytearray(b"SBCCKPT2" + hdr.tobytes())
+    out = bytearray(b"SBCCKPT3" + hdr.tobytes())
     for sec in secs:
         out += sec + np.array([_ckpt_hash(sec)], "<u8").tobytes()
-    out += b"SBCEND02"
+    out += b"SBCEND03"
     if flip is not None:
         out[flip] ^= 0x10
     return bytes(out)
@@ -201,8 +201,8 @@ def test_checkpoint_load_rejects_bad_files(tmp_path):
     with pytest.raises(sbc.SbcError) as e:
         sbc.checkpoint_load(str(p))
     assert e.value.status == sbc.SBC_ERR_ARG
-    hdr = np.array([2, 35, 35, 1, 1, 200, 0, 1], dtype=np.int64)
-    p.write_bytes(b"SBCCKPT2" + hdr.tobytes() + bytes(1000))
+    hdr = np.array([3, 35, 35, 1, 1, 200, 0, 1], dtype=np.int64)
+    p.write_bytes(b"SBCCKPT3" + hdr.tobytes() + bytes(1000))
     with pytest.raises(sbc.SbcError) as e:
         sbc.checkpoint_load(str(p))
     assert e.value.status == sbc.SBC_ERR_ARG
