@@ -38,6 +38,16 @@ from . import (SBC_ERR_ARG, SBC_ERR_BUDGET, SBC_ERR_EXISTS, BatchConfig, Context
                gen_chung_lu, resident_footprint)
 
 
+# An item goes to the whole-GPU kernels once its ascent level has this many
+# adjacency entries left to scan (the library's default is 512K).  Measured on
+# the Chung-Lu serving set (2 s per rate): 512K gives 17.2K/s sustained at an
+# offered 50K/s and p99 7.0 ms at 10K/s; 2M gives 25.9K/s and p99 5.3 ms
+# (1M with park_min 8192: 25.9K/s, 7.3 ms; no hand-overs: 19.1K/s, 97 ms).  Every hand-over runs
+# one launch per level from the graph's worker while the other graphs'
+# windows wait for the GPU, so fewer, larger ones pay off here.
+SERVE_PARK_SCAN = 2048 * 1024
+
+
 @dataclass
 class Pending:
     name: str
@@ -61,10 +71,11 @@ class Resident:
 
 class Server:
     def __init__(self, budget_bytes: int, max_window: int = 128, block_threads: int = 1024, use_graph: bool = True,
-                 num_cu: int = 0, park_min: int = 0):
+                 num_cu: int = 0, park_min: int = 0, park_scan: int = SERVE_PARK_SCAN):
         self.budget = int(budget_bytes)
         self.max_window = int(max_window)
         self.park_min = int(park_min)
+        self.park_scan = int(park_scan)
         self.block_threads = block_threads
         self.use_graph = use_graph
         self.num_cu = num_cu
@@ -126,7 +137,7 @@ class Server:
             n = t.shape[0]
             i = 0
             cfg = BatchConfig(window=self.max_window, block_threads=self.block_threads, use_graph=self.use_graph,
-                              park_min=self.park_min)
+                              park_min=self.park_min, park_scan=self.park_scan)
             while i < n:
                 now = time.perf_counter() - t0
                 if t[i] > now:
@@ -258,11 +269,13 @@ def main(argv=None) -> int:
     ap.add_argument("--max-window", type=int, default=128)
     ap.add_argument("--park-min", type=int, default=0,
                     help="sbcBatchConfig.park_min (0: the library default, < 0: never hand items to the wide kernels)")
+    ap.add_argument("--park-scan", type=int, default=SERVE_PARK_SCAN,
+                    help="sbcBatchConfig.park_scan (0: the library default of 512K adjacency entries)")
     ap.add_argument("--budget-gib", type=float, default=200.0)
     ap.add_argument("--graphs", default=",".join(f"{s}:{k}" for s, k in SERVE_GRAPHS))
     ap.add_argument("--seed", type=int, default=11)
     a = ap.parse_args(argv)
-    srv = Server(int(a.budget_gib * (1 << 30)), max_window=a.max_window, park_min=a.park_min)
+    srv = Server(int(a.budget_gib * (1 << 30)), max_window=a.max_window, park_min=a.park_min, park_scan=a.park_scan)
     specs = [tuple(int(x) for x in s.split(":")) for s in a.graphs.split(",")]
     t0 = time.perf_counter()
     hgs = {}

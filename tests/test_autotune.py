@@ -40,7 +40,7 @@ def test_tuning_windows_are_real_work(tmp_path):
     hg = sbc.gen_rmat(12, 16 << 12, seed=3)
     roots = hg.pick_roots(64, 2)
     W = 8
-    u, v = hg.insertions(W * 20, 5)
+    u, v = hg.insertions(W * (tuning_windows(2) + 4), 5)
 
     def fresh():
         ctx = sbc.Context(hg.nv)
@@ -54,7 +54,7 @@ def test_tuning_windows_are_real_work(tmp_path):
     res, used = autotune(f, g, ctx, u, v, W, hg.ne, reps=2, cache=path)
     assert used <= tuning_windows(2) * W and used % W == 0 and not res.from_cache
     stages = [r["stage"] for r in res.table]
-    assert stages == ["team size"] * 3 + ["heavy/light split"] * 2 + ["hand-over thresholds"] * 3
+    assert stages == ["team size"] * 3 + ["heavy/light split"] * 2 + ["hand-over thresholds"] * len(HANDOVER)
     assert all(r["ms_per_window"] > 0 for r in res.table)
     assert res.heavy_roots in (-1, 64 - 16) and (res.park_min, res.park_scan) in [(0, 0)] + list(HANDOVER)
     f.insert_batched(g, ctx, u[used:], v[used:], res.config(W))
