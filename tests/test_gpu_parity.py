@@ -1000,3 +1000,61 @@ def test_hub_roots_match_oracle():
         o.insert(int(a), int(b), O_FINE)
     _compare_trees(f, o, "batched insertions, hub roots")
     assert_bc_close(f.bc(), o.bc())
+
+
+def _bucket_of(w, log_t):
+    """The bucket of neighbour w in a hub table of 2^log_t slots (8 slots a
+    bucket; the multiplicative hash the library uses)."""
+    return ((int(w) * 0x9E3779B1) & 0xFFFFFFFF) >> (32 - (log_t - 3))
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_hub_table_overflow_chains(batched):
+    """A hub whose 700 neighbours all hash to 8 of its table's 512 buckets:
+    lookups of its processed children walk chains of ~90 full buckets.
+    Trees, item records and BC stay the oracle's."""
+    rng = np.random.default_rng(21)
+    n = 60000
+    log_t = 12  # 4 x 701 slots -> 4096
+    pool = [w for w in range(3, n + 1) if _bucket_of(w, log_t) < 8]
+    leaves = np.array(rng.choice(pool, size=700, replace=False), dtype=np.int64)
+    rest = np.array([w for w in range(3, n + 1) if _bucket_of(w, log_t) >= 8], dtype=np.int64)
+    lower = rng.choice(rest, size=300, replace=False).astype(np.int64)
+    eu, ev = [1], [2]  # root 1 - hub 2
+    eu += [2] * len(leaves)
+    ev += leaves.tolist()
+    for x in lower:  # level-3 vertices under 2-3 leaves each
+        for w in rng.choice(leaves, size=int(rng.integers(2, 4)), replace=False):
+            eu.append(int(x)); ev.append(int(w))
+    hg = sbc.graph_from_edges(n, eu, ev)
+    assert hg.degree()[2] == 701
+    # the premise: a long overflow chain in the hub's table
+    filled = np.bincount([_bucket_of(w, log_t) for w in hg.col[hg.rowptr[2]:hg.rowptr[3]]], minlength=512)
+    assert filled[:8].sum() == 700 and filled.max() > 8
+    roots = np.array([1, 2, int(leaves[0]), int(lower[0])], dtype=np.int64)
+    cand = np.concatenate([leaves, lower])
+    seen = set(zip(eu, ev)) | set(zip(ev, eu))
+    u, v = [], []
+    while len(u) < 64:
+        a, b = (int(x) for x in rng.choice(cand, size=2, replace=False))
+        if (a, b) not in seen:
+            seen.add((a, b)); seen.add((b, a))
+            u.append(a); v.append(b)
+    print(f"[hub-chains] nv={hg.nv} n={len(u)} batched={batched}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    _compare_trees(f, o, "hub chains static")
+    if batched:
+        rec = f.insert_batched(g, ctx, u, v, sbc.BatchConfig(window=16, block_threads=1024), records=True)
+    for i, (a, b) in enumerate(zip(u, v)):
+        if not batched:
+            f.insert_edge(g, ctx, a, b, sbc.FINE)
+            rec_g = ctx.last_records(f.k)
+        else:
+            rec_g = rec[i]
+        rec_o = o.insert(a, b, O_FINE, items=True)
+        assert np.array_equal(rec_g["affected"], rec_o["affected"]), f"insertion {i}"
+        assert np.array_equal(rec_g["scanned"], rec_o["scanned"]), f"insertion {i}"
+    _compare_trees(f, o, "hub chains")
+    assert_bc_close(f.bc(), o.bc())
