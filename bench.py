@@ -6,8 +6,8 @@ Default workload (the headline configuration): undirected R-MAT scale 20
 deduplicated, seed 1), K = 256 roots drawn from the largest component (seed
 7), random non-adjacent insertions inside it (seed 11), applied in windows of
 128 by the batched persistent kernel.  One step = one window.  The stream is
-cut into 30 untimed leading windows (on the first 16 the autotuner picks team
-size, heavy/light split and hand-over thresholds; the rest are the cost
+cut into 30 untimed leading windows (on at most the first 18 the autotuner
+picks team size and hand-over thresholds; the rest are the cost
 model's training data under --schedule mlp), the warmup windows and the
 timed windows; the plain form is then timed on exactly the insertions of the
 timed windows, on a second forest brought to the same state.
@@ -371,7 +371,7 @@ def main():
     ap.add_argument("--light-max", type=float, default=0.0,
                     help="--schedule mlp: roots predicted below this many affected vertices per window run light")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the launch-parameter search (team size, heavy/light split, hand-over thresholds) "
+                    help="skip the launch-parameter search (team size, hand-over thresholds) "
                          "that otherwise runs on the first untimed leading windows")
     ap.add_argument("--park-min", type=int, default=0,
                     help="descent frontier size at which an item is finished by whole-GPU kernels "
@@ -416,7 +416,7 @@ def main():
         W = args.window
     from streamingbc_amd.autotune import tuning_windows
     # the search takes its windows from the untimed leading ones
-    n_train = max(args.train_windows, 0 if args.no_autotune else tuning_windows())
+    n_train = max(args.train_windows, 0 if args.no_autotune else tuning_windows(ordered=False))
     n_ins = (n_train + args.warmup + args.steps) * W
     hg, roots, u, v, K, _, desc, t_gen = make_workload(args.workload, n_ins)
     print(f"[bench] {args.workload}: nv={hg.nv} directed slots={hg.ne} K={K} W={W} "

@@ -5,8 +5,11 @@ never the results (trees and BC are bit-identical under every setting):
 
 1. team size: threads per workgroup (1024, 512, 256: one root per workgroup
    team, or one per wavefront for the light phase);
-2. heavy/light split: how many roots run as whole-workgroup teams (all, or
-   all but the last quarter of the queue, which then run one wavefront each);
+2. heavy/light split, only when the caller gives a root order (heaviest
+   first): how many roots run as whole-workgroup teams (all, or all but the
+   last quarter of the queue, which then run one wavefront each, without hub
+   lookups or hand-overs; in index order that quarter holds hub roots too,
+   and one wavefront took 1.2 s over such a window on R-MAT scale 20);
 3. hand-over thresholds: the descent frontier size (``park_min``) and the
    adjacency entries left to scan (``park_scan``) at which an item is
    finished by the whole-GPU level kernels.
@@ -79,9 +82,10 @@ def _store(path, key, entry):
     os.replace(tmp, path)
 
 
-def tuning_windows(reps: int = 2) -> int:
-    """Windows a full search consumes at most."""
-    return reps * (len(BLOCKS) + 2) + (reps + 1) * len(HANDOVER)
+def tuning_windows(reps: int = 2, ordered: bool = True) -> int:
+    """Windows a full search consumes at most (``ordered``: with a root
+    order, which adds the heavy/light stage)."""
+    return reps * (len(BLOCKS) + (2 if ordered else 0)) + (reps + 1) * len(HANDOVER)
 
 
 def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int, reps: int = 2,
@@ -145,7 +149,8 @@ def autotune(forest: Forest, g: Graph, ctx: Context, u, v, window: int, ne: int,
     else:
         run_stage("team size", [{"block_threads": b} for b in BLOCKS])
         if cur["block_threads"] > 64:
-            run_stage("heavy/light split", [{"heavy_roots": -1}, {"heavy_roots": K - K // 4}])
+            if base.root_order is not None:
+                run_stage("heavy/light split", [{"heavy_roots": -1}, {"heavy_roots": K - K // 4}])
             # the thresholds matter in the few windows with a giant item, which
             # a median ignores: the mean, and a larger margin, since one such
             # window in a candidate's three moves its mean (on R-MAT scale 20

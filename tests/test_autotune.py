@@ -36,11 +36,14 @@ def test_candidates_are_valid_block_sizes():
 
 
 @pytest.mark.gpu
-def test_tuning_windows_are_real_work(tmp_path):
+@pytest.mark.parametrize("ordered", [False, True])
+def test_tuning_windows_are_real_work(tmp_path, ordered):
     hg = sbc.gen_rmat(12, 16 << 12, seed=3)
     roots = hg.pick_roots(64, 2)
     W = 8
     u, v = hg.insertions(W * (tuning_windows(2) + 4), 5)
+    # a root order turns on the heavy/light stage
+    order = np.random.default_rng(4).permutation(64).astype(np.int32) if ordered else None
 
     def fresh():
         ctx = sbc.Context(hg.nv)
@@ -51,12 +54,15 @@ def test_tuning_windows_are_real_work(tmp_path):
 
     ctx, g, f = fresh()
     path = str(tmp_path / "tune.json")
-    res, used = autotune(f, g, ctx, u, v, W, hg.ne, reps=2, cache=path)
-    assert used <= tuning_windows(2) * W and used % W == 0 and not res.from_cache
+    res, used = autotune(f, g, ctx, u, v, W, hg.ne, reps=2, cache=path,
+                         base=sbc.BatchConfig(window=W, root_order=order))
+    assert used <= tuning_windows(2, ordered) * W and used % W == 0 and not res.from_cache
     stages = [r["stage"] for r in res.table]
-    assert stages == ["team size"] * 3 + ["heavy/light split"] * 2 + ["hand-over thresholds"] * len(HANDOVER)
+    split = ["heavy/light split"] * 2 if ordered else []
+    assert stages == ["team size"] * 3 + split + ["hand-over thresholds"] * len(HANDOVER)
     assert all(r["ms_per_window"] > 0 for r in res.table)
-    assert res.heavy_roots in (-1, 64 - 16) and (res.park_min, res.park_scan) in [(0, 0)] + list(HANDOVER)
+    assert res.heavy_roots in ((-1, 64 - 16) if ordered else (-1,))
+    assert (res.park_min, res.park_scan) in [(0, 0)] + list(HANDOVER)
     f.insert_batched(g, ctx, u[used:], v[used:], res.config(W))
     f.recompute_bc(ctx)
     ctx2, g2, f2 = fresh()
