@@ -497,8 +497,10 @@ extern "C" int64_t streamingBCResidentFootprint(const int64_t* rowptr, int64_t N
         }
     }
     int64_t graph = slots * 8 + NV * (int64_t)sizeof(SbcVInfo) + NV * 16 + htab * 8;
-    // windowed-update scratch: hub term buckets for up to 4 workgroups per CU
+    // windowed-update scratch: hub term buckets for up to 4 workgroups per CU,
+    // and the lists of the whole-GPU levels of handed-over items
     int64_t scratch = htab ? 4LL * num_cu * SBC_BUCKET_SLOTS * (int64_t)sizeof(SbcBucketEnt) : 0;
+    scratch += 4 * (16 + 3 * NV + slots / 512 + 1);
     return streamingBCForestFootprint(NV, NK) + graph + scratch;
 }
 
@@ -680,6 +682,9 @@ struct extraArraysPerThread {
     int64_t* d_tprof = nullptr;
     int64_t tprof_cap = 0;
     int64_t* d_wtot = nullptr;   // [K][8] per-root work totals since the last stats reset
+    // scratch of the parked items' whole-GPU levels: counters, heavy list, aux list
+    int32_t* d_wl = nullptr;
+    int64_t wl_nv = 0, wl_aux = 0;
     int64_t wtot_k = 0;
     // per-workgroup term buckets of the hub lookup pass
     SbcBucketEnt* d_bucket = nullptr;
@@ -736,6 +741,7 @@ static void ctx_free(extraArraysPerThread* C)
         if (q) (void)hipFree(q);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->d_wtot) (void)hipFree(C->d_wtot);
+    if (C->d_wl) (void)hipFree(C->d_wl);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
     if (C->d_del) (void)hipFree(C->d_del);
@@ -894,6 +900,7 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     memset(&P, 0, sizeof(P));
     P.vinfo = G->d_vinfo;
     P.col = G->d_col;
+    P.col_cap = G->d_cap;
     P.seq = G->d_seq;
     P.hub = G->d_hub;
     P.htab = G->d_htab;
@@ -925,6 +932,9 @@ static SbcParams base_params(const sbcForestDev* F, const stinger* G, extraArray
     // test hook: fewer hubs per pass forces the batched hub passes
     const char* hb = getenv("SBC_HUB_BATCH");
     P.hub_batch = hb ? atoi(hb) : 0;
+    // test hook: a smaller giant-vertex size of the whole-GPU ascent levels
+    const char* wg = getenv("SBC_WIDE_GIANT");
+    P.wide_giant = wg ? atoi(wg) : 0;
     return P;
 }
 
@@ -937,10 +947,11 @@ static sbcStatus check_kernel_err(sbcForestDev* F, extraArraysPerThread* C)
     int32_t err = 0;
     HIPCHK(hipMemcpyAsync(&err, F->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    if (err & SBC_KERR_LEVELS) {
+    if (err & (SBC_KERR_LEVELS | SBC_KERR_WIDE)) {
         int32_t z = 0;
         HIPCHK(hipMemcpyAsync(F->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
+        if (err & SBC_KERR_WIDE) return set_err(SBC_ERR_STATE, "whole-GPU level scratch overflowed");
         return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", F->lcap);
     }
     return SBC_OK;
@@ -1616,6 +1627,31 @@ static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K)
     return SBC_OK;
 }
 
+// Scratch of the whole-GPU levels: 16 counters, a heavy list of 2 * nv and
+// an aux list of nv + slots / 512 + 1 (every work group of a frontier: each
+// frontier vertex once, a heavy one with ceil(len / 512) groups).
+static sbcStatus wide_lists(extraArraysPerThread* C, const SbcParams& P, SbcWideLists* L)
+{
+    const int64_t nv = P.nv;
+    const int64_t aux = nv + P.col_cap / 512 + 1;
+    if (aux > INT32_MAX - 16 - 2 * nv) return set_err(SBC_ERR_ARG, "graph too large for the wide-level scratch");
+    if (C->wl_nv < nv || C->wl_aux < aux) {
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (C->d_wl) HIPCHK(hipFree(C->d_wl));
+        C->d_wl = nullptr;
+        C->wl_nv = C->wl_aux = 0;
+        sbcStatus st = dalloc(&C->d_wl, (size_t)(16 + 2 * nv + aux));
+        if (st) return st;
+        C->wl_nv = nv;
+        C->wl_aux = aux;
+    }
+    L->ctr = C->d_wl;
+    L->heavy = C->d_wl + 16;
+    L->aux = C->d_wl + 16 + 2 * C->wl_nv;
+    L->aux_cap = (int32_t)C->wl_aux;
+    return SBC_OK;
+}
+
 static int wide_grid(const extraArraysPerThread* C, int64_t n)
 {
     // one wavefront per vertex, 4 per workgroup, up to 8 workgroups per CU
@@ -1652,8 +1688,11 @@ static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, co
     if (w.phase == 1) return wide_ascent_commit(F, C, P, ri, w, lv, w.j, w.cnt[0], w.cnt[1], w.lev, w.uA, w.uB, true);
     int j = w.j, fs = w.fs, fe = w.fe;
     int32_t c01[2] = {w.cnt[0], w.cnt[1]};
+    SbcWideLists L;
+    sbcStatus st = wide_lists(C, P, &L);
+    if (st) return st;
     while (fs < fe) {
-        HIPCHK(sbc_launch_wide_descent(&P, dw, ri, fs, fe, w.base + j, wide_grid(C, fe - fs), C->stream));
+        HIPCHK(sbc_launch_wide_descent(&P, dw, ri, fs, fe, w.base + j, &L, C->num_cu, C->stream));
         HIPCHK(hipMemcpyAsync(c01, &dw->cnt[0], sizeof(c01), hipMemcpyDeviceToHost, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
         j++;
@@ -1686,13 +1725,16 @@ static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, co
     const int32_t* lvlD = lv.data();
     const int32_t* lvlS = lvlD + lcap;
     int32_t u = 0;
+    SbcWideLists L;
+    sbcStatus st = wide_lists(C, P, &L);
+    if (st) return st;
     for (int lev = lev0; lev >= 0; --lev) {
         const int jj = lev - w.base;
         int32_t h[6] = {0, 0, 0, 0, uA, uB};
         if (jj >= 0) { h[0] = lvlD[jj]; h[1] = lvlD[jj + 1]; h[2] = lvlS[jj]; h[3] = lvlS[jj + 1]; }
         const int64_t tot = (int64_t)(h[1] - h[0]) + (h[3] - h[2]) + (h[5] - h[4]);
         if (tot == 0) break;
-        HIPCHK(sbc_launch_wide_ascent(&P, dw, ri, h, lev, light_counted && lev == lev0 ? 1 : 0, wide_grid(C, tot),
+        HIPCHK(sbc_launch_wide_ascent(&P, dw, ri, h, lev, light_counted && lev == lev0 ? 1 : 0, &L, C->num_cu,
                                       C->stream));
         // a deletion's p lost its child s: a parent to update once s's level is done
         if (P.del && lev == w.base) HIPCHK(sbc_launch_wide_mark(&P, dw, ri, w.p, lev - 1, C->stream));

@@ -35,7 +35,7 @@ struct SbcWinDesc {
     int32_t nvupd;   // vertex-info updates
 };
 
-enum { SBC_KERR_LEVELS = 1 };
+enum { SBC_KERR_LEVELS = 1, SBC_KERR_WIDE = 2 };
 
 // Per-hub term bucket entry of the hub lookup pass (per-workgroup scratch).
 #define SBC_BUCKET 64
@@ -68,6 +68,7 @@ static_assert(sizeof(SbcVRec) == 32, "four 8-byte fields");
 struct SbcParams {
     const SbcVInfo* vinfo;
     const int32_t* col;
+    int64_t col_cap;      // adjacency slots allocated on the device (host-side sizing of wide-level scratch)
     const int32_t* seq;
     const int4* hub;      // per vertex {offset into htab, indexed count, log2 table size, 0} (count 0: none)
     const int2* htab;     // open-addressing tables {neighbour + 1, position} of indexed vertices
@@ -112,6 +113,8 @@ struct SbcParams {
     int32_t run_cap;      // terms per wavefront run of the flat hub scan (0: full share of the scratch)
     int32_t* hubq;        // per workgroup: SBC_HUBQ_CAP queued hubs beyond the LDS queue, or nullptr
     int32_t hub_batch;    // hubs per pass (<= the LDS queue; smaller only to test the batching)
+    int32_t wide_giant;   // whole-GPU ascent: neighbours from which a vertex gets a workgroup (0: the default;
+                          // smaller only to test that path)
     // parked items: a whole-workgroup team whose next descent frontier has at
     // least park_min vertices, whose descent has scanned park_scan adjacency
     // entries, or whose ascent level has hubs with at least park_scan entries
@@ -158,6 +161,17 @@ struct SbcWideItem {
     int32_t phase, lev, uA, uB;
 };
 
+// Scratch of one whole-GPU level of a parked item: the level's vertices
+// past the one-per-lane size, found by the level's first launch and handled
+// by its second.
+struct SbcWideLists {
+    int32_t* ctr;     // [4]: heavy vertices, giant vertices (ascent) or work groups (descent), unused,
+                      // unused; zeroed before each level
+    int32_t* heavy;   // [2 * nv] ascent: vertex ids; descent: {vertex, its first work group} pairs
+    int32_t* aux;     // [aux_cap] ascent: giant vertex ids; descent: heavy-list index of each work group
+    int32_t aux_cap;  // at least nv + (adjacency slots) / 512 + 1: a frontier's work groups
+};
+
 #define SBC_APPEND_INLINE 8
 
 // Appended adjacency slots and vertex-info updates: either a few entries
@@ -202,10 +216,13 @@ hipError_t sbc_launch_features(const SbcFeatArgs* A, hipStream_t st);
 hipError_t sbc_launch_hist_update(const SbcItemRec* rec, int count, int K, float decay, float* hist, hipStream_t st);
 hipError_t sbc_cost_schedule(sbcCostModel* m, const float* dX, int n, int K, float light_max, float overhead,
                              float* d_pred, float* d_cost, int32_t* d_order, int32_t* d_heavy, hipStream_t st);
-hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc, int grid,
-                                   hipStream_t st);
-hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* here6, int lev, int hubs_only, int grid,
-                                  hipStream_t st);
+// One descent frontier / ascent level of a parked item over the whole GPU:
+// the counters of L are zeroed, then two launches (vertices one per lane;
+// the longer ones found by the first).  num_cu sizes the grids.
+hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc,
+                                   const SbcWideLists* L, int num_cu, hipStream_t st);
+hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* here6, int lev,
+                                  int hubs_only, const SbcWideLists* L, int num_cu, hipStream_t st);
 hipError_t sbc_launch_wide_commit(const SbcParams* P, SbcWideItem* wi, int ri, int nf, int nD, int nUend, int nS,
                                   int grid, hipStream_t st);
 hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, hipStream_t st);

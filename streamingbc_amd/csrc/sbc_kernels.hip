@@ -108,6 +108,31 @@ __device__ __forceinline__ double lds_chain_sum(double acc, const double* v, int
     return acc;
 }
 
+// The same sum with the next 8 values' loads issued before the current 8
+// are added (long runs).
+__device__ __forceinline__ double lds_chain_sum_pipe(double acc, const double* v, int n)
+{
+    int i = 0;
+    if (n >= 16) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = v[u];
+        for (i = 8; i + 8 <= n; i += 8) {
+            double y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) y[u] = v[i + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + x[u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = y[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + x[u];
+    }
+    for (; i < n; ++i) acc = acc + v[i];
+    return acc;
+}
+
 __device__ __forceinline__ double wave_ordered_add(double acc, bool c, double add, bool s, double sub)
 {
     double (&wrow)[16][128] = sbc_wrow;
@@ -141,6 +166,57 @@ __device__ __forceinline__ int wave_append(bool flag, int* ctr)
     if (lane_id() == 0) base = atomicAdd(ctr, __popcll(m));
     base = __shfl(base, 0);
     return flag ? base + prefix_in_wave(m) : -1;
+}
+
+// Reserves n >= 0 consecutive slots per lane from a counter with one atomic
+// per wavefront; returns the lane's first slot.  Whole wavefront.
+__device__ __forceinline__ int wave_reserve(int n, int* ctr)
+{
+    const int lane = lane_id();
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    const int tot = __shfl(incl, 63);
+    int base = 0;
+    if (tot > 0 && lane == 0) base = atomicAdd(ctr, tot);
+    base = __shfl(base, 0);
+    return base + incl - n;
+}
+
+// This lane's wavefront in the workgroup, as a wavefront-uniform (scalar)
+// value: loops indexed by it, or bounded by other readfirstlane values, are
+// uniform loops, so no lane-dependent branch inside can split a wavefront
+// around a cross-lane operation.
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+__device__ __forceinline__ int32_t wave_min32(int32_t x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const int32_t t = __shfl_xor(x, d);
+        x = t < x ? t : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int32_t wave_max32(int32_t x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const int32_t t = __shfl_xor(x, d);
+        x = t > x ? t : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = (int64_t)((uint64_t)x + (uint64_t)__shfl_xor(x, d));
+    return x;
 }
 
 template <int TW>
@@ -545,6 +621,35 @@ struct Item {
     int32_t s, p, r;  // deeper endpoint, shallower endpoint, root
 };
 
+// One neighbour k (state byte b) of descent vertex c (old level oc, new
+// level lc, added paths dc, moved: cmoved): marks k as a child at level
+// lc + 1 and adds c's added paths to it, or records it as an unmoved
+// same-level neighbour of a moved c.  appD / appS: this call marked k, which
+// then joins the next frontier / the same-level list.  Per lane.
+__device__ __forceinline__ void descent_edge(const Item& X, int32_t k, uint8_t b, int32_t oc, int32_t lc, int64_t dc,
+                                             bool cmoved, bool& appD, bool& appS)
+{
+    const uint8_t wantStay = sb_make(lc + 1, 1), wantMove = sb_make(lc + 1, 2);
+    appD = false;
+    appS = false;
+    if (sb_type(b) == 0) {
+        int64_t cd;
+        if (b & SB_INF) cd = 1;  // unreachable until now: moves up
+        else cd = (int64_t)oc + sb_rel(b, oc) - (int64_t)(lc + 1);
+        if (cd >= 0) {
+            const uint8_t want = cd > 0 ? wantMove : wantStay;
+            const uint8_t seen = cas_sb(&X.sb[k], b, want);
+            appD = (seen == b);
+            if (seen == b || seen == want)
+                add64_wg(&X.vr[k].dpath, dc);
+        } else if (cd == -1 && cmoved) {
+            appS = (cas_sb(&X.sb[k], b, sb_make(lc, 3)) == b);
+        }
+    } else if (b == wantStay || b == wantMove) {
+        add64_wg(&X.vr[k].dpath, dc);
+    }
+}
+
 // Descent over chunks [cb, ce) of vertex c (old level oc, new level lc,
 // added paths dc).  Marks children, accumulates their added paths, appends
 // them to the next frontier and records unmoved same-level neighbours of a
@@ -557,7 +662,6 @@ __device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X
                                                bool cmoved, TS& ts)
 {
     const int lane = lane_id();
-    const uint8_t wantStay = sb_make(lc + 1, 1), wantMove = sb_make(lc + 1, 2);
     const bool c_is_s = (c == X.s);
     for (int ch = cb; ch < ce; ch += SCAN_UNROLL) {
         int32_t kk[SCAN_UNROLL];
@@ -578,25 +682,7 @@ __device__ __forceinline__ void descent_chunks(const SbcParams& P, const Item& X
         for (int u = 0; u < SCAN_UNROLL; ++u) {
             bool appD = false, appS = false;
             const int32_t k = kk[u];
-            const uint8_t b = bb[u];
-            if (ok[u]) {
-                if (sb_type(b) == 0) {
-                    int64_t cd;
-                    if (b & SB_INF) cd = 1;  // unreachable until now: moves up
-                    else cd = (int64_t)oc + sb_rel(b, oc) - (int64_t)(lc + 1);
-                    if (cd >= 0) {
-                        const uint8_t want = cd > 0 ? wantMove : wantStay;
-                        const uint8_t seen = cas_sb(&X.sb[k], b, want);
-                        appD = (seen == b);
-                        if (seen == b || seen == want)
-                            add64_wg(&X.vr[k].dpath, dc);
-                    } else if (cd == -1 && cmoved) {
-                        appS = (cas_sb(&X.sb[k], b, sb_make(lc, 3)) == b);
-                    }
-                } else if (b == wantStay || b == wantMove) {
-                    add64_wg(&X.vr[k].dpath, dc);
-                }
-            }
+            if (ok[u]) descent_edge(X, k, bb[u], oc, lc, dc, cmoved, appD, appS);
             const int pd = wave_append(appD, &ts.cnt[0]);
             if (pd >= 0) X.list[pd] = k;
             const int psl = wave_append(appS, &ts.cnt[1]);
@@ -2557,60 +2643,306 @@ __global__ void sbc_wide_mark_kernel(SbcParams P, SbcWideItem* wi, int ri, int32
         X.list[atomicAdd(&wi->cnt[2], 1)] = x;
 }
 
-// Descent of frontier list[fs, fe) (new level lc of the children).
+// A level's vertices with at most WIDE_LIGHT visible neighbours are handled
+// one per lane, the lane walking its list in order (so an ascent vertex's
+// sum keeps the list order); longer ones go to a list for the level's second
+// launch: a descent vertex's list in work groups of WIDE_GROUP_CH chunks,
+// each taken by any wavefront (marks and added paths are exact in any
+// order); an ascent vertex one per wavefront, or, from WIDE_GIANT
+// neighbours on, one per workgroup whose wavefronts compute the terms while
+// one lane adds them in list order.
+#ifndef SBC_WIDE_LIGHT
+#define SBC_WIDE_LIGHT 16
+#endif
+#ifndef SBC_WIDE_GIANT
+#define SBC_WIDE_GIANT 4096
+#endif
+constexpr int WIDE_LIGHT = SBC_WIDE_LIGHT;
+constexpr int WIDE_GIANT = SBC_WIDE_GIANT;
+constexpr int WIDE_GROUP_CH = WIDE_UNROLL;  // one unrolled step of descent_chunks
+constexpr int WIDE_LU = 4;                  // neighbours whose loads a lane issues together
+static_assert(64 * WIDE_GROUP_CH == 512, "a frontier's work groups fit nv + slots / 512 + 1");
+
+__device__ __forceinline__ void wide_add_stats(SbcWideItem& w, unsigned long long a, unsigned long long b,
+                                               unsigned long long c)
+{
+    const int64_t sa = wave_sum64((int64_t)a), sb = wave_sum64((int64_t)b), sc = wave_sum64((int64_t)c);
+    if (lane_id() == 0) {
+        if (sa) atomicAdd(&w.stat[0], (unsigned long long)sa);
+        if (sb) atomicAdd(&w.stat[1], (unsigned long long)sb);
+        if (sc) atomicAdd(&w.stat[2], (unsigned long long)sc);
+    }
+}
+
+// Descent of frontier list[fs, fe) (new level lc of the children), first
+// launch: every vertex counted, light ones done, heavy ones listed.
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void sbc_wide_descent_kernel(SbcParams P, SbcWideItem* wi, int ri, int fs, int fe,
-                                                                   int32_t lc)
+__global__ __launch_bounds__(NW * 64) void sbc_wide_descent_light_kernel(SbcParams P, SbcWideItem* wi, int ri, int fs,
+                                                                         int fe, int32_t lc, SbcWideLists L)
 {
     SbcWideItem& w = *wi;
     const Item X = parked_item(P, w, ri);
-    const int nwaves = (int)gridDim.x * NW;
+    const int lane = lane_id();
+    const int n = fe - fs;
+    const int step = (int)gridDim.x * NW * 64;
     unsigned long long my_proc = 0, my_scan = 0, my_moved = 0;
-    for (int idx = fs + (int)blockIdx.x * NW + (int)(threadIdx.x >> 6); idx < fe; idx += nwaves) {
-        const int32_t c = X.list[idx];
+    for (int b0 = ((int)blockIdx.x * NW + wave_in_block()) * 64; b0 < n; b0 += step) {
+        const int i = b0 + lane;
+        const bool valid = i < n;
+        const int32_t c = valid ? X.list[fs + i] : 0;
+        SbcVInfo vi{0, 0, 0, 0};
+        if (valid) vi = P.vinfo[c];
+        const int len = valid ? visible_len_thread(P, vi, X.q) : 0;
+        const bool cmoved = valid && sb_type(ld_sb(&X.sb[c])) == 2;
+        const int64_t dc = valid ? ld_acq64(&X.vr[c].dpath) : 0;
+        const int32_t oc = valid ? X.dist[c] : 0;
+        if (valid) {
+            my_proc += 1;
+            my_scan += (unsigned long long)len;
+            my_moved += cmoved ? 1 : 0;
+        }
+        const bool light = valid && len <= WIDE_LIGHT;
+        const bool heavy = valid && !light;
+        const int ng = heavy ? (len + 64 * WIDE_GROUP_CH - 1) / (64 * WIDE_GROUP_CH) : 0;
+        const int hi = wave_append(heavy, &L.ctr[0]);
+        const int g0 = wave_reserve(ng, &L.ctr[1]);
+        if (heavy) {
+            if (g0 + ng <= L.aux_cap) {
+                L.heavy[2 * hi] = c;
+                L.heavy[2 * hi + 1] = g0;
+                for (int g = 0; g < ng; ++g) L.aux[g0 + g] = hi;
+            } else {
+                atomicOr(P.err, SBC_KERR_WIDE);
+            }
+        }
+        const bool c_is_s = c == X.s;
+        const int m = __builtin_amdgcn_readfirstlane(wave_max32(light ? len : 0));
+        for (int j0 = 0; j0 < m; j0 += WIDE_LU) {
+            int32_t kk[WIDE_LU];
+            bool ok[WIDE_LU];
+            uint8_t bb[WIDE_LU];
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) {
+                ok[u] = light && j0 + u < len;
+                kk[u] = ok[u] ? P.col[vi.start + j0 + u] : 0;
+                // the inserted edge never adds a child of s
+                if (ok[u] && c_is_s && kk[u] == X.p) ok[u] = false;
+            }
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) bb[u] = ok[u] ? ld_sb(&X.sb[kk[u]]) : (uint8_t)0;
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) {
+                bool appD = false, appS = false;
+                if (ok[u]) descent_edge(X, kk[u], bb[u], oc, lc, dc, cmoved, appD, appS);
+                const int pd = wave_append(appD, &w.cnt[0]);
+                if (pd >= 0) X.list[pd] = kk[u];
+                const int psl = wave_append(appS, &w.cnt[1]);
+                if (psl >= 0) X.list[X.nv - 1 - psl] = kk[u];
+            }
+        }
+    }
+    wide_add_stats(w, my_proc, my_scan, my_moved);
+}
+
+// Second launch: the heavy vertices' work groups, one per wavefront.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void sbc_wide_descent_heavy_kernel(SbcParams P, SbcWideItem* wi, int ri,
+                                                                         int32_t lc, SbcWideLists L)
+{
+    SbcWideItem& w = *wi;
+    const Item X = parked_item(P, w, ri);
+    const int ngr = L.ctr[1];
+    if (ngr > L.aux_cap) return;  // reported by the first launch
+    const int nwaves = (int)gridDim.x * NW;
+    for (int g = (int)blockIdx.x * NW + wave_in_block(); g < ngr; g += nwaves) {
+        const int hi = L.aux[g];
+        const int32_t c = L.heavy[2 * hi];
+        const int g0 = L.heavy[2 * hi + 1];
         const SbcVInfo vi = P.vinfo[c];
         const int len = visible_len(P, vi, X.q);
         const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
         const int64_t dc = ld_acq64(&X.vr[c].dpath);
-        descent_chunks<SbcWideItem, WIDE_UNROLL>(P, X, c, vi, len, 0, (len + 63) >> 6, X.dist[c], lc, dc, cmoved, w);
-        my_proc += 1;
-        my_scan += (unsigned long long)len;
-        my_moved += cmoved ? 1 : 0;
-    }
-    if (lane_id() == 0 && my_proc > 0) {
-        atomicAdd(&w.stat[0], my_proc);
-        atomicAdd(&w.stat[1], my_scan);
-        atomicAdd(&w.stat[2], my_moved);
+        const int cb = (g - g0) * WIDE_GROUP_CH;
+        const int ce = min(cb + WIDE_GROUP_CH, (len + 63) >> 6);
+        descent_chunks<SbcWideItem, WIDE_UNROLL>(P, X, c, vi, len, cb, ce, X.dist[c], lc, dc, cmoved, w);
     }
 }
 
-// Ascent of level lev: the processed vertices here (descent frontier,
-// same-level set, ascent set), one wavefront per vertex.
+// Ascent of level lev (the processed vertices here: descent frontier,
+// same-level set, ascent set), first launch: every vertex counted (only
+// those past HUB_MIN with hubs_only: the team counted the level's light
+// vertices), light ones done, heavy and giant ones listed.
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_kernel(SbcParams P, SbcWideItem* wi, int ri, int dA, int dB,
-                                                                  int sA, int sB, int uA, int uB, int lev,
-                                                                  int hubs_only)
+__global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_light_kernel(SbcParams P, SbcWideItem* wi, int ri, int dA,
+                                                                        int dB, int sA, int sB, int uA, int uB, int lev,
+                                                                        int hubs_only, SbcWideLists L)
 {
     SbcWideItem& w = *wi;
     const Item X = parked_item(P, w, ri);
     const LevelSets here{dA, dB, sA, sB, uA, uB};
     const int tot = here.size();
-    const int nwaves = (int)gridDim.x * NW;
+    const int lane = lane_id();
+    const int step = (int)gridDim.x * NW * 64;
     unsigned long long my_proc = 0, my_scan = 0;
-    for (int i = (int)blockIdx.x * NW + (int)(threadIdx.x >> 6); i < tot; i += nwaves) {
-        const int32_t k = here.at(X, i);
-        const SbcVInfo vi = P.vinfo[k];
-        const int len = visible_len(P, vi, X.q);
-        ascent_vertex_wave<SbcWideItem, WIDE_UNROLL>(P, X, k, vi, len, lev, w);
-        // hubs_only: the level's light vertices were counted by the team
-        if (!hubs_only || len > HUB_MIN) {
+    for (int b0 = ((int)blockIdx.x * NW + wave_in_block()) * 64; b0 < tot; b0 += step) {
+        const int i = b0 + lane;
+        const bool valid = i < tot;
+        const int32_t k = valid ? here.at(X, i) : 0;
+        SbcVInfo vi{0, 0, 0, 0};
+        if (valid) vi = P.vinfo[k];
+        const int len = valid ? visible_len_thread(P, vi, X.q) : 0;
+        if (valid && (!hubs_only || len > HUB_MIN)) {
             my_proc += 1;
             my_scan += (unsigned long long)len;
         }
+        const bool light = valid && len <= WIDE_LIGHT;
+        const bool giant = valid && !light && len >= (P.wide_giant > 0 ? P.wide_giant : WIDE_GIANT);
+        const bool heavy = valid && !light && !giant;
+        const int hi = wave_append(heavy, &L.ctr[0]);
+        if (hi >= 0) L.heavy[hi] = k;
+        const int gi = wave_append(giant, &L.ctr[1]);
+        if (gi >= 0) L.aux[gi] = k;
+        KState ks{true, 0, 0, 0.0};
+        if (light) ks = kstate(P, X, k);
+        double acc = ks.base;
+        const int m = __builtin_amdgcn_readfirstlane(wave_max32(light ? len : 0));
+        for (int j0 = 0; j0 < m; j0 += WIDE_LU) {
+            int32_t cc[WIDE_LU];
+            bool ok[WIDE_LU];
+            uint8_t ww[WIDE_LU];
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) {
+                ok[u] = light && j0 + u < len;
+                cc[u] = ok[u] ? P.col[vi.start + j0 + u] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+            Term t[WIDE_LU];
+            bool appU[WIDE_LU];
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u)
+                t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) {
+                const int pu = wave_append(appU[u], &w.cnt[2]);
+                if (pu >= 0) X.list[pu] = cc[u];
+            }
+            // the terms in list order: a child's new dependency, then the
+            // removal of its old one (the addition of the negated value)
+#pragma unroll
+            for (int u = 0; u < WIDE_LU; ++u) {
+                if (t[u].child) acc = acc + t[u].add;
+                if (t[u].hsub) acc = acc + (-t[u].sub);
+            }
+        }
+        if (light) st_reld(&X.vr[k].ndel, acc);
     }
-    if (lane_id() == 0 && my_proc > 0) {
-        atomicAdd(&w.stat[0], my_proc);
-        atomicAdd(&w.stat[1], my_scan);
+    wide_add_stats(w, my_proc, my_scan, 0ull);
+}
+
+// A giant ascent vertex's list is taken in slabs of GIANT_CH chunks:
+// wavefronts 1..GIANT_TW compute one slab's terms (loads, marks, appends)
+// and place them in LDS in list order while lane 0 of wavefront 0 adds the
+// previous slab's, one after another; two barriers a slab.
+constexpr int GIANT_TW = 15;
+constexpr int GIANT_CPW = 4;                     // chunks per term wavefront per slab
+constexpr int GIANT_CH = GIANT_TW * GIANT_CPW;  // 3840 neighbours per slab
+static_assert(GIANT_CH <= 64, "a slab's chunk counts are scanned by one wavefront");
+struct GiantShared {
+    double v[2][GIANT_CH * 128];  // a chunk gives at most two values per neighbour
+    int cnt[GIANT_CH];
+};
+
+__device__ __forceinline__ void giant_ascent(const SbcParams& P, const Item& X, int32_t k, int lev, SbcWideItem& w,
+                                             GiantShared& gs)
+{
+    const int wave = wave_in_block(), lane = lane_id();
+    const SbcVInfo vi = P.vinfo[k];
+    const int len = __builtin_amdgcn_readfirstlane(visible_len(P, vi, X.q));
+    const KState ks = kstate(P, X, k);
+    const int nch = (len + 63) >> 6;
+    const int nslab = (nch + GIANT_CH - 1) / GIANT_CH;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    double acc = ks.base;  // wavefront 0, lane 0
+    int tprev = 0;         // values of the slab wavefront 0 adds next
+    for (int sl = 0; sl <= nslab; ++sl) {
+        Term t[GIANT_CPW];
+        uint64_t cm[GIANT_CPW], sm[GIANT_CPW];
+        if (wave >= 1 && sl < nslab) {
+            int32_t cc[GIANT_CPW];
+            bool ok[GIANT_CPW];
+            uint8_t ww[GIANT_CPW];
+            bool appU[GIANT_CPW];
+#pragma unroll
+            for (int u = 0; u < GIANT_CPW; ++u) {
+                const int ch = sl * GIANT_CH + (wave - 1) * GIANT_CPW + u;
+                const int e = ch * 64 + lane;
+                ok[u] = ch < nch && e < len;
+                cc[u] = ok[u] ? P.col[vi.start + e] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < GIANT_CPW; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+#pragma unroll
+            for (int u = 0; u < GIANT_CPW; ++u)
+                t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
+#pragma unroll
+            for (int u = 0; u < GIANT_CPW; ++u) {
+                const int pu = wave_append(appU[u], &w.cnt[2]);
+                if (pu >= 0) X.list[pu] = cc[u];
+                cm[u] = ballot(t[u].child);
+                sm[u] = ballot(t[u].hsub);
+                if (lane == 0) gs.cnt[(wave - 1) * GIANT_CPW + u] = __popcll(cm[u]) + __popcll(sm[u]);
+            }
+        } else if (wave == 0 && sl >= 1 && lane == 0) {
+            acc = lds_chain_sum_pipe(acc, gs.v[(sl - 1) & 1], tprev);
+        }
+        __syncthreads();
+        int tcur = 0;
+        if (sl < nslab) {
+            // chunk offsets within the slab: an exclusive scan of the counts
+            const int cn = lane < GIANT_CH ? gs.cnt[lane] : 0;
+            int inc = cn;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(inc, d);
+                if (lane >= d) inc += y;
+            }
+            tcur = __shfl(inc, 63);
+            if (wave >= 1) {
+                double* dst = gs.v[sl & 1];
+#pragma unroll
+                for (int u = 0; u < GIANT_CPW; ++u) {
+                    const int c = (wave - 1) * GIANT_CPW + u;
+                    const int off = __shfl(inc - cn, c);
+                    int pos = off + __popcll(cm[u] & below) + __popcll(sm[u] & below);
+                    if (t[u].child) dst[pos++] = t[u].add;
+                    if (t[u].hsub) dst[pos] = -t[u].sub;
+                }
+            }
+        }
+        tprev = tcur;
+        __syncthreads();
+    }
+    if (wave == 0 && lane == 0) st_reld(&X.vr[k].ndel, acc);
+}
+
+// Second launch of an ascent level: giant vertices one per workgroup, then
+// heavy ones one per wavefront in a grid stride (the last workgroups first,
+// the ones with giants last).
+__global__ __launch_bounds__(1024) void sbc_wide_ascent_heavy_kernel(SbcParams P, SbcWideItem* wi, int ri, int lev,
+                                                                     SbcWideLists L)
+{
+    __shared__ GiantShared gs;
+    SbcWideItem& w = *wi;
+    const Item X = parked_item(P, w, ri);
+    const int nh = L.ctr[0], ngi = L.ctr[1];
+    for (int gi = (int)blockIdx.x; gi < ngi; gi += (int)gridDim.x) giant_ascent(P, X, L.aux[gi], lev, w, gs);
+    const int nwaves = (int)gridDim.x * 16;
+    for (int hi = ((int)gridDim.x - 1 - (int)blockIdx.x) * 16 + wave_in_block(); hi < nh; hi += nwaves) {
+        const int32_t k = L.heavy[hi];
+        const SbcVInfo vi = P.vinfo[k];
+        const int len = visible_len(P, vi, X.q);
+        ascent_vertex_wave<SbcWideItem, WIDE_UNROLL>(P, X, k, vi, len, lev, w);
     }
 }
 
@@ -3704,23 +4036,6 @@ __device__ __forceinline__ void dm_flag(bool want, int32_t y, int64_t flags, Sbc
     if (pos >= 0) list[pos] = y;
 }
 
-__device__ __forceinline__ int32_t wave_min32(int32_t x)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const int32_t t = __shfl_xor(x, d);
-        x = t < x ? t : x;
-    }
-    return x;
-}
-
-__device__ __forceinline__ int64_t wave_sum64(int64_t x)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x = (int64_t)((uint64_t)x + (uint64_t)__shfl_xor(x, d));
-    return x;
-}
-
 // Vertices with more neighbours than this are handled by the whole workgroup
 // in C and D (at most DM_BIGQ a level; more go to single wavefronts): one
 // wavefront would need at least 256 rounds of loads for them, while the
@@ -4310,17 +4625,31 @@ extern "C" hipError_t sbc_launch_window(const SbcParams* P, int nwb, int grid, h
 constexpr int WIDE_NW = 4;  // wavefronts per workgroup of the wide kernels
 
 extern "C" hipError_t sbc_launch_wide_descent(const SbcParams* P, SbcWideItem* wi, int ri, int fs, int fe, int lc,
-                                              int grid, hipStream_t st)
+                                              const SbcWideLists* L, int num_cu, hipStream_t st)
 {
-    hipLaunchKernelGGL(sbc_wide_descent_kernel<WIDE_NW>, dim3(grid), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, fs, fe, lc);
+    const int cu = num_cu > 0 ? num_cu : 256;
+    const int n = fe - fs;
+    int g1 = (n + WIDE_NW * 64 - 1) / (WIDE_NW * 64);
+    g1 = g1 < 1 ? 1 : (g1 < 8 * cu ? g1 : 8 * cu);
+    hipLaunchKernelGGL(sbc_reset_kernel, dim3(1), dim3(64), 0, st, L->ctr, 4);
+    hipLaunchKernelGGL(sbc_wide_descent_light_kernel<WIDE_NW>, dim3(g1), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, fs, fe,
+                       lc, *L);
+    hipLaunchKernelGGL(sbc_wide_descent_heavy_kernel<WIDE_NW>, dim3(8 * cu), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, lc,
+                       *L);
     return hipGetLastError();
 }
 
 extern "C" hipError_t sbc_launch_wide_ascent(const SbcParams* P, SbcWideItem* wi, int ri, const int32_t* h, int lev,
-                                             int hubs_only, int grid, hipStream_t st)
+                                             int hubs_only, const SbcWideLists* L, int num_cu, hipStream_t st)
 {
-    hipLaunchKernelGGL(sbc_wide_ascent_kernel<WIDE_NW>, dim3(grid), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, h[0], h[1],
-                       h[2], h[3], h[4], h[5], lev, hubs_only);
+    const int cu = num_cu > 0 ? num_cu : 256;
+    const int tot = (h[1] - h[0]) + (h[3] - h[2]) + (h[5] - h[4]);
+    int g1 = (tot + WIDE_NW * 64 - 1) / (WIDE_NW * 64);
+    g1 = g1 < 1 ? 1 : (g1 < 8 * cu ? g1 : 8 * cu);
+    hipLaunchKernelGGL(sbc_reset_kernel, dim3(1), dim3(64), 0, st, L->ctr, 4);
+    hipLaunchKernelGGL(sbc_wide_ascent_light_kernel<WIDE_NW>, dim3(g1), dim3(WIDE_NW * 64), 0, st, *P, wi, ri, h[0],
+                       h[1], h[2], h[3], h[4], h[5], lev, hubs_only, *L);
+    hipLaunchKernelGGL(sbc_wide_ascent_heavy_kernel, dim3(cu), dim3(1024), 0, st, *P, wi, ri, lev, *L);
     return hipGetLastError();
 }
 

@@ -1058,3 +1058,55 @@ def test_hub_table_overflow_chains(batched):
         assert np.array_equal(rec_g["scanned"], rec_o["scanned"]), f"insertion {i}"
     _compare_trees(f, o, "hub chains")
     assert_bc_close(f.bc(), o.bc())
+
+
+@pytest.mark.parametrize("giant", [None, "64"])
+def test_handed_over_giant_hub_matches_oracle(giant, monkeypatch):
+    """A hub of 12,000 neighbours inside items finished by the whole-GPU level
+    kernels.  In an ascent level it takes a workgroup whose wavefronts place
+    its terms in LDS in list order, slab after slab (3840 neighbours a slab,
+    two buffers), while one lane adds them; in a descent frontier its list is
+    split into work groups that any wavefront takes.  With SBC_WIDE_GIANT=64
+    every vertex past 64 neighbours takes the workgroup path.  Trees, item
+    records and BC stay the oracle's."""
+    if giant:
+        monkeypatch.setenv("SBC_WIDE_GIANT", giant)
+    rng = np.random.default_rng(31)
+    n = 20000
+    hub_nb = rng.choice(np.arange(2, n + 1), size=12000, replace=False)
+    eu = rng.integers(2, n + 1, size=3 * n).tolist() + [1] * len(hub_nb)
+    ev = rng.integers(2, n + 1, size=3 * n).tolist() + hub_nb.tolist()
+    hg = sbc.graph_from_edges(n, eu, ev)
+    assert hg.degree()[1] == 12000
+    roots = hg.pick_roots(32, 5)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    lv0 = o.trees()[0]
+    # roots two levels above the hub: an edge from such a root to a hub
+    # neighbour three or more levels down puts the hub into a descent frontier
+    pairs = []
+    for ri in np.flatnonzero(lv0[:, 1] == 2)[:6]:
+        far = hub_nb[lv0[ri, hub_nb] >= 3]
+        if far.size:
+            pairs.append((int(roots[ri]), int(far[0])))
+    assert pairs, "no root two levels above the hub"
+    seen = {(min(a, b), max(a, b)) for a, b in pairs}
+    ru, rv = hg.insertions(64, 6)
+    for a, b in zip(ru.tolist(), rv.tolist()):
+        if (min(a, b), max(a, b)) not in seen:
+            seen.add((min(a, b), max(a, b)))
+            pairs.append((a, b))
+    u = np.array([p[0] for p in pairs[:48]], dtype=np.int64)
+    v = np.array([p[1] for p in pairs[:48]], dtype=np.int64)
+    print(f"[giant hub] nv={hg.nv} k={len(roots)} n={len(u)} giant={giant}", flush=True)
+    ctx, g, f = _engine(hg, roots)
+    cfg = sbc.BatchConfig(window=16, block_threads=1024, park_min=4, recompute_bc=True)
+    rec = f.insert_batched(g, ctx, u, v, cfg, records=True)
+    assert ctx.last_parked() > 0
+    for i in range(len(u)):
+        rec_o = o.insert(int(u[i]), int(v[i]), O_FINE, items=True)
+        for key in ("kase", "affected", "scanned", "processed", "moved"):
+            assert np.array_equal(rec[i][key], rec_o[key]), f"insertion {i} {key}"
+    _compare_trees(f, o, "giant hub")
+    f.recompute_bc(ctx)
+    assert_bc_close(f.bc(), o.bc())
