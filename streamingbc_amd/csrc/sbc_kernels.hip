@@ -108,26 +108,30 @@ __device__ __forceinline__ double lds_chain_sum(double acc, const double* v, int
     return acc;
 }
 
-// The same sum with the next 8 values' loads issued before the current 8
-// are added (long runs).
+// The same sum for long runs: two register buffers of CH values take turns,
+// one being added while the other's loads are in flight (no copies between
+// them, so each wait covers only the buffer about to be added).
+template <int CH>
 __device__ __forceinline__ double lds_chain_sum_pipe(double acc, const double* v, int n)
 {
     int i = 0;
-    if (n >= 16) {
-        double x[8];
+    if (n >= 2 * CH) {
+        double a[CH], b[CH];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = v[u];
-        for (i = 8; i + 8 <= n; i += 8) {
-            double y[8];
+        for (int u = 0; u < CH; ++u) a[u] = v[u];
+        for (i = CH; i + 2 * CH <= n; i += 2 * CH) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) y[u] = v[i + u];
+            for (int u = 0; u < CH; ++u) b[u] = v[i + u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc = acc + x[u];
+            for (int u = 0; u < CH; ++u) acc = acc + a[u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = y[u];
+            for (int u = 0; u < CH; ++u) a[u] = v[i + CH + u];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) acc = acc + b[u];
         }
+        // a holds v[i - CH, i)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = acc + x[u];
+        for (int u = 0; u < CH; ++u) acc = acc + a[u];
     }
     for (; i < n; ++i) acc = acc + v[i];
     return acc;
@@ -2674,20 +2678,54 @@ __device__ __forceinline__ void wide_add_stats(SbcWideItem& w, unsigned long lon
     }
 }
 
+// Appends of one workgroup pass, staged in LDS and moved to the item's lists
+// with one global atomic per list: the next frontier from the bottom of
+// buf, same-level vertices from the top.  Capacity: a pass appends each
+// neighbour at most once.
+template <int CAP>
+struct WideStage {
+    int cnt[4];   // cnt[0]: frontier entries, cnt[1]: same-level entries (the roles of a team's counters)
+    int base[2];
+    int32_t buf[CAP];
+};
+
+// Moves a pass's staged appends to the item's lists and clears the
+// counters.  Whole workgroup; ends with a barrier.
+template <int NT, int CAP>
+__device__ __forceinline__ void wide_flush(const Item& X, SbcWideItem& w, WideStage<CAP>& st)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st.base[0] = st.cnt[0] ? atomicAdd(&w.cnt[0], st.cnt[0]) : 0;
+        st.base[1] = st.cnt[1] ? atomicAdd(&w.cnt[1], st.cnt[1]) : 0;
+    }
+    __syncthreads();
+    const int nd = st.cnt[0], ns = st.cnt[1];
+    for (int j = threadIdx.x; j < nd; j += NT) X.list[st.base[0] + j] = st.buf[j];
+    for (int j = threadIdx.x; j < ns; j += NT) X.list[X.nv - 1 - (st.base[1] + j)] = st.buf[CAP - 1 - j];
+    __syncthreads();
+    if (threadIdx.x == 0) { st.cnt[0] = 0; st.cnt[1] = 0; }
+    __syncthreads();
+}
+
 // Descent of frontier list[fs, fe) (new level lc of the children), first
-// launch: every vertex counted, light ones done, heavy ones listed.
+// launch: every vertex counted, light ones done, heavy ones listed.  The
+// workgroup takes 64 * NW frontier vertices a pass.
 template <int NW>
 __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_light_kernel(SbcParams P, SbcWideItem* wi, int ri, int fs,
                                                                          int fe, int32_t lc, SbcWideLists L)
 {
+    constexpr int CAP = NW * 64 * WIDE_LIGHT;
+    __shared__ WideStage<CAP> st;
     SbcWideItem& w = *wi;
     const Item X = parked_item(P, w, ri);
     const int lane = lane_id();
     const int n = fe - fs;
-    const int step = (int)gridDim.x * NW * 64;
+    if (threadIdx.x == 0) { st.cnt[0] = 0; st.cnt[1] = 0; }
+    __syncthreads();
     unsigned long long my_proc = 0, my_scan = 0, my_moved = 0;
-    for (int b0 = ((int)blockIdx.x * NW + wave_in_block()) * 64; b0 < n; b0 += step) {
-        const int i = b0 + lane;
+    for (int p0 = (int)blockIdx.x * NW * 64; p0 < n; p0 += (int)gridDim.x * NW * 64) {
+        const int i = p0 + wave_in_block() * 64 + lane;
         const bool valid = i < n;
         const int32_t c = valid ? X.list[fs + i] : 0;
         SbcVInfo vi{0, 0, 0, 0};
@@ -2734,37 +2772,50 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_light_kernel(SbcPara
             for (int u = 0; u < WIDE_LU; ++u) {
                 bool appD = false, appS = false;
                 if (ok[u]) descent_edge(X, kk[u], bb[u], oc, lc, dc, cmoved, appD, appS);
-                const int pd = wave_append(appD, &w.cnt[0]);
-                if (pd >= 0) X.list[pd] = kk[u];
-                const int psl = wave_append(appS, &w.cnt[1]);
-                if (psl >= 0) X.list[X.nv - 1 - psl] = kk[u];
+                const int pd = wave_append(appD, &st.cnt[0]);
+                if (pd >= 0) st.buf[pd] = kk[u];
+                const int psl = wave_append(appS, &st.cnt[1]);
+                if (psl >= 0) st.buf[CAP - 1 - psl] = kk[u];
             }
         }
+        wide_flush<NW * 64>(X, w, st);
     }
     wide_add_stats(w, my_proc, my_scan, my_moved);
 }
 
-// Second launch: the heavy vertices' work groups, one per wavefront.
+// Second launch: the heavy vertices' work groups, one per wavefront, NW a
+// workgroup pass; descent_chunks appends into the pass's LDS stage (an Item
+// whose list is the stage).
 template <int NW>
 __global__ __launch_bounds__(NW * 64) void sbc_wide_descent_heavy_kernel(SbcParams P, SbcWideItem* wi, int ri,
                                                                          int32_t lc, SbcWideLists L)
 {
+    constexpr int CAP = NW * 64 * WIDE_GROUP_CH;
+    __shared__ WideStage<CAP> st;
     SbcWideItem& w = *wi;
     const Item X = parked_item(P, w, ri);
+    Item XS = X;
+    XS.list = st.buf;
+    XS.nv = CAP;
     const int ngr = L.ctr[1];
     if (ngr > L.aux_cap) return;  // reported by the first launch
-    const int nwaves = (int)gridDim.x * NW;
-    for (int g = (int)blockIdx.x * NW + wave_in_block(); g < ngr; g += nwaves) {
-        const int hi = L.aux[g];
-        const int32_t c = L.heavy[2 * hi];
-        const int g0 = L.heavy[2 * hi + 1];
-        const SbcVInfo vi = P.vinfo[c];
-        const int len = visible_len(P, vi, X.q);
-        const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
-        const int64_t dc = ld_acq64(&X.vr[c].dpath);
-        const int cb = (g - g0) * WIDE_GROUP_CH;
-        const int ce = min(cb + WIDE_GROUP_CH, (len + 63) >> 6);
-        descent_chunks<SbcWideItem, WIDE_UNROLL>(P, X, c, vi, len, cb, ce, X.dist[c], lc, dc, cmoved, w);
+    if (threadIdx.x == 0) { st.cnt[0] = 0; st.cnt[1] = 0; }
+    __syncthreads();
+    for (int p0 = (int)blockIdx.x * NW; p0 < ngr; p0 += (int)gridDim.x * NW) {
+        const int g = p0 + wave_in_block();
+        if (g < ngr) {
+            const int hi = L.aux[g];
+            const int32_t c = L.heavy[2 * hi];
+            const int g0 = L.heavy[2 * hi + 1];
+            const SbcVInfo vi = P.vinfo[c];
+            const int len = visible_len(P, vi, X.q);
+            const bool cmoved = sb_type(ld_sb(&X.sb[c])) == 2;
+            const int64_t dc = ld_acq64(&X.vr[c].dpath);
+            const int cb = (g - g0) * WIDE_GROUP_CH;
+            const int ce = min(cb + WIDE_GROUP_CH, (len + 63) >> 6);
+            descent_chunks<WideStage<CAP>, WIDE_UNROLL>(P, XS, c, vi, len, cb, ce, X.dist[c], lc, dc, cmoved, st);
+        }
+        wide_flush<NW * 64>(X, w, st);
     }
 }
 
@@ -2777,15 +2828,18 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_light_kernel(SbcParam
                                                                         int dB, int sA, int sB, int uA, int uB, int lev,
                                                                         int hubs_only, SbcWideLists L)
 {
+    constexpr int CAP = NW * 64 * WIDE_LIGHT;
+    __shared__ WideStage<CAP> st;  // cnt[0]: parents marked in the pass
     SbcWideItem& w = *wi;
     const Item X = parked_item(P, w, ri);
     const LevelSets here{dA, dB, sA, sB, uA, uB};
     const int tot = here.size();
     const int lane = lane_id();
-    const int step = (int)gridDim.x * NW * 64;
+    if (threadIdx.x == 0) st.cnt[0] = 0;
+    __syncthreads();
     unsigned long long my_proc = 0, my_scan = 0;
-    for (int b0 = ((int)blockIdx.x * NW + wave_in_block()) * 64; b0 < tot; b0 += step) {
-        const int i = b0 + lane;
+    for (int p0 = (int)blockIdx.x * NW * 64; p0 < tot; p0 += (int)gridDim.x * NW * 64) {
+        const int i = p0 + wave_in_block() * 64 + lane;
         const bool valid = i < tot;
         const int32_t k = valid ? here.at(X, i) : 0;
         SbcVInfo vi{0, 0, 0, 0};
@@ -2824,8 +2878,8 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_light_kernel(SbcParam
                 t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
 #pragma unroll
             for (int u = 0; u < WIDE_LU; ++u) {
-                const int pu = wave_append(appU[u], &w.cnt[2]);
-                if (pu >= 0) X.list[pu] = cc[u];
+                const int pu = wave_append(appU[u], &st.cnt[0]);
+                if (pu >= 0) st.buf[pu] = cc[u];
             }
             // the terms in list order: a child's new dependency, then the
             // removal of its old one (the addition of the negated value)
@@ -2836,6 +2890,14 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_light_kernel(SbcParam
             }
         }
         if (light) st_reld(&X.vr[k].ndel, acc);
+        // the pass's marked parents to the ascent list
+        __syncthreads();
+        if (threadIdx.x == 0) st.base[0] = st.cnt[0] ? atomicAdd(&w.cnt[2], st.cnt[0]) : 0;
+        __syncthreads();
+        for (int j = threadIdx.x; j < st.cnt[0]; j += NW * 64) X.list[st.base[0] + j] = st.buf[j];
+        __syncthreads();
+        if (threadIdx.x == 0) st.cnt[0] = 0;
+        __syncthreads();
     }
     wide_add_stats(w, my_proc, my_scan, 0ull);
 }
@@ -2844,6 +2906,10 @@ __global__ __launch_bounds__(NW * 64) void sbc_wide_ascent_light_kernel(SbcParam
 // wavefronts 1..GIANT_TW compute one slab's terms (loads, marks, appends)
 // and place them in LDS in list order while lane 0 of wavefront 0 adds the
 // previous slab's, one after another; two barriers a slab.
+#ifndef SBC_GIANT_CHAIN
+#define SBC_GIANT_CHAIN 8
+#endif
+constexpr int GIANT_CHAIN = SBC_GIANT_CHAIN;  // values per register buffer of the adding lane
 constexpr int GIANT_TW = 15;
 constexpr int GIANT_CPW = 4;                     // chunks per term wavefront per slab
 constexpr int GIANT_CH = GIANT_TW * GIANT_CPW;  // 3840 neighbours per slab
@@ -2894,7 +2960,7 @@ __device__ __forceinline__ void giant_ascent(const SbcParams& P, const Item& X, 
                 if (lane == 0) gs.cnt[(wave - 1) * GIANT_CPW + u] = __popcll(cm[u]) + __popcll(sm[u]);
             }
         } else if (wave == 0 && sl >= 1 && lane == 0) {
-            acc = lds_chain_sum_pipe(acc, gs.v[(sl - 1) & 1], tprev);
+            acc = lds_chain_sum_pipe<GIANT_CHAIN>(acc, gs.v[(sl - 1) & 1], tprev);
         }
         __syncthreads();
         int tcur = 0;
