@@ -472,7 +472,8 @@ constexpr int RUN_CAP = 64;    // ascent terms per wave and hub kept in LDS
 constexpr int PHASH = 2 * HUB_CAP;      // LDS hash: vertex -> number in the level's lookup list
 constexpr int SBITS_LOG = 14;           // bits of the membership filter beside the hash
 constexpr int SBITS_WORDS = (1 << SBITS_LOG) / 32;
-constexpr int TWOHOP_CAP = 1024;        // two-hop set of the root kept for level-3 lookups
+constexpr int TWOHOP_CAP = 1024;
+constexpr int PREF_CAP = 128;           // window items a team classifies in one pass        // two-hop set of the root kept for level-3 lookups
 // Lookup buckets: a pass's SBC_BUCKET_SLOTS term slots are shared out evenly
 // between its lookup hubs, at most TEAM_SORT_CAP per hub; a bucket of up to
 // WAVE_SORT_CAP terms is sorted by one wavefront, a larger one by the whole
@@ -527,6 +528,17 @@ struct alignas(16) HubShared {
     double radd[16][RUN_CAP];
     double rsub[16][RUN_CAP];
     uint8_t rflag[16][RUN_CAP];
+    // window items [pf0, pf1) of the team's root, fetched together: their
+    // endpoints and two bits each, gap 0 (nothing to do) and an endpoint on
+    // level 0 or 1 (the root's two-hop set changes); valid until an item
+    // moves vertices (only moves change levels)
+    int pf0, pf1;
+    int32_t pu[PREF_CAP];
+    int32_t pv[PREF_CAP];
+    int32_t pdu[PREF_CAP];   // their levels
+    int32_t pdv[PREF_CAP];
+    uint32_t pzero[PREF_CAP / 32];
+    uint32_t plow[PREF_CAP / 32];
 };
 
 __device__ __forceinline__ int phash_slot(int32_t v) { return (int)(((uint32_t)v * 0x9E3779B1u) >> 21) & (PHASH - 1); }
@@ -2139,7 +2151,8 @@ __device__ __forceinline__ void park_in_ascent(const SbcParams& P, int ri, int i
 // that the untraced kernel carries none of the trace state).
 template <int TW, bool TRACE = false>
 __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int item, TeamShared& ts, HubShared* hs,
-                                             LightShared* ls, int twave)
+                                             LightShared* ls, int twave, int32_t pu = -1, int32_t pv = -1,
+                                             int32_t pdu = 0, int32_t pdv = 0)
 {
     const int lane = lane_id();
     const bool leader = (twave == 0 && lane == 0);
@@ -2154,11 +2167,13 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     X.r = P.roots[ri];
     int32_t* lvlD = P.lvl + (size_t)ri * 2 * P.lcap;
     int32_t* lvlS = lvlD + P.lcap;
-    const int32_t u = P.win ? P.wu[item] : P.u0;
-    const int32_t v = P.win ? P.wv[item] : P.v0;
+    // the endpoints and their levels: given (classified in a team pass,
+    // no vertex has moved since), or read here
+    const int32_t u = pu >= 0 ? pu : (P.win ? P.wu[item] : P.u0);
+    const int32_t v = pu >= 0 ? pv : (P.win ? P.wv[item] : P.v0);
     X.q = (P.win == nullptr) ? INF32 : P.win->seq0 + item;
 
-    const int32_t du = X.dist[u], dv = X.dist[v];
+    const int32_t du = pu >= 0 ? pdu : X.dist[u], dv = pu >= 0 ? pdv : X.dist[v];
     // an inserted edge at the root or at one of its neighbours changes the
     // root's two-hop set
     if constexpr (TW > 1)
@@ -2185,7 +2200,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
     X.p = gap > 0 ? v : u;
     const int64_t agap = gap > 0 ? gap : -gap;
     const bool smoved = agap >= 2;
-    const int32_t base = X.dist[X.p] + 1; // new level of s
+    const int32_t base = (gap > 0 ? dv : du) + 1; // new level of s (one above p)
 
     if (leader) {
         st_sb(&X.sb[X.s], sb_make(base, smoved ? 2 : 1));
@@ -2566,6 +2581,43 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
 }
 
 
+// Classifies window items [i0, i0 + PREF_CAP) of root ri for its team: one
+// thread per item reads the endpoints and their levels (what process_item
+// reads first), so items of gap 0 are passed over without a round trip
+// each.  Whole workgroup; the first barrier lets every thread finish
+// reading the previous pass.
+template <int TW>
+__device__ __forceinline__ void classify_items(const SbcParams& P, int ri, int i0, int W, HubShared& hs)
+{
+    __syncthreads();
+    const int n = min(W - i0, PREF_CAP);
+    const int32_t* dist = P.dist + (size_t)ri * (size_t)P.nv;
+    static_assert(PREF_CAP <= TW * 64, "one thread per item");
+    const int t = threadIdx.x;
+    bool zero = false, low = false;
+    if (t < n) {
+        const int32_t u = P.wu[i0 + t], v = P.wv[i0 + t];
+        const int32_t du = dist[u], dv = dist[v];
+        hs.pu[t] = u;
+        hs.pv[t] = v;
+        hs.pdu[t] = du;
+        hs.pdv[t] = dv;
+        zero = (du == INF32 && dv == INF32) || du == dv;
+        low = du <= 1 || dv <= 1;
+    }
+    if (t < PREF_CAP) {
+        const uint64_t zb = ballot(zero), lb = ballot(low);
+        if (lane_id() == 0) {
+            hs.pzero[2 * (t >> 6)] = (uint32_t)zb;
+            hs.pzero[2 * (t >> 6) + 1] = (uint32_t)(zb >> 32);
+            hs.plow[2 * (t >> 6)] = (uint32_t)lb;
+            hs.plow[2 * (t >> 6) + 1] = (uint32_t)(lb >> 32);
+        }
+    }
+    if (t == 0) { hs.pf0 = i0; hs.pf1 = i0 + n; }
+    __syncthreads();
+}
+
 // Persistent window kernel.  Phase 1: whole-workgroup teams pop root slots
 // [0, heavy) of the queue order; phase 2: each wavefront pops slots
 // [heavy, K).  A team applies every insertion of the window to its root in
@@ -2587,7 +2639,39 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
         __syncthreads();
         if (t >= P.heavy || t >= P.qlen) break;
         const int ri = P.order ? P.order[t] : t;
-        for (int i = P.start_item ? P.start_item[ri] : 0; i < W; ++i)
+        const int i0 = P.start_item ? P.start_item[ri] : 0;
+        if constexpr (NWB == 16 && !TRACE) {
+            if (P.win) {
+                // items classified a pass at a time; gap-0 items cost no
+                // round trip; a pass holds until an item moves vertices
+                bool fresh = false;
+                for (int i = i0; i < W; ++i) {
+                    if (!fresh || i >= hs.pf1) {
+                        classify_items<NWB>(P, ri, i, W, hs);
+                        fresh = true;
+                    }
+                    const int b = i - hs.pf0;
+                    if ((hs.pzero[b >> 5] >> (b & 31)) & 1u) {
+                        if (threadIdx.x == 0) {
+                            if ((hs.plow[b >> 5] >> (b & 31)) & 1u) hs.two_ok = 0;
+                            if (P.rec) {
+                                SbcItemRec* rec = &P.rec[(size_t)i * P.K + ri];
+                                rec->root_index = ri; rec->insertion = i; rec->kase = 0; rec->gap = 0;
+                                rec->processed = 0; rec->scanned = 0; rec->affected = 0; rec->moved = 0;
+                            }
+                        }
+                        continue;
+                    }
+                    if (process_item<NWB, TRACE>(P, ri, i, ts[0], &hs, &ls, wave, hs.pu[b], hs.pv[b], hs.pdu[b],
+                                                 hs.pdv[b]))
+                        break;
+                    if (ts[0].stat[2] != 0) fresh = false;
+                    __syncthreads();
+                }
+                continue;
+            }
+        }
+        for (int i = i0; i < W; ++i)
             if (process_item<NWB, TRACE>(P, ri, i, ts[0], &hs, &ls, wave)) break;
     }
     // phase 2: light, one wavefront per root
