@@ -660,6 +660,16 @@ struct extraArraysPerThread {
     int32_t* d_order2 = nullptr;
     int32_t* d_start = nullptr;
     int64_t park_cap = 0;
+    // finishing parked items while the window launch runs: a second stream
+    // (default priority), the resumed root's own queue counters and parked
+    // list, and the per-root flags the launch sets in host memory
+    hipStream_t stream2 = nullptr;
+    int32_t* d_qctr2 = nullptr;
+    int32_t* d_parked2 = nullptr;
+    int32_t* h_pflag = nullptr;   // pinned host memory, [K]; device views d_pflag / d_proot
+    int32_t* h_proot = nullptr;
+    int32_t* d_pflag = nullptr;
+    int32_t* d_proot = nullptr;
     std::vector<int32_t> h_start;
     int64_t last_parked = 0;   // items finished by the wide kernels in the last batched call
     // device scheduling (streamingBCScheduleWindow): buffers and a valid-order flag
@@ -739,6 +749,12 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_order) (void)hipFree(C->d_order);
     for (void* q : {(void*)C->d_park, (void*)C->d_parked, (void*)C->d_order2, (void*)C->d_start})
         if (q) (void)hipFree(q);
+    if (C->stream2) (void)hipStreamSynchronize(C->stream2);
+    for (void* q : {(void*)C->d_qctr2, (void*)C->d_parked2})
+        if (q) (void)hipFree(q);
+    for (void* q : {(void*)C->h_pflag, (void*)C->h_proot})
+        if (q) (void)hipHostFree(q);
+    if (C->stream2) (void)hipStreamDestroy(C->stream2);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->d_wtot) (void)hipFree(C->d_wtot);
     if (C->d_wl) (void)hipFree(C->d_wl);
@@ -1621,6 +1637,19 @@ static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K)
     if ((st = dalloc(&C->d_park, (size_t)K)) || (st = dalloc(&C->d_parked, (size_t)K)) ||
         (st = dalloc(&C->d_order2, (size_t)K)) || (st = dalloc(&C->d_start, (size_t)K)))
         return st;
+    if (!C->stream2) HIPCHK(hipStreamCreateWithFlags(&C->stream2, hipStreamNonBlocking));
+    HIPCHK(hipStreamSynchronize(C->stream2));
+    for (void* q : {(void*)C->d_qctr2, (void*)C->d_parked2})
+        if (q) HIPCHK(hipFree(q));
+    for (void* q : {(void*)C->h_pflag, (void*)C->h_proot})
+        if (q) HIPCHK(hipHostFree(q));
+    C->d_qctr2 = C->d_parked2 = C->h_pflag = C->h_proot = C->d_pflag = C->d_proot = nullptr;
+    if ((st = dalloc(&C->d_qctr2, 4)) || (st = dalloc(&C->d_parked2, (size_t)K))) return st;
+    HIPCHK(hipHostMalloc((void**)&C->h_pflag, sizeof(int32_t) * K, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostMalloc((void**)&C->h_proot, sizeof(int32_t) * K, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&C->d_pflag, C->h_pflag, 0));
+    HIPCHK(hipHostGetDevicePointer((void**)&C->d_proot, C->h_proot, 0));
+    memset(C->h_pflag, 0, sizeof(int32_t) * K);
     C->h_start.assign((size_t)K, 0);
     C->park_cap = K;
     C->gc.with_rec = -1; // cached graph holds the old pointers
@@ -1787,6 +1816,98 @@ static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const S
     }
 }
 
+// Points the context's helpers (copies, waits, timing events) at another
+// stream while the guard lives.
+struct StreamScope {
+    extraArraysPerThread* C;
+    hipStream_t saved;
+    StreamScope(extraArraysPerThread* c, hipStream_t st) : C(c), saved(c->stream) { c->stream = st; }
+    ~StreamScope() { C->stream = saved; }
+};
+
+// Finishes root ri's parked item, then resumes the root alone at its next
+// item, again after every item it parks (on the context's current stream).
+// The resumed launch has its own queue counters and parked list and the
+// per-workgroup scratch slice past the window launch's grid, so it can run
+// beside that launch.
+static sbcStatus finish_root_alone(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
+                                   int32_t ri, int64_t* total)
+{
+    for (;;) {
+        *total += 1;
+        sbcStatus st = finish_parked_item(F, C, P, ri);
+        if (st) return st;
+        SbcParams Q = P;
+        Q.order = C->d_order2;
+        Q.heavy = 1;
+        Q.qlen = 1;
+        Q.start_item = C->d_start;
+        Q.qctr = C->d_qctr2;
+        Q.parked = C->d_parked2;
+        Q.hpark_flag = nullptr;
+        Q.hpark_root = nullptr;
+        if (Q.bucket) Q.bucket += (size_t)grid * SBC_BUCKET_SLOTS;
+        if (Q.hubq) Q.hubq += (size_t)grid * SBC_HUBQ_CAP;
+        HIPCHK(hipMemcpyAsync(C->d_order2, &ri, sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_start + ri, &C->h_start[ri], sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(sbc_launch_reset(C->d_qctr2, 4, C->stream));
+        HIPCHK(sbc_launch_window(&Q, nwb, 1, C->stream));
+        int32_t np = 0;
+        HIPCHK(hipMemcpyAsync(&np, C->d_qctr2 + 2, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+        // also keeps ri and h_start[ri] unchanged until their uploads ran
+        HIPCHK(hipStreamSynchronize(C->stream));
+        if (np == 0) return SBC_OK;
+        if (np != 1) return set_err(SBC_ERR_STATE, "resumed root %d parked %d items", (int)ri, (int)np);
+    }
+}
+
+// While the window launch runs on the context stream, every root it parks
+// (published in host memory: C->h_proot[i], then C->h_pflag[i]) is finished
+// on the second stream, so the hand-over and the root's remaining items use
+// the CUs the launch's other roots have left.  Returns once the launch has
+// ended and every parked root is done.
+static sbcStatus finish_parked_concurrent(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb,
+                                          int grid, int64_t* total)
+{
+    sbcStatus err = SBC_OK;
+    int handled = 0;
+    hipEvent_t e1 = nullptr;
+    bool timed = false;
+    for (;;) {
+        // the launch's state is read first: once it has ended, every flag it
+        // set is visible
+        const hipError_t q = hipStreamQuery(C->stream);
+        const bool ended = q != hipErrorNotReady;
+        while (handled < F->K && __atomic_load_n(&C->h_pflag[handled], __ATOMIC_ACQUIRE) != 0) {
+            const int32_t ri = __atomic_load_n(&C->h_proot[handled], __ATOMIC_RELAXED);
+            handled++;
+            if (err) continue;
+            if (ri < 0 || ri >= F->K) {
+                err = set_err(SBC_ERR_STATE, "parked root %d out of range", (int)ri);
+                continue;
+            }
+            StreamScope on2(C, C->stream2);
+            if (!timed) {
+                timed = true;
+                if ((err = ev_begin(C, KSTAT_WIDE, &e1))) continue;
+            }
+            err = finish_root_alone(F, C, P, nwb, grid, ri, total);
+        }
+        if (ended) {
+            if (q != hipSuccess && !err) err = set_err(SBC_ERR_HIP, "window launch: %s", hipGetErrorString(q));
+            break;
+        }
+    }
+    if (timed) {
+        StreamScope on2(C, C->stream2);
+        const sbcStatus e2 = ev_end(C, KSTAT_WIDE, e1);
+        if (!err) err = e2;
+    }
+    const hipError_t e3 = hipStreamSynchronize(C->stream2);
+    if (!err && e3 != hipSuccess) err = set_err(SBC_ERR_HIP, "hand-over stream: %s", hipGetErrorString(e3));
+    return err;
+}
+
 static sbcStatus enqueue_window(const SbcParams& P, const SbcAppend& A, int nwb, int grid, int W,
                                 int recompute, extraArraysPerThread* C, bool timed)
 {
@@ -1866,8 +1987,9 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         int64_t cap = 2LL * C->num_cu;
         grid = (int)(g < cap ? (g > 0 ? g : 1) : cap);
     }
-    if ((st = ensure_bucket(C, G, grid))) return st;
-    if ((st = ensure_hubq(C, grid))) return st;
+    // one scratch slice more: a root resumed beside the window launch
+    if ((st = ensure_bucket(C, G, grid + 1))) return st;
+    if ((st = ensure_hubq(C, grid + 1))) return st;
     // capacity: relayout once up front if any endpoint lacks room for the whole batch
     if ((int64_t)G->h_cnt.size() != G->nv) G->h_cnt.assign((size_t)G->nv, 0);
     if ((int64_t)G->h_mark.size() != G->nv) G->h_mark.assign((size_t)G->nv, 0);
@@ -1921,6 +2043,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     if (park_min > 0 && nwb > 1) {
         P.park = C->d_park;
         P.parked = C->d_parked;
+        P.hpark_flag = C->d_pflag;
+        P.hpark_root = C->d_proot;
         P.park_min = park_min;
         // an ascent level whose hubs leave this many adjacency entries to
         // scan is handed over too: one workgroup scans 50-150 million entries
@@ -1980,6 +2104,12 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         }
     }
 
+    if (P.park) {
+        // the whole-GPU levels' scratch, allocated now: nothing is allocated
+        // or freed while a window launch runs beside the hand-over stream
+        SbcWideLists L;
+        if ((st = wide_lists(C, P, &L))) return st;
+    }
     std::vector<int32_t> prev_touched;
     int64_t nparked = 0, nparked_before = 0;
     std::vector<int32_t> wu(W), wv(W), slot(2 * W), scol(2 * W), sseq(2 * W), vv, touched;
@@ -2041,6 +2171,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         if (!vi.empty())
             HIPCHK(hipMemcpyAsync(S.d_vi, hv, sizeof(SbcVInfo) * vi.size(), hipMemcpyHostToDevice, C->stream));
         HIPCHK(hipEventRecord(S.h_done[b], C->stream));
+        // the last window's launch has ended: no flag is being set
+        if (P.park) memset(C->h_pflag, 0, sizeof(int32_t) * F->K);
         if (use_graph) {
             hipEvent_t e0;
             if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
@@ -2050,13 +2182,9 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         } else {
             if ((st = enqueue_window(P, A, nwb, grid, W, cfg->recompute_bc, C, true))) return st;
         }
-        // items this window handed over are finished before the next window
-        if (P.park) {
-            hipEvent_t e1;
-            if ((st = ev_begin(C, KSTAT_WIDE, &e1))) return st;
-            if ((st = finish_parked(F, C, P, nwb, grid, &nparked))) return st;
-            if ((st = ev_end(C, KSTAT_WIDE, e1))) return st;
-        }
+        // items this window hands over are finished while it runs, and
+        // before the next window
+        if (P.park && (st = finish_parked_concurrent(F, C, P, nwb, grid, &nparked))) return st;
         // the captured BC reduction ran before the parked items were finished
         if (nparked > nparked_before && cfg->recompute_bc) HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
         nparked_before = nparked;

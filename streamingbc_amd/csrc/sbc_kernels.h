@@ -124,6 +124,11 @@ struct SbcParams {
     // start_item[ri]
     struct SbcWideItem* park;   // [K] or nullptr (never park)
     int32_t* parked;            // [K]
+    // a parked root also published in host memory while the launch runs:
+    // hpark_root[i] = the i-th parked root, then hpark_flag[i] = 1 (a
+    // system-scope release), or nullptr
+    int32_t* hpark_root;
+    int32_t* hpark_flag;
     int32_t park_min;
     int64_t park_scan;
     const int32_t* start_item;  // [K] first item per root, or nullptr (0)

@@ -2102,6 +2102,21 @@ __device__ __forceinline__ void add_work_totals(int64_t* t, const unsigned long 
     if (affected > t[5]) t[5] = affected;
 }
 
+// Lists root ri as parked (its item saved in P.park[ri]; the caller is the
+// team leader, after the team's writes and a barrier) and, when the host
+// finishes parked items while the launch runs, publishes it in host memory:
+// the root, then its flag with a system-scope release, which makes the
+// item's state visible to work launched on another stream.
+__device__ __forceinline__ void publish_parked(const SbcParams& P, int ri)
+{
+    const int pi = atomicAdd(&P.qctr[2], 1);
+    P.parked[pi] = ri;
+    if (P.hpark_flag) {
+        P.hpark_root[pi] = ri;
+        __hip_atomic_store(&P.hpark_flag[pi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // Hands the item over to the wide kernels in ascent level lev, after the
 // team processed the level's light vertices and looked up its first pass of
 // hubs (the light vertices' counts stay in the team's; the wide kernels redo
@@ -2140,7 +2155,7 @@ __device__ __forceinline__ void park_in_ascent(const SbcParams& P, int ri, int i
         w.lev = lev;
         w.uA = uA;
         w.uB = uB;
-        P.parked[atomicAdd(&P.qctr[2], 1)] = ri;
+        publish_parked(P, ri);
     }
     team_sync<TW>();
 }
@@ -2288,7 +2303,7 @@ __device__ __forceinline__ bool process_item(const SbcParams& P, int ri, int ite
                     w.gap = (int32_t)(gap > INF32 ? INF32 : (gap < -INF32 ? -INF32 : gap));
                     w.smoved = smoved ? 1 : 0;
                     w.phase = 0;
-                    P.parked[atomicAdd(&P.qctr[2], 1)] = ri;
+                    publish_parked(P, ri);
                 }
                 team_sync<TW>();
                 return true;
