@@ -15,6 +15,7 @@
 #include <string>
 #include <algorithm>
 #include <unordered_set>
+#include <thread>
 #include <vector>
 
 #include "sbc_costmodel.h"
@@ -1897,6 +1898,8 @@ static sbcStatus finish_parked_concurrent(sbcForestDev* F, extraArraysPerThread*
             if (q != hipSuccess && !err) err = set_err(SBC_ERR_HIP, "window launch: %s", hipGetErrorString(q));
             break;
         }
+        // several serving workers may poll at once
+        std::this_thread::yield();
     }
     if (timed) {
         StreamScope on2(C, C->stream2);
