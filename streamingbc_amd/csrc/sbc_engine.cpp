@@ -87,6 +87,11 @@ struct stinger {
     std::vector<int32_t> p_hub;  // vertices whose table changed since the last upload (edge removals)
 };
 
+// (root, vertex) pairs a level of the deletion's whole-GPU delta pass hands
+// to one-workgroup-per-vertex sums; a level with more does the rest one
+// wavefront per vertex.
+static const int32_t kDmGiantCap = 65536;
+
 // Vertices with more initial neighbours than this get a sorted index (the
 // update kernel looks neighbours up in it instead of scanning hub lists).
 static const int32_t kHubIndexMin = 512;
@@ -696,6 +701,8 @@ struct extraArraysPerThread {
     // scratch of the parked items' whole-GPU levels: counters, heavy list, aux list
     int32_t* d_wl = nullptr;
     int64_t wl_nv = 0, wl_aux = 0;
+    // the deletion's whole-GPU delta pass: a count and kDmGiantCap (root, vertex) pairs
+    int32_t* d_dmg = nullptr;
     int64_t wtot_k = 0;
     // per-workgroup term buckets of the hub lookup pass
     SbcBucketEnt* d_bucket = nullptr;
@@ -759,6 +766,7 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->d_tprof) (void)hipFree(C->d_tprof);
     if (C->d_wtot) (void)hipFree(C->d_wtot);
     if (C->d_wl) (void)hipFree(C->d_wl);
+    if (C->d_dmg) (void)hipFree(C->d_dmg);
     if (C->d_bucket) (void)hipFree(C->d_bucket);
     if (C->d_hubq) (void)hipFree(C->d_hubq);
     if (C->d_del) (void)hipFree(C->d_del);
@@ -1396,7 +1404,13 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         Q.v0 = (int32_t)v;
         Q.sroots = mlist;
         Q.scnt = C->d_del + 3;
-        if (wide_delta) Q.dm_out = C->d_del + 8 + 2 * F->K;
+        if (wide_delta) {
+            Q.dm_out = C->d_del + 8 + 2 * F->K;
+            // vertices of many neighbours of a level: one workgroup each
+            if (!C->d_dmg && (st = dalloc(&C->d_dmg, (size_t)(1 + 2 * kDmGiantCap)))) return st;
+            Q.dm_giant = C->d_dmg;
+            Q.dm_giant_cap = kDmGiantCap;
+        }
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
         if ((st = ev_begin(C, KSTAT_STATIC, &e0))) return st;
         int grid = (int)(h[3] < 2LL * C->num_cu ? h[3] : 2LL * C->num_cu);

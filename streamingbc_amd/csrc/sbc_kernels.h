@@ -148,6 +148,12 @@ struct SbcParams {
     // whole GPU (sbc_delmove_level_kernel, sbc_delmove_finish_kernel), with
     // the list lengths frozen for each level launch in dm_out[2K + q]
     int32_t* dm_out;
+    // whole-GPU delta pass: vertices of a level with WIDE_GIANT or more
+    // neighbours, listed as (root number, vertex) pairs after a count in
+    // dm_giant[0] (at most dm_giant_cap pairs; past it a wavefront does the
+    // vertex), each then summed by a workgroup; or nullptr
+    int32_t* dm_giant;
+    int32_t dm_giant_cap;
 };
 
 // A parked item (one per root slot), in global memory.  cnt/stat have the

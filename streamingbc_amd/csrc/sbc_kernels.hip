@@ -3018,42 +3018,26 @@ struct GiantShared {
     int cnt[GIANT_CH];
 };
 
-__device__ __forceinline__ void giant_ascent(const SbcParams& P, const Item& X, int32_t k, int lev, SbcWideItem& w,
-                                             GiantShared& gs)
+// acc + the terms of a giant vertex's nch 64-neighbour chunks in list order,
+// slab by slab: wavefronts 1..GIANT_TW call terms(c0, t) for chunks
+// [c0, c0 + GIANT_CPW) (lane l's term of chunk c0 + u in t[u]) and place the
+// terms in LDS in list order, while lane 0 of wavefront 0 adds the previous
+// slab's.  Whole workgroup of 16 wavefronts; the sum is valid in lane 0 of
+// wavefront 0.
+template <typename Terms>
+__device__ __forceinline__ double giant_sum(double acc, int nch, GiantShared& gs, Terms terms)
 {
     const int wave = wave_in_block(), lane = lane_id();
-    const SbcVInfo vi = P.vinfo[k];
-    const int len = __builtin_amdgcn_readfirstlane(visible_len(P, vi, X.q));
-    const KState ks = kstate(P, X, k);
-    const int nch = (len + 63) >> 6;
     const int nslab = (nch + GIANT_CH - 1) / GIANT_CH;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    double acc = ks.base;  // wavefront 0, lane 0
-    int tprev = 0;         // values of the slab wavefront 0 adds next
+    int tprev = 0;  // values of the slab wavefront 0 adds next
     for (int sl = 0; sl <= nslab; ++sl) {
         Term t[GIANT_CPW];
         uint64_t cm[GIANT_CPW], sm[GIANT_CPW];
         if (wave >= 1 && sl < nslab) {
-            int32_t cc[GIANT_CPW];
-            bool ok[GIANT_CPW];
-            uint8_t ww[GIANT_CPW];
-            bool appU[GIANT_CPW];
+            terms(sl * GIANT_CH + (wave - 1) * GIANT_CPW, t);
 #pragma unroll
             for (int u = 0; u < GIANT_CPW; ++u) {
-                const int ch = sl * GIANT_CH + (wave - 1) * GIANT_CPW + u;
-                const int e = ch * 64 + lane;
-                ok[u] = ch < nch && e < len;
-                cc[u] = ok[u] ? P.col[vi.start + e] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < GIANT_CPW; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
-#pragma unroll
-            for (int u = 0; u < GIANT_CPW; ++u)
-                t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
-#pragma unroll
-            for (int u = 0; u < GIANT_CPW; ++u) {
-                const int pu = wave_append(appU[u], &w.cnt[2]);
-                if (pu >= 0) X.list[pu] = cc[u];
                 cm[u] = ballot(t[u].child);
                 sm[u] = ballot(t[u].hsub);
                 if (lane == 0) gs.cnt[(wave - 1) * GIANT_CPW + u] = __popcll(cm[u]) + __popcll(sm[u]);
@@ -3088,7 +3072,40 @@ __device__ __forceinline__ void giant_ascent(const SbcParams& P, const Item& X, 
         tprev = tcur;
         __syncthreads();
     }
-    if (wave == 0 && lane == 0) st_reld(&X.vr[k].ndel, acc);
+    return acc;
+}
+
+__device__ __forceinline__ void giant_ascent(const SbcParams& P, const Item& X, int32_t k, int lev, SbcWideItem& w,
+                                             GiantShared& gs)
+{
+    const int lane = lane_id();
+    const SbcVInfo vi = P.vinfo[k];
+    const int len = __builtin_amdgcn_readfirstlane(visible_len(P, vi, X.q));
+    const KState ks = kstate(P, X, k);
+    const int nch = (len + 63) >> 6;
+    const double acc = giant_sum(ks.base, nch, gs, [&](int c0, Term (&t)[GIANT_CPW]) {
+        int32_t cc[GIANT_CPW];
+        bool ok[GIANT_CPW];
+        uint8_t ww[GIANT_CPW];
+        bool appU[GIANT_CPW];
+#pragma unroll
+        for (int u = 0; u < GIANT_CPW; ++u) {
+            const int e = (c0 + u) * 64 + lane;
+            ok[u] = c0 + u < nch && e < len;
+            cc[u] = ok[u] ? P.col[vi.start + e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < GIANT_CPW; ++u) ww[u] = ok[u] ? ld_sb(&X.sb[cc[u]]) : (uint8_t)0;
+#pragma unroll
+        for (int u = 0; u < GIANT_CPW; ++u)
+            t[u] = ascent_lane(P, X, k, ks.isD, ks.sk_old, ks.sk_new, lev, ok[u], cc[u], ww[u], appU[u]);
+#pragma unroll
+        for (int u = 0; u < GIANT_CPW; ++u) {
+            const int pu = wave_append(appU[u], &w.cnt[2]);
+            if (pu >= 0) X.list[pu] = cc[u];
+        }
+    });
+    if (wave_in_block() == 0 && lane == 0) st_reld(&X.vr[k].ndel, acc);
 }
 
 // Second launch of an ascent level: giant vertices one per workgroup, then
@@ -4630,8 +4647,8 @@ __device__ __forceinline__ int64_t or64_dev(int64_t* p, int64_t v)
 __global__ __launch_bounds__(1024) void sbc_delmove_level_kernel(SbcParams P, int L)
 {
     const int lane = lane_id();
-    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+    const int gw = (int)blockIdx.x * 16 + wave_in_block();
+    const int nw = (int)gridDim.x * 16;
     const int nq = *P.scnt;
     const int nv = P.nv;
     const int32_t* lim = P.dm_out + 2 * P.K; // list lengths as the launch began
@@ -4652,6 +4669,20 @@ __global__ __launch_bounds__(1024) void sbc_delmove_level_kernel(SbcParams P, in
             const int64_t fx = vr[x].dpath;
             if (!(fx & (DM_SIGCH | DM_DEL)) || dist[x] != L) continue;
             const SbcVInfo vi = P.vinfo[x];
+            // a vertex of many neighbours goes to the level's second launch
+            // (one workgroup each), while the list has room
+            if (P.dm_giant && vi.deg >= (P.wide_giant > 0 ? P.wide_giant : WIDE_GIANT)) {
+                int gp = 0;
+                if (lane == 0) gp = atomicAdd(&P.dm_giant[0], 1);
+                gp = __builtin_amdgcn_readfirstlane(gp);
+                if (gp < P.dm_giant_cap) {
+                    if (lane == 0) {
+                        P.dm_giant[1 + 2 * gp] = q;
+                        P.dm_giant[2 + 2 * gp] = x;
+                    }
+                    continue;
+                }
+            }
             const int64_t sx = vr[x].sigma;
             double acc = 0.0;
             for (int e0 = 0; e0 < vi.deg; e0 += 64 * DM_SEGU) {
@@ -4697,6 +4728,71 @@ __global__ __launch_bounds__(1024) void sbc_delmove_level_kernel(SbcParams P, in
                 }
             }
         }
+    }
+}
+
+// The delta of each listed vertex of level L (the level kernel's rule: the
+// sum over children in adjacency order, from zero), one workgroup per
+// vertex, then its parents marked and listed as the level kernel does.
+__global__ __launch_bounds__(1024) void sbc_delmove_giant_kernel(SbcParams P, int L)
+{
+    __shared__ GiantShared gs;
+    __shared__ int again;
+    const int lane = lane_id(), wave = wave_in_block();
+    const int ng = min(P.dm_giant[0], P.dm_giant_cap);
+    const int nv = P.nv;
+    for (int gi = (int)blockIdx.x; gi < ng; gi += (int)gridDim.x) {
+        const int q = P.dm_giant[1 + 2 * gi];
+        const int32_t x = P.dm_giant[2 + 2 * gi];
+        const int ri = P.sroots[q];
+        const size_t off = (size_t)ri * (size_t)nv;
+        const int32_t* dist = P.dist + off;
+        SbcVRec* vr = P.vr + off;
+        int32_t* list = P.list + off;
+        const SbcVInfo vi = P.vinfo[x];
+        const int deg = __builtin_amdgcn_readfirstlane(vi.deg);
+        const int nch = (deg + 63) >> 6;
+        const int64_t fx = vr[x].dpath;
+        const int64_t sx = vr[x].sigma;
+        const double acc = giant_sum(0.0, nch, gs, [&](int c0, Term (&t)[GIANT_CPW]) {
+            int32_t cc[GIANT_CPW];
+            bool ok[GIANT_CPW];
+#pragma unroll
+            for (int u = 0; u < GIANT_CPW; ++u) {
+                const int e = (c0 + u) * 64 + lane;
+                ok[u] = c0 + u < nch && e < deg;
+                cc[u] = ok[u] ? P.col[vi.start + e] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < GIANT_CPW; ++u) {
+                const bool ch = ok[u] && dist[cc[u]] == L + 1;
+                t[u] = Term{ch, false, ch ? dep(sx, vr[cc[u]].sigma, vr[cc[u]].delta) : 0.0, 0.0};
+            }
+        });
+        if (wave == 0 && lane == 0) {
+            const double od = vr[x].delta;
+            const bool changed = __double_as_longlong(acc) != __double_as_longlong(od);
+            if (changed) {
+                vr[x].delta = acc;
+                if (P.bc_atomic && x != P.roots[ri]) atomicAdd(&P.bc[x], acc - od);
+            }
+            again = (changed || (fx & DM_SIGCH)) ? 1 : 0;
+        }
+        __syncthreads();
+        if (again) {
+            for (int ch = wave; ch < nch; ch += 16) {
+                const int e = ch * 64 + lane;
+                int32_t y = 0;
+                bool app = false;
+                if (e < deg) {
+                    y = P.col[vi.start + e];
+                    if (dist[y] == L - 1) app = !(or64_dev(&vr[y].dpath, DM_DEL | DM_LIST) & DM_LIST);
+                }
+                const int pos = wave_append(app, &P.dm_out[2 * q]);
+                if (pos >= 0) list[pos] = y;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -4879,7 +4975,9 @@ extern "C" hipError_t sbc_launch_delmove(const SbcParams* P, int grid, hipStream
 extern "C" hipError_t sbc_launch_delmove_level(const SbcParams* P, int L, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(sbc_delmove_snap_kernel, dim3(1), dim3(256), 0, st, *P);
+    if (P->dm_giant) hipLaunchKernelGGL(sbc_reset_kernel, dim3(1), dim3(64), 0, st, P->dm_giant, 1);
     hipLaunchKernelGGL(sbc_delmove_level_kernel, dim3(grid), dim3(1024), 0, st, *P, L);
+    if (P->dm_giant) hipLaunchKernelGGL(sbc_delmove_giant_kernel, dim3(grid), dim3(1024), 0, st, *P, L);
     return hipGetLastError();
 }
 

@@ -597,11 +597,14 @@ def _apply_ops(f, g, ctx, o, ops):
 
 
 # movement-case roots: updated in place (the default below 4M adjacency
-# slots), or, with SBC_DELETE_MOVE=rebuild, rebuilt by the static pass: SBC_WIDE_REBUILD_MAX =
+# slots; "incremental_wide": the delta pass over the whole GPU, "..._giant":
+# with every vertex of 8 or more neighbours summed by a workgroup), or, with
+# SBC_DELETE_MOVE=rebuild, rebuilt by the static pass: SBC_WIDE_REBUILD_MAX =
 # 1000 rebuilds them one root at a time over the whole GPU; 0 with
 # SBC_WIDE_GRAPH_MIN = 0 all together over the whole GPU; 0 with the default
 # graph-size threshold (above these graphs) one workgroup per root
-REBUILD_MODES = {"incremental": (None, None), "incremental_wide": (None, None), "single": ("1000", None),
+REBUILD_MODES = {"incremental": (None, None), "incremental_wide": (None, None),
+                 "incremental_wide_giant": (None, None), "single": ("1000", None),
                  "multi": ("0", "0"), "workgroup": ("0", None)}
 
 
@@ -611,7 +614,11 @@ def _rebuild_mode(monkeypatch, mode):
         for k in ("SBC_WIDE_REBUILD_MAX", "SBC_WIDE_GRAPH_MIN"):
             monkeypatch.delenv(k, raising=False)
         # one workgroup per root throughout, or the delta pass over the GPU
-        monkeypatch.setenv("SBC_DELETE_MOVE", "in_place_wide" if mode == "incremental_wide" else "in_place")
+        monkeypatch.setenv("SBC_DELETE_MOVE", "in_place_wide" if mode.startswith("incremental_wide") else "in_place")
+        if mode == "incremental_wide_giant":
+            monkeypatch.setenv("SBC_WIDE_GIANT", "8")
+        else:
+            monkeypatch.delenv("SBC_WIDE_GIANT", raising=False)
         return
     monkeypatch.setenv("SBC_DELETE_MOVE", "rebuild")
     monkeypatch.setenv("SBC_WIDE_REBUILD_MAX", wide)
@@ -656,7 +663,7 @@ def _graph_edges(hg, n, seed):
     return out
 
 
-@pytest.mark.parametrize("mode", ["incremental", "incremental_wide"])
+@pytest.mark.parametrize("mode", ["incremental", "incremental_wide", "incremental_wide_giant"])
 @pytest.mark.parametrize("name", ["chunglu_sparse", "rmat12"])
 def test_delete_movement_in_place_matches_oracle(name, mode, monkeypatch):
     """Many movement-case roots (subtrees cut off, moved down, hubs above
@@ -964,7 +971,8 @@ def test_wide_rebuild_of_a_root_with_a_big_hub(mode, monkeypatch):
     """Movement-case roots when a level holds a vertex with more than 16384
     neighbours: rebuilt over the whole GPU (its terms computed across the GPU
     and summed in order by one wavefront), or updated in place (the hub's
-    delta recomputed by one wavefront); bit-identical to the oracle."""
+    delta recomputed by one wavefront, or, with the delta pass over the whole
+    GPU, by a workgroup over six LDS slabs); bit-identical to the oracle."""
     _rebuild_mode(monkeypatch, mode)
     n = 20100
     edges = [(1, i) for i in range(2, 20002)] + [(i, i + 1) for i in range(2, 300)] + [(20002, 20003), (20003, 5)]
@@ -1109,4 +1117,38 @@ def test_handed_over_giant_hub_matches_oracle(giant, monkeypatch):
             assert np.array_equal(rec[i][key], rec_o[key]), f"insertion {i} {key}"
     _compare_trees(f, o, "giant hub")
     f.recompute_bc(ctx)
+    assert_bc_close(f.bc(), o.bc())
+
+
+def test_delete_giant_hub_delta_pass_matches_oracle(monkeypatch):
+    """Deletions around a 12,000-neighbour hub with the movement case's delta
+    pass over the whole GPU: the hub's sum (a fresh sum over its children in
+    adjacency order) crosses four LDS slabs of one workgroup, and its parents
+    are marked by every wavefront.  Case counts, trees and BC stay the
+    oracle's."""
+    monkeypatch.setenv("SBC_DELETE_MOVE", "in_place_wide")
+    monkeypatch.delenv("SBC_WIDE_GIANT", raising=False)
+    rng = np.random.default_rng(41)
+    n = 20000
+    hub_nb = rng.choice(np.arange(2, n + 1), size=12000, replace=False)
+    eu = rng.integers(2, n + 1, size=3 * n).tolist() + [1] * len(hub_nb)
+    ev = rng.integers(2, n + 1, size=3 * n).tolist() + hub_nb.tolist()
+    hg = sbc.graph_from_edges(n, eu, ev)
+    assert hg.degree()[1] == 12000
+    roots = hg.pick_roots(32, 7)
+    ctx, g, f = _engine(hg, roots)
+    o = Oracle(hg.rowptr, hg.col, roots)
+    o.static(PULL)
+    # edges of the hub and of its neighbours: deleting them moves vertices
+    # below the hub and changes its children
+    ops = [(0, 1, int(w)) for w in hub_nb[:6]]
+    for w in hub_nb[6:12]:
+        nb = hg.col[hg.rowptr[w]:hg.rowptr[w + 1]]
+        nb = nb[nb != 1]
+        if nb.size:
+            ops.append((0, int(w), int(nb[0])))
+    print(f"[delete giant] nv={hg.nv} k={len(roots)} ops={len(ops)}", flush=True)
+    c = _apply_ops(f, g, ctx, o, ops)
+    assert c[:, 2].sum() > 0, "no movement-case root"
+    _compare_trees(f, o, "deletions around a giant hub")
     assert_bc_close(f.bc(), o.bc())
