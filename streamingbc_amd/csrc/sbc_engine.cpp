@@ -1047,6 +1047,9 @@ extern "C" void streamingBCInitStaticApproximate(bcForest* forest, struct stinge
 // ------------------------------------------------------------ plain update
 
 static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K);
+static sbcStatus wide_lists(extraArraysPerThread* C, const SbcParams& P, SbcWideLists* L);
+static sbcStatus finish_parked_concurrent(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb,
+                                          int grid, int nitems, int64_t* total);
 static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
                               int64_t* total);
 
@@ -1077,6 +1080,8 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
         if ((st = ensure_park(C, F->K))) return st;
         P.park = C->d_park;
         P.parked = C->d_parked;
+        P.hpark_flag = C->d_pflag;
+        P.hpark_root = C->d_proot;
         P.park_min = 4096;
         P.park_scan = kParkScan;
         const char* ps = getenv("SBC_PARK_SCAN");  // test hooks: the scan and frontier thresholds
@@ -1088,17 +1093,21 @@ static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerTh
         int64_t g = (F->K + nwb - 1) / nwb;
         grid = (int)(g < 4LL * C->num_cu ? g : 4LL * C->num_cu);
     }
+    if (P.park) {
+        // the whole-GPU levels' scratch before the launch; no flag is set now
+        SbcWideLists L;
+        if ((st = wide_lists(C, P, &L))) return st;
+        memset(C->h_pflag, 0, sizeof(int32_t) * F->K);
+    }
     HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
     hipEvent_t e0;
     if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
     HIPCHK(sbc_launch_window(&P, nwb, grid, C->stream));
     if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
     int64_t parked = 0;
-    if (P.park) {
-        if ((st = ev_begin(C, KSTAT_WIDE, &e0))) return st;
-        if ((st = finish_parked(F, C, P, nwb, grid, &parked))) return st;
-        if ((st = ev_end(C, KSTAT_WIDE, e0))) return st;
-    }
+    // items the launch hands over are finished while it runs (the one
+    // insertion leaves no item to resume)
+    if (P.park && (st = finish_parked_concurrent(F, C, P, nwb, grid, 1, &parked))) return st;
     C->last_parked = parked;
     C->last_rec_n = F->K;
     return check_kernel_err(F, C);
@@ -1846,12 +1855,13 @@ struct StreamScope {
 // per-workgroup scratch slice past the window launch's grid, so it can run
 // beside that launch.
 static sbcStatus finish_root_alone(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
-                                   int32_t ri, int64_t* total)
+                                   int nitems, int32_t ri, int64_t* total)
 {
     for (;;) {
         *total += 1;
         sbcStatus st = finish_parked_item(F, C, P, ri);
         if (st) return st;
+        if (C->h_start[ri] >= nitems) return SBC_OK;  // the root has no item left
         SbcParams Q = P;
         Q.order = C->d_order2;
         Q.heavy = 1;
@@ -1876,13 +1886,13 @@ static sbcStatus finish_root_alone(sbcForestDev* F, extraArraysPerThread* C, con
     }
 }
 
-// While the window launch runs on the context stream, every root it parks
-// (published in host memory: C->h_proot[i], then C->h_pflag[i]) is finished
-// on the second stream, so the hand-over and the root's remaining items use
-// the CUs the launch's other roots have left.  Returns once the launch has
-// ended and every parked root is done.
+// While the window launch (nitems items) runs on the context stream, every
+// root it parks (published in host memory: C->h_proot[i], then
+// C->h_pflag[i]) is finished on the second stream, so the hand-over and the
+// root's remaining items use the CUs the launch's other roots have left.
+// Returns once the launch has ended and every parked root is done.
 static sbcStatus finish_parked_concurrent(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb,
-                                          int grid, int64_t* total)
+                                          int grid, int nitems, int64_t* total)
 {
     sbcStatus err = SBC_OK;
     int handled = 0;
@@ -1906,7 +1916,7 @@ static sbcStatus finish_parked_concurrent(sbcForestDev* F, extraArraysPerThread*
                 timed = true;
                 if ((err = ev_begin(C, KSTAT_WIDE, &e1))) continue;
             }
-            err = finish_root_alone(F, C, P, nwb, grid, ri, total);
+            err = finish_root_alone(F, C, P, nwb, grid, nitems, ri, total);
         }
         if (ended) {
             if (q != hipSuccess && !err) err = set_err(SBC_ERR_HIP, "window launch: %s", hipGetErrorString(q));
@@ -2201,7 +2211,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         }
         // items this window hands over are finished while it runs, and
         // before the next window
-        if (P.park && (st = finish_parked_concurrent(F, C, P, nwb, grid, &nparked))) return st;
+        if (P.park && (st = finish_parked_concurrent(F, C, P, nwb, grid, n, &nparked))) return st;
         // the captured BC reduction ran before the parked items were finished
         if (nparked > nparked_before && cfg->recompute_bc) HIPCHK(sbc_launch_bc_reduce(&P, C->stream));
         nparked_before = nparked;
