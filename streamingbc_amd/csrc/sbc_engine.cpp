@@ -1050,8 +1050,6 @@ static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K);
 static sbcStatus wide_lists(extraArraysPerThread* C, const SbcParams& P, SbcWideLists* L);
 static sbcStatus finish_parked_concurrent(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb,
                                           int grid, int nitems, int64_t* total);
-static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
-                              int64_t* total);
 
 static sbcStatus run_insert_plain(bcForest* forest, stinger* G, extraArraysPerThread* C, int64_t u, int64_t v,
                                   uint32_t granularity)
@@ -1145,8 +1143,6 @@ extern "C" StreamingExtraInfo insertEdgeStreamingBC(bcForest* forest, struct sti
 // before their rebuild and back after.  counts: same-level, adjacent and
 // movement roots, as the reference's StreamingExtraInfo.
 static sbcStatus ensure_park(extraArraysPerThread* C, int64_t K);
-static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
-                              int64_t* total);
 
 // Movement-case roots up to this many per deletion are rebuilt one at a
 // time by the whole GPU (one launch per level); more, one workgroup each.
@@ -1378,19 +1374,27 @@ static sbcStatus run_delete(bcForest* forest, stinger* G, extraArraysPerThread* 
         // as in the windowed form
         Q.park = C->d_park;
         Q.parked = C->d_parked;
+        Q.hpark_flag = C->d_pflag;
+        Q.hpark_root = C->d_proot;
         Q.park_min = 4096;
         Q.park_scan = kParkScan;
         const char* ps = getenv("SBC_PARK_SCAN");  // test hook: the scan threshold alone
         if (ps) Q.park_scan = atoll(ps);
         int grid = (int)(h[2] < 2LL * C->num_cu ? h[2] : 2LL * C->num_cu);
+        {
+            // the whole-GPU levels' scratch before the launch; no flag is set now
+            SbcWideLists L;
+            if ((st = wide_lists(C, Q, &L))) return st;
+            memset(C->h_pflag, 0, sizeof(int32_t) * F->K);
+        }
         HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
         if ((st = ev_begin(C, KSTAT_UPDATE, &e0))) return st;
         HIPCHK(sbc_launch_window(&Q, 16, grid, C->stream));
         if ((st = ev_end(C, KSTAT_UPDATE, e0))) return st;
+        // items handed over are finished while the launch runs (one item per
+        // root: none is resumed)
         int64_t parked = 0;
-        if ((st = ev_begin(C, KSTAT_WIDE, &e0))) return st;
-        if ((st = finish_parked(F, C, Q, 16, grid, &parked))) return st;
-        if ((st = ev_end(C, KSTAT_WIDE, e0))) return st;
+        if ((st = finish_parked_concurrent(F, C, Q, 16, grid, 1, &parked))) return st;
         C->last_parked = parked;
     }
     // movement roots: updated in place (BC by the deltas that change), one
@@ -1802,42 +1806,6 @@ static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, co
     HIPCHK(sbc_launch_wide_commit(&P, dw, ri, nf, nD, uB, nS, wide_grid(C, ncommit / 64 + 1), C->stream));
     C->h_start[ri] = w.item + 1;
     return SBC_OK;
-}
-
-// After a window launch: finishes every parked item and resumes its root at
-// the next item, until no root parks.
-static sbcStatus finish_parked(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int nwb, int grid,
-                              int64_t* total)
-{
-    for (;;) {
-        int32_t np = 0;
-        HIPCHK(hipMemcpyAsync(&np, C->d_qctr + 2, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
-        HIPCHK(hipStreamSynchronize(C->stream));
-        if (np == 0) return SBC_OK;
-        *total += np;
-        if (np < 0 || np > F->K) return set_err(SBC_ERR_STATE, "parked count %d out of range", (int)np);
-        std::vector<int32_t> roots(np);
-        HIPCHK(hipMemcpyAsync(roots.data(), C->d_parked, sizeof(int32_t) * np, hipMemcpyDeviceToHost, C->stream));
-        HIPCHK(hipStreamSynchronize(C->stream));
-        for (int32_t ri : roots) {
-            if (ri < 0 || ri >= F->K) return set_err(SBC_ERR_STATE, "parked root %d out of range", (int)ri);
-            sbcStatus st = finish_parked_item(F, C, P, ri);
-            if (st) return st;
-        }
-        HIPCHK(hipMemcpyAsync(C->d_order2, roots.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, C->stream));
-        HIPCHK(hipMemcpyAsync(C->d_start, C->h_start.data(), sizeof(int32_t) * F->K, hipMemcpyHostToDevice,
-                              C->stream));
-        SbcParams Q = P;
-        Q.order = C->d_order2;
-        Q.heavy = np;
-        Q.qlen = np;
-        Q.start_item = C->d_start;
-        HIPCHK(sbc_launch_reset(C->d_qctr, 4, C->stream));
-        // at most the window's grid: per-workgroup scratch is sized for it
-        HIPCHK(sbc_launch_window(&Q, nwb, np < grid ? np : grid, C->stream));
-        // the host copy of start items must not change before the upload ran
-        HIPCHK(hipStreamSynchronize(C->stream));
-    }
 }
 
 // Points the context's helpers (copies, waits, timing events) at another
