@@ -1718,9 +1718,10 @@ static int wide_grid(const extraArraysPerThread* C, int64_t n)
 }
 
 // Finishes the parked item of root ri: the remaining descent frontiers, the
-// ascent levels and the commit, one whole-GPU launch per level; the level
-// offsets and list counters are read back between launches to drive the
-// same loop as a team's.
+// ascent levels and the commit, with whole-GPU launches per level (one for
+// the light vertices, one lane each, then one for the heavy and giant
+// vertices); the level offsets and list counters are read back between
+// levels to drive the same loop as a team's.
 static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, const SbcParams& P, int32_t ri,
                                     const SbcWideItem& w, const std::vector<int32_t>& lv, int nf, int nD, int nS,
                                     int lev0, int uA, int uB, bool light_counted);

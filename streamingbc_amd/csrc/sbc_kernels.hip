@@ -2711,7 +2711,7 @@ __global__ __launch_bounds__(NWB * 64) void sbc_window_kernel(SbcParams P)
 #endif
 constexpr int WIDE_UNROLL = SBC_WIDE_UNROLL;
 //
-// A parked item is finished by kernels that span the whole GPU, one launch
+// A parked item is finished by kernels that span the whole GPU, two launches
 // per BFS level; the launch boundaries order the levels and make each
 // level's writes visible to the next.  Within a level every wavefront takes
 // vertices of the level in a grid stride and runs the same per-vertex code
