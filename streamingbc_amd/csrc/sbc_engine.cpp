@@ -14,8 +14,8 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
-#include <unordered_set>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "sbc_costmodel.h"
@@ -205,6 +205,67 @@ static bool graph_has_edge(const stinger* G, int32_t u, int32_t v)
     for (int32_t i = 0; i < G->deg[u]; i++)
         if (a[i] == v) return true;
     return false;
+}
+
+// Checks a batch of insertions before anything is touched: the first
+// insertion (in batch order) that is out of range, a self loop, a repeat of
+// an earlier one or an existing edge fails the batch.  The host mirror is
+// read at random (a window's endpoints are spread over arrays of hundreds of
+// MB), so the passes prefetch what the next pass and the window's host
+// staging read: the misses overlap instead of queueing one after another.
+static sbcStatus validate_batch(const stinger* G, const int64_t* u, const int64_t* v, int64_t count)
+{
+    int64_t limit = count, bad_range = -1, bad_loop = -1;
+    for (int64_t i = 0; i < count; i++) {
+        const int64_t a = u[i], b = v[i];
+        if (a < 0 || b < 0 || a >= G->nv || b >= G->nv) { bad_range = i; limit = i; break; }
+        if (a == b) { bad_loop = i; limit = i; break; }
+        for (int64_t x : {a, b}) {
+            __builtin_prefetch(&G->deg[x]);
+            __builtin_prefetch(&G->start[x]);
+            __builtin_prefetch(&G->cap[x]);
+            __builtin_prefetch(&G->deg0[x], 1);
+            if ((int64_t)G->h_mark.size() == G->nv) __builtin_prefetch(&G->h_mark[x], 1);
+            if ((int64_t)G->h_cnt.size() == G->nv) __builtin_prefetch(&G->h_cnt[x], 1);
+        }
+    }
+    // the smaller endpoint's list is scanned; both lists' ends are written
+    for (int64_t i = 0; i < limit; i++) {
+        const int32_t a = (int32_t)u[i], b = (int32_t)v[i];
+        const int32_t s = G->deg[b] < G->deg[a] ? b : a;
+        if ((int64_t)G->start[s] < (int64_t)G->col.size()) __builtin_prefetch(&G->col[G->start[s]]);
+        for (int32_t x : {a, b}) {
+            const int64_t end = (int64_t)G->start[x] + G->deg[x];
+            if (end < (int64_t)G->col.size()) {
+                __builtin_prefetch(&G->col[end], 1);
+                __builtin_prefetch(&G->seq[end], 1);
+            }
+        }
+    }
+    // repeats: sorted (edge, position) pairs; in each run of one edge every
+    // position but the first repeats it
+    std::vector<std::pair<uint64_t, int64_t>> keys((size_t)limit);
+    for (int64_t i = 0; i < limit; i++) {
+        const uint64_t a = (uint64_t)u[i], b = (uint64_t)v[i];
+        keys[(size_t)i] = {(a < b ? a : b) << 32 | (a < b ? b : a), i};
+    }
+    std::sort(keys.begin(), keys.end());
+    int64_t bad_dup = -1;
+    for (size_t k = 1; k < keys.size(); k++)
+        if (keys[k].first == keys[k - 1].first && (bad_dup < 0 || keys[k].second < bad_dup)) bad_dup = keys[k].second;
+    const int64_t edge_limit = bad_dup >= 0 ? bad_dup : limit;
+    int64_t bad_edge = -1;
+    for (int64_t i = 0; i < edge_limit; i++)
+        if (graph_has_edge(G, (int32_t)u[i], (int32_t)v[i])) { bad_edge = i; break; }
+    int64_t bad_exists = bad_dup;
+    if (bad_edge >= 0 && (bad_exists < 0 || bad_edge < bad_exists)) bad_exists = bad_edge;
+    if (bad_exists >= 0)
+        return set_err(SBC_ERR_EXISTS, "insertion %ld (%ld,%ld) duplicates an edge", (long)bad_exists,
+                       (long)u[bad_exists], (long)v[bad_exists]);
+    if (bad_range >= 0) return set_err(SBC_ERR_ARG, "insertion %ld out of range", (long)bad_range);
+    if (bad_loop >= 0) return set_err(SBC_ERR_EXISTS, "insertion %ld is a self loop", (long)bad_loop);
+    if ((int64_t)G->seq_next + count >= INT32_MAX) return set_err(SBC_ERR_ARG, "sequence space exhausted");
+    return SBC_OK;
 }
 
 // Appends v to u's list in the host mirror; returns the slot.
@@ -672,6 +733,7 @@ struct extraArraysPerThread {
     hipStream_t stream2 = nullptr;
     int32_t* d_qctr2 = nullptr;
     int32_t* d_parked2 = nullptr;
+    int32_t* h_err = nullptr;     // pinned host word the kernel error flags are read back into
     int32_t* h_pflag = nullptr;   // pinned host memory, [K]; device views d_pflag / d_proot
     int32_t* h_proot = nullptr;
     int32_t* d_pflag = nullptr;
@@ -736,6 +798,8 @@ static sbcStatus ctx_init(extraArraysPerThread* C)
     HIPCHK(hipMalloc(&C->d_qctr, sizeof(int32_t) * 4));
     HIPCHK(hipMemsetAsync(C->d_qctr, 0, sizeof(int32_t) * 4, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipHostMalloc((void**)&C->h_err, sizeof(int32_t)));
+    *C->h_err = 0;
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -760,7 +824,7 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->stream2) (void)hipStreamSynchronize(C->stream2);
     for (void* q : {(void*)C->d_qctr2, (void*)C->d_parked2})
         if (q) (void)hipFree(q);
-    for (void* q : {(void*)C->h_pflag, (void*)C->h_proot})
+    for (void* q : {(void*)C->h_pflag, (void*)C->h_proot, (void*)C->h_err})
         if (q) (void)hipHostFree(q);
     if (C->stream2) (void)hipStreamDestroy(C->stream2);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
@@ -969,9 +1033,11 @@ static sbcStatus check_kernel_err(sbcForestDev* F, extraArraysPerThread* C)
 {
     sbcStatus st0 = forest_mark(F, C->stream);
     if (st0) return st0;
-    int32_t err = 0;
-    HIPCHK(hipMemcpyAsync(&err, F->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+    // into pinned memory: a copy into pageable memory is staged by the
+    // runtime and costs a few tens of microseconds more per call
+    HIPCHK(hipMemcpyAsync(C->h_err, F->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
+    const int32_t err = *C->h_err;
     if (err & (SBC_KERR_LEVELS | SBC_KERR_WIDE)) {
         int32_t z = 0;
         HIPCHK(hipMemcpyAsync(F->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
@@ -1939,20 +2005,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         return set_err(SBC_ERR_ARG, "block_threads must be 64, 128, 256, 512 or 1024");
     const int nwb = bt / 64;
     // validate the whole batch before touching anything
-    {
-        std::unordered_set<uint64_t> seen;
-        seen.reserve((size_t)count * 2);
-        for (int64_t i = 0; i < count; i++) {
-            int64_t a = u[i], b = v[i];
-            if (a < 0 || b < 0 || a >= G->nv || b >= G->nv) return set_err(SBC_ERR_ARG, "insertion %ld out of range", (long)i);
-            if (a == b) return set_err(SBC_ERR_EXISTS, "insertion %ld is a self loop", (long)i);
-            uint64_t key = ((uint64_t)(a < b ? a : b) << 32) | (uint64_t)(a < b ? b : a);
-            if (!seen.insert(key).second || graph_has_edge(G, (int32_t)a, (int32_t)b))
-                return set_err(SBC_ERR_EXISTS, "insertion %ld (%ld,%ld) duplicates an edge", (long)i, (long)a, (long)b);
-        }
-        if ((int64_t)G->seq_next + count >= INT32_MAX) return set_err(SBC_ERR_ARG, "sequence space exhausted");
-    }
     sbcStatus st;
+    if ((st = validate_batch(G, u, v, count))) return st;
     if ((st = graph_flush(G, C->stream))) return st;
     if ((st = ensure_stage(C, W))) return st;
     if ((st = ensure_rec(C, (int64_t)W * F->K))) return st;

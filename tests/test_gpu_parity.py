@@ -232,6 +232,41 @@ def test_errors_are_reported():
         f.insert_edge(g, ctx, 1, 10_000, sbc.FINE)
 
 
+def test_batch_validation_reports_the_first_offender():
+    """A batched call is checked whole before anything changes: the first
+    insertion in batch order that is out of range, a self loop, a repeat of
+    an earlier one (either direction) or an existing edge fails it, and a
+    refused batch leaves graph and trees as they were."""
+    hg = sbc.read_dimacs(karate_path())
+    ctx, g, f = _engine(hg, np.arange(hg.nv))
+    before = f.trees()
+    deg = [g.degree(x) for x in range(hg.nv)]
+    cases = [
+        ([(3, 6), (6, 3)], sbc.SBC_ERR_EXISTS, 1),            # repeat, reversed
+        ([(3, 6), (1, 2)], sbc.SBC_ERR_EXISTS, 1),            # existing edge
+        ([(3, 6), (5, 10_000)], sbc.SBC_ERR_ARG, 1),          # out of range
+        ([(3, 6), (1, 2), (5, 10_000)], sbc.SBC_ERR_EXISTS, 1),
+        ([(3, 6), (5, 10_000), (1, 2)], sbc.SBC_ERR_ARG, 1),
+        ([(3, 6), (4, 7), (7, 4), (1, 2)], sbc.SBC_ERR_EXISTS, 2),
+        ([(3, 6), (4, 7), (1, 2), (7, 4)], sbc.SBC_ERR_EXISTS, 2),
+        ([(3, 6), (9, 9), (4, 7), (7, 4)], sbc.SBC_ERR_EXISTS, 1),  # self loop first
+        ([(3, 6), (4, 7), (7, 4), (-1, 2)], sbc.SBC_ERR_EXISTS, 2),
+    ]
+    for pairs, status, pos in cases:
+        u = [a for a, _ in pairs]
+        v = [b for _, b in pairs]
+        with pytest.raises(sbc.SbcError) as e:
+            f.insert_batched(g, ctx, u, v, sbc.BatchConfig(window=2, block_threads=1024))
+        assert e.value.status == status, pairs
+        assert f"insertion {pos} " in str(e.value), (pairs, str(e.value))
+    assert [g.degree(x) for x in range(hg.nv)] == deg
+    for p, q in zip(before, f.trees()):
+        assert np.array_equal(p, q)
+    # the batch goes through once it is clean
+    f.insert_batched(g, ctx, [3, 4], [6, 7], sbc.BatchConfig(window=2, block_threads=1024))
+    assert g.degree(3) == deg[3] + 1 and g.degree(7) == deg[7] + 1
+
+
 @pytest.mark.parametrize("name", ["karate", "dolphins", "jazz"])
 def test_gpu_matches_reference_golden(name):
     """Device results against the reference's own outputs (fixtures written
