@@ -689,10 +689,14 @@ extern "C" void streamingBCDeleteForestApproximate(bcForestPtr* deadForest, int6
 enum { KSTAT_UPDATE = 0, KSTAT_STATIC = 1, KSTAT_BC = 2, KSTAT_APPEND = 3, KSTAT_WIDE = 4, KSTAT_N = 5 };
 
 struct WinStage {
-    // device window descriptor and its pinned host twin
+    // device window staging and its pinned host twins, one block each with
+    // the same layout for a window of W insertions, so one copy moves it:
+    // desc at 0 | ints at 64: wu, wv, slot, scol, sseq, vv at stride 4W |
+    // vinfo at 64 + 96W.  The views below are set per call (stage_views).
+    char* d_base = nullptr;
     SbcWinDesc* d_desc = nullptr;
-    int32_t* d_ints = nullptr;     // wu, wv, slot, scol, sseq, vv  (6 * cap)
-    SbcVInfo* d_vi = nullptr;      // cap2
+    int32_t* d_ints = nullptr;
+    SbcVInfo* d_vi = nullptr;
     void* h_buf[2] = {nullptr, nullptr};
     hipEvent_t h_done[2] = {nullptr, nullptr};
     int h_cur = 0;
@@ -840,9 +844,7 @@ static void ctx_free(extraArraysPerThread* C)
     for (void* q : {(void*)C->d_suv, (void*)C->d_shist, (void*)C->d_sX, (void*)C->d_spred, (void*)C->d_scost,
                     (void*)C->d_sheavy, (void*)C->d_hist})
         if (q) (void)hipFree(q);
-    if (C->win.d_desc) (void)hipFree(C->win.d_desc);
-    if (C->win.d_ints) (void)hipFree(C->win.d_ints);
-    if (C->win.d_vi) (void)hipFree(C->win.d_vi);
+    if (C->win.d_base) (void)hipFree(C->win.d_base);
     for (int i = 0; i < 2; i++) {
         if (C->win.h_buf[i]) (void)hipHostFree(C->win.h_buf[i]);
         if (C->win.h_done[i]) (void)hipEventDestroy(C->win.h_done[i]);
@@ -1683,24 +1685,40 @@ static sbcStatus ensure_stage(extraArraysPerThread* C, int W)
     WinStage& S = C->win;
     if (S.cap >= W) return SBC_OK;
     HIPCHK(hipStreamSynchronize(C->stream));
-    if (S.d_desc) HIPCHK(hipFree(S.d_desc));
-    if (S.d_ints) HIPCHK(hipFree(S.d_ints));
-    if (S.d_vi) HIPCHK(hipFree(S.d_vi));
+    if (S.d_base) HIPCHK(hipFree(S.d_base));
+    S.d_base = nullptr;
+    S.d_desc = nullptr;
+    S.d_ints = nullptr;
+    S.d_vi = nullptr;
     for (int i = 0; i < 2; i++) {
         if (S.h_buf[i]) HIPCHK(hipHostFree(S.h_buf[i]));
         S.h_buf[i] = nullptr;
         if (!S.h_done[i]) HIPCHK(hipEventCreateWithFlags(&S.h_done[i], hipEventDisableTiming));
     }
     S.cap = W;
-    HIPCHK(hipMalloc(&S.d_desc, sizeof(SbcWinDesc)));
-    HIPCHK(hipMalloc(&S.d_ints, sizeof(int32_t) * 6 * 4 * (size_t)W));
-    HIPCHK(hipMalloc(&S.d_vi, sizeof(SbcVInfo) * 4 * (size_t)W));
+    HIPCHK(hipMalloc((void**)&S.d_base, stage_bytes(W)));
     for (int i = 0; i < 2; i++) {
         HIPCHK(hipHostMalloc(&S.h_buf[i], stage_bytes(W)));
         HIPCHK(hipEventRecord(S.h_done[i], C->stream));
     }
     C->gc.W = -1; // invalidate cached graph
     return SBC_OK;
+}
+
+static_assert(sizeof(SbcWinDesc) <= 64, "window descriptor exceeds its staging slot");
+
+// Device views of the staging block for windows of W insertions (W <= cap).
+static void stage_views(WinStage& S, int W)
+{
+    S.d_desc = (SbcWinDesc*)S.d_base;
+    S.d_ints = (int32_t*)(S.d_base + 64);
+    S.d_vi = (SbcVInfo*)(S.d_base + 64 + sizeof(int32_t) * 6 * 4 * (size_t)W);
+}
+
+// Bytes from the start of the staging block through nvi vertex records.
+static size_t stage_span(int W, size_t nvi)
+{
+    return 64 + sizeof(int32_t) * 6 * 4 * (size_t)W + sizeof(SbcVInfo) * nvi;
 }
 
 static sbcStatus ensure_order(extraArraysPerThread* C, int64_t K)
@@ -2009,6 +2027,7 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
     if ((st = validate_batch(G, u, v, count))) return st;
     if ((st = graph_flush(G, C->stream))) return st;
     if ((st = ensure_stage(C, W))) return st;
+    stage_views(C->win, W);
     if ((st = ensure_rec(C, (int64_t)W * F->K))) return st;
     if ((st = ensure_order(C, F->K))) return st;
     const int park_min = cfg->park_min == 0 ? 4096 : cfg->park_min;
@@ -2216,10 +2235,8 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
         memcpy(hi + 5 * 4 * W, vv.data(), sizeof(int32_t) * vv.size());
         SbcVInfo* hv = (SbcVInfo*)(hb + 64 + sizeof(int32_t) * 6 * 4 * (size_t)W);
         memcpy(hv, vi.data(), sizeof(SbcVInfo) * vi.size());
-        HIPCHK(hipMemcpyAsync(S.d_desc, hd, sizeof(SbcWinDesc), hipMemcpyHostToDevice, C->stream));
-        HIPCHK(hipMemcpyAsync(S.d_ints, hi, sizeof(int32_t) * 6 * 4 * (size_t)W, hipMemcpyHostToDevice, C->stream));
-        if (!vi.empty())
-            HIPCHK(hipMemcpyAsync(S.d_vi, hv, sizeof(SbcVInfo) * vi.size(), hipMemcpyHostToDevice, C->stream));
+        // one copy: the host block has the device block's layout
+        HIPCHK(hipMemcpyAsync(S.d_base, hb, stage_span(W, vi.size()), hipMemcpyHostToDevice, C->stream));
         HIPCHK(hipEventRecord(S.h_done[b], C->stream));
         // the last window's launch has ended: no flag is being set
         if (P.park) memset(C->h_pflag, 0, sizeof(int32_t) * F->K);
@@ -2273,10 +2290,9 @@ extern "C" sbcStatus streamingBCInsertEdgesBatched(bcForest* forest, struct stin
             hvv[i] = prev_touched[i];
             hv[i] = vinfo_of(G, prev_touched[i]);
         }
-        HIPCHK(hipMemcpyAsync(S.d_desc, hd, sizeof(SbcWinDesc), hipMemcpyHostToDevice, C->stream));
-        HIPCHK(hipMemcpyAsync(S.d_ints + 5 * 4 * W, hvv, sizeof(int32_t) * prev_touched.size(), hipMemcpyHostToDevice,
-                              C->stream));
-        HIPCHK(hipMemcpyAsync(S.d_vi, hv, sizeof(SbcVInfo) * prev_touched.size(), hipMemcpyHostToDevice, C->stream));
+        // one copy of the block (the window's own ints in it are not read by
+        // this launch and are staged again by the next window)
+        HIPCHK(hipMemcpyAsync(S.d_base, hb, stage_span(W, prev_touched.size()), hipMemcpyHostToDevice, C->stream));
         HIPCHK(hipEventRecord(S.h_done[b], C->stream));
         SbcAppend B;
         memset(&B, 0, sizeof(B));
