@@ -737,7 +737,15 @@ struct extraArraysPerThread {
     hipStream_t stream2 = nullptr;
     int32_t* d_qctr2 = nullptr;
     int32_t* d_parked2 = nullptr;
-    int32_t* h_err = nullptr;     // pinned host word the kernel error flags are read back into
+    // pinned host words read back every call (copies into pageable memory
+    // are staged by the runtime and cost tens of microseconds more): the
+    // kernel error flags, the scheduler's heavy-root count, a resumed root's
+    // parked count, a hand-over's list counters after each level
+    int32_t* h_err = nullptr;
+    int32_t* h_heavy = nullptr;   // h_err + 1
+    int32_t* h_np = nullptr;      // h_err + 2
+    int32_t* h_wend = nullptr;    // h_err + 4, two words
+    int32_t* h_suv = nullptr;     // pinned [2 * sched_cnt]: a scheduled window's endpoints
     int32_t* h_pflag = nullptr;   // pinned host memory, [K]; device views d_pflag / d_proot
     int32_t* h_proot = nullptr;
     int32_t* d_pflag = nullptr;
@@ -802,8 +810,11 @@ static sbcStatus ctx_init(extraArraysPerThread* C)
     HIPCHK(hipMalloc(&C->d_qctr, sizeof(int32_t) * 4));
     HIPCHK(hipMemsetAsync(C->d_qctr, 0, sizeof(int32_t) * 4, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    HIPCHK(hipHostMalloc((void**)&C->h_err, sizeof(int32_t)));
-    *C->h_err = 0;
+    HIPCHK(hipHostMalloc((void**)&C->h_err, 8 * sizeof(int32_t)));
+    C->h_heavy = C->h_err + 1;
+    C->h_np = C->h_err + 2;
+    C->h_wend = C->h_err + 4;
+    memset(C->h_err, 0, 8 * sizeof(int32_t));
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -828,7 +839,7 @@ static void ctx_free(extraArraysPerThread* C)
     if (C->stream2) (void)hipStreamSynchronize(C->stream2);
     for (void* q : {(void*)C->d_qctr2, (void*)C->d_parked2})
         if (q) (void)hipFree(q);
-    for (void* q : {(void*)C->h_pflag, (void*)C->h_proot, (void*)C->h_err})
+    for (void* q : {(void*)C->h_pflag, (void*)C->h_proot, (void*)C->h_err, (void*)C->h_suv})
         if (q) (void)hipHostFree(q);
     if (C->stream2) (void)hipStreamDestroy(C->stream2);
     if (C->d_tprof) (void)hipFree(C->d_tprof);
@@ -1835,8 +1846,10 @@ static sbcStatus finish_parked_item(sbcForestDev* F, extraArraysPerThread* C, co
     if (st) return st;
     while (fs < fe) {
         HIPCHK(sbc_launch_wide_descent(&P, dw, ri, fs, fe, w.base + j, &L, C->num_cu, C->stream));
-        HIPCHK(hipMemcpyAsync(c01, &dw->cnt[0], sizeof(c01), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipMemcpyAsync(C->h_wend, &dw->cnt[0], sizeof(c01), hipMemcpyDeviceToHost, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
+        c01[0] = C->h_wend[0];
+        c01[1] = C->h_wend[1];
         j++;
         if (j + 1 >= lcap) return set_err(SBC_ERR_LEVELS, "BFS depth exceeds the level capacity %d", lcap);
         if (c01[0] < fe || c01[0] > F->nv || c01[1] < 0 || c01[1] > F->nv)
@@ -1880,8 +1893,9 @@ static sbcStatus wide_ascent_commit(sbcForestDev* F, extraArraysPerThread* C, co
                                       C->stream));
         // a deletion's p lost its child s: a parent to update once s's level is done
         if (P.del && lev == w.base) HIPCHK(sbc_launch_wide_mark(&P, dw, ri, w.p, lev - 1, C->stream));
-        HIPCHK(hipMemcpyAsync(&u, &dw->cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipMemcpyAsync(C->h_wend, &dw->cnt[2], sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
         HIPCHK(hipStreamSynchronize(C->stream));
+        u = *C->h_wend;
         if (u < uB || (int64_t)u + nS > F->nv)
             return set_err(SBC_ERR_STATE, "parked item of root %d: ascent list out of range", (int)ri);
         uA = uB;
@@ -1930,10 +1944,10 @@ static sbcStatus finish_root_alone(sbcForestDev* F, extraArraysPerThread* C, con
         HIPCHK(hipMemcpyAsync(C->d_start + ri, &C->h_start[ri], sizeof(int32_t), hipMemcpyHostToDevice, C->stream));
         HIPCHK(sbc_launch_reset(C->d_qctr2, 4, C->stream));
         HIPCHK(sbc_launch_window(&Q, nwb, 1, C->stream));
-        int32_t np = 0;
-        HIPCHK(hipMemcpyAsync(&np, C->d_qctr2 + 2, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+        HIPCHK(hipMemcpyAsync(C->h_np, C->d_qctr2 + 2, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
         // also keeps ri and h_start[ri] unchanged until their uploads ran
         HIPCHK(hipStreamSynchronize(C->stream));
+        const int32_t np = *C->h_np;
         if (np == 0) return SBC_OK;
         if (np != 1) return set_err(SBC_ERR_STATE, "resumed root %d parked %d items", (int)ri, (int)np);
     }
@@ -2530,24 +2544,29 @@ extern "C" sbcStatus streamingBCScheduleWindow(bcForest* forest, struct stinger*
             if (q) HIPCHK(hipFree(q));
         C->d_suv = nullptr; C->d_shist = nullptr; C->d_sX = nullptr; C->d_spred = nullptr; C->d_scost = nullptr;
         C->d_sheavy = nullptr;
+        if (C->h_suv) HIPCHK(hipHostFree(C->h_suv));
+        C->h_suv = nullptr;
         const int64_t ni = items > C->sched_items ? items : C->sched_items;
         const int64_t nk = F->K > C->sched_k ? F->K : C->sched_k;
         const int64_t nc = count > C->sched_cnt ? count : C->sched_cnt;
+        C->sched_items = C->sched_k = C->sched_cnt = 0;  // until every buffer is in place
         if ((st = dalloc(&C->d_suv, (size_t)(2 * nc))) || (st = dalloc(&C->d_shist, (size_t)nk)) ||
             (st = dalloc(&C->d_sX, (size_t)ni * SBC_COST_FEATURES)) || (st = dalloc(&C->d_spred, (size_t)ni)) ||
             (st = dalloc(&C->d_scost, (size_t)nk)) || (st = dalloc(&C->d_sheavy, 1)))
             return st;
+        HIPCHK(hipHostMalloc((void**)&C->h_suv, sizeof(int32_t) * 2 * (size_t)nc));
         C->sched_items = ni;
         C->sched_k = nk;
         C->sched_cnt = nc;
     }
-    std::vector<int32_t> uv(2 * count);
+    // the last call's copy out of h_suv ended with that call's synchronisation
+    int32_t* uv = C->h_suv;
     for (int64_t i = 0; i < count; i++) {
         if (u[i] < 0 || v[i] < 0 || u[i] >= F->nv || v[i] >= F->nv) return set_err(SBC_ERR_ARG, "vertex out of range");
         uv[i] = (int32_t)u[i];
         uv[count + i] = (int32_t)v[i];
     }
-    HIPCHK(hipMemcpyAsync(C->d_suv, uv.data(), sizeof(int32_t) * 2 * count, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipMemcpyAsync(C->d_suv, uv, sizeof(int32_t) * 2 * count, hipMemcpyHostToDevice, C->stream));
     if (root_hist)
         HIPCHK(hipMemcpyAsync(C->d_shist, root_hist, sizeof(float) * F->K, hipMemcpyHostToDevice, C->stream));
     SbcFeatArgs A;
@@ -2564,11 +2583,11 @@ extern "C" sbcStatus streamingBCScheduleWindow(bcForest* forest, struct stinger*
     HIPCHK(sbc_launch_features(&A, C->stream));
     HIPCHK(sbc_cost_schedule(model, C->d_sX, (int)count, (int)F->K, light_max > 0 ? light_max : -1.0f, item_overhead,
                              C->d_spred, C->d_scost, C->d_order, C->d_sheavy, C->stream));
-    int32_t heavy = 0;
-    HIPCHK(hipMemcpyAsync(&heavy, C->d_sheavy, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipMemcpyAsync(C->h_heavy, C->d_sheavy, sizeof(int32_t), hipMemcpyDeviceToHost, C->stream));
     if (pred_out)
         HIPCHK(hipMemcpyAsync(pred_out, C->d_spred, sizeof(float) * (size_t)items, hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
+    const int32_t heavy = *C->h_heavy;
     if (heavy_out) *heavy_out = light_max > 0 ? heavy : -1;
     C->order_ready = true;
     C->order_forest = F;
